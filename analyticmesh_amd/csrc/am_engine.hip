@@ -1,0 +1,800 @@
+// Host side of the engine: the C-ABI of include/am.h.
+//
+// The marching loop is breadth first: a level is the set of regions found by
+// the previous level.  Per level the engine launches the region kernel (one
+// wave64 per region), inserts the level's neighbour candidates into the
+// visited set, and compacts the new regions into the region store, which
+// becomes the next level.  The reference instead pops LIFO batches of 1024
+// (backend/src/cuam_kernel.cu:53-98); both orders reach the same regions.
+#include <cstring>
+#include <hip/hip_runtime.h>
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/am.h"
+#include "am_hash.h"
+#include "am_kernels.h"
+#include "am_fast.h"
+
+using namespace am;
+
+namespace {
+
+std::string g_create_error;
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct HipErr {
+  hipError_t e;
+  const char* what;
+};
+
+#define HIPCHK(x)                                 \
+  do {                                            \
+    hipError_t e_ = (x);                          \
+    if (e_ != hipSuccess) throw HipErr{e_, #x};   \
+  } while (0)
+
+struct ArgErr {
+  std::string msg;
+};
+struct CapErr {
+  std::string msg;
+};
+
+template <typename T>
+void ensure(DevBuf<T>& b, size_t n, bool keep, hipStream_t st) {
+  if (b.n >= n) return;
+  size_t cap = std::max<size_t>(n, b.n ? b.n * 2 : 1024);
+  T* q = nullptr;
+  HIPCHK(hipMalloc(&q, cap * sizeof(T)));
+  if (keep && b.p && b.n) HIPCHK(hipMemcpyAsync(q, b.p, b.n * sizeof(T), hipMemcpyDeviceToDevice, st));
+  if (b.p) {
+    HIPCHK(hipStreamSynchronize(st));
+    HIPCHK(hipFree(b.p));
+  }
+  b.p = q;
+  b.n = cap;
+}
+
+inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
+
+}  // namespace
+
+struct am_ctx {
+  // architecture
+  std::vector<int> nodes, off;
+  int D = 0, L = 0, E = 0, C = 0, Wd = 0, vm = 20, precision = AM_PREC_FP64;
+  std::vector<std::vector<std::pair<int, int>>> skips;  // per linear layer index
+  std::string err;
+  hipStream_t stream = nullptr;
+  int dev = 0;
+  size_t max_lds = 65536;
+  long long max_regions = -1;
+  int table_log2 = 22;
+
+  // network (device)
+  DevBuf<double> W0, ew, eb;
+  std::vector<DevBuf<double>> WT, b, tm, tmT;
+  std::vector<int> tm_h, tm_w;
+  NetDev net{};
+  FastNet fast{};
+  FastBuffers fbuf;
+
+  // region store
+  DevBuf<uint32_t> reg_mask;
+  DevBuf<int> reg_start, reg_info, reg_tab, reg_vpair;
+  DevBuf<double> reg_point, reg_verts;
+  size_t reg_cap = 0;
+  long long R = 0;
+
+  // level candidates
+  DevBuf<uint32_t> cand_mask;
+  DevBuf<int> cand_start, status, winner, slot, owner, first, link, rank, scan_tmp_i;
+  DevBuf<double> cand_point;
+  DevBuf<char> scan_tmp;
+  DevBuf<unsigned> rflags;
+
+  // visited table
+  DevBuf<uint32_t> table;
+  size_t table_cap = 0;
+
+  // mesh
+  bool has_run = false, has_mesh = false;
+  std::vector<double> mesh_v;
+  std::vector<int> mesh_off, mesh_idx;
+
+  double stats[AM_STAT_COUNT] = {0};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void grow_regions(am_ctx* c, size_t need) {
+  if (need <= c->reg_cap) return;
+  size_t cap = std::max<size_t>(need, c->reg_cap ? c->reg_cap * 2 : (1u << 14));
+  const hipStream_t st = c->stream;
+  ensure(c->reg_mask, cap * c->Wd, true, st);
+  ensure(c->reg_start, cap, true, st);
+  ensure(c->reg_point, cap * 3, true, st);
+  ensure(c->reg_info, cap * AM_INFO_STRIDE, true, st);
+  ensure(c->reg_tab, cap * (c->vm + 1), true, st);
+  ensure(c->reg_verts, cap * c->vm * 3, true, st);
+  ensure(c->reg_vpair, cap * c->vm * 2, true, st);
+  c->reg_cap = cap;
+}
+
+void grow_cands(am_ctx* c, size_t n) {
+  const hipStream_t st = c->stream;
+  ensure(c->cand_mask, n * c->Wd, false, st);
+  ensure(c->cand_start, n, false, st);
+  ensure(c->cand_point, n * 3, false, st);
+  ensure(c->status, n, false, st);
+  ensure(c->winner, n, false, st);
+  ensure(c->slot, n, false, st);
+  ensure(c->owner, n, false, st);
+  ensure(c->first, n, false, st);
+  ensure(c->link, n, false, st);
+  ensure(c->rank, n, false, st);
+}
+
+void exclusive_scan(am_ctx* c, const int* in, int* out, size_t n) {
+  size_t bytes = 0;
+  HIPCHK(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, n, rocprim::plus<int>(), c->stream));
+  ensure(c->scan_tmp, bytes + 16, false, c->stream);
+  HIPCHK(rocprim::exclusive_scan(c->scan_tmp.p, bytes, in, out, 0, n, rocprim::plus<int>(), c->stream));
+}
+
+int scan_total(am_ctx* c, const int* flags, int* rank, size_t n) {
+  if (n == 0) return 0;
+  exclusive_scan(c, flags, rank, n);
+  int a = 0, b = 0;
+  HIPCHK(hipMemcpyAsync(&a, rank + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&b, flags + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return a + b;
+}
+
+void ensure_table(am_ctx* c, size_t entries) {
+  size_t need = c->table_cap ? c->table_cap : (size_t(1) << c->table_log2);
+  while (entries * 2 > need) need *= 2;
+  if (need > (size_t(1) << 31)) throw CapErr{"visited set would exceed 2^31 slots"};
+  if (need == c->table_cap) return;
+  const hipStream_t st = c->stream;
+  c->table.release();
+  ensure(c->table, need, false, st);
+  c->table_cap = need;
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(need, 256)), dim3(256), 0, st, c->table.p, need, 0xFFFFFFFFu);
+  if (c->R > 0)
+    hipLaunchKernelGGL(k_rehash, dim3(blocks(c->R, 256)), dim3(256), 0, st, c->reg_mask.p, (int)c->R, c->Wd,
+                       c->table.p, (uint32_t)(need - 1));
+  HIPCHK(hipGetLastError());
+}
+
+// Insert n candidates (already in cand_*) and append the new ones as regions.
+int insert_level(am_ctx* c, size_t n) {
+  if (n == 0) return 0;
+  const hipStream_t st = c->stream;
+  if (n > (size_t)INT_MAX / 2) throw CapErr{"too many candidates in one level"};
+  ensure_table(c, (size_t)c->R + n);
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, c->winner.p, n, INT_MAX);
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, n, 0);
+  InsertIO io;
+  io.cand_mask = c->cand_mask.p;
+  io.cand_start = c->cand_start.p;
+  io.reg_mask = c->reg_mask.p;
+  io.table = c->table.p;
+  io.status = c->status.p;
+  io.winner = c->winner.p;
+  io.slot = c->slot.p;
+  io.owner = c->owner.p;
+  io.n = (int)n;
+  io.Wd = c->Wd;
+  io.cap_mask = (uint32_t)(c->table_cap - 1);
+  hipLaunchKernelGGL(k_insert, dim3(blocks(n, 256)), dim3(256), 0, st, io);
+  hipLaunchKernelGGL(k_mark_first, dim3(blocks(n, 256)), dim3(256), 0, st, c->status.p, c->winner.p, c->first.p,
+                     c->link.p, (int)n);
+  HIPCHK(hipGetLastError());
+  const int total = scan_total(c, c->first.p, c->rank.p, n);
+  if (total == 0) return 0;
+  if ((long long)c->R + total > (long long)INT_MAX / 2) throw CapErr{"region count exceeds 2^30"};
+  grow_regions(c, (size_t)c->R + total);
+  hipLaunchKernelGGL(k_commit, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, c->rank.p, c->link.p, c->slot.p,
+                     c->cand_mask.p, c->cand_start.p, c->cand_point.p, c->reg_mask.p, c->reg_start.p,
+                     c->reg_point.p, c->table.p, (int)n, c->Wd, (int)c->R);
+  HIPCHK(hipGetLastError());
+  c->R += total;
+  return total;
+}
+
+size_t region_lds_bytes(const am_ctx* c, int& Cs, int& Csort) {
+  Cs = c->C;
+  Csort = 1;
+  while (Csort < c->C) Csort <<= 1;
+  return (size_t)4 * Cs * sizeof(double) + (size_t)Csort * (sizeof(double) + sizeof(int)) +
+         (size_t)((c->C + 31) / 32) * 4 + (size_t)c->Wd * 4 + 64;
+}
+
+void launch_region_fp64(am_ctx* c, int first, int count, float* ms) {
+  int Cs, Csort;
+  const size_t lds = region_lds_bytes(c, Cs, Csort);
+  RegionIO io;
+  io.reg_mask = c->reg_mask.p;
+  io.reg_start = c->reg_start.p;
+  io.reg_point = c->reg_point.p;
+  io.reg_info = c->reg_info.p;
+  io.reg_tab = c->reg_tab.p;
+  io.reg_verts = c->reg_verts.p;
+  io.reg_vpair = c->reg_vpair.p;
+  io.cand_mask = c->cand_mask.p;
+  io.cand_start = c->cand_start.p;
+  io.cand_point = c->cand_point.p;
+  io.first = first;
+  io.count = count;
+  io.Cs = Cs;
+  io.Csort = Csort;
+  io.flags = nullptr;
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  hipLaunchKernelGGL(k_region_fp64, dim3(count), dim3(64), lds, c->stream, c->net, io);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+}
+
+void upload(DevBuf<double>& dst, const void* src, size_t n, int dtype, hipStream_t st) {
+  ensure(dst, std::max<size_t>(n, 1), false, st);
+  if (n == 0) return;
+  std::vector<double> h(n);
+  if (dtype == AM_DTYPE_F64) {
+    HIPCHK(hipMemcpy(h.data(), src, n * sizeof(double), hipMemcpyDefault));
+  } else {
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpy(f.data(), src, n * sizeof(float), hipMemcpyDefault));
+    for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
+  }
+  HIPCHK(hipMemcpy(dst.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+}
+
+void upload_T(DevBuf<double>& dst, const void* src, int rows, int cols, int dtype, hipStream_t st) {
+  // (rows, cols) row-major -> (cols, rows) row-major
+  const size_t n = (size_t)rows * cols;
+  std::vector<double> h(n), t(n);
+  if (dtype == AM_DTYPE_F64) {
+    HIPCHK(hipMemcpy(h.data(), src, n * sizeof(double), hipMemcpyDefault));
+  } else {
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpy(f.data(), src, n * sizeof(float), hipMemcpyDefault));
+    for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
+  }
+  for (int r = 0; r < rows; ++r)
+    for (int q = 0; q < cols; ++q) t[(size_t)q * rows + r] = h[(size_t)r * cols + q];
+  ensure(dst, std::max<size_t>(n, 1), false, st);
+  HIPCHK(hipMemcpy(dst.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+}
+
+am_status fail(am_ctx* c, am_status s, const std::string& m) {
+  if (c) c->err = m;
+  return s;
+}
+
+template <typename F>
+am_status guarded(am_ctx* c, F&& f) {
+  try {
+    return f();
+  } catch (const HipErr& e) {
+    return fail(c, AM_ERR_HIP, std::string("HIP error: ") + hipGetErrorString(e.e) + " in " + e.what);
+  } catch (const ArgErr& e) {
+    return fail(c, AM_ERR_ARG, e.msg);
+  } catch (const CapErr& e) {
+    return fail(c, AM_ERR_CAPACITY, e.msg);
+  } catch (const std::exception& e) {
+    return fail(c, AM_ERR_HIP, std::string("error: ") + e.what());
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_table, int arc_rows,
+                    int arc_cols, int num_extra_constraints, int precision) {
+  if (!out) return AM_ERR_ARG;
+  *out = nullptr;
+  if (n_nodes < 3 || !nodes || nodes[0] != 3 || nodes[n_nodes - 1] != 1) {
+    g_create_error = "nodes must start with 3, end with 1 and have at least 3 entries";
+    return AM_ERR_ARG;
+  }
+  if (n_nodes - 1 > AM_MAX_LINEAR) {
+    g_create_error = "too many layers";
+    return AM_ERR_ARG;
+  }
+  if (arc_rows != n_nodes - 2 || arc_cols < 1 || (arc_cols + 1) % 2 != 0 || !arc_table) {
+    g_create_error = "arc_table must be int32 [len(nodes)-2, 2k+1]";
+    return AM_ERR_ARG;
+  }
+  if (num_extra_constraints < 0) {
+    g_create_error = "num_extra_constraints must be >= 0";
+    return AM_ERR_ARG;
+  }
+  if (precision != AM_PREC_FP32 && precision != AM_PREC_SPLIT_BF16X3 && precision != AM_PREC_FP64) {
+    g_create_error = "unknown precision";
+    return AM_ERR_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    g_create_error = "no HIP device available";
+    return AM_ERR_NOGPU;
+  }
+  am_ctx* c = new am_ctx();
+  c->nodes.assign(nodes, nodes + n_nodes);
+  c->D = n_nodes - 2;
+  c->off.assign(c->D + 1, 0);
+  for (int k = 0; k < c->D; ++k) c->off[k + 1] = c->off[k] + nodes[k + 1];
+  c->L = c->off[c->D];
+  c->E = num_extra_constraints;
+  c->C = c->L + c->E;
+  c->Wd = (c->L + 31) / 32;
+  c->precision = precision;
+  if (c->Wd > AM_MAX_WORDS) {
+    g_create_error = "more than 2048 hidden neurons";
+    delete c;
+    return AM_ERR_ARG;
+  }
+  c->skips.assign(c->D + 1, {});
+  for (int row = 0; row < arc_rows; ++row) {
+    const int cnt = arc_table[row * arc_cols];
+    if (cnt < 0 || 1 + 2 * cnt > arc_cols) {
+      g_create_error = "arc_table row declares more skips than it holds";
+      delete c;
+      return AM_ERR_ARG;
+    }
+    if (row + 1 >= c->D) continue;  // output-layer skips: ignored, as the reference does
+    if (cnt > AM_MAX_SKIPS) {
+      g_create_error = "too many skips into one layer";
+      delete c;
+      return AM_ERR_ARG;
+    }
+    for (int k = 0; k < cnt; ++k)
+      c->skips[row + 1].push_back({arc_table[row * arc_cols + 1 + 2 * k], arc_table[row * arc_cols + 2 + 2 * k]});
+  }
+  am_status s = guarded(c, [&]() -> am_status {
+    HIPCHK(hipGetDevice(&c->dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, c->dev));
+    c->max_lds = prop.sharedMemPerBlock;
+    int optin = 0;
+    if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->dev) == hipSuccess && optin > 0)
+      c->max_lds = std::max<size_t>(c->max_lds, (size_t)optin);
+    HIPCHK(hipEventCreate(&c->ev0));
+    HIPCHK(hipEventCreate(&c->ev1));
+    return AM_OK;
+  });
+  if (s != AM_OK) {
+    g_create_error = c->err;
+    delete c;
+    return s;
+  }
+  *out = c;
+  return AM_OK;
+}
+
+am_status am_set_option(am_ctx* c, const char* key, long long value) {
+  if (!c || !key) return AM_ERR_ARG;
+  const std::string k(key);
+  if (k == "vm") {
+    if (value < 3 || value > AM_VM_MAX) return fail(c, AM_ERR_ARG, "vm must be in [3, 64]");
+    if (c->has_run) return fail(c, AM_ERR_STATE, "vm must be set before the first am_run");
+    c->vm = (int)value;
+  } else if (k == "table_log2") {
+    if (value < 10 || value > 31) return fail(c, AM_ERR_ARG, "table_log2 must be in [10, 31]");
+    c->table_log2 = (int)value;
+  } else if (k == "max_regions") {
+    c->max_regions = value;
+  } else {
+    return fail(c, AM_ERR_ARG, "unknown option " + k);
+  }
+  return AM_OK;
+}
+
+am_status am_run(am_ctx* c, const void* const* weights, const void* const* biases, int n_tm,
+                 const int* tm_shapes, const void* const* tms, int dtype, const void* points,
+                 const uint8_t* states, int n_seeds, const void* extra_w, const void* extra_b, double iso,
+                 int flip_insideout, void* stream) {
+  if (!c) return AM_ERR_ARG;
+  return guarded(c, [&]() -> am_status {
+    if (!weights || !biases) throw ArgErr{"weights and biases are required"};
+    if (dtype != AM_DTYPE_F32 && dtype != AM_DTYPE_F64) throw ArgErr{"dtype must be AM_DTYPE_F32 or AM_DTYPE_F64"};
+    if (n_seeds < 1 || !points || !states) throw ArgErr{"at least one seed (point + state) is required"};
+    if (c->E > 0 && (!extra_w || !extra_b)) throw ArgErr{"extra constraints declared but not given"};
+    if (n_tm < 0 || n_tm > AM_MAX_TMS || (n_tm > 0 && (!tm_shapes || !tms))) throw ArgErr{"bad transform matrices"};
+    for (auto& row : c->skips)
+      for (auto& s : row) {
+        if (s.second < 0 || s.second >= n_tm) throw ArgErr{"arc_table names a missing transform matrix"};
+        if (s.first < 0 || s.first > c->D) throw ArgErr{"arc_table names a missing source layer"};
+      }
+    c->stream = (hipStream_t)stream;
+    const hipStream_t st = c->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < AM_STAT_COUNT; ++i) c->stats[i] = 0;
+    c->has_run = false;
+    c->has_mesh = false;
+
+    // ---- network upload (fp64, weights transposed for the chain)
+    const int nl = c->D + 1;
+    c->WT.resize(nl);
+    c->b.resize(nl);
+    upload(c->W0, weights[0], (size_t)c->nodes[1] * 3, dtype, st);
+    for (int li = 0; li < nl; ++li) {
+      upload(c->b[li], biases[li], (size_t)c->nodes[li + 1], dtype, st);
+      if (li >= 1) upload_T(c->WT[li], weights[li], c->nodes[li + 1], c->nodes[li], dtype, st);
+    }
+    c->tm.resize(n_tm);
+    c->tmT.resize(n_tm);
+    c->tm_h.assign(n_tm, 0);
+    c->tm_w.assign(n_tm, 0);
+    for (int t = 0; t < n_tm; ++t) {
+      const int h = tm_shapes[2 * t], w = tm_shapes[2 * t + 1];
+      c->tm_h[t] = h;
+      c->tm_w[t] = w;
+      if (h > 0 && w > 0) {
+        upload(c->tm[t], tms[t], (size_t)h * w, dtype, st);
+        upload_T(c->tmT[t], tms[t], h, w, dtype, st);
+      }
+    }
+    // skip shape checks (backend/src/cuam.cpp:160-172)
+    for (int li = 1; li < c->D; ++li)
+      for (auto& s : c->skips[li]) {
+        const int h = c->tm_h[s.second], w = c->tm_w[s.second];
+        if (h == 0 && w == 0) {
+          if (s.first > 0 && c->nodes[s.first] != c->nodes[li + 1])
+            throw ArgErr{"identity skip between layers of different widths"};
+        } else {
+          if (h != c->nodes[li + 1]) throw ArgErr{"transform matrix rows must match the destination width"};
+          if (w != c->nodes[s.first]) throw ArgErr{"transform matrix columns must match the source width"};
+        }
+      }
+    upload(c->ew, extra_w, (size_t)c->E * 3, dtype, st);
+    upload(c->eb, extra_b, (size_t)c->E, dtype, st);
+
+    NetDev& n = c->net;
+    std::memset(&n, 0, sizeof(n));
+    n.D = c->D; n.L = c->L; n.E = c->E; n.C = c->C; n.Wd = c->Wd; n.vm = c->vm;
+    for (int i = 0; i < (int)c->nodes.size(); ++i) n.nodes[i] = c->nodes[i];
+    for (int i = 0; i <= c->D; ++i) n.off[i] = c->off[i];
+    n.W0 = c->W0.p;
+    for (int li = 0; li < nl; ++li) {
+      n.WT[li] = li >= 1 ? c->WT[li].p : nullptr;
+      n.b[li] = c->b[li].p;
+    }
+    for (int li = 0; li < nl; ++li) {
+      n.nskip[li] = (int)c->skips[li].size();
+      for (size_t s = 0; s < c->skips[li].size(); ++s) {
+        n.skip_from[li][s] = c->skips[li][s].first;
+        n.skip_tm[li][s] = c->skips[li][s].second;
+      }
+    }
+    for (int t = 0; t < n_tm; ++t) {
+      n.tm[t] = c->tm[t].p;
+      n.tmT[t] = c->tmT[t].p;
+      n.tm_h[t] = c->tm_h[t];
+      n.tm_w[t] = c->tm_w[t];
+    }
+    n.ew = c->ew.p;
+    n.eb = c->eb.p;
+    n.iso = iso;
+    n.flip = flip_insideout ? 1 : 0;
+
+    int Cs, Csort;
+    const size_t lds = region_lds_bytes(c, Cs, Csort);
+    if (lds > c->max_lds)
+      throw CapErr{"network too wide for the FP64 region kernel's LDS (" + std::to_string(lds) + " bytes)"};
+    if (lds > 65536)
+      HIPCHK(hipFuncSetAttribute((const void*)k_region_fp64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (c->precision != AM_PREC_FP64) fast_prepare(c->fast, c->fbuf, n, c->precision, weights, biases, dtype, st);
+
+    // ---- seeds -> first level (backend/inc/states.h:147-176)
+    c->R = 0;
+    c->table_cap = 0;
+    c->table.release();
+    grow_cands(c, (size_t)n_seeds);
+    {
+      std::vector<uint32_t> m((size_t)n_seeds * c->Wd, 0u);
+      std::vector<uint8_t> hs((size_t)n_seeds * c->L);
+      HIPCHK(hipMemcpy(hs.data(), states, hs.size(), hipMemcpyDefault));
+      for (int s = 0; s < n_seeds; ++s)
+        for (int g = 0; g < c->L; ++g)
+          if (hs[(size_t)s * c->L + g]) m[(size_t)s * c->Wd + (g >> 5)] |= 1u << (g & 31);
+      std::vector<double> pts((size_t)n_seeds * 3);
+      if (dtype == AM_DTYPE_F64) {
+        HIPCHK(hipMemcpy(pts.data(), points, pts.size() * sizeof(double), hipMemcpyDefault));
+      } else {
+        std::vector<float> f(pts.size());
+        HIPCHK(hipMemcpy(f.data(), points, f.size() * sizeof(float), hipMemcpyDefault));
+        for (size_t i = 0; i < f.size(); ++i) pts[i] = f[i];
+      }
+      std::vector<int> startv(n_seeds, -1);
+      HIPCHK(hipMemcpy(c->cand_mask.p, m.data(), m.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->cand_point.p, pts.data(), pts.size() * 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->cand_start.p, startv.data(), startv.size() * 4, hipMemcpyHostToDevice));
+    }
+    long long lvl_first = 0;
+    long long lvl_count = insert_level(c, (size_t)n_seeds);
+    double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0;
+    long long levels = 0, cands = 0, redo = 0, launches = 0;
+    while (lvl_count > 0) {
+      if (c->max_regions >= 0 && lvl_first >= c->max_regions) break;
+      const size_t ncand = (size_t)lvl_count * c->vm;
+      grow_cands(c, ncand);
+      float ms = 0;
+      if (c->precision == AM_PREC_FP64) {
+        launch_region_fp64(c, (int)lvl_first, (int)lvl_count, &ms);
+        t_region += ms;
+        launches += 1;
+      } else {
+        FastLevel fl;
+        fl.first = (int)lvl_first;
+        fl.count = (int)lvl_count;
+        fl.reg_mask = c->reg_mask.p; fl.reg_start = c->reg_start.p; fl.reg_point = c->reg_point.p;
+        fl.reg_info = c->reg_info.p; fl.reg_tab = c->reg_tab.p; fl.reg_verts = c->reg_verts.p;
+        fl.reg_vpair = c->reg_vpair.p; fl.cand_mask = c->cand_mask.p; fl.cand_start = c->cand_start.p;
+        fl.cand_point = c->cand_point.p;
+        FastTimes ft{};
+        int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->ev0, c->ev1, ft);
+        if (nredo > 0) {
+          // FP64 re-decision of the undecided regions: rerun them through the
+          // FP64 kernel in place (their candidate slots are rewritten too).
+          float m2 = 0;
+          redo_fp64(c->fbuf, n, fl, nredo, st, c->ev0, c->ev1, &m2, lds, Cs, Csort);
+          t_redo += m2;
+          redo += nredo;
+        }
+        t_gemm += ft.gemm_ms;
+        t_walk += ft.walk_ms;
+        t_region += ft.gemm_ms + ft.walk_ms;
+        launches += ft.launches;
+      }
+      HIPCHK(hipEventRecord(c->ev0, st));
+      const long long before = c->R;
+      const int added = insert_level(c, ncand);
+      HIPCHK(hipEventRecord(c->ev1, st));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+      t_insert += ms;
+      cands += (long long)ncand;
+      levels += 1;
+      lvl_first = before;
+      lvl_count = added;
+    }
+    if (lvl_count > 0 && c->max_regions >= 0) {
+      // regions found but not processed: drop them from the store
+      c->R = lvl_first;
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    const auto t1 = std::chrono::steady_clock::now();
+    c->stats[AM_STAT_RUN_MS] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->stats[AM_STAT_REGION_MS] = t_region;
+    c->stats[AM_STAT_INSERT_MS] = t_insert;
+    c->stats[AM_STAT_GEMM_MS] = t_gemm;
+    c->stats[AM_STAT_WALK_MS] = t_walk;
+    c->stats[AM_STAT_REDO_MS] = t_redo;
+    c->stats[AM_STAT_LEVELS] = (double)levels;
+    c->stats[AM_STAT_REGIONS] = (double)c->R;
+    c->stats[AM_STAT_CANDIDATES] = (double)cands;
+    c->stats[AM_STAT_REDO_REGIONS] = (double)redo;
+    c->stats[AM_STAT_REGION_LAUNCHES] = (double)launches;
+    c->has_run = true;
+    return AM_OK;
+  });
+}
+
+am_status am_combine(am_ctx* c, double scale, const double* center3) {
+  if (!c) return AM_ERR_ARG;
+  if (!c->has_run) return fail(c, AM_ERR_STATE, "am_combine called before am_run");
+  return guarded(c, [&]() -> am_status {
+    const auto t0 = std::chrono::steady_clock::now();
+    const hipStream_t st = c->stream;
+    const double cx = center3 ? center3[0] : 0.0, cy = center3 ? center3[1] : 0.0, cz = center3 ? center3[2] : 0.0;
+    const size_t R = (size_t)c->R;
+    const size_t n = R * c->vm;
+    c->mesh_v.clear();
+    c->mesh_off.assign(1, 0);
+    c->mesh_idx.clear();
+    if (R == 0) {
+      c->has_mesh = true;
+      return AM_OK;
+    }
+    size_t cap = 1024;
+    while (cap < 2 * n) cap *= 2;
+    DevBuf<uint32_t> vt;
+    DevBuf<int> vstat, vwin, vown, vfirst, vrank, vid, fflag, fnv, frank, foff, face_off, face_idx;
+    DevBuf<double> xyz;
+    ensure(vt, cap, false, st);
+    ensure(vstat, n, false, st);
+    ensure(vwin, n, false, st);
+    ensure(vown, n, false, st);
+    ensure(vfirst, n, false, st);
+    ensure(vrank, n, false, st);
+    ensure(vid, n, false, st);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(cap, 256)), dim3(256), 0, st, vt.p, cap, 0xFFFFFFFFu);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, vwin.p, n, INT_MAX);
+    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, vfirst.p, n, 0);
+    VertexIO io;
+    io.reg_mask = c->reg_mask.p;
+    io.info = c->reg_info.p;
+    io.vpair = c->reg_vpair.p;
+    io.verts = c->reg_verts.p;
+    io.table = vt.p;
+    io.status = vstat.p;
+    io.winner = vwin.p;
+    io.owner = vown.p;
+    io.n = (int)n;
+    io.vm = c->vm;
+    io.L = c->L;
+    io.Wd = c->Wd;
+    io.cap_mask = (uint32_t)(cap - 1);
+    hipLaunchKernelGGL(k_vinsert, dim3(blocks(n, 256)), dim3(256), 0, st, io);
+    hipLaunchKernelGGL(k_vmark_first, dim3(blocks(n, 256)), dim3(256), 0, st, vstat.p, vwin.p, vfirst.p, (int)n);
+    HIPCHK(hipGetLastError());
+    const int V = scan_total(c, vfirst.p, vrank.p, n);
+    ensure(xyz, std::max<size_t>((size_t)V * 3, 1), false, st);
+    hipLaunchKernelGGL(k_vassign, dim3(blocks(n, 256)), dim3(256), 0, st, io, vfirst.p, vrank.p, vid.p, xyz.p,
+                       scale, cx, cy, cz);
+    ensure(fflag, R, false, st);
+    ensure(fnv, R, false, st);
+    ensure(frank, R, false, st);
+    ensure(foff, R, false, st);
+    hipLaunchKernelGGL(k_face_flags, dim3(blocks(R, 256)), dim3(256), 0, st, c->reg_info.p, (int)R, fflag.p, fnv.p);
+    HIPCHK(hipGetLastError());
+    const int F = scan_total(c, fflag.p, frank.p, R);
+    const int NI = scan_total(c, fnv.p, foff.p, R);
+    ensure(face_off, std::max(F, 1), false, st);
+    ensure(face_idx, std::max(NI, 1), false, st);
+    hipLaunchKernelGGL(k_face_write, dim3(blocks(R, 256)), dim3(256), 0, st, c->reg_info.p, fflag.p, frank.p, foff.p,
+                       vid.p, (int)R, c->vm, face_off.p, face_idx.p);
+    HIPCHK(hipGetLastError());
+    c->mesh_v.resize((size_t)V * 3);
+    c->mesh_off.resize((size_t)F + 1);
+    c->mesh_idx.resize((size_t)NI);
+    if (V) HIPCHK(hipMemcpyAsync(c->mesh_v.data(), xyz.p, (size_t)V * 3 * 8, hipMemcpyDeviceToHost, st));
+    if (F) HIPCHK(hipMemcpyAsync(c->mesh_off.data(), face_off.p, (size_t)F * 4, hipMemcpyDeviceToHost, st));
+    if (NI) HIPCHK(hipMemcpyAsync(c->mesh_idx.data(), face_idx.p, (size_t)NI * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->mesh_off[F] = NI;
+    c->has_mesh = true;
+    c->stats[AM_STAT_COMBINE_MS] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return AM_OK;
+  });
+}
+
+am_status am_get_counts(am_ctx* c, long long* nr, long long* nf, long long* nv, long long* ni) {
+  if (!c) return AM_ERR_ARG;
+  if (nr) *nr = c->R;
+  if (nf) *nf = c->has_mesh ? (long long)c->mesh_off.size() - 1 : 0;
+  if (nv) *nv = c->has_mesh ? (long long)c->mesh_v.size() / 3 : 0;
+  if (ni) *ni = c->has_mesh ? (long long)c->mesh_idx.size() : 0;
+  return AM_OK;
+}
+
+am_status am_get_mesh(am_ctx* c, double* vertices, int* face_offsets, int* face_indices) {
+  if (!c) return AM_ERR_ARG;
+  if (!c->has_mesh) return fail(c, AM_ERR_STATE, "am_get_mesh called before am_combine");
+  if (vertices) std::memcpy(vertices, c->mesh_v.data(), c->mesh_v.size() * 8);
+  if (face_offsets) std::memcpy(face_offsets, c->mesh_off.data(), c->mesh_off.size() * 4);
+  if (face_indices) std::memcpy(face_indices, c->mesh_idx.data(), c->mesh_idx.size() * 4);
+  return AM_OK;
+}
+
+am_status am_get_regions(am_ctx* c, uint32_t* masks, int* info, int* tabs, double* verts, int* vpairs,
+                         double* points) {
+  if (!c) return AM_ERR_ARG;
+  if (!c->has_run) return fail(c, AM_ERR_STATE, "am_get_regions called before am_run");
+  return guarded(c, [&]() -> am_status {
+    const size_t R = (size_t)c->R;
+    if (R == 0) return AM_OK;
+    if (masks) HIPCHK(hipMemcpy(masks, c->reg_mask.p, R * c->Wd * 4, hipMemcpyDeviceToHost));
+    if (info) HIPCHK(hipMemcpy(info, c->reg_info.p, R * AM_INFO_STRIDE * 4, hipMemcpyDeviceToHost));
+    if (tabs) HIPCHK(hipMemcpy(tabs, c->reg_tab.p, R * (c->vm + 1) * 4, hipMemcpyDeviceToHost));
+    if (verts) HIPCHK(hipMemcpy(verts, c->reg_verts.p, R * c->vm * 3 * 8, hipMemcpyDeviceToHost));
+    if (vpairs) HIPCHK(hipMemcpy(vpairs, c->reg_vpair.p, R * c->vm * 2 * 4, hipMemcpyDeviceToHost));
+    if (points) HIPCHK(hipMemcpy(points, c->reg_point.p, R * 3 * 8, hipMemcpyDeviceToHost));
+    return AM_OK;
+  });
+}
+
+am_status am_get_stats(am_ctx* c, double* out, int n) {
+  if (!c || !out) return AM_ERR_ARG;
+  for (int i = 0; i < n && i < AM_STAT_COUNT; ++i) out[i] = c->stats[i];
+  return AM_OK;
+}
+
+am_status am_export_ply(am_ctx* c, const char* path, int is_polymesh, int is_float32) {
+  if (!c || !path) return AM_ERR_ARG;
+  if (!c->has_mesh) return fail(c, AM_ERR_STATE, "am_export_ply called before am_combine");
+  const long long V = (long long)c->mesh_v.size() / 3;
+  const long long F = (long long)c->mesh_off.size() - 1;
+  long long T = 0;
+  for (long long f = 0; f < F; ++f) T += c->mesh_off[f + 1] - c->mesh_off[f] - 2;
+  std::ofstream o(path, std::ios::out | std::ios::binary);
+  if (!o) return fail(c, AM_ERR_IO, std::string("cannot open ") + path);
+  // header layout of backend/inc/polymesh.h:368-377
+  o << "ply\n";
+  o << "format binary_little_endian 1.0\n";
+  o << "element vertex " << V << "\n";
+  const char* ft = is_float32 ? "float" : "double";
+  o << "property " << ft << " x\n";
+  o << "property " << ft << " y\n";
+  o << "property " << ft << " z\n";
+  o << "element face " << (is_polymesh ? F : T) << "\n";
+  o << "property list uchar int vertex_index\n";
+  o << "end_header\n";
+  if (is_float32) {
+    std::vector<float> f(c->mesh_v.size());
+    for (size_t i = 0; i < f.size(); ++i) f[i] = (float)c->mesh_v[i];
+    o.write((const char*)f.data(), f.size() * 4);
+  } else {
+    o.write((const char*)c->mesh_v.data(), c->mesh_v.size() * 8);
+  }
+  std::vector<char> buf;
+  buf.reserve((size_t)(is_polymesh ? F * 5 + c->mesh_idx.size() * 4 : T * 13));
+  auto put_i = [&](int v) {
+    const char* p = (const char*)&v;
+    buf.insert(buf.end(), p, p + 4);
+  };
+  for (long long f = 0; f < F; ++f) {
+    const int a = c->mesh_off[f], n = c->mesh_off[f + 1] - a;
+    if (is_polymesh) {
+      buf.push_back((char)(unsigned char)n);
+      for (int j = 0; j < n; ++j) put_i(c->mesh_idx[a + j]);
+    } else {
+      for (int j = 0; j < n - 2; ++j) {  // fan (backend/inc/polymesh.h:405-419)
+        buf.push_back((char)3);
+        put_i(c->mesh_idx[a]);
+        put_i(c->mesh_idx[a + j + 1]);
+        put_i(c->mesh_idx[a + j + 2]);
+      }
+    }
+  }
+  o.write(buf.data(), buf.size());
+  o.close();
+  if (!o) return fail(c, AM_ERR_IO, std::string("write failed: ") + path);
+  return AM_OK;
+}
+
+am_status am_destroy(am_ctx* c) {
+  if (!c) return AM_ERR_ARG;
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  fast_release(c->fbuf);
+  delete c;
+  return AM_OK;
+}
+
+const char* am_last_error(const am_ctx* c) {
+  if (!c) return g_create_error.c_str();
+  return c->err.c_str();
+}
+
+}  // extern "C"

@@ -1,0 +1,215 @@
+// Visited set, frontier compaction and vertex deduplication.
+//
+// Visited set (replaces backend/inc/hash.h:33-67 and backend/src/states.cu:30-79):
+// an open-addressing table in HBM whose slots hold either a region id or,
+// while a level is being inserted, "pending | candidate index".  Keys are the
+// full packed masks, compared word by word against immutable copies (the
+// region store or the level's candidate buffer), so membership is exact.
+// Among equal candidates of one level the lowest candidate index wins, which
+// makes the next frontier's order independent of atomic timing.
+//
+// Vertex deduplication (replaces the quantised-coordinate hash of
+// backend/inc/polymesh.h:144-233): a vertex is keyed by its two tight
+// constraints (lo, hi) and the region mask with those two bits cleared; every
+// region around the vertex produces the same key.  The first occurrence in
+// (region, vertex) order gives the vertex its index and coordinates.
+#include "am_device.h"
+#include "am_kernels.h"
+#include "am_hash.h"
+
+namespace am {
+
+#define AM_EMPTY 0xFFFFFFFFu
+#define AM_PEND 0x80000000u
+#define AM_HSEED 0x6a09e667f3bcc909ULL
+
+__device__ __forceinline__ bool words_equal(const uint32_t* a, const uint32_t* b, int n) {
+  for (int w = 0; w < n; ++w)
+    if (a[w] != b[w]) return false;
+  return true;
+}
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_i32(int* p, size_t n, int v) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_insert(InsertIO io) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= io.n) return;
+  if (io.cand_start[c] == -2) { io.status[c] = 0; return; }
+  const uint32_t* key = io.cand_mask + (size_t)c * io.Wd;
+  uint32_t s = (uint32_t)hash_words(key, io.Wd, AM_HSEED) & io.cap_mask;
+  for (uint32_t probes = 0; probes <= io.cap_mask; ++probes) {
+    const uint32_t prev = atomicCAS(&io.table[s], AM_EMPTY, AM_PEND | (uint32_t)c);
+    if (prev == AM_EMPTY) {
+      io.status[c] = 1;
+      io.slot[c] = (int)s;
+      atomicMin(&io.winner[c], c);
+      return;
+    }
+    if (prev & AM_PEND) {
+      const int o = (int)(prev & ~AM_PEND);
+      if (words_equal(key, io.cand_mask + (size_t)o * io.Wd, io.Wd)) {
+        io.status[c] = 2;
+        io.owner[c] = o;
+        atomicMin(&io.winner[o], c);
+        return;
+      }
+    } else if (words_equal(key, io.reg_mask + (size_t)prev * io.Wd, io.Wd)) {
+      io.status[c] = 3;
+      return;
+    }
+    s = (s + 1) & io.cap_mask;
+  }
+  io.status[c] = 4;  // table full: the host sizes the table so this never happens
+}
+
+// first[winner] = 1 and link[winner] = owner for every group of equal candidates.
+__global__ void k_mark_first(const int* status, const int* winner, int* first, int* link, int n) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  if (status[c] == 1) {
+    const int w = winner[c];
+    first[w] = 1;
+    link[w] = c;
+  }
+}
+
+// New regions get ids R0 + rank in candidate order; the table slot is finalised.
+__global__ void k_commit(const int* first, const int* rank, const int* link, const int* slot,
+                         const uint32_t* cand_mask, const int* cand_start, const double* cand_point,
+                         uint32_t* reg_mask, int* reg_start, double* reg_point, uint32_t* table,
+                         int n, int Wd, int R0) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n || !first[c]) return;
+  const int rid = R0 + rank[c];
+  for (int w = 0; w < Wd; ++w) reg_mask[(size_t)rid * Wd + w] = cand_mask[(size_t)c * Wd + w];
+  reg_start[rid] = cand_start[c];
+  reg_point[(size_t)rid * 3 + 0] = cand_point[(size_t)c * 3 + 0];
+  reg_point[(size_t)rid * 3 + 1] = cand_point[(size_t)c * 3 + 1];
+  reg_point[(size_t)rid * 3 + 2] = cand_point[(size_t)c * 3 + 2];
+  table[slot[link[c]]] = (uint32_t)rid;
+}
+
+// Re-insert every region id into an empty (larger) table.
+__global__ void k_rehash(const uint32_t* reg_mask, int R, int Wd, uint32_t* table, uint32_t cap_mask) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  uint32_t s = (uint32_t)hash_words(reg_mask + (size_t)r * Wd, Wd, AM_HSEED) & cap_mask;
+  for (uint32_t probes = 0; probes <= cap_mask; ++probes) {
+    if (atomicCAS(&table[s], AM_EMPTY, (uint32_t)r) == AM_EMPTY) return;
+    s = (s + 1) & cap_mask;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// mesh assembly
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void vkey_parts(const int* vpair, const int* info, int vm, int L, int q,
+                                           int& r, int& lo, int& hi) {
+  r = q / vm;
+  const int k = q - r * vm;
+  const int a = vpair[(size_t)r * vm * 2 + k * 2 + 0];
+  const int b = vpair[(size_t)r * vm * 2 + k * 2 + 1];
+  lo = a < b ? a : b;
+  hi = a < b ? b : a;
+  (void)info;
+  (void)L;
+}
+
+__device__ __forceinline__ uint32_t masked_word(const uint32_t* m, int w, int lo, int hi, int L) {
+  uint32_t x = m[w];
+  if (lo >= 0 && lo < L && (lo >> 5) == w) x &= ~(1u << (lo & 31));
+  if (hi >= 0 && hi < L && (hi >> 5) == w) x &= ~(1u << (hi & 31));
+  return x;
+}
+
+__global__ void k_vinsert(VertexIO io) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= io.n) return;
+  const int r = q / io.vm, k = q - (q / io.vm) * io.vm;
+  const int nv = io.info[(size_t)r * AM_INFO_STRIDE + 2];
+  if (nv < 3 || k >= nv) { io.status[q] = 0; return; }
+  int rr, lo, hi;
+  vkey_parts(io.vpair, io.info, io.vm, io.L, q, rr, lo, hi);
+  const uint32_t* m = io.reg_mask + (size_t)r * io.Wd;
+  uint64_t h = mix64(AM_HSEED ^ (uint64_t(uint32_t(lo)) << 32 | uint32_t(hi)));
+  for (int w = 0; w < io.Wd; ++w)
+    h = mix64(h ^ (uint64_t(masked_word(m, w, lo, hi, io.L)) + 0x9e3779b97f4a7c15ULL * uint64_t(w + 1)));
+  uint32_t s = (uint32_t)h & io.cap_mask;
+  for (uint32_t probes = 0; probes <= io.cap_mask; ++probes) {
+    const uint32_t prev = atomicCAS(&io.table[s], AM_EMPTY, (uint32_t)q);
+    if (prev == AM_EMPTY) {
+      io.status[q] = 1;
+      atomicMin(&io.winner[q], q);
+      return;
+    }
+    int r2, lo2, hi2;
+    vkey_parts(io.vpair, io.info, io.vm, io.L, (int)prev, r2, lo2, hi2);
+    bool eq = (lo2 == lo && hi2 == hi);
+    if (eq) {
+      const uint32_t* m2 = io.reg_mask + (size_t)r2 * io.Wd;
+      for (int w = 0; w < io.Wd && eq; ++w)
+        eq = masked_word(m, w, lo, hi, io.L) == masked_word(m2, w, lo2, hi2, io.L);
+    }
+    if (eq) {
+      io.status[q] = 2;
+      io.owner[q] = (int)prev;
+      atomicMin(&io.winner[prev], q);
+      return;
+    }
+    s = (s + 1) & io.cap_mask;
+  }
+  io.status[q] = 4;
+}
+
+__global__ void k_vmark_first(const int* status, const int* winner, int* first, int n) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  if (status[q] == 1) first[winner[q]] = 1;
+}
+
+// Vertex index of every (region, vertex) item and the unique vertex coordinates.
+__global__ void k_vassign(VertexIO io, const int* first, const int* rank, int* vid, double* out_xyz,
+                          double scale, double cx, double cy, double cz) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= io.n) return;
+  const int st = io.status[q];
+  if (st != 1 && st != 2) { vid[q] = -1; return; }
+  const int grp = st == 1 ? q : io.owner[q];
+  const int w = io.winner[grp];
+  const int id = rank[w];
+  vid[q] = id;
+  if (first[q]) {
+    out_xyz[(size_t)id * 3 + 0] = io.verts[(size_t)q * 3 + 0] * scale + cx;
+    out_xyz[(size_t)id * 3 + 1] = io.verts[(size_t)q * 3 + 1] * scale + cy;
+    out_xyz[(size_t)id * 3 + 2] = io.verts[(size_t)q * 3 + 2] * scale + cz;
+  }
+}
+
+__global__ void k_face_flags(const int* info, int R, int* fflag, int* fnv) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
+  fflag[r] = nv >= 3 ? 1 : 0;
+  fnv[r] = nv >= 3 ? nv : 0;
+}
+
+__global__ void k_face_write(const int* info, const int* fflag, const int* frank, const int* foff, const int* vid,
+                             int R, int vm, int* face_off, int* face_idx) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R || !fflag[r]) return;
+  const int f = frank[r];
+  const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
+  const int o = foff[r];
+  face_off[f] = o;
+  for (int k = 0; k < nv; ++k) face_idx[o + k] = vid[(size_t)r * vm + k];
+}
+
+}  // namespace am

@@ -1,0 +1,46 @@
+// Kernel interfaces of the marching engine (device side).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "am_limits.h"
+
+namespace am {
+
+// Region store + the current level's candidate slots.
+struct RegionIO {
+  const uint32_t* reg_mask;   // [R][Wd]
+  const int* reg_start;       // [R]   start plane (-1: seed)
+  const double* reg_point;    // [R][3] seed point of the region
+  int* reg_info;              // [R][AM_INFO_STRIDE]
+  int* reg_tab;               // [R][vm + 1]
+  double* reg_verts;          // [R][vm][3]
+  int* reg_vpair;             // [R][vm][2]
+  uint32_t* cand_mask;        // [count * vm][Wd]
+  int* cand_start;            // [count * vm]   -2: no candidate in this slot
+  double* cand_point;         // [count * vm][3]
+  int first;                  // region id of blockIdx.x == 0
+  int count;                  // regions in this launch
+  int Cs;                     // stride of one SoA plane array in LDS
+  int Csort;                  // power of two >= C (sort width)
+  unsigned* flags;            // [count] per-region flags (bit 0: undecided in reduced precision)
+};
+
+// Visited-set insertion of one level's candidates.
+struct InsertIO {
+  const uint32_t* cand_mask;  // [n][Wd]
+  const int* cand_start;      // [n]  -2 = empty slot
+  const uint32_t* reg_mask;   // [R][Wd]
+  uint32_t* table;            // [cap]  0xFFFFFFFF empty, 0x80000000|cand pending, else region id
+  int* status;                // [n]  0 empty, 1 new (slot owner), 2 duplicate, 3 visited
+  int* winner;                // [n]  min candidate index of the owner's group
+  int* slot;                  // [n]  table slot of an owner
+  int* owner;                 // [n]  owner candidate of a duplicate
+  int n;
+  int Wd;
+  uint32_t cap_mask;
+};
+
+__global__ void k_region_fp64(NetDev net, RegionIO io);
+
+}  // namespace am

@@ -1,0 +1,145 @@
+// pybind11 module `_am`: a thin object wrapper over the C-ABI of include/am.h.
+// Buffers cross as integer addresses (host or device), so no torch types
+// appear in the native signatures.
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/am.h"
+
+namespace py = pybind11;
+
+namespace {
+
+void check(am_ctx* c, am_status s) {
+  if (s != AM_OK) throw std::runtime_error("am error " + std::to_string(s) + ": " + am_last_error(c));
+}
+
+struct Engine {
+  am_ctx* c = nullptr;
+  int vm = 20;
+
+  Engine(const std::vector<int>& nodes, py::array_t<int, py::array::c_style | py::array::forcecast> arc,
+         int num_extra, int precision) {
+    if (arc.ndim() != 2) throw std::runtime_error("arc_table must be 2-D");
+    const am_status s = am_create(&c, (int)nodes.size(), nodes.data(), arc.data(), (int)arc.shape(0),
+                                  (int)arc.shape(1), num_extra, precision);
+    if (s != AM_OK) throw std::runtime_error("am error " + std::to_string(s) + ": " + am_last_error(nullptr));
+  }
+  ~Engine() {
+    if (c) am_destroy(c);
+  }
+
+  void set_option(const std::string& key, long long value) {
+    check(c, am_set_option(c, key.c_str(), value));
+    if (key == "vm") vm = (int)value;
+  }
+
+  void run(const std::vector<uintptr_t>& weights, const std::vector<uintptr_t>& biases,
+           const std::vector<std::pair<int, int>>& tm_shapes, const std::vector<uintptr_t>& tms, int dtype,
+           uintptr_t points, uintptr_t states, int n_seeds, uintptr_t extra_w, uintptr_t extra_b, double iso,
+           bool flip, uintptr_t stream) {
+    std::vector<const void*> w(weights.size()), b(biases.size()), t(tms.size());
+    for (size_t i = 0; i < w.size(); ++i) w[i] = (const void*)weights[i];
+    for (size_t i = 0; i < b.size(); ++i) b[i] = (const void*)biases[i];
+    for (size_t i = 0; i < t.size(); ++i) t[i] = (const void*)tms[i];
+    std::vector<int> shp;
+    for (auto& s : tm_shapes) {
+      shp.push_back(s.first);
+      shp.push_back(s.second);
+    }
+    am_status s;
+    {
+      py::gil_scoped_release nogil;
+      s = am_run(c, w.data(), b.data(), (int)t.size(), shp.data(), t.data(), dtype, (const void*)points,
+                 (const uint8_t*)states, n_seeds, (const void*)extra_w, (const void*)extra_b, iso, flip ? 1 : 0,
+                 (void*)stream);
+    }
+    check(c, s);
+  }
+
+  void combine(double scale, const std::vector<double>& center) {
+    if (center.size() != 3) throw std::runtime_error("center must have 3 entries");
+    am_status s;
+    {
+      py::gil_scoped_release nogil;
+      s = am_combine(c, scale, center.data());
+    }
+    check(c, s);
+  }
+
+  void export_ply(const std::string& path, bool is_poly, bool is_f32) {
+    check(c, am_export_ply(c, path.c_str(), is_poly ? 1 : 0, is_f32 ? 1 : 0));
+  }
+
+  py::tuple counts() {
+    long long r, f, v, n;
+    check(c, am_get_counts(c, &r, &f, &v, &n));
+    return py::make_tuple(r, f, v, n);
+  }
+
+  py::tuple mesh() {
+    long long r, f, v, n;
+    check(c, am_get_counts(c, &r, &f, &v, &n));
+    py::array_t<double> V({(py::ssize_t)v, (py::ssize_t)3});
+    py::array_t<int> off((py::ssize_t)(f + 1));
+    py::array_t<int> idx((py::ssize_t)n);
+    check(c, am_get_mesh(c, V.mutable_data(), off.mutable_data(), idx.mutable_data()));
+    return py::make_tuple(V, off, idx);
+  }
+
+  py::dict regions(int Wd) {
+    long long r, f, v, n;
+    check(c, am_get_counts(c, &r, &f, &v, &n));
+    py::array_t<uint32_t> masks({(py::ssize_t)r, (py::ssize_t)Wd});
+    py::array_t<int> info({(py::ssize_t)r, (py::ssize_t)8});
+    py::array_t<int> tabs({(py::ssize_t)r, (py::ssize_t)(vm + 1)});
+    py::array_t<double> verts({(py::ssize_t)r, (py::ssize_t)vm, (py::ssize_t)3});
+    py::array_t<int> vpairs({(py::ssize_t)r, (py::ssize_t)vm, (py::ssize_t)2});
+    py::array_t<double> points({(py::ssize_t)r, (py::ssize_t)3});
+    check(c, am_get_regions(c, masks.mutable_data(), info.mutable_data(), tabs.mutable_data(), verts.mutable_data(),
+                            vpairs.mutable_data(), points.mutable_data()));
+    py::dict d;
+    d["masks"] = masks;
+    d["info"] = info;
+    d["tabs"] = tabs;
+    d["verts"] = verts;
+    d["vpairs"] = vpairs;
+    d["points"] = points;
+    return d;
+  }
+
+  py::array_t<double> stats() {
+    py::array_t<double> s((py::ssize_t)AM_STAT_COUNT);
+    check(c, am_get_stats(c, s.mutable_data(), AM_STAT_COUNT));
+    return s;
+  }
+};
+
+}  // namespace
+
+PYBIND11_MODULE(_am, m) {
+  m.doc() = "MI355X analytic-marching engine (C-ABI wrapper)";
+  m.attr("PREC_FP32") = AM_PREC_FP32;
+  m.attr("PREC_SPLIT_BF16X3") = AM_PREC_SPLIT_BF16X3;
+  m.attr("PREC_FP64") = AM_PREC_FP64;
+  m.attr("DTYPE_F32") = AM_DTYPE_F32;
+  m.attr("DTYPE_F64") = AM_DTYPE_F64;
+  py::class_<Engine>(m, "Engine")
+      .def(py::init<const std::vector<int>&, py::array_t<int, py::array::c_style | py::array::forcecast>, int, int>(),
+           py::arg("nodes"), py::arg("arc_table"), py::arg("num_extra_constraints"), py::arg("precision"))
+      .def("set_option", &Engine::set_option)
+      .def("run", &Engine::run, py::arg("weights"), py::arg("biases"), py::arg("tm_shapes"), py::arg("tms"),
+           py::arg("dtype"), py::arg("points"), py::arg("states"), py::arg("n_seeds"), py::arg("extra_w"),
+           py::arg("extra_b"), py::arg("iso"), py::arg("flip"), py::arg("stream"))
+      .def("combine", &Engine::combine)
+      .def("export_ply", &Engine::export_ply)
+      .def("counts", &Engine::counts)
+      .def("mesh", &Engine::mesh)
+      .def("regions", &Engine::regions)
+      .def("stats", &Engine::stats);
+}

@@ -1,0 +1,155 @@
+"""Python front end of the native engine (``libam.so`` through ``_am``).
+
+``march()`` runs one network from explicit seeds and returns the engine's
+records; it is what the reference-style entry points and the benchmark use.
+The extension must be built in-tree (``python -m analyticmesh_amd.build``);
+there is no fallback path.
+"""
+import os
+
+import numpy as np
+
+from .canonical import RegionRecords
+
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp64": 2}
+
+_mod = None
+
+
+def native():
+    """Import the compiled module, failing loudly when it is missing."""
+    global _mod
+    if _mod is None:
+        try:
+            from . import _am  # noqa: F401
+        except ImportError as e:
+            here = os.path.dirname(os.path.abspath(__file__))
+            raise ImportError(
+                "analyticmesh_amd native engine is not built; run `python -m analyticmesh_amd.build` "
+                f"(looked in {here}): {e}") from e
+        _mod = _am
+    return _mod
+
+
+def _as_ptr(a):
+    """Address of a torch tensor or numpy array (host or device)."""
+    if hasattr(a, "data_ptr"):
+        return int(a.data_ptr())
+    return int(a.ctypes.data)
+
+
+class MarchResult:
+    def __init__(self, eng, spec_L, Wd, vm):
+        self._eng = eng
+        self.L = spec_L
+        self.Wd = Wd
+        self.vm = vm
+        r = eng.regions(Wd)
+        self.masks = r["masks"]
+        self.info = r["info"]
+        self.tabs = r["tabs"]
+        self.verts = r["verts"]
+        self.vpairs = r["vpairs"]
+        self.points = r["points"]
+        self.stats = eng.stats()
+
+    def records(self):
+        return RegionRecords(self.masks, self.info, self.tabs, self.verts, self.vpairs, L=self.L)
+
+    @property
+    def run_ms(self):
+        return float(self.stats[0])
+
+    def combine(self, scale=1.0, center=(0.0, 0.0, 0.0)):
+        self._eng.combine(float(scale), [float(c) for c in center])
+        V, off, idx = self._eng.mesh()
+        faces = [idx[off[i]:off[i + 1]] for i in range(len(off) - 1)]
+        return V, faces
+
+    def export_ply(self, path, is_polymesh=True, is_float32=True):
+        self._eng.export_ply(str(path), bool(is_polymesh), bool(is_float32))
+
+
+class Engine:
+    """One engine context for a fixed architecture (mirrors ``Init``)."""
+
+    def __init__(self, nodes, arc_table=None, num_extra_constraints=0, precision="fp64", vm=20,
+                 table_log2=None):
+        m = native()
+        nodes = [int(n) for n in nodes]
+        D = len(nodes) - 2
+        if arc_table is None:
+            arc = np.zeros((D, 1), dtype=np.int32)
+        else:
+            arc_table = [list(r) for r in arc_table]
+            width = max(len(r) for r in arc_table)
+            arc = np.zeros((len(arc_table), width), dtype=np.int32)
+            for i, r in enumerate(arc_table):
+                arc[i, :len(r)] = r
+        self.nodes = nodes
+        self.L = sum(nodes[1:-1])
+        self.Wd = (self.L + 31) // 32
+        self.E = int(num_extra_constraints)
+        self.vm = int(vm)
+        self._eng = m.Engine(nodes, arc, self.E, PRECISIONS[precision])
+        self._eng.set_option("vm", self.vm)
+        if table_log2 is not None:
+            self._eng.set_option("table_log2", int(table_log2))
+
+    def set_option(self, key, value):
+        self._eng.set_option(key, int(value))
+
+    def run(self, weights, biases, points, states, tms=(), extra_w=None, extra_b=None, iso=0.0, flip=False,
+            stream=0):
+        """Buffers may be numpy arrays or torch tensors (CPU or GPU), all float32 or all float64."""
+        first = weights[0]
+        is64 = str(getattr(first, "dtype", "")) in ("float64", "torch.float64")
+        dtype = 1 if is64 else 0
+        keep = []
+
+        def cont(a):
+            if hasattr(a, "contiguous"):
+                a = a.contiguous()
+            else:
+                a = np.ascontiguousarray(a)
+            keep.append(a)
+            return a
+
+        wp = [_as_ptr(cont(w)) for w in weights]
+        bp = [_as_ptr(cont(b)) for b in biases]
+        def shape2(t):
+            n = t.numel() if hasattr(t, "numel") else np.size(t)
+            return (int(t.shape[0]), int(t.shape[1])) if n else (0, 0)
+
+        tm_shapes = [shape2(t) for t in tms]
+        tp = [_as_ptr(cont(t)) if s != (0, 0) else 0 for t, s in zip(tms, tm_shapes)]
+        pts = cont(points)
+        st = states
+        if hasattr(st, "to"):
+            import torch
+            st = st.to(torch.uint8).contiguous()
+        else:
+            st = np.ascontiguousarray(st, dtype=np.uint8)
+        keep.append(st)
+        n = int(pts.shape[0])
+        ew = cont(extra_w) if extra_w is not None and len(extra_w) else None
+        eb = cont(extra_b) if extra_b is not None and len(extra_b) else None
+        self._eng.run(wp, bp, tm_shapes, tp, dtype, _as_ptr(pts), _as_ptr(st), n,
+                      _as_ptr(ew) if ew is not None else 0, _as_ptr(eb) if eb is not None else 0,
+                      float(iso), bool(flip), int(stream))
+        return MarchResult(self._eng, self.L, self.Wd, self.vm)
+
+
+def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip=False, max_regions=None,
+          table_log2=None):
+    """Mesh one NetSpec from explicit seeds (points (N,3) float64, states (N,L) bool)."""
+    eng = Engine(spec.nodes, None, 0, precision=precision, vm=vm, table_log2=table_log2)
+    if max_regions is not None:
+        eng.set_option("max_regions", max_regions)
+    pts = np.ascontiguousarray(seed_points, dtype=np.float64).reshape(-1, 3)
+    st = np.ascontiguousarray(seed_states, dtype=np.uint8).reshape(len(pts), spec.L)
+    W = [np.ascontiguousarray(w, dtype=np.float64) for w in spec.weights]
+    B = [np.ascontiguousarray(b, dtype=np.float64) for b in spec.biases]
+    res = eng.run(W, B, pts, st, iso=iso, flip=flip)
+    res.engine = eng
+    return res

@@ -1,0 +1,116 @@
+/* C-ABI of the MI355X analytic-marching engine (libam.so).
+ *
+ * Drop-in for the reference's native module `build.cuam`
+ * (backend/src/cuam.cpp:186-217): the same five calls, with plain pointers,
+ * status codes instead of exit(), and one context per caller instead of the
+ * process-global singleton (backend/src/cuam_kernel.cu:22-25).
+ *
+ * Pointers to weights, seeds and planes may be host or device memory; the
+ * engine copies them into its own device buffers during the call.  All
+ * results stay in engine-owned memory until the next am_run or am_destroy.
+ */
+#ifndef AM_H
+#define AM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct am_ctx am_ctx;
+typedef int am_status;
+
+#define AM_OK 0
+#define AM_ERR_ARG 1        /* invalid argument (reference: TORCH_CHECK, cuam.cpp:69-180) */
+#define AM_ERR_STATE 2      /* call out of order (reference: prints and returns, cuam.cpp:107-111) */
+#define AM_ERR_HIP 3        /* HIP runtime error (reference: exit(), utilities.h:73-95) */
+#define AM_ERR_CAPACITY 4   /* a configured capacity was exceeded */
+#define AM_ERR_NOGPU 5      /* no usable gfx950 device */
+#define AM_ERR_IO 6         /* file could not be written */
+
+/* Arithmetic of the constraint planes and of the walk. */
+#define AM_PREC_FP32 0            /* FP32 MFMA planes, FP64 re-decision of undecided regions */
+#define AM_PREC_SPLIT_BF16X3 1    /* split-BF16 (3 products) MFMA planes, FP64 re-decision */
+#define AM_PREC_FP64 2            /* FP64 everywhere (the reference's default dtype) */
+
+/* Element type of caller buffers. */
+#define AM_DTYPE_F32 0
+#define AM_DTYPE_F64 1
+
+/* Replaces Init(float_type, nodesnum, arc_table, num_extra_constraints)
+ * (backend/src/cuam.cpp:58-95, backend/src/cuam_kernel.cu:128-148).
+ * nodes: n_nodes widths, nodes[0] == 3, nodes[n_nodes-1] == 1, n_nodes >= 3.
+ * arc_table: row-major int32 [arc_rows][arc_cols], arc_rows == n_nodes - 2,
+ * row r = [count, from0, tm0, from1, tm1, ...] (backend/model.py:23-31).
+ * precision: AM_PREC_*.  Fails with AM_ERR_NOGPU when no device is present. */
+am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_table, int arc_rows,
+                    int arc_cols, int num_extra_constraints, int precision);
+
+/* Replaces AnalyticMarching(weights, biases, states, points, arc_tm,
+ * w_extra_constraints, b_extra_constraints, iso, flip_insideout)
+ * (backend/src/cuam.cpp:97-184, backend/src/cuam_kernel.cu:30-123).
+ * weights[i]: (nodes[i+1], nodes[i]) row-major; biases[i]: (nodes[i+1],);
+ * tms[t]: (tm_shapes[2t], tm_shapes[2t+1]) row-major, a (0, 0) shape is the
+ * identity; points: (n_seeds, 3); states: (n_seeds, L) bytes, nonzero = active
+ * (L = sum of hidden widths); extra_w: (E, 3); extra_b: (E,).  Element type of
+ * every floating-point buffer: dtype (AM_DTYPE_*).  stream: a hipStream_t or
+ * NULL.  The call returns once the mesh records are complete. */
+am_status am_run(am_ctx* ctx, const void* const* weights, const void* const* biases, int n_tm,
+                 const int* tm_shapes, const void* const* tms, int dtype, const void* points,
+                 const uint8_t* states, int n_seeds, const void* extra_w, const void* extra_b, double iso,
+                 int flip_insideout, void* stream);
+
+/* Replaces CombineMesh(scale, center) (backend/src/cuam_kernel.cu:185-200):
+ * deduplicates vertices and indexes faces; output vertex = v * scale + center. */
+am_status am_combine(am_ctx* ctx, double scale, const double* center3);
+
+/* Replaces ExportMesh(file_path, is_polymesh, is_float32)
+ * (backend/src/cuam_kernel.cu:205-246, backend/inc/polymesh.h:350-423):
+ * binary little-endian PLY, polygons or fan triangles, float or double xyz. */
+am_status am_export_ply(am_ctx* ctx, const char* path, int is_polymesh, int is_float32);
+
+/* Replaces Destroy() (backend/src/cuam_kernel.cu:153-179). */
+am_status am_destroy(am_ctx* ctx);
+
+/* Message of the last failing call on ctx (or of a failed am_create when ctx is NULL). */
+const char* am_last_error(const am_ctx* ctx);
+
+/* Sizes after am_run / am_combine. */
+am_status am_get_counts(am_ctx* ctx, long long* n_regions, long long* n_faces, long long* n_vertices,
+                        long long* n_face_indices);
+
+/* Combined mesh: vertices (V x 3 double), face_offsets (F + 1), face_indices. */
+am_status am_get_mesh(am_ctx* ctx, double* vertices, int* face_offsets, int* face_indices);
+
+/* Per-region records in visiting order (parity view): masks [R][W] u32,
+ * info [R][8] (start queued, start used, vertices, tab entries, reversed,
+ * status, flags, 0), tabs [R][vm+1], verts [R][vm][3], vpairs [R][vm][2],
+ * points [R][3] (seed point each region was queued with).  Any pointer may be NULL. */
+am_status am_get_regions(am_ctx* ctx, uint32_t* masks, int* info, int* tabs, double* verts, int* vpairs,
+                         double* points);
+
+/* Tunables: "vm" (polygon vertex cap, 3..64, default 20 like the reference's
+ * _VAR_VERT_MAX), "table_log2" (initial visited-table size), "max_regions"
+ * (stop after this many regions, -1 = unlimited). */
+am_status am_set_option(am_ctx* ctx, const char* key, long long value);
+
+/* Timings and counters of the last am_run/am_combine; see AM_STAT_* indices. */
+#define AM_STAT_RUN_MS 0        /* am_run wall time, seeds in -> records in device memory */
+#define AM_STAT_REGION_MS 1     /* region kernels (HIP events) */
+#define AM_STAT_INSERT_MS 2     /* visited-set insertion + compaction (HIP events) */
+#define AM_STAT_GEMM_MS 3       /* plane GEMM kernels (HIP events; reduced precisions) */
+#define AM_STAT_WALK_MS 4       /* walk kernels after the GEMM (HIP events; reduced precisions) */
+#define AM_STAT_REDO_MS 5       /* FP64 re-decision kernels (HIP events) */
+#define AM_STAT_LEVELS 6
+#define AM_STAT_REGIONS 7
+#define AM_STAT_CANDIDATES 8
+#define AM_STAT_REDO_REGIONS 9
+#define AM_STAT_COMBINE_MS 10
+#define AM_STAT_REGION_LAUNCHES 11
+#define AM_STAT_COUNT 16
+am_status am_get_stats(am_ctx* ctx, double* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

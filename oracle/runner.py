@@ -1,0 +1,149 @@
+"""ctypes front end of ``libamoracle.so`` (test infrastructure only)."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libamoracle.so")
+INFO_STRIDE = 8
+ST_OK, ST_NO_START, ST_GAVE_UP, ST_BAD_START = 0, 1, 2, 3
+
+_lib = None
+
+
+def ensure_built():
+    if not os.path.exists(LIB_PATH):
+        from .build import build
+        build()
+    return LIB_PATH
+
+
+def load_oracle():
+    global _lib
+    if _lib is None:
+        ensure_built()
+        lib = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        lib.amo_create.restype = P
+        lib.amo_create.argtypes = [ctypes.c_int, P, P, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P,
+                                   ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        lib.amo_run.restype = ctypes.c_int
+        lib.amo_run.argtypes = [P, ctypes.c_int, P, P, ctypes.c_double, ctypes.c_int, ctypes.c_long]
+        for name in ("amo_num_regions", "amo_tag_collisions", "amo_num_batches"):
+            getattr(lib, name).restype = ctypes.c_long
+            getattr(lib, name).argtypes = [P]
+        for name in ("amo_mask_words", "amo_num_constraints", "amo_truncated"):
+            getattr(lib, name).restype = ctypes.c_int
+            getattr(lib, name).argtypes = [P]
+        lib.amo_seconds.restype = ctypes.c_double
+        lib.amo_seconds.argtypes = [P]
+        lib.amo_get.restype = None
+        lib.amo_get.argtypes = [P] * 7
+        lib.amo_planes.restype = None
+        lib.amo_planes.argtypes = [P, P, ctypes.c_double, P, P]
+        lib.amo_destroy.restype = None
+        lib.amo_destroy.argtypes = [P]
+        lib.amo_fnv1a.restype = ctypes.c_uint32
+        lib.amo_fnv1a.argtypes = [P, ctypes.c_int]
+        lib.amo_fnv1.restype = ctypes.c_uint32
+        lib.amo_fnv1.argtypes = [P, ctypes.c_int]
+        lib.amo_get_i_j.restype = None
+        lib.amo_get_i_j.argtypes = [ctypes.c_int, ctypes.c_int, P, P]
+        _lib = lib
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class OracleRun:
+    """One oracle configuration (network + precision + traversal)."""
+
+    def __init__(self, spec, precision="fp64", traversal="lifo", vm=20, threads=0,
+                 extra_w=None, extra_b=None, arc_table=None, tms=None):
+        lib = load_oracle()
+        self.lib = lib
+        self.spec = spec
+        nodes = np.asarray(spec.nodes, dtype=np.int32)
+        self._W = [np.ascontiguousarray(w, dtype=np.float64) for w in spec.weights]
+        self._b = [np.ascontiguousarray(b, dtype=np.float64) for b in spec.biases]
+        Wp = (ctypes.c_void_p * len(self._W))(*[w.ctypes.data for w in self._W])
+        bp = (ctypes.c_void_p * len(self._b))(*[b.ctypes.data for b in self._b])
+        D = len(spec.nodes) - 2
+        if arc_table is None:
+            arc = np.zeros((D, 1), dtype=np.int32)
+        else:
+            width = max(len(r) for r in arc_table)
+            arc = np.zeros((len(arc_table), width), dtype=np.int32)
+            for i, r in enumerate(arc_table):
+                arc[i, :len(r)] = r
+        self._arc = arc
+        tms = tms or []
+        self._tms = [np.ascontiguousarray(t, dtype=np.float64).reshape(t.shape if np.size(t) else (0, 0)) for t in tms]
+        shapes = np.array([[t.shape[0], t.shape[1]] if t.size else [0, 0] for t in self._tms] or [[0, 0]],
+                          dtype=np.int32)
+        self._tm_shapes = shapes
+        tmp = (ctypes.c_void_p * max(1, len(self._tms)))(*([t.ctypes.data for t in self._tms] or [0]))
+        E = 0 if extra_w is None else len(extra_w)
+        self._ew = np.ascontiguousarray(extra_w if E else np.zeros((1, 3)), dtype=np.float64)
+        self._eb = np.ascontiguousarray(extra_b if E else np.zeros(1), dtype=np.float64)
+        self.E = E
+        self.vm = vm
+        self.h = lib.amo_create(len(nodes), _ptr(nodes), Wp, bp, _ptr(arc), arc.shape[0], arc.shape[1],
+                                len(self._tms), _ptr(shapes), tmp, E, _ptr(self._ew), _ptr(self._eb),
+                                0 if precision == "fp64" else 1, 0 if traversal == "lifo" else 1, vm, threads)
+        if not self.h:
+            raise ValueError("oracle rejected the network description")
+        self.L = spec.L
+        self.Wd = lib.amo_mask_words(self.h)
+        self.C = lib.amo_num_constraints(self.h)
+
+    def run(self, seed_points, seed_states, iso=0.0, flip=False, max_regions=-1):
+        pts = np.ascontiguousarray(seed_points, dtype=np.float64).reshape(-1, 3)
+        st = np.ascontiguousarray(seed_states, dtype=np.uint8).reshape(len(pts), self.L)
+        rc = self.lib.amo_run(self.h, len(pts), _ptr(pts), _ptr(st), float(iso), int(flip), int(max_regions))
+        if rc != 0:
+            raise RuntimeError(f"oracle run failed ({rc})")
+        R = self.lib.amo_num_regions(self.h)
+        vm = self.vm
+        self.masks = np.zeros((R, self.Wd), dtype=np.uint32)
+        self.info = np.zeros((R, INFO_STRIDE), dtype=np.int32)
+        self.tabs = np.zeros((R, vm + 1), dtype=np.int32)
+        self.verts = np.zeros((R, vm, 3), dtype=np.float64)
+        self.vpairs = np.zeros((R, vm, 2), dtype=np.int32)
+        self.points = np.zeros((R, 3), dtype=np.float64)
+        self.lib.amo_get(self.h, _ptr(self.masks), _ptr(self.info), _ptr(self.tabs), _ptr(self.verts),
+                         _ptr(self.vpairs), _ptr(self.points))
+        self.seconds = self.lib.amo_seconds(self.h)
+        self.tag_collisions = self.lib.amo_tag_collisions(self.h)
+        self.batches = self.lib.amo_num_batches(self.h)
+        self.truncated = bool(self.lib.amo_truncated(self.h))
+        return self
+
+    def planes(self, mask_words, iso=0.0):
+        m = np.ascontiguousarray(mask_words, dtype=np.uint32)
+        A = np.zeros((self.C, 4), dtype=np.float64)
+        eq = np.zeros(4, dtype=np.float64)
+        self.lib.amo_planes(self.h, _ptr(m), float(iso), _ptr(A), _ptr(eq))
+        return A, eq
+
+    def records(self):
+        """Per-region records in the layout the GPU engine also returns."""
+        from analyticmesh_amd.canonical import RegionRecords
+        return RegionRecords(self.masks, self.info, self.tabs, self.verts, self.vpairs, L=self.L)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.amo_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+def run_oracle(spec, seed_points, seed_states, precision="fp64", traversal="lifo", vm=20, iso=0.0,
+               flip=False, max_regions=-1, threads=0, **kw):
+    r = OracleRun(spec, precision=precision, traversal=traversal, vm=vm, threads=threads, **kw)
+    return r.run(seed_points, seed_states, iso=iso, flip=flip, max_regions=max_regions)

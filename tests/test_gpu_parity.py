@@ -1,0 +1,86 @@
+"""GPU engine vs the CPU restatement (oracle) on the same seeded inputs."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from _common import assert_same_topology, max_vertex_error, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def oracle_mod():
+    import oracle
+    return oracle
+
+
+@pytest.mark.parametrize("name", ["polytope", "sphere"])
+def test_fp64_bitexact_vs_oracle_bfs(name, oracle_mod):
+    """FP64 path = the oracle's breadth-first order, record for record, bit for bit."""
+    from analyticmesh_amd.engine import march
+    spec, pts, st = workload(name)
+    _say(f"[fp64 bit-exact] {name} L={spec.L}")
+    g = march(spec, pts, st, precision="fp64")
+    o = oracle_mod.run_oracle(spec, pts, st, traversal="bfs")
+    assert len(g.masks) == len(o.masks)
+    np.testing.assert_array_equal(g.masks, o.masks)
+    np.testing.assert_array_equal(g.info[:, :6], o.info[:, :6])
+    np.testing.assert_array_equal(g.tabs, o.tabs)
+    np.testing.assert_array_equal(g.vpairs, o.vpairs)
+    np.testing.assert_array_equal(g.points, o.points)
+    assert np.array_equal(g.verts.view(np.uint64), o.verts.view(np.uint64)), "vertex bits differ"
+
+
+@pytest.mark.parametrize("name", ["polytope", "sphere"])
+def test_fp64_topology_vs_oracle_lifo(name, oracle_mod):
+    """Same regions, faces and vertex keys as the reference's own LIFO order."""
+    from analyticmesh_amd.engine import march
+    spec, pts, st = workload(name)
+    _say(f"[fp64 vs lifo] {name}")
+    g = march(spec, pts, st, precision="fp64").records()
+    o = oracle_mod.run_oracle(spec, pts, st, traversal="lifo").records()
+    assert_same_topology(g, o)
+    # the same vertex solved from different regions / plane orders: rounding
+    # amplified by the condition number (~1e7 for the polytope's +- plane pairs)
+    assert max_vertex_error(g, o) < (1e-7 if name == "polytope" else 1e-10)
+
+
+def test_fp64_mesh_and_ply(tmp_path, oracle_mod):
+    from analyticmesh_amd.engine import march
+    from analyticmesh_amd.ply import read_ply
+    spec, pts, st = workload("sphere")
+    _say("[mesh+ply] sphere")
+    res = march(spec, pts, st, precision="fp64")
+    V, F = res.combine(scale=2.0, center=(0.5, -1.0, 0.25))
+    Vr, Fr = res.records().keyed_mesh()
+    assert len(V) == len(Vr) and len(F) == len(Fr)
+    np.testing.assert_array_equal(V, Vr * 2.0 + np.array([0.5, -1.0, 0.25]))
+    for a, b in zip(F, Fr):
+        assert list(a) == list(b)
+    p = tmp_path / "m.ply"
+    res.export_ply(p, is_polymesh=True, is_float32=False)
+    V2, F2 = read_ply(str(p))
+    np.testing.assert_array_equal(V2, V)
+    assert [list(f) for f in F2] == [list(f) for f in F]
+    p3 = tmp_path / "t.ply"
+    res.export_ply(p3, is_polymesh=False, is_float32=True)
+    V3, F3 = read_ply(str(p3))
+    assert len(F3) == sum(len(f) - 2 for f in F)
+    np.testing.assert_array_equal(V3, V.astype(np.float32))
+
+
+def test_multi_seed_dedup(oracle_mod):
+    from analyticmesh_amd.engine import march
+    spec, pts, st = workload("sphere", n_seeds=64)
+    _say("[multi-seed] sphere x64")
+    g = march(spec, pts, st, precision="fp64")
+    o = oracle_mod.run_oracle(spec, pts, st, traversal="bfs")
+    np.testing.assert_array_equal(g.masks, o.masks)
+    np.testing.assert_array_equal(g.tabs, o.tabs)
