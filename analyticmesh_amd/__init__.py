@@ -1,1 +1,15 @@
-"""MI355X-native analytic marching (exact meshing of ReLU-MLP level sets)."""
+"""MI355X-native analytic marching: exact meshing of ReLU-MLP level sets.
+
+Public surface of the reference package (backend/__init__.py:3-6):
+``AnalyticMarching``, ``load_model``, ``save_model``, ``MLP``, ``get_boundary``.
+"""
+from .model import MLP  # noqa: F401
+from .onnx_io import load_model, save_model  # noqa: F401
+
+
+def __getattr__(name):
+    if name in ("AnalyticMarching", "get_boundary", "DICHOTOMY_DICT", "SPHERE_TRACING_DICT",
+                "GRADIENT_DESCENT_DICT", "VOXEL_DICT"):
+        from . import main
+        return getattr(main, name)
+    raise AttributeError(name)

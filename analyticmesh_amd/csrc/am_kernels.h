@@ -24,6 +24,7 @@ struct RegionIO {
   int Cs;                     // stride of one SoA plane array in LDS
   int Csort;                  // power of two >= C (sort width)
   unsigned* flags;            // [count] per-region flags (bit 0: undecided in reduced precision)
+  const int* ids;             // optional: region ids to process (FP64 re-decision); else first + blockIdx.x
 };
 
 // Visited-set insertion of one level's candidates.

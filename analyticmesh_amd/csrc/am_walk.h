@@ -1,0 +1,351 @@
+// Per-region walk shared by the FP64 kernel and the reduced-precision walk
+// kernel: distance order, seed start search, pivot walk, orientation, record
+// and neighbour candidates (backend/inc/process.h:208-264,
+// backend/inc/kernel.h:799-1506).  One wave64 runs one region; the constraint
+// planes are already in LDS as SoA arrays (x, y, z, c).
+//
+// CHECKED = false: plain evaluation in T (the FP64 path; the operation order
+// matches the CPU restatement exactly).
+// CHECKED = true: every predicate also carries an error bound derived from the
+// planes' relative error bound `eps`; when a sign cannot be decided within its
+// bound the walk stops and reports the region as undecided, and the FP64
+// kernel redoes it.  Candidate order and which of two feasible candidates is
+// taken first only change how a face is written down, never which face it is.
+#pragma once
+#include "am_device.h"
+#include "am_kernels.h"
+
+namespace am {
+
+#define AM_UNDECIDED 0x1u
+
+struct WalkParams {
+  float eps;        // relative error bound of the planes (CHECKED only)
+  float eps_turn;   // relative bound for the turn and orientation tests
+  float h_max;      // largest Hadamard ratio of a vertex system that is trusted
+};
+
+template <typename T>
+__device__ __forceinline__ T hadamard_ratio(const T* p1, const T* p2, const T* p3) {
+  const T d = AM_DET3(p1[0], p1[1], p1[2], p2[0], p2[1], p2[2], p3[0], p3[1], p3[2]);
+  const T n1 = __builtin_sqrt(plane_nn(p1[0], p1[1], p1[2]));
+  const T n2 = __builtin_sqrt(plane_nn(p2[0], p2[1], p2[2]));
+  const T n3 = __builtin_sqrt(plane_nn(p3[0], p3[1], p3[2]));
+  return (n1 * n2 * n3) / absT(d);
+}
+
+// Verdict of one vertex against all constraints owned by this lane.
+// 0 = satisfied, 1 = certainly violated, 2 = undecided.
+template <typename T, bool CHECKED>
+__device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int lane,
+                                          int ex1, int ex2, const T* v, T epsv) {
+  int verdict = 0;
+  for (int g = lane; g < C; g += 64) {
+    if (g == ex1 || g == ex2) continue;
+    const T ax = Px[g], ay = Py[g], az = Pz[g], c = Pc[g];
+    if (!CHECKED) {
+      if (violates(ax, ay, az, c, v)) return 1;
+    } else {
+      const T s = plane_eval(ax, ay, az, c, v);
+      const T m = absT(ax) * absT(v[0]) + absT(ay) * absT(v[1]) + absT(az) * absT(v[2]) + absT(c);
+      const T err = epsv * m;
+      if (s > err) return 1;
+      if (s >= -err) verdict = 2;
+    }
+  }
+  return verdict;
+}
+
+template <typename T, bool CHECKED>
+__device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
+                            T* Px, T* Py, T* Pz, T* Pc, T* keys, int* order, uint32_t* used,
+                            const uint32_t* smask, const T* eq_in, const T* p) {
+  __shared__ int s_tab[AM_VM_MAX + 2];
+  __shared__ T s_verts[AM_VM_MAX * 3];
+  __shared__ int s_vpair[AM_VM_MAX * 2];
+
+  const int C = net.C, L = net.L, vm = net.vm, N2 = io.Csort;
+  const int lane = threadIdx.x;
+  const T eq[4] = {eq_in[0], eq_in[1], eq_in[2], eq_in[3]};
+  const int start_in = io.reg_start[rid];
+  bool undecided = false;
+
+  // ---- distance keys s|s|/|a|^2 (orders like s/|a|), stable descending sort
+  for (int g = lane; g < N2; g += 64) {
+    T key = -__builtin_inf();
+    if (g < C) {
+      const T s = plane_eval(Px[g], Py[g], Pz[g], Pc[g], p);
+      const T nn = plane_nn(Px[g], Py[g], Pz[g]);
+      key = nn > T(0) ? s * absT(s) / nn : (s > T(0) ? T(__builtin_inf()) : T(-__builtin_inf()));
+      if (key != key) key = -__builtin_inf();  // NaN sorts last
+    }
+    keys[g] = key;
+    order[g] = g;
+  }
+  __syncthreads();
+  for (int k = 2; k <= N2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < N2; i += 64) {
+        const int l = i ^ j;
+        if (l > i) {
+          const T ki = keys[i], kl = keys[l];
+          const int oi = order[i], ol = order[l];
+          const bool i_first = (ki > kl) || (ki == kl && oi < ol);
+          if (((i & k) == 0) != i_first) {
+            keys[i] = kl; keys[l] = ki;
+            order[i] = ol; order[l] = oi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  int start = start_in;
+  int status = AM_ST_OK;
+  // ---- seed start search: lowest feasible pair index (backend/inc/kernel.h:799-856)
+  if (start == -1) {
+    const int xmax = min((C - 1) * (C - 2), 812);
+    for (int xb = 0; xb <= xmax; xb += 64) {
+      const int x = xb + lane;
+      int verdict = x <= xmax ? 0 : 1;
+      int ri = 0, rj = 0;
+      T v[3] = {T(0), T(0), T(0)};
+      T epsv = T(0);
+      if (x <= xmax) {
+        int i, j;
+        get_i_j(x, C, i, j);
+        ri = order[i];
+        rj = order[j];
+        const T pi4[4] = {Px[ri], Py[ri], Pz[ri], Pc[ri]};
+        const T pj4[4] = {Px[rj], Py[rj], Pz[rj], Pc[rj]};
+        solve3(pi4, pj4, eq, v);
+        if (CHECKED) {
+          const T h = hadamard_ratio(pi4, pj4, eq);
+          epsv = T(wp.eps) * (T(1) + h);
+          if (!(h <= T(wp.h_max))) verdict = 2;
+        }
+      }
+      for (int g = 0; g < C; ++g) {
+        if (!__any(verdict == 0 || verdict == 2)) break;
+        if ((verdict == 0 || (CHECKED && verdict == 2)) && g != ri && g != rj) {
+          if (!CHECKED) {
+            if (violates(Px[g], Py[g], Pz[g], Pc[g], v)) verdict = 1;
+          } else {
+            const T s = plane_eval(Px[g], Py[g], Pz[g], Pc[g], v);
+            const T m = absT(Px[g]) * absT(v[0]) + absT(Py[g]) * absT(v[1]) + absT(Pz[g]) * absT(v[2]) + absT(Pc[g]);
+            if (s > epsv * m) verdict = 1;
+            else if (s >= -epsv * m) verdict = 2;
+          }
+        }
+      }
+      const unsigned long long open = __ballot(verdict == 0 || verdict == 2);
+      if (open) {
+        const int wl = __builtin_ctzll(open);
+        const int vw = __shfl(verdict, wl);
+        if (CHECKED && vw == 2) { undecided = true; break; }
+        start = __shfl(ri, wl);
+        break;
+      }
+    }
+    if (start == -1) status = AM_ST_NO_START;
+  }
+
+  int ntab = 0, nv = 0;
+  int reversed = 0;
+  if (status == AM_ST_OK && !undecided) {
+    // ---- pivot walk (backend/inc/kernel.h:861-1320, backend/src/kernel.cu:203-299)
+    if (lane == 0) {
+      s_tab[0] = start;
+      used[start >> 5] |= 1u << (start & 31);
+    }
+    __syncthreads();
+    ntab = 1;
+    int cur = start;
+    while (true) {
+      if ((ntab >= 4 && s_tab[0] == s_tab[ntab - 1]) || ntab >= vm + 1) break;
+      if (cur < 0 || cur >= C) { status = AM_ST_BAD_START; break; }
+      int win = -1;
+      T v[3] = {T(0), T(0), T(0)};
+      T vepsv = T(0);
+      const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
+      const int first_entry = s_tab[0];
+      for (int w = 0; w < C; ++w) {
+        const int r = order[w];
+        const bool cand = (r == first_entry && ntab >= 3) || !((used[r >> 5] >> (r & 31)) & 1u);
+        if (!cand) continue;
+        const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
+        T vt[3];
+        solve3(pr4, pc4, eq, vt);
+        T epsv = T(0);
+        bool trusted = true;
+        if (CHECKED) {
+          const T h = hadamard_ratio(pr4, pc4, eq);
+          epsv = T(wp.eps) * (T(1) + h);
+          trusted = h <= T(wp.h_max);
+        }
+        const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, C, lane, cur, r, vt, epsv);
+        if (__any(verdict == 1)) continue;          // certainly infeasible
+        if (CHECKED && (__any(verdict == 2) || !trusted)) { undecided = true; break; }
+        win = r;
+        v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
+        vepsv = epsv;
+        break;
+      }
+      if (undecided) break;
+      if (win < 0) { status = AM_ST_GAVE_UP; break; }
+      int pl = -1, pr = -1, cnt = 0;
+      for (int idx = ntab - 1; idx >= 0 && cnt < 2; --idx) {
+        if (s_tab[idx] >= 0) { if (cnt == 0) pl = idx; else pr = idx; ++cnt; }
+      }
+      int valid = -1;
+      if (cnt == 2) {
+        const int a = s_tab[pr], b = s_tab[pl];
+        const T n1[3] = {Px[a], Py[a], Pz[a]};
+        const T n2[3] = {Px[b], Py[b], Pz[b]};
+        const T n3[3] = {Px[win], Py[win], Pz[win]};
+        valid = pivot_valid(eq, n1, n2, n3) ? 1 : 0;
+        if (CHECKED) {
+          const T A12 = AM_DOT3(n1, n2), A23 = AM_DOT3(n2, n3), A13 = AM_DOT3(n1, n3), A22 = AM_DOT3(n2, n2);
+          const T C1 = AM_DOT3(n1, eq), C2 = AM_DOT3(n2, eq), C3 = AM_DOT3(n3, eq), B = AM_DOT3(eq, eq);
+          const T val = (A12 * A23 - A22 * A13) - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3) / B;
+          const T mag = absT(A12 * A23) + absT(A22 * A13) +
+                        (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
+          if (!(absT(val) > T(wp.eps_turn) * mag)) { undecided = true; break; }
+        }
+      }
+      bool same = false;
+      if (valid != 0 && nv > 0) {
+        const T last[3] = {s_verts[(nv - 1) * 3 + 0], s_verts[(nv - 1) * 3 + 1], s_verts[(nv - 1) * 3 + 2]};
+        if (!CHECKED) {
+          same = same_vertex(last, v);
+        } else {
+          // decide the FP64 test (L1 < 3e-14) from reduced-precision vertices
+          const T d = absT(last[0] - v[0]) + absT(last[1] - v[1]) + absT(last[2] - v[2]);
+          const T mag = absT(v[0]) + absT(v[1]) + absT(v[2]) + absT(last[0]) + absT(last[1]) + absT(last[2]);
+          if (!(d > T(4) * vepsv * (mag + T(1)))) { undecided = true; break; }
+        }
+      }
+      __syncthreads();
+      if (valid == 0) {
+        if (CHECKED) { undecided = true; break; }   // degenerate turn: leave it to FP64
+        if (lane == 0) {
+          s_tab[ntab] = -win - 1;
+          used[win >> 5] |= 1u << (win & 31);
+        }
+        ntab += 1;
+        cur = pl + 1;   // the reference keeps a 1-based tab position here
+      } else if (same) {
+        if (lane == 0) {
+          s_tab[ntab] = win;
+          used[win >> 5] |= 1u << (win & 31);
+          if (pl >= 0) s_tab[pl] = -s_tab[pl] - 1;
+        }
+        ntab += 1;
+        cur = win;
+      } else {
+        if (lane == 0) {
+          s_verts[nv * 3 + 0] = v[0];
+          s_verts[nv * 3 + 1] = v[1];
+          s_verts[nv * 3 + 2] = v[2];
+          s_vpair[nv * 2 + 0] = win;
+          s_vpair[nv * 2 + 1] = cur;
+          s_tab[ntab] = win;
+          used[win >> 5] |= 1u << (win & 31);
+        }
+        nv += 1;
+        ntab += 1;
+        cur = win;
+      }
+      __syncthreads();
+    }
+    if (CHECKED && !undecided) {
+      const bool closed = ntab >= 4 && s_tab[0] == s_tab[ntab - 1];
+      if (!closed || status != AM_ST_OK) undecided = true;   // truncated or stuck: FP64 decides
+    }
+    // ---- orientation (backend/inc/kernel.h:1325-1371)
+    if (nv >= 3 && !undecided) {
+      const T* q = s_verts;
+      const T ax = q[3] - q[0], ay = q[4] - q[1], az = q[5] - q[2];
+      const T bx = q[6] - q[3], by = q[7] - q[4], bz = q[8] - q[5];
+      const T cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+      const T dir = cx * eq[0] + cy * eq[1] + cz * eq[2];
+      if (CHECKED) {
+        const T mag = (absT(cx) + absT(cy) + absT(cz)) * (absT(eq[0]) + absT(eq[1]) + absT(eq[2]));
+        if (!(absT(dir) > T(wp.eps_turn) * mag)) undecided = true;
+      }
+      reversed = ((dir < T(0)) != (net.flip != 0)) ? 1 : 0;
+      __syncthreads();
+      if (!undecided && reversed && lane < nv / 2) {
+        const int a = lane, b = nv - 1 - lane;
+        for (int c = 0; c < 3; ++c) {
+          const T t = s_verts[a * 3 + c];
+          s_verts[a * 3 + c] = s_verts[b * 3 + c];
+          s_verts[b * 3 + c] = t;
+        }
+        for (int c = 0; c < 2; ++c) {
+          const int t = s_vpair[a * 2 + c];
+          s_vpair[a * 2 + c] = s_vpair[b * 2 + c];
+          s_vpair[b * 2 + c] = t;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (CHECKED && (status != AM_ST_OK)) undecided = true;
+  if (CHECKED && undecided) {
+    if (lane == 0) io.flags[slot_region] = AM_UNDECIDED;
+    return;   // the FP64 redo writes this region's record and candidate slots
+  }
+  if (CHECKED && lane == 0) io.flags[slot_region] = 0u;
+
+  // ---- region record
+  if (lane == 0) {
+    int* inf = io.reg_info + (size_t)rid * AM_INFO_STRIDE;
+    inf[0] = start_in; inf[1] = start; inf[2] = nv; inf[3] = ntab;
+    inf[4] = reversed; inf[5] = status; inf[6] = CHECKED ? 2 : (io.ids ? 1 : 0); inf[7] = 0;
+  }
+  for (int i = lane; i < vm + 1; i += 64) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
+  for (int i = lane; i < vm * 3; i += 64)
+    io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[i] : 0.0;
+  for (int i = lane; i < vm * 2; i += 64) io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[i] : 0;
+
+  // ---- neighbour candidates (backend/inc/kernel.h:1390-1474)
+  const size_t slot0 = (size_t)slot_region * vm;
+  if (lane < vm) {
+    const int m = lane;
+    int t = -1;
+    if (m + 1 < ntab) t = s_tab[m];
+    const bool valid = t >= 0 && t < L;
+    T mid[3] = {T(0), T(0), T(0)};
+    if (valid) {
+      const int m_ = (m == 0) ? ntab - 1 : m;
+      int cnt = -1, v1 = -1;
+      for (int i = 1; i < ntab; ++i) {
+        if (s_tab[i] >= 0) ++cnt;
+        if (i == m_) v1 = cnt;
+      }
+      if (nv > 0 && v1 >= 0) {
+        const int v2 = (v1 + 1) % nv;
+        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[v1 * 3 + c] + s_verts[v2 * 3 + c]) / T(2);
+      }
+    }
+    io.cand_start[slot0 + m] = valid ? t : -2;
+    io.cand_point[(slot0 + m) * 3 + 0] = (double)mid[0];
+    io.cand_point[(slot0 + m) * 3 + 1] = (double)mid[1];
+    io.cand_point[(slot0 + m) * 3 + 2] = (double)mid[2];
+  }
+  const int Wd = net.Wd;
+  for (int idx = lane; idx < vm * Wd; idx += 64) {
+    const int m = idx / Wd, w = idx - m * Wd;
+    int t = -1;
+    if (m + 1 < ntab) t = s_tab[m];
+    if (t >= 0 && t < L) {
+      uint32_t word = smask[w];
+      if ((t >> 5) == w) word ^= 1u << (t & 31);
+      io.cand_mask[(slot0 + m) * Wd + w] = word;
+    }
+  }
+}
+
+}  // namespace am

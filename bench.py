@@ -1,0 +1,178 @@
+#!/usr/bin/env python3
+"""Mesh faces/s of the marching engine on the 4x256 ReLU MLP (one MI355X).
+
+Workload: the deep network 3->256->256->256->256->1 (geometric init, r=0.6,
+N(0, 0.05^2) hidden-weight perturbation, PCG64 seed 0, float32 parameters)
+with 1024 seeds found by FP64 bisection along Fibonacci-sphere directions.
+One step = one complete marching run: seeds and weights in, every region's
+face walked, vertices deduplicated and the indexed mesh copied to the host
+(``am_run`` + ``am_combine``).  Seed search and PLY writing are outside the
+step, as in the reference's ``am_time``.
+
+Prints one JSON line.  ``value`` is the plain FP32 form's faces/s (the
+drop-in); the split-BF16 form and the FP64 form are measured in the same
+process and reported beside it, with a topology comparison.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp64": 78.6}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(n_seeds):
+    from analyticmesh_amd import configs
+    spec = configs.deep(0)
+    pts = configs.seeds_fibonacci(spec, n_seeds)
+    states = configs.mask_of(spec, pts)
+    return spec, pts, states
+
+
+def run_form(spec, pts, states, precision, steps, warmup):
+    import torch
+    from analyticmesh_amd.engine import Engine
+    eng = Engine(spec.nodes, None, 0, precision=precision, vm=20)
+    dev = torch.device("cuda:0")
+    W = [torch.from_numpy(w).to(dev) for w in spec.weights]
+    B = [torch.from_numpy(b).to(dev) for b in spec.biases]
+    P = torch.from_numpy(pts.astype(np.float32)).to(dev)
+    S = torch.from_numpy(states.astype(np.uint8)).to(dev)
+    res = None
+    for _ in range(warmup):
+        res = eng.run(W, B, P, S)
+        res.combine()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for _ in range(steps):
+        res = eng.run(W, B, P, S)
+        res.combine()
+        stats.append(np.array(res.stats))
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    st = np.mean(stats, axis=0)
+    faces = int((res.info[:, 2] >= 3).sum())
+    return dict(sec_per_step=dt, faces=faces, regions=int(st[7]), stats=st, res=res, eng=eng)
+
+
+def cpu_baseline(spec, pts, states, max_regions):
+    import oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    r = oracle.run_oracle(spec, pts, states, precision="fp64", traversal="lifo", max_regions=max_regions,
+                          threads=threads)
+    faces = int((r.info[:, 2] >= 3).sum())
+    return dict(value=faces / r.seconds, unit="faces/s", cores=threads, kind="port",
+                sample=f"first {len(r.masks)} regions of the LIFO march of the same 4x256 net and seeds, "
+                       f"FP64 (the reference default dtype), {r.seconds:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--seeds", type=int, default=1024)
+    ap.add_argument("--cpu-regions", type=int, default=131072)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--forms", default="fp32,bf16x3,fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but not built",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print(json.dumps({"error": "no GPU visible"}))
+        sys.exit(1)
+
+    spec, pts, states = make_workload(args.seeds)
+    flops_region = spec.gemm_flops_per_region()
+    forms = [f for f in args.forms.split(",") if f]
+    out = {}
+    for f in forms:
+        log(f"[bench] form {f}: warmup {args.warmup}, steps {args.steps}")
+        try:
+            out[f] = run_form(spec, pts, states, f, args.steps, args.warmup)
+            log(f"[bench]   {out[f]['faces']} faces, {out[f]['sec_per_step']*1e3:.1f} ms/step")
+        except Exception as e:  # a form that is not built yet is reported, not fatal
+            log(f"[bench]   form {f} failed: {e}")
+            out[f] = None
+
+    head = "fp32" if out.get("fp32") else next(f for f in forms if out.get(f))
+    h = out[head]
+    st = h["stats"]
+    # dominant kernel: the plane GEMM in reduced precision, the region kernel in FP64
+    if head == "fp64":
+        kern_ms, launches, kname = st[1], max(1.0, st[11]), "k_region_fp64"
+    else:
+        kern_ms, launches, kname = st[3], max(1.0, st[11]), "k_gemm_" + head
+    flops = flops_region * (h["regions"] - st[9])
+    achieved = flops / (kern_ms / 1e3) / 1e12 if kern_ms > 0 else 0.0
+    roofline = {"bound": "mfma", "kernel": kname, "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[head],
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 5), "traffic": None,
+                "avg_launch_ms": round(kern_ms / launches, 4),
+                "flops_per_region": flops_region}
+
+    topo = {}
+    if out.get("fp64"):
+        ref = out["fp64"]["res"].records()
+        for f in forms:
+            if f != "fp64" and out.get(f):
+                r = out[f]["res"].records()
+                same = (r.visited() == ref.visited() and r.faces() == ref.faces()
+                        and r.vertex_keys() == ref.vertex_keys())
+                topo[f] = "identical" if same else "DIFFERENT"
+
+    line = {
+        "metric": "mesh faces/s, 4x256 ReLU MLP: plain FP32 vs split-BF16 MFMA, same topology",
+        "value": round(h["faces"] / h["sec_per_step"], 1),
+        "unit": "faces/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(h["sec_per_step"] * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64"}[head],
+        "data": "synthetic (seeded geometric-init weights, bisection seeds)",
+        "config": {"workload": "deep 3->256x4->1 ReLU MLP, r=0.6, sigma=0.05, seed 0, 1024 seeds",
+                   "faces": h["faces"], "regions": h["regions"], "form": head},
+        "forms": {f: (None if not out.get(f) else {
+            "faces_per_s": round(out[f]["faces"] / out[f]["sec_per_step"], 1),
+            "ms_per_step": round(out[f]["sec_per_step"] * 1e3, 3),
+            "faces": out[f]["faces"],
+            "run_ms": round(float(out[f]["stats"][0]), 3),
+            "region_kernels_ms": round(float(out[f]["stats"][1]), 3),
+            "gemm_ms": round(float(out[f]["stats"][3]), 3),
+            "walk_ms": round(float(out[f]["stats"][4]), 3),
+            "fp64_redo_ms": round(float(out[f]["stats"][5]), 3),
+            "fp64_redo_regions": int(out[f]["stats"][9]),
+            "insert_ms": round(float(out[f]["stats"][2]), 3),
+            "combine_ms": round(float(out[f]["stats"][10]), 3),
+            "levels": int(out[f]["stats"][6]),
+        }) for f in forms},
+        "topology_vs_fp64": topo,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        log(f"[bench] cpu baseline: oracle on the first {args.cpu_regions} regions")
+        line["cpu_baseline"] = cpu_baseline(spec, pts, states, args.cpu_regions)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

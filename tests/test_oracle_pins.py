@@ -1,0 +1,229 @@
+"""Pins of the CPU restatement (oracle) against independent facts.
+
+Every check here fails on a plausible mistake in the restatement: a dropped
+mask, a wrong sign flip, a transposed weight, a wrong enumeration or hash
+constant, an inverted orientation or a broken neighbour rule.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import _bruteforce as bf
+from _common import assert_same_topology, workload
+from analyticmesh_amd import configs
+from analyticmesh_amd.canonical import edges_of_faces
+import oracle
+from oracle.runner import load_oracle
+
+
+def py_fnv1a(data):
+    h = 0x811C9DC5
+    for b in data:
+        h = ((h ^ b) * 0x01000193) & 0xFFFFFFFF
+    return h
+
+
+def py_fnv1(data):
+    h = 0x811C9DC5
+    for b in data:
+        h = ((h * 0x01000193) & 0xFFFFFFFF) ^ b
+    return h
+
+
+def test_fnv_known_answers():
+    # published FNV-1a / FNV-1 32-bit test vectors
+    lib = load_oracle()
+    kat_1a = {b"": 0x811C9DC5, b"a": 0xE40C292C, b"foobar": 0xBF9CF968}
+    kat_1 = {b"": 0x811C9DC5, b"a": 0x050C5D7E, b"foobar": 0x31F0B262}
+    for s, h in kat_1a.items():
+        buf = np.frombuffer(s + b"\0", dtype=np.uint8)
+        assert lib.amo_fnv1a(buf.ctypes.data, len(s)) == h == py_fnv1a(s)
+    for s, h in kat_1.items():
+        buf = np.frombuffer(s + b"\0", dtype=np.uint8)
+        assert lib.amo_fnv1(buf.ctypes.data, len(s)) == h == py_fnv1(s)
+
+
+def test_pair_enumeration_matches_float_formula():
+    """Integer get_i_j == the reference's float ceil(sqrt(2k+2.25)-1.5) form."""
+    import ctypes
+    lib = load_oracle()
+    i = ctypes.c_int()
+    j = ctypes.c_int()
+    seen = set()
+    for ij_max in (14, 64, 1024):
+        for x in range(0, 813):
+            k = x // 2
+            a = int(math.ceil(float(np.sqrt(np.float32(2 * k + 2.25))) - 1.5))
+            r = x - a * (a + 1)
+            A = 2 * a + int(r / (a + 1))
+            B = int(math.fmod(r, a + 1))
+            ii = (A + 1) // 2 + B + 1
+            jj = A + 1 - ii
+            ii, jj = min(ii, ij_max - 1), min(jj, ij_max - 1)
+            lib.amo_get_i_j(x, ij_max, ctypes.byref(i), ctypes.byref(j))
+            assert (i.value, j.value) == (ii, jj)
+            assert 0 <= jj <= ii
+            if ij_max == 1024:
+                seen.add((ii, jj))
+    assert len(seen) == 813  # every x names a distinct pair
+
+
+@pytest.mark.parametrize("name", ["sphere", "deep"])
+def test_planes_equal_frozen_network(name):
+    """Masked chain == planes read off the frozen network at 0, e_x, e_y, e_z."""
+    spec = getattr(configs, name)(0)
+    run = oracle.OracleRun(spec)
+    rng = np.random.default_rng(1)
+    for _ in range(5):
+        bits = rng.integers(0, 2, spec.L).astype(np.uint8)
+        A, eq = run.planes(bf.pack_mask(bits))
+        Ab, eqb = bf.region_planes(spec, bits)
+        scale = np.abs(Ab).max()
+        assert np.abs(A - Ab).max() <= 1e-12 * scale
+        assert np.abs(eq - eqb).max() <= 1e-12 * max(1.0, np.abs(eqb).max())
+
+
+def test_polytope_against_bruteforce_enumeration():
+    """All 2^14 masks clipped independently: same regions, same polygons, 32 large faces."""
+    spec, pts, st = workload("polytope")
+    rec = oracle.run_oracle(spec, pts, st, traversal="lifo").records()
+    found = bf.enumerate_regions(spec)
+    assert rec.visited() == set(found)
+    for i in range(len(rec)):
+        P = bf.clip_polygon(*bf.region_planes(spec, bf.unpack_mask(rec.masks[i], spec.L)), half=10.0)
+        nv = rec.info[i, 2]
+        V = rec.verts[i, :nv]
+        assert len(P) == nv
+        for p in P:
+            assert np.abs(V - p).sum(1).min() < 1e-7
+    # closed form of the noise-free polyhedron: 32 faces, 18 vertices, 48 edges
+    big = [i for i in range(len(rec)) if bf.polygon_area(rec.verts[i, :rec.info[i, 2]]) > 1e-6]
+    assert len(big) == 32
+    pts_big = np.concatenate([rec.verts[i, :rec.info[i, 2]] for i in big])
+    cand = bf.dedup_points(pts_big, 1e-5)
+    # a corner is a point shared by at least three large faces (points where a
+    # sliver meets an edge of the polyhedron touch only two)
+    share = [sum(np.abs(rec.verts[i, :rec.info[i, 2]] - c).sum(1).min() < 1e-5 for i in big) for c in cand]
+    corners = cand[np.array(share) >= 3]
+    assert len(corners) == 18
+    # degrees: 12 corners where six large faces meet, 6 where four meet
+    assert sorted(np.array(share)[np.array(share) >= 3].tolist()) == [4] * 6 + [6] * 12
+    # every corner lies on the noise-free surface: sum of relu over the 14
+    # rows = half the sum of |row . x| (rows come in +- pairs), minus 2
+    W0 = np.array(configs._POLY_W0)
+    fz = 0.5 * np.abs(corners @ W0.T).sum(1) - 2.0
+    assert np.abs(fz).max() < 1e-5
+
+
+def test_sphere_is_closed_manifold_and_orders_agree():
+    spec, pts, st = workload("sphere")
+    lifo = oracle.run_oracle(spec, pts, st, traversal="lifo").records()
+    bfs = oracle.run_oracle(spec, pts, st, traversal="bfs").records()
+    assert_same_topology(lifo, bfs)
+    V, F = lifo.keyed_mesh()
+    E = edges_of_faces(F)
+    assert set(E.values()) == {2}, "every edge is shared by exactly two faces"
+    assert len(V) - len(E) + len(F) == 2, "Euler characteristic of a sphere"
+
+
+def test_sphere_vertices_on_their_planes():
+    """Each vertex: its two tight neurons and f vanish (independent FP64 forward)."""
+    spec, pts, st = workload("sphere")
+    rec = oracle.run_oracle(spec, pts, st, traversal="bfs").records()
+    worst = 0.0
+    for i in range(0, len(rec), 7):
+        bits = bf.unpack_mask(rec.masks[i], spec.L)
+        nv = rec.info[i, 2]
+        V = rec.verts[i, :nv]
+        pre, out = bf.frozen_preacts(spec, bits, V)
+        for k in range(nv):
+            a, b = rec.vpairs[i, k]
+            worst = max(worst, abs(pre[k, a]), abs(pre[k, b]), abs(out[k]))
+            # every other constraint holds (with the region's signs)
+            sgn = np.where(bits.astype(bool), -1.0, 1.0)
+            assert (sgn * pre[k] <= 1e-9).all()
+    assert worst < 1e-12
+
+
+def test_sphere_faces_match_clipping_sample():
+    spec, pts, st = workload("sphere")
+    rec = oracle.run_oracle(spec, pts, st, traversal="bfs").records()
+    for i in range(0, len(rec), 97):
+        P = bf.clip_polygon(*bf.region_planes(spec, bf.unpack_mask(rec.masks[i], spec.L)), half=10.0)
+        nv = rec.info[i, 2]
+        assert len(P) == nv
+        for p in P:
+            assert np.abs(rec.verts[i, :nv] - p).sum(1).min() < 1e-9
+
+
+def test_neighbour_symmetry_and_orientation():
+    spec, pts, st = workload("sphere")
+    rec = oracle.run_oracle(spec, pts, st, traversal="bfs").records()
+    faces = rec.faces()
+    index = {rec.masks[i].tobytes(): i for i in range(len(rec))}
+    for i in range(0, len(rec), 11):
+        m = rec.masks[i]
+        nv = rec.info[i, 2]
+        edges = set()
+        for k in range(nv):
+            edges.update(int(x) for x in rec.vpairs[i, k])
+        for t in edges:
+            n = m.copy()
+            n[t >> 5] ^= np.uint32(1 << (t & 31))
+            j = index[n.tobytes()]
+            other = set()
+            for k in range(rec.info[j, 2]):
+                other.update(int(x) for x in rec.vpairs[j, k])
+            assert t in other
+        # orientation: polygon normal along the gradient of f (flip_insideout=False)
+        V = rec.verts[i, :nv]
+        bits = bf.unpack_mask(m, spec.L)
+        _, eq = bf.region_planes(spec, bits)
+        nrm = np.zeros(3)
+        for k in range(nv):
+            nrm += np.cross(V[k], V[(k + 1) % nv])
+        assert nrm @ eq[:3] > 0
+    assert len(faces) == len(rec)
+
+
+def test_flip_insideout_reverses_every_face():
+    spec, pts, st = workload("sphere")
+    a = oracle.run_oracle(spec, pts, st, traversal="bfs", flip=False).records().faces()
+    b = oracle.run_oracle(spec, pts, st, traversal="bfs", flip=True).records().faces()
+    assert a.keys() == b.keys()
+    for k in a:
+        ra = list(a[k])
+        rb = list(reversed(b[k]))
+        i0 = rb.index(min(rb))
+        assert ra == rb[i0:] + rb[:i0]
+
+
+def test_extra_planes_bound_the_mesh():
+    """Extra half-spaces x <= 0.2 clip the sphere: all vertices satisfy them."""
+    spec, pts, st = workload("sphere")
+    ew = np.array([[1.0 + 1e-8, 1e-8, 1e-8]])
+    eb = np.array([-0.2])
+    seed = configs.bisect_seed(spec, (-1.0, 0.0, 0.0))[None]
+    r = oracle.run_oracle(spec, seed, configs.mask_of(spec, seed), traversal="bfs",
+                          extra_w=ew, extra_b=eb).records()
+    V, F = r.keyed_mesh()
+    # the 1e-8 jitter on the plane normal moves the cut by up to ~1e-8
+    assert (V[:, 0] <= 0.2 + 1e-7).all()
+    assert (V[:, 0] > 0.2 - 1e-6).any()
+    # boundary edges appear exactly once (open mesh along the cut)
+    E = edges_of_faces(F)
+    assert set(E.values()) <= {1, 2}
+
+
+def test_max_regions_truncates():
+    spec, pts, st = workload("sphere")
+    r = oracle.run_oracle(spec, pts, st, traversal="lifo", max_regions=1000)
+    assert r.truncated and 1000 <= len(r.masks) <= 1000 + 1024
+
+
+def test_oracle_fp32_mode_runs():
+    spec, pts, st = workload("sphere")
+    r = oracle.run_oracle(spec, pts, st, precision="fp32", traversal="lifo")
+    rec = r.records()
+    assert rec.face_count() > 10000
