@@ -233,7 +233,7 @@ size_t region_lds_bytes(const am_ctx* c, int& Cs, int& Csort) {
          (size_t)((c->C + 31) / 32) * 4 + (size_t)c->Wd * 4 + 64;
 }
 
-void launch_region_fp64(am_ctx* c, int first, int count, float* ms) {
+void launch_region_fp64(am_ctx* c, int first, int count, float* ms, const int* ids = nullptr) {
   int Cs, Csort;
   const size_t lds = region_lds_bytes(c, Cs, Csort);
   RegionIO io;
@@ -252,6 +252,7 @@ void launch_region_fp64(am_ctx* c, int first, int count, float* ms) {
   io.Cs = Cs;
   io.Csort = Csort;
   io.flags = nullptr;
+  io.ids = ids;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   hipLaunchKernelGGL(k_region_fp64, dim3(count), dim3(64), lds, c->stream, c->net, io);
   HIPCHK(hipGetLastError());
@@ -561,12 +562,13 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.reg_vpair = c->reg_vpair.p; fl.cand_mask = c->cand_mask.p; fl.cand_start = c->cand_start.p;
         fl.cand_point = c->cand_point.p;
         FastTimes ft{};
-        int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->ev0, c->ev1, ft);
+        int* ids = nullptr;
+        const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->ev0, c->ev1, ft, &ids);
         if (nredo > 0) {
           // FP64 re-decision of the undecided regions: rerun them through the
-          // FP64 kernel in place (their candidate slots are rewritten too).
+          // FP64 kernel in place (their records and candidate slots are rewritten).
           float m2 = 0;
-          redo_fp64(c->fbuf, n, fl, nredo, st, c->ev0, c->ev1, &m2, lds, Cs, Csort);
+          launch_region_fp64(c, (int)lvl_first, nredo, &m2, ids);
           t_redo += m2;
           redo += nredo;
         }

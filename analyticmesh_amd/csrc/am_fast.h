@@ -1,21 +1,47 @@
-// Reduced-precision plane path (FP32 MFMA / split-BF16 MFMA) with FP64
-// re-decision of regions whose predicates fall inside the error bound.
+// Reduced-precision plane path: the masked affine chain of a tile of regions
+// as real GEMMs on MFMA (FP32, or split-BF16 with three products), a checked
+// FP32 walk, and a list of the regions whose predicates it could not decide
+// (the engine re-runs those through the FP64 kernel).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "am_kernels.h"
+#include "am_walk.h"
 
 namespace am {
+
+#define AM_GT 8                 // regions per GEMM workgroup
+#define AM_GN (4 * AM_GT)       // GEMM columns: x, y, z, c of each region
+
+struct FastNetDev {
+  int D, L, E, C, Wd;
+  int Cs;                       // stride of one plane component array in HBM (C rounded up to 64)
+  int Kpad;                     // LDS row count (largest width rounded up to 32)
+  int nodes[AM_MAX_LINEAR + 1];
+  int off[AM_MAX_LINEAR + 1];
+  int nrb[AM_MAX_LINEAR];       // 16-row blocks of linear layer li
+  int nkb[AM_MAX_LINEAR];       // 4-wide (fp32) or 32-wide (bf16) k blocks of layer li
+  const float* W0;              // (n1, 3)
+  const float* bias[AM_MAX_LINEAR];
+  const float* Wp[AM_MAX_LINEAR];       // fp32 A operands, [nrb][nkb][64 lanes]
+  const uint16_t* Wh[AM_MAX_LINEAR];    // bf16 hi A operands, [nrb][nkb][64 lanes][8]
+  const uint16_t* Wl[AM_MAX_LINEAR];    // bf16 lo A operands
+  const float* ew;
+  const float* eb;
+  float iso;
+};
 
 struct FastNet {
   int precision = 0;
   bool ready = false;
+  FastNetDev dev{};
+  WalkParams wp{};
 };
 
 struct FastBuffers {
-  void* mem[16] = {nullptr};
-  size_t bytes[16] = {0};
+  void* mem[40] = {nullptr};
+  size_t bytes[40] = {0};
 };
 
 struct FastLevel {
@@ -37,12 +63,13 @@ struct FastTimes {
   int launches;
 };
 
+// Upload weights in MFMA operand order; `weights`/`biases` as given to am_run.
 void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, const void* const* weights,
                   const void* const* biases, int dtype, hipStream_t st);
+// Planes + checked walk for one level; returns the number of undecided
+// regions and their ids in *ids (device memory owned by `b`).
 int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-               hipEvent_t e0, hipEvent_t e1, FastTimes& t);
-void redo_fp64(FastBuffers& b, const NetDev& net, const FastLevel& lvl, int nredo, hipStream_t st, hipEvent_t e0,
-               hipEvent_t e1, float* ms, size_t lds, int Cs, int Csort);
+               hipEvent_t e0, hipEvent_t e1, FastTimes& t, int** ids);
 void fast_release(FastBuffers& b);
 
 }  // namespace am

@@ -1,31 +1,491 @@
-// Reduced-precision plane path: placeholder until the MFMA kernels land.
+// Reduced-precision plane path.
+//
+// Face kernel: for a tile of AM_GT regions the masked affine chain
+//   P_{l+1} = W_l diag(m_l) P_l   (columns: x, y, z and the bias column c)
+// is one GEMM per layer, W_l[n_out x n_in] x X[n_in x 4*AM_GT], on MFMA with
+// X staged in LDS (backend/inc/process.h:18-193 does the same chain as
+// per-region cuBLAS calls).  Two arithmetic forms:
+//   * FP32: v_mfma_f32_16x16x4f32;
+//   * split-BF16: w = wh + wl, x = xh + xl (bf16 parts), y = wh.xh + wh.xl + wl.xh
+//     with v_mfma_f32_16x16x32_bf16 and FP32 accumulation.
+// The signed planes go to HBM, one SoA record per region, and the checked FP32
+// walk reads them back into LDS.
+#include <algorithm>
+#include <cstring>
 #include <stdexcept>
+#include <vector>
 
 #include "am_fast.h"
 
 namespace am {
 
-void fast_prepare(FastNet& f, FastBuffers&, const NetDev&, int precision, const void* const*, const void* const*,
-                  int, hipStream_t) {
-  f.precision = precision;
-  throw std::runtime_error("reduced-precision planes are not available in this build");
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+#define AM_XS (AM_GN + 4)   // fp32 LDS row stride (floats)
+
+struct GemmIO {
+  const uint32_t* reg_mask;
+  int first;        // region id of local index 0
+  int count;        // regions in this chunk
+  float* planes;    // [count][4][Cs]
+  float* eq;        // [count][4]
+};
+
+__device__ __forceinline__ uint16_t bf16_rn(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
+// Stage layer-1 planes: X = masked rows of W0 (and b0); signed planes to HBM.
+template <int PREC>
+__device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32_t* mk, int r0, int nreg,
+                             float* Xs, uint16_t* Xh, uint16_t* Xl, int KS) {
+  const int tid = threadIdx.x;
+  const int n1 = f.nodes[1];
+  for (int idx = tid; idx < n1 * AM_GN; idx += 256) {
+    const int k = idx / AM_GN, col = idx - k * AM_GN, t = col >> 2, q = col & 3;
+    const float val = q < 3 ? f.W0[k * 3 + q] : f.bias[0][k];
+    const float x = mbit(mk + t * f.Wd, k) ? val : 0.f;
+    if (PREC == 0) {
+      Xs[k * AM_XS + col] = x;
+    } else {
+      const uint16_t h = bf16_rn(x);
+      Xh[col * KS + k] = h;
+      Xl[col * KS + k] = bf16_rn(x - bf16_to_f(h));
+    }
+  }
+  for (int idx = tid; idx < AM_GN * n1; idx += 256) {
+    const int col = idx / n1, k = idx - col * n1, t = col >> 2, q = col & 3;
+    if (t < nreg) {
+      const float val = q < 3 ? f.W0[k * 3 + q] : f.bias[0][k];
+      io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + k] = mbit(mk + t * f.Wd, k) ? -val : val;
+    }
+  }
+  for (int idx = tid; idx < AM_GN * f.E; idx += 256) {
+    const int col = idx / f.E, e = idx - col * f.E, t = col >> 2, q = col & 3;
+    if (t < nreg) io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + f.L + e] = q < 3 ? f.ew[e * 3 + q] : f.eb[e];
+  }
 }
 
-int fast_level(FastNet&, FastBuffers&, const NetDev&, const FastLevel&, hipStream_t, hipEvent_t, hipEvent_t,
-               FastTimes& t) {
+template <int PREC>
+__global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int KS = f.Kpad + 8;   // bf16 column stride (halves)
+  float* Ys = reinterpret_cast<float*>(smem_raw);                   // [Kpad][AM_XS]
+  float* Xs = Ys + (size_t)f.Kpad * AM_XS;                           // fp32: [Kpad][AM_XS]
+  uint16_t* Xh = reinterpret_cast<uint16_t*>(Xs);                    // bf16: [AM_GN][KS]
+  uint16_t* Xl = Xh + (size_t)AM_GN * KS;
+  uint32_t* mk = reinterpret_cast<uint32_t*>(PREC == 0 ? (unsigned char*)(Xs + (size_t)f.Kpad * AM_XS)
+                                                       : (unsigned char*)(Xl + (size_t)AM_GN * KS));
+  const int r0 = blockIdx.x * AM_GT;
+  const int nreg = min(AM_GT, io.count - r0);
+
+  for (int idx = tid; idx < AM_GT * f.Wd; idx += 256) {
+    const int t = idx / f.Wd, w = idx - t * f.Wd;
+    mk[idx] = t < nreg ? io.reg_mask[(size_t)(io.first + r0 + t) * f.Wd + w] : 0u;
+  }
+  if (PREC == 0) {
+    for (int i = tid; i < f.Kpad * AM_XS; i += 256) Xs[i] = 0.f;
+  } else {
+    for (int i = tid; i < 2 * AM_GN * KS; i += 256) Xh[i] = 0;
+  }
+  __syncthreads();
+  stage_layer1<PREC>(f, io, mk, r0, nreg, Xs, Xh, Xl, KS);
+  __syncthreads();
+
+  for (int li = 1; li <= f.D; ++li) {
+    const int nout = f.nodes[li + 1];
+    const int nrb = f.nrb[li], nkb = f.nkb[li];
+    const float* bias = f.bias[li];
+    for (int rb = wave; rb < nrb; rb += 4) {
+      f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+      if (PREC == 0) {
+        const float* Wp = f.Wp[li] + (size_t)rb * nkb * 64 + lane;
+        const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
+        int kb = 0;
+        for (; kb + 4 <= nkb; kb += 4) {
+          const float a0 = Wp[(kb + 0) * 64], a1 = Wp[(kb + 1) * 64], a2 = Wp[(kb + 2) * 64], a3 = Wp[(kb + 3) * 64];
+          const float* x0 = xb + (kb * 4) * AM_XS;
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[0], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[16], acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, x0[4 * AM_XS], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, x0[4 * AM_XS + 16], acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, x0[8 * AM_XS], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, x0[8 * AM_XS + 16], acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS + 16], acc1, 0, 0, 0);
+        }
+        for (; kb < nkb; ++kb) {
+          const float a = Wp[kb * 64];
+          const float* x0 = xb + (kb * 4) * AM_XS;
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[0], acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[16], acc1, 0, 0, 0);
+        }
+      } else {
+        const uint4* Ah = reinterpret_cast<const uint4*>(f.Wh[li]) + (size_t)rb * nkb * 64 + lane;
+        const uint4* Al = reinterpret_cast<const uint4*>(f.Wl[li]) + (size_t)rb * nkb * 64 + lane;
+        const int kq = 8 * (lane >> 4);
+        const uint16_t* xh0 = Xh + (lane & 15) * KS + kq;
+        const uint16_t* xl0 = Xl + (lane & 15) * KS + kq;
+        const uint16_t* xh1 = xh0 + 16 * KS;
+        const uint16_t* xl1 = xl0 + 16 * KS;
+        for (int kb = 0; kb < nkb; ++kb) {
+          const uint4 ah = Ah[kb * 64], al = Al[kb * 64];
+          const uint4 bh0 = *reinterpret_cast<const uint4*>(xh0 + kb * 32);
+          const uint4 bl0 = *reinterpret_cast<const uint4*>(xl0 + kb * 32);
+          const uint4 bh1 = *reinterpret_cast<const uint4*>(xh1 + kb * 32);
+          const uint4 bl1 = *reinterpret_cast<const uint4*>(xl1 + kb * 32);
+          const bf16x8 vah = __builtin_bit_cast(bf16x8, ah), val_ = __builtin_bit_cast(bf16x8, al);
+          const bf16x8 vbh0 = __builtin_bit_cast(bf16x8, bh0), vbl0 = __builtin_bit_cast(bf16x8, bl0);
+          const bf16x8 vbh1 = __builtin_bit_cast(bf16x8, bh1), vbl1 = __builtin_bit_cast(bf16x8, bl1);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbh0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbh1, acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbl0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbl1, acc1, 0, 0, 0);
+          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh0, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh1, acc1, 0, 0, 0);
+        }
+      }
+      const int c0 = lane & 15;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int row = rb * 16 + 4 * (lane >> 4) + i;
+        if (row < nout) {
+          const float b = bias[row];
+          Ys[row * AM_XS + c0] = acc0[i] + ((c0 & 3) == 3 ? b : 0.f);
+          Ys[row * AM_XS + 16 + c0] = acc1[i] + (((16 + c0) & 3) == 3 ? b : 0.f);
+        }
+      }
+    }
+    __syncthreads();
+    if (li < f.D) {
+      const int out0 = f.off[li];
+      for (int idx = tid; idx < AM_GN * nout; idx += 256) {
+        const int col = idx / nout, j = idx - col * nout, t = col >> 2, q = col & 3;
+        if (t < nreg) {
+          const float y = Ys[j * AM_XS + col];
+          io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + out0 + j] = mbit(mk + t * f.Wd, out0 + j) ? -y : y;
+        }
+      }
+      const int rows = PREC == 0 ? ((nout + 3) & ~3) : ((nout + 31) & ~31);
+      for (int idx = tid; idx < rows * AM_GN; idx += 256) {
+        const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2;
+        const float x = (j < nout && mbit(mk + t * f.Wd, out0 + j)) ? Ys[j * AM_XS + col] : 0.f;
+        if (PREC == 0) {
+          Xs[j * AM_XS + col] = x;
+        } else {
+          const uint16_t h = bf16_rn(x);
+          Xh[col * KS + j] = h;
+          Xl[col * KS + j] = bf16_rn(x - bf16_to_f(h));
+        }
+      }
+    } else if (tid < AM_GN) {
+      const int t = tid >> 2, q = tid & 3;
+      if (t < nreg) {
+        float v = Ys[tid];
+        if (q == 3) v = v - f.iso;
+        io.eq[(size_t)(r0 + t) * 4 + q] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Checked FP32 walk of one region whose planes the face kernel wrote.
+__global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
+                                                 const float* eqbuf, int Cs_hbm, int chunk_first) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int C = net.C;
+  const int Cs = io.Cs, N2 = io.Csort;
+  float* Px = reinterpret_cast<float*>(smem_raw);
+  float* Py = Px + Cs;
+  float* Pz = Py + Cs;
+  float* Pc = Pz + Cs;
+  float* keys = Pc + Cs;
+  int* order = reinterpret_cast<int*>(keys + N2);
+  uint32_t* used = reinterpret_cast<uint32_t*>(order + N2);
+  uint32_t* smask = used + ((C + 31) >> 5);
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;          // index within the chunk
+  if (b >= io.count) return;
+  const int rid = chunk_first + b;
+  const float* src = planes + (size_t)b * 4 * Cs_hbm;
+  for (int g = lane; g < C; g += 64) {
+    Px[g] = src[g];
+    Py[g] = src[Cs_hbm + g];
+    Pz[g] = src[2 * Cs_hbm + g];
+    Pc[g] = src[3 * Cs_hbm + g];
+  }
+  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
+  const float eq[4] = {eqbuf[b * 4 + 0], eqbuf[b * 4 + 1], eqbuf[b * 4 + 2], eqbuf[b * 4 + 3]};
+  const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
+                      (float)io.reg_point[(size_t)rid * 3 + 2]};
+  __syncthreads();
+  walk_region<float, true>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
+}
+
+__global__ void k_collect_undecided(const unsigned* flags, int n, int first, int* cnt, int* ids) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (flags[i] & AM_UNDECIDED)) ids[atomicAdd(cnt, 1)] = first + i;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+namespace {
+
+#define FCHK(x)                                                                   \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) throw std::runtime_error(std::string("HIP error: ") +   \
+                                                   hipGetErrorString(e_) + " in " #x); \
+  } while (0)
+
+enum Slot {
+  S_W0 = 0, S_EW, S_EB, S_PLANES, S_EQ, S_FLAGS, S_IDS, S_CNT,
+  S_BIAS0 = 8,      // 8..23: biases per linear layer
+  S_WP0 = 24,       // 24..39: operand buffers per linear layer (fp32, or hi+lo packed)
+};
+
+void* slot(FastBuffers& b, int s, size_t bytes) {
+  if (b.bytes[s] < bytes) {
+    if (b.mem[s]) (void)hipFree(b.mem[s]);
+    b.mem[s] = nullptr;
+    b.bytes[s] = 0;
+    FCHK(hipMalloc(&b.mem[s], std::max<size_t>(bytes, 256)));
+    b.bytes[s] = std::max<size_t>(bytes, 256);
+  }
+  return b.mem[s];
+}
+
+std::vector<float> to_float(const void* src, size_t n, int dtype) {
+  std::vector<float> out(n);
+  if (n == 0) return out;
+  if (dtype == 1) {
+    std::vector<double> d(n);
+    FCHK(hipMemcpy(d.data(), src, n * 8, hipMemcpyDefault));
+    for (size_t i = 0; i < n; ++i) out[i] = (float)d[i];
+  } else {
+    FCHK(hipMemcpy(out.data(), src, n * 4, hipMemcpyDefault));
+  }
+  return out;
+}
+
+uint16_t host_bf16(float x) {
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+float host_bf16_f(uint16_t h) {
+  uint32_t u = uint32_t(h) << 16;
+  float x;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+
+size_t gemm_lds(const FastNetDev& d, int prec) {
+  const size_t ys = (size_t)d.Kpad * AM_XS * 4;
+  const size_t xs = prec == 0 ? (size_t)d.Kpad * AM_XS * 4 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;
+  return ys + xs + (size_t)AM_GT * d.Wd * 4 + 16;
+}
+
+}  // namespace
+
+void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, const void* const* weights,
+                  const void* const* biases, int dtype, hipStream_t st) {
+  (void)st;
+  FastNetDev& d = f.dev;
+  std::memset(&d, 0, sizeof(d));
+  f.precision = precision;
+  for (int li = 1; li < net.D; ++li)
+    if (net.nskip[li] > 0) throw std::runtime_error("skip connections run on the FP64 path only");
+  d.D = net.D; d.L = net.L; d.E = net.E; d.C = net.C; d.Wd = net.Wd;
+  d.Cs = (net.C + 63) & ~63;
+  int maxw = 0;
+  for (int i = 0; i <= net.D + 1; ++i) {
+    d.nodes[i] = net.nodes[i];
+    maxw = std::max(maxw, net.nodes[i]);
+  }
+  for (int i = 0; i <= net.D; ++i) d.off[i] = net.off[i];
+  d.Kpad = (maxw + 31) & ~31;
+  d.iso = (float)net.iso;
+  const int prec = precision == 1 ? 1 : 0;
+  // W0, biases, extras
+  {
+    auto w0 = to_float(weights[0], (size_t)net.nodes[1] * 3, dtype);
+    float* p = (float*)slot(b, S_W0, w0.size() * 4);
+    FCHK(hipMemcpy(p, w0.data(), w0.size() * 4, hipMemcpyHostToDevice));
+    d.W0 = p;
+  }
+  for (int li = 0; li <= net.D; ++li) {
+    auto bb = to_float(biases[li], (size_t)net.nodes[li + 1], dtype);
+    float* p = (float*)slot(b, S_BIAS0 + li, bb.size() * 4);
+    FCHK(hipMemcpy(p, bb.data(), bb.size() * 4, hipMemcpyHostToDevice));
+    d.bias[li] = p;
+  }
+  {
+    std::vector<float> ew(std::max(1, net.E * 3)), eb(std::max(1, net.E));
+    if (net.E) {
+      std::vector<double> t(net.E * 3);
+      FCHK(hipMemcpy(t.data(), net.ew, t.size() * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < t.size(); ++i) ew[i] = (float)t[i];
+      std::vector<double> u(net.E);
+      FCHK(hipMemcpy(u.data(), net.eb, u.size() * 8, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < u.size(); ++i) eb[i] = (float)u[i];
+    }
+    float* pw = (float*)slot(b, S_EW, ew.size() * 4);
+    float* pb = (float*)slot(b, S_EB, eb.size() * 4);
+    FCHK(hipMemcpy(pw, ew.data(), ew.size() * 4, hipMemcpyHostToDevice));
+    FCHK(hipMemcpy(pb, eb.data(), eb.size() * 4, hipMemcpyHostToDevice));
+    d.ew = pw;
+    d.eb = pb;
+  }
+  // MFMA A operands of every layer after the first
+  for (int li = 1; li <= net.D; ++li) {
+    const int nin = net.nodes[li], nout = net.nodes[li + 1];
+    auto w = to_float(weights[li], (size_t)nout * nin, dtype);
+    const int nrb = (nout + 15) / 16;
+    d.nrb[li] = nrb;
+    if (prec == 0) {
+      const int nkb = (nin + 3) / 4;
+      d.nkb[li] = nkb;
+      std::vector<float> a((size_t)nrb * nkb * 64, 0.f);
+      for (int rb = 0; rb < nrb; ++rb)
+        for (int kb = 0; kb < nkb; ++kb)
+          for (int l = 0; l < 64; ++l) {
+            const int row = rb * 16 + (l & 15), k = kb * 4 + (l >> 4);
+            if (row < nout && k < nin) a[((size_t)rb * nkb + kb) * 64 + l] = w[(size_t)row * nin + k];
+          }
+      float* p = (float*)slot(b, S_WP0 + li, a.size() * 4);
+      FCHK(hipMemcpy(p, a.data(), a.size() * 4, hipMemcpyHostToDevice));
+      d.Wp[li] = p;
+    } else {
+      const int nkb = (nin + 31) / 32;
+      d.nkb[li] = nkb;
+      const size_t n = (size_t)nrb * nkb * 64 * 8;
+      std::vector<uint16_t> hi(2 * n, 0);
+      for (int rb = 0; rb < nrb; ++rb)
+        for (int kb = 0; kb < nkb; ++kb)
+          for (int l = 0; l < 64; ++l)
+            for (int e = 0; e < 8; ++e) {
+              const int row = rb * 16 + (l & 15), k = kb * 32 + 8 * (l >> 4) + e;
+              float x = (row < nout && k < nin) ? w[(size_t)row * nin + k] : 0.f;
+              const uint16_t h = host_bf16(x);
+              const size_t at = (((size_t)rb * nkb + kb) * 64 + l) * 8 + e;
+              hi[at] = h;
+              hi[n + at] = host_bf16(x - host_bf16_f(h));
+            }
+      uint16_t* p = (uint16_t*)slot(b, S_WP0 + li, hi.size() * 2);
+      FCHK(hipMemcpy(p, hi.data(), hi.size() * 2, hipMemcpyHostToDevice));
+      d.Wh[li] = p;
+      d.Wl[li] = p + n;
+    }
+  }
+  const size_t lds = gemm_lds(d, prec);
+  if (lds > 65536) {
+    if (prec == 0)
+      FCHK(hipFuncSetAttribute((const void*)k_gemm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    else
+      FCHK(hipFuncSetAttribute((const void*)k_gemm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  }
+  // error bounds of the planes, relative to each plane's largest coefficient
+  if (prec == 0) f.wp = WalkParams{2e-5f, 1e-3f, 1e3f};
+  else f.wp = WalkParams{1e-3f, 2e-2f, 1e2f};
+  f.ready = true;
+}
+
+int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
+               hipEvent_t e0, hipEvent_t e1, FastTimes& t, int** ids) {
+  const FastNetDev& d = f.dev;
+  const int prec = f.precision == 1 ? 1 : 0;
   t.gemm_ms = t.walk_ms = 0;
   t.launches = 0;
-  throw std::runtime_error("reduced-precision planes are not available in this build");
-}
+  const int CH = 1 << 16;
+  const int chunk = std::min(lvl.count, CH);
+  float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * 4 * d.Cs * 4);
+  float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * 4 * 4);
+  unsigned* flags = (unsigned*)slot(b, S_FLAGS, (size_t)lvl.count * 4);
+  int* idl = (int*)slot(b, S_IDS, (size_t)lvl.count * 4);
+  int* cnt = (int*)slot(b, S_CNT, 16);
 
-void redo_fp64(FastBuffers&, const NetDev&, const FastLevel&, int, hipStream_t, hipEvent_t, hipEvent_t, float* ms,
-               size_t, int, int) {
-  *ms = 0;
+  int Csort = 1;
+  while (Csort < net.C) Csort <<= 1;
+  const int CsL = net.C;
+  const size_t wlds = (size_t)4 * CsL * 4 + (size_t)Csort * 8 + (size_t)((net.C + 31) / 32) * 4 + (size_t)net.Wd * 4 + 64;
+  if (wlds > 65536)
+    FCHK(hipFuncSetAttribute((const void*)k_walk_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wlds));
+  const size_t glds = gemm_lds(d, prec);
+
+  for (int c0 = 0; c0 < lvl.count; c0 += CH) {
+    const int n = std::min(CH, lvl.count - c0);
+    GemmIO g;
+    g.reg_mask = lvl.reg_mask;
+    g.first = lvl.first + c0;
+    g.count = n;
+    g.planes = planes;
+    g.eq = eqb;
+    float ms = 0;
+    FCHK(hipEventRecord(e0, st));
+    if (prec == 0)
+      hipLaunchKernelGGL(k_gemm<0>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
+    else
+      hipLaunchKernelGGL(k_gemm<1>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
+    FCHK(hipGetLastError());
+    FCHK(hipEventRecord(e1, st));
+    FCHK(hipEventSynchronize(e1));
+    FCHK(hipEventElapsedTime(&ms, e0, e1));
+    t.gemm_ms += ms;
+
+    RegionIO io;
+    io.reg_mask = lvl.reg_mask;
+    io.reg_start = lvl.reg_start;
+    io.reg_point = lvl.reg_point;
+    io.reg_info = lvl.reg_info;
+    io.reg_tab = lvl.reg_tab;
+    io.reg_verts = lvl.reg_verts;
+    io.reg_vpair = lvl.reg_vpair;
+    io.cand_mask = lvl.cand_mask;
+    io.cand_start = lvl.cand_start;
+    io.cand_point = lvl.cand_point;
+    io.first = lvl.first;          // candidate slots / flags are level-relative
+    io.count = n;
+    io.Cs = CsL;
+    io.Csort = Csort;
+    io.flags = flags;
+    io.ids = nullptr;
+    FCHK(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_walk_f32, dim3(n), dim3(64), wlds, st, net, io, f.wp, planes, eqb, d.Cs,
+                       lvl.first + c0);
+    FCHK(hipGetLastError());
+    FCHK(hipEventRecord(e1, st));
+    FCHK(hipEventSynchronize(e1));
+    FCHK(hipEventElapsedTime(&ms, e0, e1));
+    t.walk_ms += ms;
+    t.launches += 1;
+  }
+  FCHK(hipMemsetAsync(cnt, 0, 4, st));
+  hipLaunchKernelGGL(k_collect_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count,
+                     lvl.first, cnt, idl);
+  FCHK(hipGetLastError());
+  int h = 0;
+  FCHK(hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, st));
+  FCHK(hipStreamSynchronize(st));
+  *ids = idl;
+  return h;
 }
 
 void fast_release(FastBuffers& b) {
-  for (int i = 0; i < 16; ++i)
+  for (int i = 0; i < 40; ++i) {
     if (b.mem[i]) (void)hipFree(b.mem[i]);
+    b.mem[i] = nullptr;
+    b.bytes[i] = 0;
+  }
 }
 
 }  // namespace am

@@ -47,8 +47,9 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
       if (violates(ax, ay, az, c, v)) return 1;
     } else {
       const T s = plane_eval(ax, ay, az, c, v);
-      const T m = absT(ax) * absT(v[0]) + absT(ay) * absT(v[1]) + absT(az) * absT(v[2]) + absT(c);
-      const T err = epsv * m;
+      // plane errors are bounded relative to the plane's largest coefficient
+      const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
+      const T err = epsv * pm * (T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
       if (s > err) return 1;
       if (s >= -err) verdict = 2;
     }
@@ -133,7 +134,8 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
             if (violates(Px[g], Py[g], Pz[g], Pc[g], v)) verdict = 1;
           } else {
             const T s = plane_eval(Px[g], Py[g], Pz[g], Pc[g], v);
-            const T m = absT(Px[g]) * absT(v[0]) + absT(Py[g]) * absT(v[1]) + absT(Pz[g]) * absT(v[2]) + absT(Pc[g]);
+            const T pm = fmax(fmax(absT(Px[g]), absT(Py[g])), fmax(absT(Pz[g]), absT(Pc[g])));
+            const T m = pm * (T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
             if (s > epsv * m) verdict = 1;
             else if (s >= -epsv * m) verdict = 2;
           }

@@ -38,6 +38,26 @@ def _as_ptr(a):
     return int(a.ctypes.data)
 
 
+class Faces:
+    """Polygon faces in CSR form; indexing yields one face's vertex indices."""
+
+    def __init__(self, offsets, indices):
+        self.offsets = offsets
+        self.indices = indices
+
+    def __len__(self):
+        return len(self.offsets) - 1
+
+    def __getitem__(self, i):
+        if i < 0:
+            i += len(self)
+        return self.indices[self.offsets[i]:self.offsets[i + 1]]
+
+    def __iter__(self):
+        for i in range(len(self)):
+            yield self[i]
+
+
 class MarchResult:
     def __init__(self, eng, spec_L, Wd, vm):
         self._eng = eng
@@ -61,10 +81,14 @@ class MarchResult:
         return float(self.stats[0])
 
     def combine(self, scale=1.0, center=(0.0, 0.0, 0.0)):
+        """Deduplicate vertices and index faces on the GPU.
+
+        Returns (vertices (V, 3) float64, Faces) where Faces is a list-like view
+        over the CSR arrays ``offsets`` (F + 1) and ``indices``."""
         self._eng.combine(float(scale), [float(c) for c in center])
         V, off, idx = self._eng.mesh()
-        faces = [idx[off[i]:off[i + 1]] for i in range(len(off) - 1)]
-        return V, faces
+        self.combine_ms = float(self._eng.stats()[10])
+        return V, Faces(off, idx)
 
     def export_ply(self, path, is_polymesh=True, is_float32=True):
         self._eng.export_ply(str(path), bool(is_polymesh), bool(is_float32))
