@@ -23,6 +23,7 @@
 #include "am_hash.h"
 #include "am_kernels.h"
 #include "am_fast.h"
+#include "am_adjoint.h"
 
 using namespace am;
 
@@ -93,7 +94,7 @@ struct am_ctx {
 
   // network (device)
   DevBuf<double> W0, ew, eb;
-  std::vector<DevBuf<double>> WT, b, tm, tmT;
+  std::vector<DevBuf<double>> WT, Wrow, b, tm, tmT;
   std::vector<int> tm_h, tm_w;
   NetDev net{};
   FastNet fast{};
@@ -119,6 +120,7 @@ struct am_ctx {
 
   // mesh
   bool has_run = false, has_mesh = false;
+  DevBuf<int> m_vfirst, m_vid;   // per (region, vertex slot): first occurrence flag, vertex index
   std::vector<double> mesh_v;
   std::vector<int> mesh_off, mesh_idx;
 
@@ -445,11 +447,13 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     // ---- network upload (fp64, weights transposed for the chain)
     const int nl = c->D + 1;
     c->WT.resize(nl);
+    c->Wrow.resize(nl);
     c->b.resize(nl);
     upload(c->W0, weights[0], (size_t)c->nodes[1] * 3, dtype, st);
     for (int li = 0; li < nl; ++li) {
       upload(c->b[li], biases[li], (size_t)c->nodes[li + 1], dtype, st);
       if (li >= 1) upload_T(c->WT[li], weights[li], c->nodes[li + 1], c->nodes[li], dtype, st);
+      if (li >= 1) upload(c->Wrow[li], weights[li], (size_t)c->nodes[li + 1] * c->nodes[li], dtype, st);
     }
     c->tm.resize(n_tm);
     c->tmT.resize(n_tm);
@@ -487,6 +491,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     n.W0 = c->W0.p;
     for (int li = 0; li < nl; ++li) {
       n.WT[li] = li >= 1 ? c->WT[li].p : nullptr;
+      n.Wrow[li] = li >= 1 ? c->Wrow[li].p : nullptr;
       n.b[li] = c->b[li].p;
     }
     for (int li = 0; li < nl; ++li) {
@@ -630,7 +635,9 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
     size_t cap = 1024;
     while (cap < 2 * n) cap *= 2;
     DevBuf<uint32_t> vt;
-    DevBuf<int> vstat, vwin, vown, vfirst, vrank, vid, fflag, fnv, frank, foff, face_off, face_idx;
+    DevBuf<int> vstat, vwin, vown, vrank, fflag, fnv, frank, foff, face_off, face_idx;
+    DevBuf<int>& vfirst = c->m_vfirst;
+    DevBuf<int>& vid = c->m_vid;
     DevBuf<double> xyz;
     ensure(vt, cap, false, st);
     ensure(vstat, n, false, st);
@@ -730,6 +737,94 @@ am_status am_get_stats(am_ctx* c, double* out, int n) {
   if (!c || !out) return AM_ERR_ARG;
   for (int i = 0; i < n && i < AM_STAT_COUNT; ++i) out[i] = c->stats[i];
   return AM_OK;
+}
+
+am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* const* grad_weights,
+                     void* const* grad_biases) {
+  if (!c) return AM_ERR_ARG;
+  if (!c->has_mesh) return fail(c, AM_ERR_STATE, "am_adjoint called before am_combine");
+  return guarded(c, [&]() -> am_status {
+    if (!grad_vertices || !grad_weights || !grad_biases) throw ArgErr{"null gradient buffer"};
+    if (dtype != AM_DTYPE_F32 && dtype != AM_DTYPE_F64) throw ArgErr{"bad dtype"};
+    for (auto& row : c->skips)
+      if (!row.empty()) throw ArgErr{"the adjoint does not support skip connections"};
+    const hipStream_t st = c->stream;
+    const size_t V = c->mesh_v.size() / 3;
+    const int nl = c->D + 1;
+    DevBuf<double> gv;
+    upload(gv, grad_vertices, V * 3, dtype, st);
+    std::vector<DevBuf<double>> dW(nl), db(nl), Gst(nl), Pst(nl);
+    for (int li = 0; li < nl; ++li) {
+      const size_t nw = (size_t)c->nodes[li + 1] * c->nodes[li];
+      ensure(dW[li], nw, false, st);
+      ensure(db[li], (size_t)c->nodes[li + 1], false, st);
+      HIPCHK(hipMemsetAsync(dW[li].p, 0, nw * 8, st));
+      HIPCHK(hipMemsetAsync(db[li].p, 0, (size_t)c->nodes[li + 1] * 8, st));
+    }
+    const int R = (int)c->R;
+    const int CH = 16384;
+    const int chunk = std::min(R, CH);
+    for (int li = 0; li < nl; ++li) {
+      ensure(Gst[li], (size_t)chunk * 4 * c->nodes[li + 1], false, st);
+      if (li >= 1) ensure(Pst[li], (size_t)chunk * 4 * c->nodes[li], false, st);
+    }
+    const size_t lds = (size_t)8 * c->L * 8 + (size_t)c->Wd * 4 + 64;
+    if (lds > c->max_lds) throw CapErr{"network too wide for the adjoint kernel's LDS"};
+    if (lds > 65536)
+      HIPCHK(hipFuncSetAttribute((const void*)k_adjoint_fp64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int r0 = 0; r0 < R; r0 += CH) {
+      const int cnt = std::min(CH, R - r0);
+      AdjointIO io;
+      io.reg_mask = c->reg_mask.p;
+      io.info = c->reg_info.p;
+      io.vpair = c->reg_vpair.p;
+      io.vfirst = c->m_vfirst.p;
+      io.vid = c->m_vid.p;
+      io.gradv = gv.p;
+      io.r0 = r0;
+      io.count = cnt;
+      io.vm = c->vm;
+      for (int li = 0; li < nl; ++li) {
+        io.Gst[li] = Gst[li].p;
+        io.Pst[li] = li >= 1 ? Pst[li].p : nullptr;
+      }
+      hipLaunchKernelGGL(k_adjoint_fp64, dim3(cnt), dim3(64), lds, st, c->net, io);
+      HIPCHK(hipGetLastError());
+      const int K = cnt * 4, kchunk = 2048;
+      for (int li = 1; li < nl; ++li) {
+        const int M = c->nodes[li + 1], N = c->nodes[li];
+        hipLaunchKernelGGL(k_dw_gemm, dim3((N + 15) / 16, (M + 15) / 16, (K + kchunk - 1) / kchunk), dim3(256), 0,
+                           st, Gst[li].p, Pst[li].p, K, M, N, dW[li].p, kchunk);
+        hipLaunchKernelGGL(k_col_sum, dim3((M + 255) / 256, (cnt + 1023) / 1024), dim3(256), 0, st, Gst[li].p,
+                           cnt, M, 3, db[li].p, 1, 1024);
+      }
+      const int n1 = c->nodes[1];
+      for (int q = 0; q < 3; ++q)
+        hipLaunchKernelGGL(k_col_sum, dim3((n1 + 255) / 256, (cnt + 1023) / 1024), dim3(256), 0, st, Gst[0].p, cnt,
+                           n1, q, dW[0].p + q, 3, 1024);
+      hipLaunchKernelGGL(k_col_sum, dim3((n1 + 255) / 256, (cnt + 1023) / 1024), dim3(256), 0, st, Gst[0].p, cnt, n1,
+                         3, db[0].p, 1, 1024);
+      HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    c->stats[12] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (int li = 0; li < nl; ++li) {
+      const size_t nw = (size_t)c->nodes[li + 1] * c->nodes[li], nb = (size_t)c->nodes[li + 1];
+      std::vector<double> hw(nw), hb(nb);
+      HIPCHK(hipMemcpy(hw.data(), dW[li].p, nw * 8, hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(hb.data(), db[li].p, nb * 8, hipMemcpyDeviceToHost));
+      if (dtype == AM_DTYPE_F64) {
+        HIPCHK(hipMemcpy(grad_weights[li], hw.data(), nw * 8, hipMemcpyDefault));
+        HIPCHK(hipMemcpy(grad_biases[li], hb.data(), nb * 8, hipMemcpyDefault));
+      } else {
+        std::vector<float> fw(hw.begin(), hw.end()), fb(hb.begin(), hb.end());
+        HIPCHK(hipMemcpy(grad_weights[li], fw.data(), nw * 4, hipMemcpyDefault));
+        HIPCHK(hipMemcpy(grad_biases[li], fb.data(), nb * 4, hipMemcpyDefault));
+      }
+    }
+    return AM_OK;
+  });
 }
 
 am_status am_export_ply(am_ctx* c, const char* path, int is_polymesh, int is_float32) {

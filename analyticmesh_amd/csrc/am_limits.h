@@ -31,6 +31,7 @@ struct NetDev {
   int off[AM_MAX_LINEAR + 1];
   const double* W0;                 // (n1, 3) row-major, fp64
   const double* WT[AM_MAX_LINEAR];  // li >= 1: (n_li, n_li+1) fp64
+  const double* Wrow[AM_MAX_LINEAR];  // li >= 1: (n_li+1, n_li) fp64, as given (adjoint)
   const double* b[AM_MAX_LINEAR];
   int nskip[AM_MAX_LINEAR];
   int skip_from[AM_MAX_LINEAR][AM_MAX_SKIPS];

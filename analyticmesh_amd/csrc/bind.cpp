@@ -113,6 +113,18 @@ struct Engine {
     return d;
   }
 
+  void adjoint(uintptr_t gradv, int dtype, const std::vector<uintptr_t>& gw, const std::vector<uintptr_t>& gb) {
+    std::vector<void*> w(gw.size()), b(gb.size());
+    for (size_t i = 0; i < w.size(); ++i) w[i] = (void*)gw[i];
+    for (size_t i = 0; i < b.size(); ++i) b[i] = (void*)gb[i];
+    am_status s;
+    {
+      py::gil_scoped_release nogil;
+      s = am_adjoint(c, (const void*)gradv, dtype, w.data(), b.data());
+    }
+    check(c, s);
+  }
+
   py::array_t<double> stats() {
     py::array_t<double> s((py::ssize_t)AM_STAT_COUNT);
     check(c, am_get_stats(c, s.mutable_data(), AM_STAT_COUNT));
@@ -141,5 +153,6 @@ PYBIND11_MODULE(_am, m) {
       .def("counts", &Engine::counts)
       .def("mesh", &Engine::mesh)
       .def("regions", &Engine::regions)
+      .def("adjoint", &Engine::adjoint)
       .def("stats", &Engine::stats);
 }

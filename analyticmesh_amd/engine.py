@@ -90,6 +90,17 @@ class MarchResult:
         self.combine_ms = float(self._eng.stats()[10])
         return V, Faces(off, idx)
 
+    def adjoint(self, grad_vertices, layer_shapes):
+        """dL/d(weights, biases) for L = sum <grad_vertices, v> (regions held fixed).
+
+        grad_vertices: (V, 3) array in the order of ``combine()``'s vertices.
+        layer_shapes: [(out, in), ...] per linear layer.  Returns (dW list, db list), float64."""
+        g = np.ascontiguousarray(grad_vertices, dtype=np.float64)
+        dW = [np.zeros(s, dtype=np.float64) for s in layer_shapes]
+        db = [np.zeros(s[0], dtype=np.float64) for s in layer_shapes]
+        self._eng.adjoint(int(g.ctypes.data), 1, [int(a.ctypes.data) for a in dW], [int(a.ctypes.data) for a in db])
+        return dW, db
+
     def export_ply(self, path, is_polymesh=True, is_float32=True):
         self._eng.export_ply(str(path), bool(is_polymesh), bool(is_float32))
 

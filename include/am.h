@@ -89,6 +89,15 @@ am_status am_get_mesh(am_ctx* ctx, double* vertices, int* face_offsets, int* fac
 am_status am_get_regions(am_ctx* ctx, uint32_t* masks, int* info, int* tabs, double* verts, int* vpairs,
                          double* points);
 
+/* Gradient of L = sum_v <dL/dv, v> with respect to every weight and bias,
+ * with the activation regions held fixed (no counterpart in the reference,
+ * which has no differentiation).  grad_vertices: (V, 3) in the vertex order
+ * of am_get_mesh (before scale/center); grad_weights[i] / grad_biases[i]
+ * receive arrays shaped like weights[i] / biases[i] of am_run, in `dtype`.
+ * Requires am_combine and a network without skip connections. */
+am_status am_adjoint(am_ctx* ctx, const void* grad_vertices, int dtype, void* const* grad_weights,
+                     void* const* grad_biases);
+
 /* Tunables: "vm" (polygon vertex cap, 3..64, default 20 like the reference's
  * _VAR_VERT_MAX), "table_log2" (initial visited-table size), "max_regions"
  * (stop after this many regions, -1 = unlimited). */
@@ -107,6 +116,7 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_REDO_REGIONS 9
 #define AM_STAT_COMBINE_MS 10
 #define AM_STAT_REGION_LAUNCHES 11
+#define AM_STAT_ADJOINT_MS 12   /* am_adjoint kernels (wall time) */
 #define AM_STAT_COUNT 16
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 
