@@ -87,6 +87,7 @@ struct am_ctx {
   std::vector<std::vector<std::pair<int, int>>> skips;  // per linear layer index
   std::string err;
   hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;   // used when the caller passes no stream
   int dev = 0;
   size_t max_lds = 65536;
   long long max_regions = -1;
@@ -392,6 +393,8 @@ am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_
       c->max_lds = std::max<size_t>(c->max_lds, (size_t)optin);
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+    c->stream = c->own_stream;
     return AM_OK;
   });
   if (s != AM_OK) {
@@ -415,6 +418,9 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
     c->table_log2 = (int)value;
   } else if (k == "max_regions") {
     c->max_regions = value;
+  } else if (k == "walk") {
+    if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
+    c->fast.walk_mode = (int)value;
   } else {
     return fail(c, AM_ERR_ARG, "unknown option " + k);
   }
@@ -437,7 +443,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         if (s.second < 0 || s.second >= n_tm) throw ArgErr{"arc_table names a missing transform matrix"};
         if (s.first < 0 || s.first > c->D) throw ArgErr{"arc_table names a missing source layer"};
       }
-    c->stream = (hipStream_t)stream;
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
     const hipStream_t st = c->stream;
     const auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < AM_STAT_COUNT; ++i) c->stats[i] = 0;
@@ -885,7 +891,10 @@ am_status am_destroy(am_ctx* c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   fast_release(c->fbuf);
+  if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+  hipStream_t own = c->own_stream;
   delete c;
+  if (own) (void)hipStreamDestroy(own);
   return AM_OK;
 }
 

@@ -34,6 +34,7 @@ struct FastNetDev {
 
 struct FastNet {
   int precision = 0;
+  int walk_mode = 0;            // 0: ray-shooting walk, 1: the reference's candidate scan (checked)
   bool ready = false;
   FastNetDev dev{};
   WalkParams wp{};

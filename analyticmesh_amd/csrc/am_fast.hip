@@ -229,6 +229,36 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
   walk_region<float, true>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
 }
 
+// Ray-shooting FP32 walk: LDS holds only the planes and the mask.
+__global__ __launch_bounds__(64) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
+                                                     const float* eqbuf, int Cs_hbm, int chunk_first) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int C = net.C;
+  const int Cs = io.Cs;
+  float* Px = reinterpret_cast<float*>(smem_raw);
+  float* Py = Px + Cs;
+  float* Pz = Py + Cs;
+  float* Pc = Pz + Cs;
+  uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cs);
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;
+  if (b >= io.count) return;
+  const int rid = chunk_first + b;
+  const float* src = planes + (size_t)b * 4 * Cs_hbm;
+  for (int g = lane; g < C; g += 64) {
+    Px[g] = src[g];
+    Py[g] = src[Cs_hbm + g];
+    Pz[g] = src[2 * Cs_hbm + g];
+    Pc[g] = src[3 * Cs_hbm + g];
+  }
+  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  const float eq[4] = {eqbuf[b * 4 + 0], eqbuf[b * 4 + 1], eqbuf[b * 4 + 2], eqbuf[b * 4 + 3]};
+  const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
+                      (float)io.reg_point[(size_t)rid * 3 + 2]};
+  __syncthreads();
+  walk_ray<float>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+}
+
 __global__ void k_collect_undecided(const unsigned* flags, int n, int first, int* cnt, int* ids) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && (flags[i] & AM_UNDECIDED)) ids[atomicAdd(cnt, 1)] = first + i;
@@ -420,6 +450,9 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
   const size_t wlds = (size_t)4 * CsL * 4 + (size_t)Csort * 8 + (size_t)((net.C + 31) / 32) * 4 + (size_t)net.Wd * 4 + 64;
   if (wlds > 65536)
     FCHK(hipFuncSetAttribute((const void*)k_walk_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wlds));
+  const size_t rlds = (size_t)4 * CsL * 4 + (size_t)net.Wd * 4 + 64;
+  if (rlds > 65536)
+    FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
   const size_t glds = gemm_lds(d, prec);
 
   for (int c0 = 0; c0 < lvl.count; c0 += CH) {
@@ -460,8 +493,12 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     io.flags = flags;
     io.ids = nullptr;
     FCHK(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_walk_f32, dim3(n), dim3(64), wlds, st, net, io, f.wp, planes, eqb, d.Cs,
-                       lvl.first + c0);
+    if (f.walk_mode == 0)
+      hipLaunchKernelGGL(k_walk_ray_f32, dim3(n), dim3(64), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
+                         lvl.first + c0);
+    else
+      hipLaunchKernelGGL(k_walk_f32, dim3(n), dim3(64), wlds, st, net, io, f.wp, planes, eqb, d.Cs,
+                         lvl.first + c0);
     FCHK(hipGetLastError());
     FCHK(hipEventRecord(e1, st));
     FCHK(hipEventSynchronize(e1));

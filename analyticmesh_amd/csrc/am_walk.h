@@ -57,6 +57,100 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
   return verdict;
 }
 
+// Orientation, region record and neighbour candidates of a finished walk
+// (backend/inc/kernel.h:1325-1474).  In CHECKED mode an undecided region only
+// raises its flag: the FP64 redo writes its record and candidate slots.
+template <typename T, bool CHECKED>
+__device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
+                              const uint32_t* smask, const T* eq, int* s_tab, T* s_verts, int* s_vpair, int ntab,
+                              int nv, int start_in, int start, int status, bool undecided) {
+  const int L = net.L, vm = net.vm;
+  const int lane = threadIdx.x;
+  // ---- orientation (backend/inc/kernel.h:1325-1371)
+  int reversed = 0;
+  if (nv >= 3 && !undecided) {
+      const T* q = s_verts;
+      const T ax = q[3] - q[0], ay = q[4] - q[1], az = q[5] - q[2];
+      const T bx = q[6] - q[3], by = q[7] - q[4], bz = q[8] - q[5];
+      const T cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+      const T dir = cx * eq[0] + cy * eq[1] + cz * eq[2];
+      if (CHECKED) {
+        const T mag = (absT(cx) + absT(cy) + absT(cz)) * (absT(eq[0]) + absT(eq[1]) + absT(eq[2]));
+        if (!(absT(dir) > T(wp.eps_turn) * mag)) undecided = true;
+      }
+      reversed = ((dir < T(0)) != (net.flip != 0)) ? 1 : 0;
+      __syncthreads();
+      if (!undecided && reversed && lane < nv / 2) {
+        const int a = lane, b = nv - 1 - lane;
+        for (int c = 0; c < 3; ++c) {
+          const T t = s_verts[a * 3 + c];
+          s_verts[a * 3 + c] = s_verts[b * 3 + c];
+          s_verts[b * 3 + c] = t;
+        }
+        for (int c = 0; c < 2; ++c) {
+          const int t = s_vpair[a * 2 + c];
+          s_vpair[a * 2 + c] = s_vpair[b * 2 + c];
+          s_vpair[b * 2 + c] = t;
+        }
+      }
+      __syncthreads();
+  }
+  if (CHECKED && (status != AM_ST_OK)) undecided = true;
+  if (CHECKED && undecided) {
+    if (lane == 0) io.flags[slot_region] = AM_UNDECIDED;
+    return;   // the FP64 redo writes this region's record and candidate slots
+  }
+  if (CHECKED && lane == 0) io.flags[slot_region] = 0u;
+
+  // ---- region record
+  if (lane == 0) {
+    int* inf = io.reg_info + (size_t)rid * AM_INFO_STRIDE;
+    inf[0] = start_in; inf[1] = start; inf[2] = nv; inf[3] = ntab;
+    inf[4] = reversed; inf[5] = status; inf[6] = CHECKED ? 2 : (io.ids ? 1 : 0); inf[7] = 0;
+  }
+  for (int i = lane; i < vm + 1; i += 64) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
+  for (int i = lane; i < vm * 3; i += 64)
+    io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[i] : 0.0;
+  for (int i = lane; i < vm * 2; i += 64) io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[i] : 0;
+
+  // ---- neighbour candidates (backend/inc/kernel.h:1390-1474)
+  const size_t slot0 = (size_t)slot_region * vm;
+  if (lane < vm) {
+    const int m = lane;
+    int t = -1;
+    if (m + 1 < ntab) t = s_tab[m];
+    const bool valid = t >= 0 && t < L;
+    T mid[3] = {T(0), T(0), T(0)};
+    if (valid) {
+      const int m_ = (m == 0) ? ntab - 1 : m;
+      int cnt = -1, v1 = -1;
+      for (int i = 1; i < ntab; ++i) {
+        if (s_tab[i] >= 0) ++cnt;
+        if (i == m_) v1 = cnt;
+      }
+      if (nv > 0 && v1 >= 0) {
+        const int v2 = (v1 + 1) % nv;
+        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[v1 * 3 + c] + s_verts[v2 * 3 + c]) / T(2);
+      }
+    }
+    io.cand_start[slot0 + m] = valid ? t : -2;
+    io.cand_point[(slot0 + m) * 3 + 0] = (double)mid[0];
+    io.cand_point[(slot0 + m) * 3 + 1] = (double)mid[1];
+    io.cand_point[(slot0 + m) * 3 + 2] = (double)mid[2];
+  }
+  const int Wd = net.Wd;
+  for (int idx = lane; idx < vm * Wd; idx += 64) {
+    const int m = idx / Wd, w = idx - m * Wd;
+    int t = -1;
+    if (m + 1 < ntab) t = s_tab[m];
+    if (t >= 0 && t < L) {
+      uint32_t word = smask[w];
+      if ((t >> 5) == w) word ^= 1u << (t & 31);
+      io.cand_mask[(slot0 + m) * Wd + w] = word;
+    }
+  }
+}
+
 template <typename T, bool CHECKED>
 __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                             T* Px, T* Py, T* Pz, T* Pc, T* keys, int* order, uint32_t* used,
@@ -154,7 +248,6 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
   }
 
   int ntab = 0, nv = 0;
-  int reversed = 0;
   if (status == AM_ST_OK && !undecided) {
     // ---- pivot walk (backend/inc/kernel.h:861-1320, backend/src/kernel.cu:203-299)
     if (lane == 0) {
@@ -265,89 +358,157 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
       const bool closed = ntab >= 4 && s_tab[0] == s_tab[ntab - 1];
       if (!closed || status != AM_ST_OK) undecided = true;   // truncated or stuck: FP64 decides
     }
-    // ---- orientation (backend/inc/kernel.h:1325-1371)
-    if (nv >= 3 && !undecided) {
-      const T* q = s_verts;
-      const T ax = q[3] - q[0], ay = q[4] - q[1], az = q[5] - q[2];
-      const T bx = q[6] - q[3], by = q[7] - q[4], bz = q[8] - q[5];
-      const T cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-      const T dir = cx * eq[0] + cy * eq[1] + cz * eq[2];
-      if (CHECKED) {
-        const T mag = (absT(cx) + absT(cy) + absT(cz)) * (absT(eq[0]) + absT(eq[1]) + absT(eq[2]));
-        if (!(absT(dir) > T(wp.eps_turn) * mag)) undecided = true;
-      }
-      reversed = ((dir < T(0)) != (net.flip != 0)) ? 1 : 0;
-      __syncthreads();
-      if (!undecided && reversed && lane < nv / 2) {
-        const int a = lane, b = nv - 1 - lane;
-        for (int c = 0; c < 3; ++c) {
-          const T t = s_verts[a * 3 + c];
-          s_verts[a * 3 + c] = s_verts[b * 3 + c];
-          s_verts[b * 3 + c] = t;
-        }
-        for (int c = 0; c < 2; ++c) {
-          const int t = s_vpair[a * 2 + c];
-          s_vpair[a * 2 + c] = s_vpair[b * 2 + c];
-          s_vpair[b * 2 + c] = t;
-        }
-      }
-      __syncthreads();
-    }
   }
-  if (CHECKED && (status != AM_ST_OK)) undecided = true;
-  if (CHECKED && undecided) {
-    if (lane == 0) io.flags[slot_region] = AM_UNDECIDED;
-    return;   // the FP64 redo writes this region's record and candidate slots
-  }
-  if (CHECKED && lane == 0) io.flags[slot_region] = 0u;
+  finish_region<T, CHECKED>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv, start_in,
+                            start, status, undecided);
+}
 
-  // ---- region record
-  if (lane == 0) {
-    int* inf = io.reg_info + (size_t)rid * AM_INFO_STRIDE;
-    inf[0] = start_in; inf[1] = start; inf[2] = nv; inf[3] = ntab;
-    inf[4] = reversed; inf[5] = status; inf[6] = CHECKED ? 2 : (io.ids ? 1 : 0); inf[7] = 0;
+// Minimum of (t, idx) over the wave, ties to the smaller index.
+template <typename T>
+__device__ __forceinline__ void wave_argmin(T& t, int& idx) {
+  for (int off = 32; off > 0; off >>= 1) {
+    const T t2 = __shfl_xor(t, off);
+    const int i2 = __shfl_xor(idx, off);
+    if (t2 < t || (t2 == t && i2 < idx)) { t = t2; idx = i2; }
   }
-  for (int i = lane; i < vm + 1; i += 64) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
-  for (int i = lane; i < vm * 3; i += 64)
-    io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[i] : 0.0;
-  for (int i = lane; i < vm * 2; i += 64) io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[i] : 0;
+}
 
-  // ---- neighbour candidates (backend/inc/kernel.h:1390-1474)
-  const size_t slot0 = (size_t)slot_region * vm;
-  if (lane < vm) {
-    const int m = lane;
-    int t = -1;
-    if (m + 1 < ntab) t = s_tab[m];
-    const bool valid = t >= 0 && t < L;
-    T mid[3] = {T(0), T(0), T(0)};
-    if (valid) {
-      const int m_ = (m == 0) ? ntab - 1 : m;
-      int cnt = -1, v1 = -1;
-      for (int i = 1; i < ntab; ++i) {
-        if (s_tab[i] >= 0) ++cnt;
-        if (i == m_) v1 = cnt;
-      }
-      if (nv > 0 && v1 >= 0) {
-        const int v2 = (v1 + 1) % nv;
-        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[v1 * 3 + c] + s_verts[v2 * 3 + c]) / T(2);
-      }
-    }
-    io.cand_start[slot0 + m] = valid ? t : -2;
-    io.cand_point[(slot0 + m) * 3 + 0] = (double)mid[0];
-    io.cand_point[(slot0 + m) * 3 + 1] = (double)mid[1];
-    io.cand_point[(slot0 + m) * 3 + 2] = (double)mid[2];
-  }
-  const int Wd = net.Wd;
-  for (int idx = lane; idx < vm * Wd; idx += 64) {
-    const int m = idx / Wd, w = idx - m * Wd;
-    int t = -1;
-    if (m + 1 < ntab) t = s_tab[m];
-    if (t >= 0 && t < L) {
-      uint32_t word = smask[w];
-      if ((t >> 5) == w) word ^= 1u << (t & 31);
-      io.cand_mask[(slot0 + m) * Wd + w] = word;
+// Nearest plane crossed by the ray x + t d (t may be negative when x lies
+// outside a half-space), skipping ex1/ex2; INT_MAX when no plane bounds it.
+template <typename T>
+__device__ __forceinline__ void shoot(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* x,
+                                      const T* d, int ex1, int ex2, T& tmin, int& gmin) {
+  T tb = T(__builtin_inff());
+  int gb = 0x7fffffff;
+  for (int g = threadIdx.x; g < C; g += 64) {
+    if (g == ex1 || g == ex2) continue;
+    const T al = fmaT(Px[g], d[0], fmaT(Py[g], d[1], Pz[g] * d[2]));
+    if (al > T(0)) {
+      const T tg = -plane_eval(Px[g], Py[g], Pz[g], Pc[g], x) / al;
+      if (tg < tb || (tg == tb && g < gb)) { tb = tg; gb = g; }
     }
   }
+  wave_argmin(tb, gb);
+  tmin = tb;
+  gmin = gb;
+}
+
+// Reduced-precision walk by ray shooting.  From the start plane's edge each
+// step proposes the next edge plane as the nearest plane crossed along the
+// current edge line (one pass over the constraints), then certifies the new
+// vertex with the checked feasibility test.  On generic input the proposal
+// is the unique feasible candidate the reference's scan would accept, so the
+// face is the same; anything the bounds cannot decide goes to FP64.
+template <typename T>
+__device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
+                         const T* Px, const T* Py, const T* Pz, const T* Pc, const uint32_t* smask, const T* eq,
+                         const T* p) {
+  __shared__ int s_tab[AM_VM_MAX + 2];
+  __shared__ T s_verts[AM_VM_MAX * 3];
+  __shared__ int s_vpair[AM_VM_MAX * 2];
+  const int C = net.C, vm = net.vm;
+  const int lane = threadIdx.x;
+  const int start_in = io.reg_start[rid];
+  const int NONE = 0x7fffffff;
+  bool undecided = false;
+  int start = start_in;
+  T tdummy;
+
+  if (start == -1) {
+    // seed: project p onto the f = iso plane and shoot inside it
+    const T nn = plane_nn(eq[0], eq[1], eq[2]);
+    const T se = plane_eval(eq[0], eq[1], eq[2], eq[3], p) / nn;
+    const T q[3] = {p[0] - se * eq[0], p[1] - se * eq[1], p[2] - se * eq[2]};
+    const T ax = absT(eq[0]), ay = absT(eq[1]), az = absT(eq[2]);
+    const T e[3] = {T(ax <= ay && ax <= az ? 1 : 0), T(ay < ax && ay <= az ? 1 : 0), T(az < ax && az < ay ? 1 : 0)};
+    const T d0[3] = {eq[1] * e[2] - eq[2] * e[1], eq[2] * e[0] - eq[0] * e[2], eq[0] * e[1] - eq[1] * e[0]};
+    int g0;
+    shoot(Px, Py, Pz, Pc, C, q, d0, -1, -1, tdummy, g0);
+    if (g0 == NONE) undecided = true;
+    start = g0;
+  }
+  int ntab = 0, nv = 0;
+  int r = NONE;
+  if (!undecided && (start < 0 || start >= C)) undecided = true;
+  if (!undecided) {
+    // edge on the start plane: its line's interval inside every other half-space
+    const T ps[4] = {Px[start], Py[start], Pz[start], Pc[start]};
+    const T d[3] = {ps[1] * eq[2] - ps[2] * eq[1], ps[2] * eq[0] - ps[0] * eq[2], ps[0] * eq[1] - ps[1] * eq[0]};
+    const T pd[4] = {d[0], d[1], d[2], -(d[0] * p[0] + d[1] * p[1] + d[2] * p[2])};
+    T x0[3];
+    solve3(ps, eq, pd, x0);
+    const T dm[3] = {-d[0], -d[1], -d[2]};
+    T th, tl;
+    int rh, rl;
+    shoot(Px, Py, Pz, Pc, C, x0, d, start, -1, th, rh);
+    shoot(Px, Py, Pz, Pc, C, x0, dm, start, -1, tl, rl);
+    if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) undecided = true;
+    r = rh;
+  }
+  if (!undecided) {
+    if (lane == 0) s_tab[0] = start;
+    __syncthreads();
+    ntab = 1;
+    int cur = start;
+    while (true) {
+      // r is the proposed next edge plane
+      bool in_tab = false;
+      for (int i = 0; i < ntab; ++i) in_tab = in_tab || (s_tab[i] == r);
+      if (in_tab && !(r == s_tab[0] && ntab >= 3)) { undecided = true; break; }
+      const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
+      const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
+      T v[3];
+      solve3(pr4, pc4, eq, v);
+      const T h = hadamard_ratio(pr4, pc4, eq);
+      const T epsv = T(wp.eps) * (T(1) + h);
+      const int verdict = lane_check<T, true>(Px, Py, Pz, Pc, C, lane, cur, r, v, epsv);
+      if (__any(verdict != 0) || !(h <= T(wp.h_max))) { undecided = true; break; }
+      if (ntab >= 2) {
+        const int a = s_tab[ntab - 2], b = s_tab[ntab - 1];
+        const T n1[3] = {Px[a], Py[a], Pz[a]};
+        const T n2[3] = {Px[b], Py[b], Pz[b]};
+        const T n3[3] = {pr4[0], pr4[1], pr4[2]};
+        const T A12 = AM_DOT3(n1, n2), A23 = AM_DOT3(n2, n3), A13 = AM_DOT3(n1, n3), A22 = AM_DOT3(n2, n2);
+        const T C1 = AM_DOT3(n1, eq), C2 = AM_DOT3(n2, eq), C3 = AM_DOT3(n3, eq), B = AM_DOT3(eq, eq);
+        const T val = (A12 * A23 - A22 * A13) - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3) / B;
+        const T mag = absT(A12 * A23) + absT(A22 * A13) +
+                      (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
+        if (!(val > T(wp.eps_turn) * mag)) { undecided = true; break; }   // must certainly turn the right way
+      }
+      if (nv > 0) {
+        const T* last = s_verts + (nv - 1) * 3;
+        const T dd = absT(last[0] - v[0]) + absT(last[1] - v[1]) + absT(last[2] - v[2]);
+        const T mag = absT(v[0]) + absT(v[1]) + absT(v[2]) + absT(last[0]) + absT(last[1]) + absT(last[2]);
+        if (!(dd > T(4) * epsv * (mag + T(1)))) { undecided = true; break; }
+      }
+      __syncthreads();
+      if (lane == 0) {
+        s_verts[nv * 3 + 0] = v[0];
+        s_verts[nv * 3 + 1] = v[1];
+        s_verts[nv * 3 + 2] = v[2];
+        s_vpair[nv * 2 + 0] = r;
+        s_vpair[nv * 2 + 1] = cur;
+        s_tab[ntab] = r;
+      }
+      __syncthreads();
+      nv += 1;
+      ntab += 1;
+      if (ntab >= 4 && s_tab[0] == s_tab[ntab - 1]) break;          // closed
+      if (ntab >= vm + 1) { undecided = true; break; }               // truncated: FP64 decides
+      // walk along the new edge line, away from the plane we came along
+      const int prev = cur;
+      cur = r;
+      const T pc[3] = {Px[cur], Py[cur], Pz[cur]};
+      T d[3] = {pc[1] * eq[2] - pc[2] * eq[1], pc[2] * eq[0] - pc[0] * eq[2], pc[0] * eq[1] - pc[1] * eq[0]};
+      if (Px[prev] * d[0] + Py[prev] * d[1] + Pz[prev] * d[2] > T(0)) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
+      int nx;
+      shoot(Px, Py, Pz, Pc, C, v, d, cur, prev, tdummy, nx);
+      if (nx == NONE) { undecided = true; break; }
+      r = nx;
+    }
+  }
+  finish_region<T, true>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv, start_in, start,
+                         AM_ST_OK, undecided);
 }
 
 }  // namespace am

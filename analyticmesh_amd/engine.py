@@ -109,7 +109,7 @@ class Engine:
     """One engine context for a fixed architecture (mirrors ``Init``)."""
 
     def __init__(self, nodes, arc_table=None, num_extra_constraints=0, precision="fp64", vm=20,
-                 table_log2=None):
+                 table_log2=None, walk="ray"):
         m = native()
         nodes = [int(n) for n in nodes]
         D = len(nodes) - 2
@@ -130,6 +130,7 @@ class Engine:
         self._eng.set_option("vm", self.vm)
         if table_log2 is not None:
             self._eng.set_option("table_log2", int(table_log2))
+        self._eng.set_option("walk", {"ray": 0, "scan": 1}[walk])
 
     def set_option(self, key, value):
         self._eng.set_option(key, int(value))
@@ -176,9 +177,12 @@ class Engine:
 
 
 def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip=False, max_regions=None,
-          table_log2=None):
-    """Mesh one NetSpec from explicit seeds (points (N,3) float64, states (N,L) bool)."""
-    eng = Engine(spec.nodes, None, 0, precision=precision, vm=vm, table_log2=table_log2)
+          table_log2=None, walk="ray"):
+    """Mesh one NetSpec from explicit seeds (points (N,3) float64, states (N,L) bool).
+
+    walk: reduced-precision forms only; "ray" (ray shooting, default) or
+    "scan" (the reference's candidate scan with checked predicates)."""
+    eng = Engine(spec.nodes, None, 0, precision=precision, vm=vm, table_log2=table_log2, walk=walk)
     if max_regions is not None:
         eng.set_option("max_regions", max_regions)
     pts = np.ascontiguousarray(seed_points, dtype=np.float64).reshape(-1, 3)
@@ -188,3 +192,28 @@ def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip
     res = eng.run(W, B, pts, st, iso=iso, flip=flip)
     res.engine = eng
     return res
+
+
+def march_batch(specs, seeds, precision="fp32", workers=16, vm=20, walk="ray", combine=True):
+    """Mesh many independent networks in one job.
+
+    Every network gets its own engine context and HIP stream; a pool of host
+    threads drives them so their levels overlap on the GPU.  ``seeds`` is a
+    list of (points, states) per network.  Returns the per-network results
+    (with ``.mesh`` = (V, faces) when ``combine``) and the wall time in seconds.
+    """
+    import concurrent.futures
+    import time
+
+    def one(i):
+        spec = specs[i]
+        pts, st = seeds[i]
+        r = march(spec, pts, st, precision=precision, vm=vm, walk=walk)
+        if combine:
+            r.mesh = r.combine()
+        return r
+
+    t0 = time.perf_counter()
+    with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
+        out = list(ex.map(one, range(len(specs))))
+    return out, time.perf_counter() - t0

@@ -67,6 +67,74 @@ def run_form(spec, pts, states, precision, steps, warmup):
     return dict(sec_per_step=dt, faces=faces, regions=int(st[7]), stats=st, res=res, eng=eng)
 
 
+def copy_calibration(mb):
+    """A plain device copy of known size, for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = mb * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}
+
+
+def traffic_from_profile(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 2 and parts[0] == kernel:
+            try:
+                return float(parts[1])
+            except ValueError:
+                return None
+    return None
+
+
+def batch64_extra(n_nets, seeds_per_net):
+    from analyticmesh_amd import configs
+    from analyticmesh_amd.engine import march_batch
+    specs = configs.batch64(0)[:n_nets]
+    seeds = []
+    for s in specs:
+        p = configs.seeds_fibonacci(s, seeds_per_net)
+        seeds.append((p, configs.mask_of(s, p)))
+    out = {}
+    recs = {}
+    for f in ("fp32", "fp64"):
+        march_batch(specs[:2], seeds[:2], precision=f, workers=2)   # warm up
+        res, dt = march_batch(specs, seeds, precision=f, workers=16)
+        faces = sum(int((r.info[:, 2] >= 3).sum()) for r in res)
+        out[f] = {"faces": faces, "seconds": round(dt, 4), "faces_per_s": round(faces / dt, 1)}
+        recs[f] = [r.records() for r in res]
+    same = sum(1 for a, b in zip(recs["fp32"], recs["fp64"])
+               if a.visited() == b.visited() and a.faces() == b.faces())
+    out["nets"] = len(specs)
+    out["nets_with_identical_topology"] = same
+    return out
+
+
+def adjoint_extra():
+    from analyticmesh_amd import configs
+    from analyticmesh_amd.engine import march
+    spec = configs.diff(0)
+    pts = configs.seeds_fibonacci(spec, 64)
+    st = configs.mask_of(spec, pts)
+    res = march(spec, pts, st, precision="fp64")
+    V, _ = res.combine()
+    g = np.random.Generator(np.random.PCG64(0)).normal(size=V.shape)
+    shapes = [w.shape for w in spec.weights]
+    res.adjoint(g, shapes)                      # warm up
+    t0 = time.perf_counter()
+    res.adjoint(g, shapes)
+    dt = time.perf_counter() - t0
+    return {"workload": "diff 3->128x4->1, r=0.5, sigma=0.03, 64 seeds", "vertices": int(len(V)),
+            "backward_ms": round(dt * 1e3, 3), "forward_run_ms": round(float(res.stats[0]), 3)}
+
+
 def cpu_baseline(spec, pts, states, max_regions):
     import oracle
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
@@ -74,8 +142,8 @@ def cpu_baseline(spec, pts, states, max_regions):
                           threads=threads)
     faces = int((r.info[:, 2] >= 3).sum())
     return dict(value=faces / r.seconds, unit="faces/s", cores=threads, kind="port",
-                sample=f"first {len(r.masks)} regions of the LIFO march of the same 4x256 net and seeds, "
-                       f"FP64 (the reference default dtype), {r.seconds:.1f} s")
+                sample=(f"{'first ' if r.truncated else 'all '}{len(r.masks)} regions of the LIFO march of the "
+                        f"same 4x256 net and seeds, FP64 (the reference default dtype), {r.seconds:.1f} s"))
 
 
 def main():
@@ -84,9 +152,11 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seeds", type=int, default=1024)
-    ap.add_argument("--cpu-regions", type=int, default=131072)
+    ap.add_argument("--cpu-regions", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,bf16x3,fp64")
+    ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
+    ap.add_argument("--extras", action="store_true", help="also measure the 64-MLP batch and the adjoint")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is designed but not built",
@@ -122,7 +192,8 @@ def main():
     flops = flops_region * (h["regions"] - st[9])
     achieved = flops / (kern_ms / 1e3) / 1e12 if kern_ms > 0 else 0.0
     roofline = {"bound": "mfma", "kernel": kname, "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[head],
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 5), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 5),
+                "traffic": traffic_from_profile(kname),
                 "avg_launch_ms": round(kern_ms / launches, 4),
                 "flops_per_region": flops_region}
 
@@ -162,12 +233,18 @@ def main():
             "fp64_redo_ms": round(float(out[f]["stats"][5]), 3),
             "fp64_redo_regions": int(out[f]["stats"][9]),
             "insert_ms": round(float(out[f]["stats"][2]), 3),
-            "combine_ms": round(float(out[f]["stats"][10]), 3),
+            "combine_ms": round(float(getattr(out[f]["res"], "combine_ms", 0.0)), 3),
             "levels": int(out[f]["stats"][6]),
         }) for f in forms},
         "topology_vs_fp64": topo,
         "roofline": roofline,
     }
+    if args.extras:
+        log("[bench] extras: 64-MLP batch and adjoint")
+        line["batch64"] = batch64_extra(64, 16)
+        line["adjoint"] = adjoint_extra()
+    if args.calib_copy_mb:
+        line["copy_calibration"] = copy_calibration(args.calib_copy_mb)
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle on the first {args.cpu_regions} regions")
         line["cpu_baseline"] = cpu_baseline(spec, pts, states, args.cpu_regions)

@@ -17,14 +17,14 @@ def _say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+@pytest.mark.parametrize("prec,walk", [("fp32", "ray"), ("bf16x3", "ray"), ("fp32", "scan")])
 @pytest.mark.parametrize("name", ["polytope", "sphere"])
-def test_fast_topology_vs_oracle(name, prec):
+def test_fast_topology_vs_oracle(name, prec, walk):
     import oracle
     from analyticmesh_amd.engine import march
     spec, pts, st = workload(name)
-    _say(f"[{prec}] {name}")
-    g = march(spec, pts, st, precision=prec)
+    _say(f"[{prec}/{walk}] {name}")
+    g = march(spec, pts, st, precision=prec, walk=walk)
     o = oracle.run_oracle(spec, pts, st, traversal="lifo")
     a, b = g.records(), o.records()
     assert_same_topology(a, b)
