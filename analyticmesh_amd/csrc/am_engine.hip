@@ -342,7 +342,8 @@ am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_
     g_create_error = "num_extra_constraints must be >= 0";
     return AM_ERR_ARG;
   }
-  if (precision != AM_PREC_FP32 && precision != AM_PREC_SPLIT_BF16X3 && precision != AM_PREC_FP64) {
+  if (precision != AM_PREC_FP32 && precision != AM_PREC_SPLIT_BF16X3 && precision != AM_PREC_FP64 &&
+      precision != AM_PREC_FP64_MFMA) {
     g_create_error = "unknown precision";
     return AM_ERR_ARG;
   }
@@ -735,6 +736,30 @@ am_status am_get_regions(am_ctx* c, uint32_t* masks, int* info, int* tabs, doubl
     if (verts) HIPCHK(hipMemcpy(verts, c->reg_verts.p, R * c->vm * 3 * 8, hipMemcpyDeviceToHost));
     if (vpairs) HIPCHK(hipMemcpy(vpairs, c->reg_vpair.p, R * c->vm * 2 * 4, hipMemcpyDeviceToHost));
     if (points) HIPCHK(hipMemcpy(points, c->reg_point.p, R * 3 * 8, hipMemcpyDeviceToHost));
+    return AM_OK;
+  });
+}
+
+am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, float* eq) {
+  if (!c || !masks || !planes || !eq || n < 1) return AM_ERR_ARG;
+  if (!c->has_run || c->precision == AM_PREC_FP64)
+    return fail(c, AM_ERR_STATE, "am_planes needs a reduced-precision context after am_run");
+  return guarded(c, [&]() -> am_status {
+    const hipStream_t st = c->stream;
+    const int Cs = c->fast.dev.Cs;
+    DevBuf<uint32_t> m;
+    DevBuf<float> p, e;
+    ensure(m, (size_t)n * c->Wd, false, st);
+    ensure(p, (size_t)n * 4 * Cs, false, st);
+    ensure(e, (size_t)n * 4, false, st);
+    HIPCHK(hipMemcpy(m.p, masks, (size_t)n * c->Wd * 4, hipMemcpyDefault));
+    fast_planes(c->fast, m.p, n, p.p, e.p, st);
+    std::vector<float> hp((size_t)n * 4 * Cs);
+    HIPCHK(hipMemcpy(hp.data(), p.p, hp.size() * 4, hipMemcpyDeviceToHost));
+    for (int r = 0; r < n; ++r)
+      for (int q = 0; q < 4; ++q)
+        std::memcpy(planes + ((size_t)r * 4 + q) * c->C, hp.data() + ((size_t)r * 4 + q) * Cs, (size_t)c->C * 4);
+    HIPCHK(hipMemcpy(eq, e.p, (size_t)n * 4 * 4, hipMemcpyDeviceToHost));
     return AM_OK;
   });
 }

@@ -27,11 +27,15 @@ struct FastNetDev {
   const float* Wp[AM_MAX_LINEAR];       // fp32 A operands, [nrb][nkb][64 lanes]
   const uint16_t* Wh[AM_MAX_LINEAR];    // bf16 hi A operands, [nrb][nkb][64 lanes][8]
   const uint16_t* Wl[AM_MAX_LINEAR];    // bf16 lo A operands
+  const double* Wp64[AM_MAX_LINEAR];    // fp64 A operands, [nrb][nkb][64 lanes]
+  const double* W0d;                    // (n1, 3) fp64
+  const double* biasd[AM_MAX_LINEAR];
   const float* ew;
   const float* eb;
   float iso;
 };
 
+// Forms: 0 FP32 MFMA, 1 split-BF16 MFMA, 3 FP64 MFMA (planes rounded to FP32 for the walk).
 struct FastNet {
   int precision = 0;
   int walk_mode = 0;            // 0: ray-shooting walk, 1: the reference's candidate scan (checked)
@@ -71,6 +75,9 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
 // regions and their ids in *ids (device memory owned by `b`).
 int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
                hipEvent_t e0, hipEvent_t e1, FastTimes& t, int** ids);
+// Planes of n given regions (their masks in `reg_mask`) with the prepared
+// reduced-precision form: planes [n][4][Cs] f32 (signed), eq [n][4] f32.
+void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st);
 void fast_release(FastBuffers& b);
 
 }  // namespace am

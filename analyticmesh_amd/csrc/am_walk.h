@@ -60,30 +60,30 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
 // Orientation, region record and neighbour candidates of a finished walk
 // (backend/inc/kernel.h:1325-1474).  In CHECKED mode an undecided region only
 // raises its flag: the FP64 redo writes its record and candidate slots.
-template <typename T, bool CHECKED>
+template <typename T, bool CHECKED, typename V = T>
 __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
-                              const uint32_t* smask, const T* eq, int* s_tab, T* s_verts, int* s_vpair, int ntab,
+                              const uint32_t* smask, const T* eq, int* s_tab, V* s_verts, int* s_vpair, int ntab,
                               int nv, int start_in, int start, int status, bool undecided) {
   const int L = net.L, vm = net.vm;
   const int lane = threadIdx.x;
   // ---- orientation (backend/inc/kernel.h:1325-1371)
   int reversed = 0;
   if (nv >= 3 && !undecided) {
-      const T* q = s_verts;
-      const T ax = q[3] - q[0], ay = q[4] - q[1], az = q[5] - q[2];
-      const T bx = q[6] - q[3], by = q[7] - q[4], bz = q[8] - q[5];
-      const T cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-      const T dir = cx * eq[0] + cy * eq[1] + cz * eq[2];
+      const V* q = s_verts;
+      const V ax = q[3] - q[0], ay = q[4] - q[1], az = q[5] - q[2];
+      const V bx = q[6] - q[3], by = q[7] - q[4], bz = q[8] - q[5];
+      const V cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+      const V dir = cx * V(eq[0]) + cy * V(eq[1]) + cz * V(eq[2]);
       if (CHECKED) {
-        const T mag = (absT(cx) + absT(cy) + absT(cz)) * (absT(eq[0]) + absT(eq[1]) + absT(eq[2]));
-        if (!(absT(dir) > T(wp.eps_turn) * mag)) undecided = true;
+        const V mag = (absT(cx) + absT(cy) + absT(cz)) * (absT(V(eq[0])) + absT(V(eq[1])) + absT(V(eq[2])));
+        if (!(absT(dir) > V(wp.eps_turn) * mag)) undecided = true;
       }
-      reversed = ((dir < T(0)) != (net.flip != 0)) ? 1 : 0;
+      reversed = ((dir < V(0)) != (net.flip != 0)) ? 1 : 0;
       __syncthreads();
       if (!undecided && reversed && lane < nv / 2) {
         const int a = lane, b = nv - 1 - lane;
         for (int c = 0; c < 3; ++c) {
-          const T t = s_verts[a * 3 + c];
+          const V t = s_verts[a * 3 + c];
           s_verts[a * 3 + c] = s_verts[b * 3 + c];
           s_verts[b * 3 + c] = t;
         }
@@ -120,7 +120,7 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
     int t = -1;
     if (m + 1 < ntab) t = s_tab[m];
     const bool valid = t >= 0 && t < L;
-    T mid[3] = {T(0), T(0), T(0)};
+    V mid[3] = {V(0), V(0), V(0)};
     if (valid) {
       const int m_ = (m == 0) ? ntab - 1 : m;
       int cnt = -1, v1 = -1;
@@ -130,7 +130,7 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
       }
       if (nv > 0 && v1 >= 0) {
         const int v2 = (v1 + 1) % nv;
-        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[v1 * 3 + c] + s_verts[v2 * 3 + c]) / T(2);
+        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[v1 * 3 + c] + s_verts[v2 * 3 + c]) / V(2);
       }
     }
     io.cand_start[slot0 + m] = valid ? t : -2;
@@ -314,12 +314,10 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
         const T last[3] = {s_verts[(nv - 1) * 3 + 0], s_verts[(nv - 1) * 3 + 1], s_verts[(nv - 1) * 3 + 2]};
         if (!CHECKED) {
           same = same_vertex(last, v);
-        } else {
-          // decide the FP64 test (L1 < 3e-14) from reduced-precision vertices
-          const T d = absT(last[0] - v[0]) + absT(last[1] - v[1]) + absT(last[2] - v[2]);
-          const T mag = absT(v[0]) + absT(v[1]) + absT(v[2]) + absT(last[0]) + absT(last[1]) + absT(last[2]);
-          if (!(d > T(4) * vepsv * (mag + T(1)))) { undecided = true; break; }
         }
+        // CHECKED: the accepted vertex was certified strictly inside the plane
+        // of the last vertex (the previous tab entry), so it is not the same
+        // vertex in the FP64 sense (L1 < 3e-14) either.
       }
       __syncthreads();
       if (valid == 0) {
@@ -404,7 +402,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
                          const T* Px, const T* Py, const T* Pz, const T* Pc, const uint32_t* smask, const T* eq,
                          const T* p) {
   __shared__ int s_tab[AM_VM_MAX + 2];
-  __shared__ T s_verts[AM_VM_MAX * 3];
+  __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
   __shared__ int s_vpair[AM_VM_MAX * 2];
   const int C = net.C, vm = net.vm;
   const int lane = threadIdx.x;
@@ -475,17 +473,22 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
                       (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
         if (!(val > T(wp.eps_turn) * mag)) { undecided = true; break; }   // must certainly turn the right way
       }
-      if (nv > 0) {
-        const T* last = s_verts + (nv - 1) * 3;
-        const T dd = absT(last[0] - v[0]) + absT(last[1] - v[1]) + absT(last[2] - v[2]);
-        const T mag = absT(v[0]) + absT(v[1]) + absT(v[2]) + absT(last[0]) + absT(last[1]) + absT(last[2]);
-        if (!(dd > T(4) * epsv * (mag + T(1)))) { undecided = true; break; }
+      double vd[3];
+      {
+        const double a4[4] = {pr4[0], pr4[1], pr4[2], pr4[3]};
+        const double b4[4] = {pc4[0], pc4[1], pc4[2], pc4[3]};
+        const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
+        solve3(a4, b4, e4, vd);
       }
+      // The reference's "same vertex" test (L1 < 3e-14) needs no separate
+      // bound: the certified check above found the new vertex strictly inside
+      // the plane of the previous vertex by more than its error bound, so the
+      // two are farther apart than 3e-14.
       __syncthreads();
       if (lane == 0) {
-        s_verts[nv * 3 + 0] = v[0];
-        s_verts[nv * 3 + 1] = v[1];
-        s_verts[nv * 3 + 2] = v[2];
+        s_verts[nv * 3 + 0] = vd[0];
+        s_verts[nv * 3 + 1] = vd[1];
+        s_verts[nv * 3 + 2] = vd[2];
         s_vpair[nv * 2 + 0] = r;
         s_vpair[nv * 2 + 1] = cur;
         s_tab[ntab] = r;
@@ -507,8 +510,8 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       r = nx;
     }
   }
-  finish_region<T, true>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv, start_in, start,
-                         AM_ST_OK, undecided);
+  finish_region<T, true, double>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv,
+                                 start_in, start, AM_ST_OK, undecided);
 }
 
 }  // namespace am

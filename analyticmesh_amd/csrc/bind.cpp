@@ -125,6 +125,14 @@ struct Engine {
     check(c, s);
   }
 
+  py::tuple planes(py::array_t<uint32_t, py::array::c_style | py::array::forcecast> masks, int C) {
+    const int n = (int)masks.shape(0);
+    py::array_t<float> P({(py::ssize_t)n, (py::ssize_t)4, (py::ssize_t)C});
+    py::array_t<float> E({(py::ssize_t)n, (py::ssize_t)4});
+    check(c, am_planes(c, masks.data(), n, P.mutable_data(), E.mutable_data()));
+    return py::make_tuple(P, E);
+  }
+
   py::array_t<double> stats() {
     py::array_t<double> s((py::ssize_t)AM_STAT_COUNT);
     check(c, am_get_stats(c, s.mutable_data(), AM_STAT_COUNT));
@@ -139,6 +147,7 @@ PYBIND11_MODULE(_am, m) {
   m.attr("PREC_FP32") = AM_PREC_FP32;
   m.attr("PREC_SPLIT_BF16X3") = AM_PREC_SPLIT_BF16X3;
   m.attr("PREC_FP64") = AM_PREC_FP64;
+  m.attr("PREC_FP64_MFMA") = AM_PREC_FP64_MFMA;
   m.attr("DTYPE_F32") = AM_DTYPE_F32;
   m.attr("DTYPE_F64") = AM_DTYPE_F64;
   py::class_<Engine>(m, "Engine")
@@ -154,5 +163,6 @@ PYBIND11_MODULE(_am, m) {
       .def("mesh", &Engine::mesh)
       .def("regions", &Engine::regions)
       .def("adjoint", &Engine::adjoint)
+      .def("planes", &Engine::planes)
       .def("stats", &Engine::stats);
 }

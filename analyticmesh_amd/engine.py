@@ -11,7 +11,7 @@ import numpy as np
 
 from .canonical import RegionRecords
 
-PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp64": 2}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp64": 2, "fp64mfma": 3}
 
 _mod = None
 
@@ -134,6 +134,10 @@ class Engine:
 
     def set_option(self, key, value):
         self._eng.set_option(key, int(value))
+
+    def planes(self, masks):
+        """Reduced-precision planes of the given packed masks: ((n, 4, C) f32 signed, (n, 4) eq)."""
+        return self._eng.planes(np.ascontiguousarray(masks, dtype=np.uint32), self.L + self.E)
 
     def run(self, weights, biases, points, states, tms=(), extra_w=None, extra_b=None, iso=0.0, flip=False,
             stream=0):

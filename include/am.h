@@ -32,6 +32,7 @@ typedef int am_status;
 #define AM_PREC_FP32 0            /* FP32 MFMA planes, FP64 re-decision of undecided regions */
 #define AM_PREC_SPLIT_BF16X3 1    /* split-BF16 (3 products) MFMA planes, FP64 re-decision */
 #define AM_PREC_FP64 2            /* FP64 everywhere (the reference's default dtype) */
+#define AM_PREC_FP64_MFMA 3       /* FP64 MFMA planes, FP32 walk, FP64 re-decision */
 
 /* Element type of caller buffers. */
 #define AM_DTYPE_F32 0
@@ -97,6 +98,13 @@ am_status am_get_regions(am_ctx* ctx, uint32_t* masks, int* info, int* tabs, dou
  * Requires am_combine and a network without skip connections. */
 am_status am_adjoint(am_ctx* ctx, const void* grad_vertices, int dtype, void* const* grad_weights,
                      void* const* grad_biases);
+
+/* Constraint planes of n regions (packed masks [n][W]) computed by the
+ * context's reduced-precision face kernel, for accuracy checks: planes
+ * [n][4][C] f32 (x, y, z, c arrays, signed so that a.x + c <= 0 inside),
+ * eq [n][4] f32 (f = iso plane).  Needs a prior am_run on this context.
+ * No counterpart in the reference (its planes stay internal). */
+am_status am_planes(am_ctx* ctx, const uint32_t* masks, int n, float* planes, float* eq);
 
 /* Tunables: "vm" (polygon vertex cap, 3..64, default 20 like the reference's
  * _VAR_VERT_MAX), "table_log2" (initial visited-table size), "max_regions"
