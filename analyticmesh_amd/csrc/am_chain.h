@@ -2,7 +2,7 @@
 // kernel and the adjoint kernel (backend/inc/process.h:18-146).  One wave64;
 // the unsigned pre-activation planes of every hidden neuron land in LDS as
 // SoA arrays and the output plane in s_eq.  Canonical order: for each output
-// neuron the accumulators start at (0, 0, 0, b_j) and take fma(W[j][k], P_k)
+// neuron the accumulators start at (0, 0, 0, b_j + 0) and take fma(W[j][k], P_k)
 // over active k ascending; skips are added afterwards in connection order.
 #pragma once
 #include "am_device.h"
@@ -36,7 +36,7 @@ __device__ __forceinline__ void chain_fp64(const NetDev& net, const uint32_t* sm
         jj[t] = jb + lane + 64 * t;
         ok[t] = jj[t] < nout;
         ax[t] = 0.0; ay[t] = 0.0; az[t] = 0.0;
-        ac[t] = ok[t] ? bb[jj[t]] : 0.0;
+        ac[t] = ok[t] ? bb[jj[t]] + 0.0 : 0.0;   // -0 bias -> +0: the tiled chain zero-fills inactive inputs
       }
       for (int k = 0; k < nin; ++k) {
         if (!mbit(smask, in0 + k)) continue;

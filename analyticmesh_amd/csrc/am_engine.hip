@@ -125,8 +125,12 @@ struct am_ctx {
   std::vector<double> mesh_v;
   std::vector<int> mesh_off, mesh_idx;
 
+  // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
+  bool tile_ok = false;
+  DevBuf<double> tplanes, teq;
+
   double stats[AM_STAT_COUNT] = {0};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
 namespace {
@@ -230,7 +234,7 @@ int insert_level(am_ctx* c, size_t n) {
 
 size_t region_lds_bytes(const am_ctx* c, int& Cs, int& Csort) {
   Cs = c->C;
-  Csort = 1;
+  Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
   while (Csort < c->C) Csort <<= 1;
   return (size_t)4 * Cs * sizeof(double) + (size_t)Csort * (sizeof(double) + sizeof(int)) +
          (size_t)((c->C + 31) / 32) * 4 + (size_t)c->Wd * 4 + 64;
@@ -262,6 +266,82 @@ void launch_region_fp64(am_ctx* c, int first, int count, float* ms, const int* i
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   HIPCHK(hipEventSynchronize(c->ev1));
   HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+}
+
+size_t chain_lds_bytes(const am_ctx* c) {
+  return (size_t)AM_CHAIN_MAXW * AM_CXS * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
+}
+
+// FP64 planes + walk of `count` regions (region ids first.., or ids[0..count)),
+// through the tiled chain and the walk kernel in chunks; candidate slots are
+// numbered from `lvl_first`.  Falls back to the one-region kernel where the
+// tiled chain does not apply.
+void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, float* chain_ms,
+                      float* walk_ms) {
+  *chain_ms = 0;
+  *walk_ms = 0;
+  if (!c->tile_ok) {
+    // one region per workgroup: the chain and the walk in one kernel
+    float ms = 0;
+    if (ids) {
+      launch_region_fp64(c, lvl_first, count, &ms, ids);
+    } else {
+      launch_region_fp64(c, first, count, &ms);
+    }
+    *walk_ms = ms;
+    return;
+  }
+  int Cs, Csort;
+  const size_t wlds = region_lds_bytes(c, Cs, Csort);
+  const int Ch = (c->C + 63) & ~63;
+  size_t chunk = ((size_t)1 << 30) / ((size_t)32 * Ch);
+  chunk = std::max<size_t>(1024, std::min<size_t>(chunk, 65536)) & ~(size_t)(AM_CT - 1);
+  const size_t cap = std::min<size_t>(chunk, (size_t)count);
+  ensure(c->tplanes, cap * 4 * Ch, false, c->stream);
+  ensure(c->teq, cap * 4, false, c->stream);
+  const size_t clds = chain_lds_bytes(c);
+  for (int off = 0; off < count; off += (int)chunk) {
+    const int n = std::min<int>((int)chunk, count - off);
+    ChainIO cio;
+    cio.reg_mask = c->reg_mask.p;
+    cio.ids = ids ? ids + off : nullptr;
+    cio.first = first + off;
+    cio.count = n;
+    cio.Cs = Ch;
+    cio.planes = c->tplanes.p;
+    cio.eq = c->teq.p;
+    RegionIO io;
+    io.reg_mask = c->reg_mask.p;
+    io.reg_start = c->reg_start.p;
+    io.reg_point = c->reg_point.p;
+    io.reg_info = c->reg_info.p;
+    io.reg_tab = c->reg_tab.p;
+    io.reg_verts = c->reg_verts.p;
+    io.reg_vpair = c->reg_vpair.p;
+    io.cand_mask = c->cand_mask.p;
+    io.cand_start = c->cand_start.p;
+    io.cand_point = c->cand_point.p;
+    io.first = lvl_first;
+    io.count = n;
+    io.Cs = Cs;
+    io.Csort = Csort;
+    io.flags = nullptr;
+    io.ids = ids ? ids + off : nullptr;
+    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(256), clds, c->stream, c->net, cio);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, c->tplanes.p,
+                       c->teq.p, Ch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(c->ev2, c->stream));
+    HIPCHK(hipEventSynchronize(c->ev2));
+    float a = 0, b = 0;
+    HIPCHK(hipEventElapsedTime(&a, c->ev0, c->ev1));
+    HIPCHK(hipEventElapsedTime(&b, c->ev1, c->ev2));
+    *chain_ms += a;
+    *walk_ms += b;
+  }
 }
 
 void upload(DevBuf<double>& dst, const void* src, size_t n, int dtype, hipStream_t st) {
@@ -394,6 +474,7 @@ am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_
       c->max_lds = std::max<size_t>(c->max_lds, (size_t)optin);
     HIPCHK(hipEventCreate(&c->ev0));
     HIPCHK(hipEventCreate(&c->ev1));
+    HIPCHK(hipEventCreate(&c->ev2));
     HIPCHK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     return AM_OK;
@@ -523,8 +604,22 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     const size_t lds = region_lds_bytes(c, Cs, Csort);
     if (lds > c->max_lds)
       throw CapErr{"network too wide for the FP64 region kernel's LDS (" + std::to_string(lds) + " bytes)"};
-    if (lds > 65536)
+    if (lds > 65536) {
       HIPCHK(hipFuncSetAttribute((const void*)k_region_fp64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      HIPCHK(hipFuncSetAttribute((const void*)k_walk_fp64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    c->tile_ok = true;
+    for (int li = 0; li <= c->D; ++li)
+      if (!c->skips[li].empty()) c->tile_ok = false;
+    for (int i = 1; i <= c->D + 1; ++i)
+      if (c->nodes[i] > AM_CHAIN_MAXW) c->tile_ok = false;
+    if (c->tile_ok) {
+      const size_t clds = chain_lds_bytes(c);
+      if (clds > c->max_lds) c->tile_ok = false;
+      else if (clds > 65536)
+        HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)clds));
+    }
     if (c->precision != AM_PREC_FP64) fast_prepare(c->fast, c->fbuf, n, c->precision, weights, biases, dtype, st);
 
     // ---- seeds -> first level (backend/inc/states.h:147-176)
@@ -554,7 +649,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     }
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
-    double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0;
+    double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0, t_chain64 = 0, t_walk64 = 0;
     long long levels = 0, cands = 0, redo = 0, launches = 0;
     while (lvl_count > 0) {
       if (c->max_regions >= 0 && lvl_first >= c->max_regions) break;
@@ -562,8 +657,13 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       grow_cands(c, ncand);
       float ms = 0;
       if (c->precision == AM_PREC_FP64) {
-        launch_region_fp64(c, (int)lvl_first, (int)lvl_count, &ms);
-        t_region += ms;
+        float cm = 0, wm = 0;
+        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr, &cm, &wm);
+        t_region += cm + wm;
+        t_gemm += cm;
+        t_walk += wm;
+        t_chain64 += cm;
+        t_walk64 += wm;
         launches += 1;
       } else {
         FastLevel fl;
@@ -579,9 +679,11 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         if (nredo > 0) {
           // FP64 re-decision of the undecided regions: rerun them through the
           // FP64 kernel in place (their records and candidate slots are rewritten).
-          float m2 = 0;
-          launch_region_fp64(c, (int)lvl_first, nredo, &m2, ids);
-          t_redo += m2;
+          float cm = 0, wm = 0;
+          run_fp64_regions(c, (int)lvl_first, (int)lvl_first, nredo, ids, &cm, &wm);
+          t_redo += cm + wm;
+          t_chain64 += cm;
+          t_walk64 += wm;
           redo += nredo;
         }
         t_gemm += ft.gemm_ms;
@@ -618,6 +720,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_CANDIDATES] = (double)cands;
     c->stats[AM_STAT_REDO_REGIONS] = (double)redo;
     c->stats[AM_STAT_REGION_LAUNCHES] = (double)launches;
+    c->stats[AM_STAT_CHAIN64_MS] = t_chain64;
+    c->stats[AM_STAT_WALK64_MS] = t_walk64;
     c->has_run = true;
     return AM_OK;
   });
@@ -915,6 +1019,7 @@ am_status am_destroy(am_ctx* c) {
   if (!c) return AM_ERR_ARG;
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->ev2) (void)hipEventDestroy(c->ev2);
   fast_release(c->fbuf);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   hipStream_t own = c->own_stream;

@@ -266,7 +266,8 @@ __global__ __launch_bounds__(256) void k_gemm_f64(FastNetDev f, GemmIO io) {
       if (rb >= nrb) continue;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = rb * 16 + 4 * (lane >> 4) + i;
+        // f64 16x16x4 result layout: element i of lane l is D[(l >> 4) + 4 i][l & 15]
+        const int row = rb * 16 + (lane >> 4) + 4 * i;
         if (row < nout) {
           const double b = bias[row];
           Xs[row * AM_XS64 + c0] = acc[j][0][i] + ((c0 & 3) == 3 ? b : 0.0);
@@ -579,7 +580,7 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
   int* idl = (int*)slot(b, S_IDS, (size_t)lvl.count * 4);
   int* cnt = (int*)slot(b, S_CNT, 16);
 
-  int Csort = 1;
+  int Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
   while (Csort < net.C) Csort <<= 1;
   const int CsL = net.C;
   const size_t wlds = (size_t)4 * CsL * 4 + (size_t)Csort * 8 + (size_t)((net.C + 31) / 32) * 4 + (size_t)net.Wd * 4 + 64;

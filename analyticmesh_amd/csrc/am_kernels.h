@@ -44,4 +44,25 @@ struct InsertIO {
 
 __global__ void k_region_fp64(NetDev net, RegionIO io);
 
+// Tiled FP64 chain: the planes of a tile of regions in HBM, for the walk kernel.
+#define AM_CT 8                 // regions per chain workgroup
+#define AM_CN (4 * AM_CT)       // chain columns: x, y, z, c of each region
+#define AM_CXS (AM_CN + 2)      // LDS row stride in doubles (16-byte aligned rows)
+#define AM_CHAIN_MAXW 256       // widest layer the tiled chain takes
+
+struct ChainIO {
+  const uint32_t* reg_mask;   // [R][Wd]
+  const int* ids;             // optional region ids; else first + tile-local index
+  int first, count;           // regions of this launch
+  int Cs;                     // stride of one component array in `planes` (>= C)
+  double* planes;             // [count][4][Cs]: signed constraint planes, extras appended
+  double* eq;                 // [count][4]: output plane with iso subtracted
+};
+
+__global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
+// The walk of k_region_fp64 on planes from k_chain_tile_fp64 (blockIdx.x indexes both).
+// Region of block b: io.ids[b] when ids are given, else rid0 + b; candidate slots
+// are numbered from io.first as in k_region_fp64.
+__global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const double* planes, const double* eq, int Cs_hbm);
+
 }  // namespace am

@@ -7,6 +7,7 @@
 #define AM_MAX_TMS 16           // transform matrices
 #define AM_MAX_WORDS 64         // packed mask words: up to 2048 hidden neurons
 #define AM_VM_MAX 64            // polygon vertex cap (one lane per vertex)
+#define AM_TOPK 64              // distance-order positions the walk builds up front (sort width >= this)
 #define AM_INFO_STRIDE 8
 
 // Region status codes (shared with the host API).

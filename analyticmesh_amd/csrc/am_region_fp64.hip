@@ -24,7 +24,7 @@ __global__ __launch_bounds__(64) void k_region_fp64(NetDev net, RegionIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C, L = net.L, vm = net.vm;
   const int Cs = io.Cs;        // stride of one SoA plane array (>= C)
-  const int N2 = io.Csort;     // power of two >= C
+  const int N2 = io.Csort;     // power of two >= max(C, 64)
   double* Px = reinterpret_cast<double*>(smem_raw);
   double* Py = Px + Cs;
   double* Pz = Py + Cs;

@@ -151,6 +151,100 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
   }
 }
 
+// Distance key of constraint g at p: s|s|/|a|^2 orders like the signed
+// distance s/|a| (backend/inc/process.h:208-239) without a square root.
+template <typename T>
+__device__ __forceinline__ T distance_key(const T* Px, const T* Py, const T* Pz, const T* Pc, int g, const T* p) {
+  const T s = plane_eval(Px[g], Py[g], Pz[g], Pc[g], p);
+  const T nn = plane_nn(Px[g], Py[g], Pz[g]);
+  T key = nn > T(0) ? s * absT(s) / nn : (s > T(0) ? T(__builtin_inf()) : T(-__builtin_inf()));
+  if (key != key) key = -__builtin_inf();  // NaN sorts last
+  return key;
+}
+
+// (key, index) a comes before b: larger key first, ties to the lower index.
+template <typename T>
+__device__ __forceinline__ bool key_before(T ka, int ia, T kb, int ib) {
+  return ka > kb || (ka == kb && ia < ib);
+}
+
+// Full stable descending order of all constraints: keys[0..N2), order[0..N2)
+// in LDS, bitonic (backend/inc/process.h:244-264 sorts the same keys).
+template <typename T>
+__device__ void full_sort(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int N2, const T* p, T* keys,
+                          int* order) {
+  const int lane = threadIdx.x;
+  __syncthreads();
+  for (int g = lane; g < N2; g += 64) {
+    keys[g] = g < C ? distance_key<T>(Px, Py, Pz, Pc, g, p) : T(-__builtin_inf());
+    order[g] = g;
+  }
+  __syncthreads();
+  for (int k = 2; k <= N2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = lane; i < N2; i += 64) {
+        const int l = i ^ j;
+        if (l > i) {
+          const T ki = keys[i], kl = keys[l];
+          const int oi = order[i], ol = order[l];
+          const bool i_first = key_before(ki, oi, kl, ol);
+          if (((i & k) == 0) != i_first) {
+            keys[i] = kl; keys[l] = ki;
+            order[i] = ol; order[l] = oi;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// The first 64 positions of that order, in registers: lane l ends holding
+// position l (sentinel (-inf, INT_MAX) past C).  Each round sorts 64 new
+// constraints across the wave with a bitonic network of shuffles, then keeps
+// the better half of (best, reversed batch) -- a bitonic sequence holding the
+// 64 best of both -- and cleans it.  The comparisons are those of full_sort,
+// so the 64 entries are exactly its first 64.
+template <typename T>
+__device__ void wave_top64(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* p, T& bk, int& bi) {
+  const int lane = threadIdx.x;
+  bk = T(-__builtin_inf());
+  bi = 0x7fffffff;
+  for (int base = 0; base < C; base += 64) {
+    const int g = base + lane;
+    T ek = T(-__builtin_inf());
+    int ei = 0x7fffffff;
+    if (g < C) { ek = distance_key<T>(Px, Py, Pz, Pc, g, p); ei = g; }
+    // bitonic sort of the batch, descending across lanes
+    for (int k = 2; k <= 64; k <<= 1) {
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const T pk = __shfl_xor(ek, j);
+        const int pi = __shfl_xor(ei, j);
+        const bool lower = (lane & j) == 0;
+        const bool desc = (lane & k) == 0 || k == 64;
+        const bool mine_first = key_before(ek, ei, pk, pi);
+        if ((lower == desc) != mine_first) { ek = pk; ei = pi; }
+      }
+    }
+    if (base == 0) {
+      bk = ek;
+      bi = ei;
+      continue;
+    }
+    // better of best[l] and batch[63 - l]: a bitonic sequence of the 64 best
+    const T rk = __shfl(ek, 63 - lane);
+    const int ri = __shfl(ei, 63 - lane);
+    if (!key_before(bk, bi, rk, ri)) { bk = rk; bi = ri; }
+    for (int j = 32; j > 0; j >>= 1) {
+      const T pk = __shfl_xor(bk, j);
+      const int pi = __shfl_xor(bi, j);
+      const bool lower = (lane & j) == 0;
+      const bool mine_first = key_before(bk, bi, pk, pi);
+      if (lower != mine_first) { bk = pk; bi = pi; }
+    }
+  }
+}
+
 template <typename T, bool CHECKED>
 __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                             T* Px, T* Py, T* Pz, T* Pc, T* keys, int* order, uint32_t* used,
@@ -165,36 +259,19 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
   const int start_in = io.reg_start[rid];
   bool undecided = false;
 
-  // ---- distance keys s|s|/|a|^2 (orders like s/|a|), stable descending sort
-  for (int g = lane; g < N2; g += 64) {
-    T key = -__builtin_inf();
-    if (g < C) {
-      const T s = plane_eval(Px[g], Py[g], Pz[g], Pc[g], p);
-      const T nn = plane_nn(Px[g], Py[g], Pz[g]);
-      key = nn > T(0) ? s * absT(s) / nn : (s > T(0) ? T(__builtin_inf()) : T(-__builtin_inf()));
-      if (key != key) key = -__builtin_inf();  // NaN sorts last
-    }
-    keys[g] = key;
-    order[g] = g;
+  // ---- distance order (keys s|s|/|a|^2, stable descending).  Only the first
+  // AM_TOPK positions are built up front (the start search reads positions
+  // <= 56 and the walk's winner is nearly always among the first few); the
+  // full order is sorted the first time the walk scans past them.
+  bool full_order = C <= AM_TOPK;
+  {
+    T bk;
+    int bi;
+    wave_top64<T>(Px, Py, Pz, Pc, C, p, bk, bi);
+    keys[lane] = bk;
+    order[lane] = bi;
   }
   __syncthreads();
-  for (int k = 2; k <= N2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = lane; i < N2; i += 64) {
-        const int l = i ^ j;
-        if (l > i) {
-          const T ki = keys[i], kl = keys[l];
-          const int oi = order[i], ol = order[l];
-          const bool i_first = (ki > kl) || (ki == kl && oi < ol);
-          if (((i & k) == 0) != i_first) {
-            keys[i] = kl; keys[l] = ki;
-            order[i] = ol; order[l] = oi;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
 
   int start = start_in;
   int status = AM_ST_OK;
@@ -266,6 +343,10 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
       const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
       const int first_entry = s_tab[0];
       for (int w = 0; w < C; ++w) {
+        if (w == AM_TOPK && !full_order) {
+          full_sort<T>(Px, Py, Pz, Pc, C, N2, p, keys, order);
+          full_order = true;
+        }
         const int r = order[w];
         const bool cand = (r == first_entry && ntab >= 3) || !((used[r >> 5] >> (r & 31)) & 1u);
         if (!cand) continue;
@@ -278,6 +359,25 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
           const T h = hadamard_ratio(pr4, pc4, eq);
           epsv = T(wp.eps) * (T(1) + h);
           trusted = h <= T(wp.h_max);
+        }
+        if (!CHECKED) {
+          // infeasible as soon as one constraint is violated: test them in
+          // distance order, a wave-wide chunk at a time, so that a rejected
+          // candidate usually stops after the nearest 64 planes (the verdict
+          // does not depend on the order of the tests)
+          // (first the nearest 64, then every constraint by index)
+          bool viol = false;
+          for (int base = -64; base < C; base += 64) {
+            const int pos = base + lane;
+            bool vl = false;
+            const int g = base < 0 ? (lane < C ? order[lane] : C) : pos;
+            if (g < C && g != cur && g != r) vl = violates(Px[g], Py[g], Pz[g], Pc[g], vt);
+            if (__any(vl)) { viol = true; break; }
+          }
+          if (viol) continue;
+          win = r;
+          v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
+          break;
         }
         const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, C, lane, cur, r, vt, epsv);
         if (__any(verdict == 1)) continue;          // certainly infeasible

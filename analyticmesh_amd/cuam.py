@@ -10,7 +10,9 @@ One process-wide context, as in the reference; create
 
 ``float_type`` selects the arithmetic: ``"float64"`` runs the FP64 form (the
 reference's default dtype); ``"float32"`` runs the FP32-MFMA form with FP64
-re-decision; ``"bf16x3"`` runs the split-BF16 MFMA form.
+re-decision; ``"bf16x3"`` runs the split-BF16 MFMA form; ``"fp64mfma"`` builds
+the planes on FP64 MFMA and walks them in FP32.  The last two take float32
+tensors, like ``"float32"``.
 """
 import numpy as np
 import torch
@@ -19,7 +21,7 @@ from .engine import Engine, MarchResult
 
 _S = {"engine": None, "float_type": "", "nodes": None, "arc": None, "E": 0, "result": None}
 
-_PRECISION = {"float64": "fp64", "float32": "fp32", "bf16x3": "bf16x3"}
+_PRECISION = {"float64": "fp64", "float32": "fp32", "bf16x3": "bf16x3", "fp64mfma": "fp64mfma"}
 
 
 def _check(cond, msg):

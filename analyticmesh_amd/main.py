@@ -218,7 +218,8 @@ def AnalyticMarching(model, save_ply_path="mesh.ply", iso=0.0, scale=1.0, center
     """Mesh the iso level set of ``model`` into ``save_ply_path`` (backend/main.py:335-559).
 
     ``precision`` (optional) overrides the arithmetic chosen by ``dtype``:
-    "fp64", "fp32" (FP32 MFMA + FP64 re-decision) or "bf16x3" (split-BF16 MFMA).
+    "fp64", "fp32" (FP32 MFMA + FP64 re-decision), "bf16x3" (split-BF16 MFMA) or
+    "fp64mfma" (FP64 MFMA planes, FP32 walk, FP64 re-decision).
     Returns {"init_point_time", "init_cuda_time", "am_time", "export_time"} in seconds.
     """
     assert w_extra_constraints_.shape[0] == b_extra_constraints_.shape[0]
@@ -237,7 +238,7 @@ def AnalyticMarching(model, save_ply_path="mesh.ply", iso=0.0, scale=1.0, center
     out["init_point_time"] = time.time() - t
 
     info = model.get_info()
-    float_type = {"fp64": "float64", "fp32": "float32", "bf16x3": "bf16x3"}.get(
+    float_type = {"fp64": "float64", "fp32": "float32", "bf16x3": "bf16x3", "fp64mfma": "fp64mfma"}.get(
         precision, "float64" if dtype == torch.float64 else "float32")
     tdt = torch.float64 if float_type == "float64" else torch.float32
     # states at the seeds, in the marching precision's own arithmetic

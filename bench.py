@@ -25,7 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp64": 78.6}
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp64": 78.6, "fp64mfma": 78.6}
+GEMM_KERNEL = {"fp32": "k_gemm<0>", "bf16x3": "k_gemm<1>", "fp64mfma": "k_gemm_f64", "fp64": "k_chain_tile_fp64"}
 
 
 def log(*a):
@@ -154,7 +155,7 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--cpu-regions", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,bf16x3,fp64")
+    ap.add_argument("--forms", default="fp32,bf16x3,fp64mfma,fp64")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
     ap.add_argument("--extras", action="store_true", help="also measure the 64-MLP batch and the adjoint")
     args = ap.parse_args()
@@ -184,12 +185,9 @@ def main():
     head = "fp32" if out.get("fp32") else next(f for f in forms if out.get(f))
     h = out[head]
     st = h["stats"]
-    # dominant kernel: the plane GEMM in reduced precision, the region kernel in FP64
-    if head == "fp64":
-        kern_ms, launches, kname = st[1], max(1.0, st[11]), "k_region_fp64"
-    else:
-        kern_ms, launches, kname = st[3], max(1.0, st[11]), "k_gemm_" + head
-    flops = flops_region * (h["regions"] - st[9])
+    # the plane GEMM of the form (FLOPs per region x regions it built planes for)
+    kern_ms, launches, kname = st[3], max(1.0, st[11]), GEMM_KERNEL[head]
+    flops = flops_region * (h["regions"] - (st[9] if head != "fp64" else 0))
     achieved = flops / (kern_ms / 1e3) / 1e12 if kern_ms > 0 else 0.0
     roofline = {"bound": "mfma", "kernel": kname, "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[head],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 5),
@@ -218,7 +216,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64"}[head],
+        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64", "fp64mfma": "f64 planes, f32 walk"}[head],
         "data": "synthetic (seeded geometric-init weights, bisection seeds)",
         "config": {"workload": "deep 3->256x4->1 ReLU MLP, r=0.6, sigma=0.05, seed 0, 1024 seeds",
                    "faces": h["faces"], "regions": h["regions"], "form": head},
@@ -232,6 +230,8 @@ def main():
             "walk_ms": round(float(out[f]["stats"][4]), 3),
             "fp64_redo_ms": round(float(out[f]["stats"][5]), 3),
             "fp64_redo_regions": int(out[f]["stats"][9]),
+            "fp64_chain_ms": round(float(out[f]["stats"][13]), 3),
+            "fp64_walk_ms": round(float(out[f]["stats"][14]), 3),
             "insert_ms": round(float(out[f]["stats"][2]), 3),
             "combine_ms": round(float(getattr(out[f]["res"], "combine_ms", 0.0)), 3),
             "levels": int(out[f]["stats"][6]),

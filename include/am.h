@@ -125,6 +125,8 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_COMBINE_MS 10
 #define AM_STAT_REGION_LAUNCHES 11
 #define AM_STAT_ADJOINT_MS 12   /* am_adjoint kernels (wall time) */
+#define AM_STAT_CHAIN64_MS 13   /* FP64 tiled plane chain, all launches (HIP events) */
+#define AM_STAT_WALK64_MS 14    /* FP64 walk after the tiled chain, all launches (HIP events) */
 #define AM_STAT_COUNT 16
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 

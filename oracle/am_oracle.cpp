@@ -138,7 +138,9 @@ void compute_planes(const Net& n, const std::vector<std::vector<T>>& Wt,
     const int in0 = n.off[li - 1];
     const T* Wl = Wt[li].data();
     for (int j = 0; j < nout; ++j) {
-      T ax = 0, ay = 0, az = 0, c = bt[li][j];
+      // +0 turns a -0 bias into +0, so that skipping an inactive input and
+      // adding fma(w, +0, acc) give the same bits (no accumulator is ever -0)
+      T ax = 0, ay = 0, az = 0, c = bt[li][j] + T(0);
       for (int k = 0; k < nin; ++k) {
         if (!bit(mask, in0 + k)) continue;
         const T w = Wl[(size_t)j * nin + k];
