@@ -2,18 +2,22 @@
 //
 // The masked affine chain (backend/inc/process.h:18-193) is a GEMM per layer:
 // Y = W_li * X with X = diag(m) P_{li-1}, one column per plane component (x, y,
-// z, c) of each region.  A workgroup takes AM_CT regions (AM_CN columns); each
-// thread owns 4 output rows x 8 columns and runs the whole k loop in ascending
-// order with one fma per step, so every output is the same sequence of
-// roundings as the canonical chain: accumulators start at (0, 0, 0, b_j + 0)
-// and take fma(W[j][k], m_k ? P_k : +0).  An inactive k adds fma(w, +0, acc) =
-// acc exactly (acc is never -0: every accumulator starts at +0 or a nonzero
-// bias, and a sum of opposite-signed zeros rounds to +0), so zero-filling the
-// inactive inputs reproduces skipping them.  W streams from L2 (coalesced
-// rows of W^T, shared by the four waves); X and the outputs live in one LDS
-// buffer: after the k loop every wave parks its accumulators there, then the
-// tile is written to HBM signed (a.x + c <= 0 inside) and masked in place as
-// the next layer's input.
+// z, c) of each region.  A workgroup takes AM_CT regions = 64 columns, one per
+// lane, and AM_CHAIN_MAXW / AM_CR waves, each owning AM_CR output rows.  Every
+// lane runs the whole k loop in ascending order with one fma per step, so each
+// output is the same sequence of roundings as the canonical chain:
+// accumulators start at (0, 0, 0, b_j + 0) and take fma(W[j][k], m_k ? P_k : +0).
+// An inactive k adds fma(w, +0, acc) = acc exactly (no accumulator is ever -0:
+// each starts at +0 or a nonzero bias, and a sum of opposite-signed zeros
+// rounds to +0), so zero-filling the inactive inputs reproduces skipping them.
+//
+// Operand delivery: X[k][lane] is a private LDS read per lane; the wave's 16
+// weights W[k][row0..row0+15] arrive as one coalesced vector load (lane l
+// holds row l & 15) and each fma takes its row's weight through a DPP row
+// broadcast, so a weight costs no extra instruction.  The next k's weights and
+// X are loaded while this k's fmas run.  After the k loop each wave writes its rows to HBM (signed,
+// a.x + c <= 0 inside), and, once every wave is done reading X, parks them in
+// LDS masked as the next layer's input.
 //
 // k_walk_fp64 is the polygon walk of the region kernel on those planes.
 #include "am_device.h"
@@ -22,15 +26,46 @@
 
 namespace am {
 
-__global__ __launch_bounds__(256) void k_chain_tile_fp64(NetDev net, ChainIO io) {
+#define AM_CPF 4   // k steps of operands in flight
+
+// acc[j] = fma(w from lane j of this lane's 16-lane row, x, acc[j]), j < 16:
+// v_fmac_f64 with a row_newbcast DPP source is one fused multiply-add per
+// output.  The leading s_nop covers the DPP read-after-VALU-write hazard on w.
+__device__ __forceinline__ void chain_step16(double (&acc)[AM_CR], double w, double x) {
+  static_assert(AM_CR == 16, "one DPP row broadcast per output row");
+  asm volatile(
+      "s_nop 1\n"
+      "v_fmac_f64_dpp %0, %16, %17 row_newbcast:0 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %16, %17 row_newbcast:1 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %16, %17 row_newbcast:2 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %16, %17 row_newbcast:3 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %4, %16, %17 row_newbcast:4 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %5, %16, %17 row_newbcast:5 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %6, %16, %17 row_newbcast:6 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %7, %16, %17 row_newbcast:7 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %8, %16, %17 row_newbcast:8 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %9, %16, %17 row_newbcast:9 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %10, %16, %17 row_newbcast:10 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %11, %16, %17 row_newbcast:11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %12, %16, %17 row_newbcast:12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %13, %16, %17 row_newbcast:13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %14, %16, %17 row_newbcast:14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %15, %16, %17 row_newbcast:15 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]), "+v"(acc[6]),
+        "+v"(acc[7]), "+v"(acc[8]), "+v"(acc[9]), "+v"(acc[10]), "+v"(acc[11]), "+v"(acc[12]), "+v"(acc[13]),
+        "+v"(acc[14]), "+v"(acc[15])
+      : "v"(w), "v"(x));
+}
+
+__global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int Kp = AM_CHAIN_MAXW;
-  double* Xs = reinterpret_cast<double*>(smem_raw);                          // [Kp][AM_CXS]
-  uint32_t* mk = reinterpret_cast<uint32_t*>(Xs + (size_t)Kp * AM_CXS);       // [AM_CT][Wd]
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  double* Xs = reinterpret_cast<double*>(smem_raw);                              // [AM_CHAIN_MAXW][AM_CN]
+  uint32_t* mk = reinterpret_cast<uint32_t*>(Xs + (size_t)AM_CHAIN_MAXW * AM_CN);  // [AM_CT][Wd]
   const int r0 = blockIdx.x * AM_CT;
   const int nreg = min(AM_CT, io.count - r0);
-  for (int idx = tid; idx < AM_CT * net.Wd; idx += 256) {
+  for (int idx = tid; idx < AM_CT * net.Wd; idx += AM_CHAIN_THREADS) {
     const int t = idx / net.Wd, w = idx - t * net.Wd;
     uint32_t v = 0u;
     if (t < nreg) {
@@ -43,83 +78,87 @@ __global__ __launch_bounds__(256) void k_chain_tile_fp64(NetDev net, ChainIO io)
   const size_t Cs = (size_t)io.Cs;
   // hidden node 1: the rows of W0 (unsigned into X, signed into HBM)
   const int n1 = net.nodes[1];
-  for (int idx = tid; idx < n1 * AM_CN; idx += 256) {
+  const int n1p = (n1 + AM_CPF - 1) / AM_CPF * AM_CPF;
+  for (int idx = tid; idx < n1p * AM_CN; idx += AM_CHAIN_THREADS) {
     const int k = idx / AM_CN, col = idx - k * AM_CN, t = col >> 2, q = col & 3;
-    const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
-    Xs[k * AM_CXS + col] = mbit(mk + t * net.Wd, k) ? val : 0.0;
+    double x = 0.0;
+    if (k < n1) {
+      const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
+      x = mbit(mk + t * net.Wd, k) ? val : 0.0;
+    }
+    Xs[k * AM_CN + col] = x;
   }
-  for (int idx = tid; idx < AM_CN * n1; idx += 256) {
+  for (int idx = tid; idx < AM_CN * n1; idx += AM_CHAIN_THREADS) {
     const int col = idx / n1, k = idx - col * n1, t = col >> 2, q = col & 3;
     if (t < nreg) {
       const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
       io.planes[((size_t)(r0 + t) * 4 + q) * Cs + k] = mbit(mk + t * net.Wd, k) ? -val : val;
     }
   }
-  for (int idx = tid; idx < AM_CN * net.E; idx += 256) {
+  for (int idx = tid; idx < AM_CN * net.E; idx += AM_CHAIN_THREADS) {
     const int col = idx / net.E, e = idx - col * net.E, t = col >> 2, q = col & 3;
     if (t < nreg) io.planes[((size_t)(r0 + t) * 4 + q) * Cs + net.L + e] = q < 3 ? net.ew[e * 3 + q] : net.eb[e];
   }
   __syncthreads();
-  const int c0 = 8 * wave;            // this wave's columns: regions 2*wave, 2*wave + 1
+  const int t_l = lane >> 2, q_l = lane & 3;
+  const bool col_live = t_l < nreg;
+  const uint32_t* mk_l = mk + t_l * net.Wd;
   for (int li = 1; li <= net.D; ++li) {
-    const int nin = net.nodes[li], nout = net.nodes[li + 1];
-    const double* WT = net.WT[li];
-    const double* bias = net.b[li];
-    double acc[4][8];
-    bool ok[4];
+    const int nin = net.nodes[li], nout = net.nodes[li + 1], noutp = io.noutp[li];
+    const int row0 = wave * AM_CR;
+    const bool active = row0 < nout;   // wave-uniform
+    double acc[AM_CR];
+    if (active) {
+      const double* bias = io.bp[li] + row0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = lane + 64 * i;
-      ok[i] = row < nout;
-      const double bz = ok[i] ? bias[row] + 0.0 : 0.0;
+      for (int j = 0; j < AM_CR; ++j) acc[j] = q_l == 3 ? bias[j] + 0.0 : 0.0;
+      // lane l carries W[k][row0 + (l & 15)]; each fma takes row j's weight by a
+      // DPP row broadcast of that register
+      const double* wcol = io.Wp[li] + row0 + (lane & 15);
+      // operands of the next AM_CPF k steps are in flight while a step's fmas
+      // run (past the end the ring reloads row nin-1 and is not used)
+      // k runs over nin rounded up to AM_CPF: the padding rows of W^T and of X
+      // are +0, and fma(+0, +0, acc) = acc
+      const int ninp = (nin + AM_CPF - 1) / AM_CPF * AM_CPF;
+      double wq[AM_CPF], xq[AM_CPF];
 #pragma unroll
-      for (int j = 0; j < 8; ++j) acc[i][j] = (j & 3) == 3 ? bz : 0.0;
-    }
-    const double* wk = WT + lane;
-#pragma unroll 2
-    for (int k = 0; k < nin; ++k) {
-      const double* xr = Xs + k * AM_CXS + c0;
-      double x[8];
+      for (int u = 0; u < AM_CPF; ++u) {
+        wq[u] = wcol[(size_t)u * noutp];
+        xq[u] = Xs[u * AM_CN + lane];
+      }
+      for (int k = 0; k < ninp; k += AM_CPF) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) x[j] = xr[j];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        if (ok[i]) {
-          const double w = wk[(size_t)k * nout + 64 * i];
-#pragma unroll
-          for (int j = 0; j < 8; ++j) acc[i][j] = fmaT(w, x[j], acc[i][j]);
+        for (int u = 0; u < AM_CPF; ++u) {
+          chain_step16(acc, wq[u], xq[u]);
+          // refill the slot just used, AM_CPF steps ahead
+          const int kn = min(k + u + AM_CPF, ninp - 1);
+          wq[u] = wcol[(size_t)kn * noutp];
+          xq[u] = Xs[kn * AM_CN + lane];
         }
+      }
+      if (li < net.D && col_live) {
+        const int out0 = net.off[li];
+        double* dst = io.planes + ((size_t)(r0 + t_l) * 4 + q_l) * Cs + out0 + row0;
+#pragma unroll
+        for (int j = 0; j < AM_CR; ++j)
+          if (row0 + j < nout) dst[j] = mbit(mk_l, out0 + row0 + j) ? -acc[j] : acc[j];
       }
     }
     __syncthreads();                  // every wave is done reading this layer's input
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if (!ok[i]) continue;
-      double* yr = Xs + (lane + 64 * i) * AM_CXS + c0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) yr[j] = acc[i][j];
-    }
-    __syncthreads();
     if (li < net.D) {
-      const int out0 = net.off[li];
-      for (int idx = tid; idx < AM_CN * nout; idx += 256) {
-        const int col = idx / nout, j = idx - col * nout, t = col >> 2, q = col & 3;
-        if (t < nreg) {
-          const double y = Xs[j * AM_CXS + col];
-          io.planes[((size_t)(r0 + t) * 4 + q) * Cs + out0 + j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
+      // masked input of the next layer; rows nout..round_up(nout, AM_CPF) are +0
+      const int noutk = (nout + AM_CPF - 1) / AM_CPF * AM_CPF;
+      if (active) {
+        const int out0 = net.off[li];
+#pragma unroll
+        for (int j = 0; j < AM_CR; ++j) {
+          const int row = row0 + j;
+          if (row < nout) Xs[row * AM_CN + lane] = mbit(mk_l, out0 + row) ? acc[j] : 0.0;
+          else if (row < noutk) Xs[row * AM_CN + lane] = 0.0;
         }
       }
-      __syncthreads();
-      for (int idx = tid; idx < nout * AM_CN; idx += 256) {
-        const int j = idx / AM_CN, col = idx - j * AM_CN, t = col >> 2;
-        if (!mbit(mk + t * net.Wd, out0 + j)) Xs[j * AM_CXS + col] = 0.0;
-      }
-    } else if (tid < AM_CN) {
-      const int t = tid >> 2, q = tid & 3;
-      if (t < nreg) {
-        const double v = Xs[tid];
-        io.eq[(size_t)(r0 + t) * 4 + q] = q == 3 ? v - net.iso : v;
-      }
+    } else if (wave == 0 && col_live) {
+      io.eq[(size_t)(r0 + t_l) * 4 + q_l] = q_l == 3 ? acc[0] - net.iso : acc[0];
     }
     __syncthreads();
   }

@@ -128,6 +128,7 @@ struct am_ctx {
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
   DevBuf<double> tplanes, teq;
+  std::vector<DevBuf<double>> WTp, bpad;   // W_li^T and b_li with n_out padded to AM_CR (zeros)
 
   double stats[AM_STAT_COUNT] = {0};
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
@@ -269,7 +270,7 @@ void launch_region_fp64(am_ctx* c, int first, int count, float* ms, const int* i
 }
 
 size_t chain_lds_bytes(const am_ctx* c) {
-  return (size_t)AM_CHAIN_MAXW * AM_CXS * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
+  return (size_t)AM_CHAIN_MAXW * AM_CN * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
 }
 
 // FP64 planes + walk of `count` regions (region ids first.., or ids[0..count)),
@@ -310,6 +311,11 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     cio.Cs = Ch;
     cio.planes = c->tplanes.p;
     cio.eq = c->teq.p;
+    for (int li = 0; li < AM_MAX_LINEAR; ++li) {
+      cio.Wp[li] = li >= 1 && li <= c->D ? c->WTp[li].p : nullptr;
+      cio.bp[li] = li >= 1 && li <= c->D ? c->bpad[li].p : nullptr;
+      cio.noutp[li] = li >= 1 && li <= c->D ? (c->nodes[li + 1] + AM_CR - 1) / AM_CR * AM_CR : 0;
+    }
     RegionIO io;
     io.reg_mask = c->reg_mask.p;
     io.reg_start = c->reg_start.p;
@@ -328,7 +334,8 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.flags = nullptr;
     io.ids = ids ? ids + off : nullptr;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
-    hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(256), clds, c->stream, c->net, cio);
+    hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
+                       c->net, cio);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, c->tplanes.p,
@@ -373,6 +380,36 @@ void upload_T(DevBuf<double>& dst, const void* src, int rows, int cols, int dtyp
     for (int q = 0; q < cols; ++q) t[(size_t)q * rows + r] = h[(size_t)r * cols + q];
   ensure(dst, std::max<size_t>(n, 1), false, st);
   HIPCHK(hipMemcpy(dst.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+}
+
+// W (rows x cols, row-major) -> W^T padded: [round_up(cols, 4)][round_up(rows, AM_CR)],
+// zeros past `rows` and `cols`; b -> b padded to round_up(rows, AM_CR).  The tiled
+// chain reads whole AM_CR-row blocks and k in steps of 4.
+void upload_pad(DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const void* b, int rows, int cols,
+                int dtype, hipStream_t st) {
+  const int rp = (rows + AM_CR - 1) / AM_CR * AM_CR;
+  const size_t n = (size_t)rows * cols;
+  std::vector<double> h(n), bh(rows);
+  if (dtype == AM_DTYPE_F64) {
+    HIPCHK(hipMemcpy(h.data(), w, n * sizeof(double), hipMemcpyDefault));
+    HIPCHK(hipMemcpy(bh.data(), b, rows * sizeof(double), hipMemcpyDefault));
+  } else {
+    std::vector<float> f(n), fb(rows);
+    HIPCHK(hipMemcpy(f.data(), w, n * sizeof(float), hipMemcpyDefault));
+    HIPCHK(hipMemcpy(fb.data(), b, rows * sizeof(float), hipMemcpyDefault));
+    for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
+    for (int i = 0; i < rows; ++i) bh[i] = (double)fb[i];
+  }
+  const int cp = (cols + 3) / 4 * 4;
+  std::vector<double> t((size_t)cp * rp, 0.0), bp(rp, 0.0);
+  for (int r = 0; r < rows; ++r) {
+    for (int q = 0; q < cols; ++q) t[(size_t)q * rp + r] = h[(size_t)r * cols + q];
+    bp[r] = bh[r];
+  }
+  ensure(wdst, t.size(), false, st);
+  ensure(bdst, bp.size(), false, st);
+  HIPCHK(hipMemcpy(wdst.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(bdst.p, bp.data(), bp.size() * sizeof(double), hipMemcpyHostToDevice));
 }
 
 am_status fail(am_ctx* c, am_status s, const std::string& m) {
@@ -543,6 +580,10 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       if (li >= 1) upload_T(c->WT[li], weights[li], c->nodes[li + 1], c->nodes[li], dtype, st);
       if (li >= 1) upload(c->Wrow[li], weights[li], (size_t)c->nodes[li + 1] * c->nodes[li], dtype, st);
     }
+    c->WTp.resize(nl);
+    c->bpad.resize(nl);
+    for (int li = 1; li < nl; ++li)
+      upload_pad(c->WTp[li], c->bpad[li], weights[li], biases[li], c->nodes[li + 1], c->nodes[li], dtype, st);
     c->tm.resize(n_tm);
     c->tmT.resize(n_tm);
     c->tm_h.assign(n_tm, 0);

@@ -45,10 +45,11 @@ struct InsertIO {
 __global__ void k_region_fp64(NetDev net, RegionIO io);
 
 // Tiled FP64 chain: the planes of a tile of regions in HBM, for the walk kernel.
-#define AM_CT 8                 // regions per chain workgroup
-#define AM_CN (4 * AM_CT)       // chain columns: x, y, z, c of each region
-#define AM_CXS (AM_CN + 2)      // LDS row stride in doubles (16-byte aligned rows)
+#define AM_CT 16                // regions per chain workgroup
+#define AM_CN (4 * AM_CT)       // chain columns (one per lane): x, y, z, c of each region
+#define AM_CR 16                // output rows per wave
 #define AM_CHAIN_MAXW 256       // widest layer the tiled chain takes
+#define AM_CHAIN_THREADS (64 * AM_CHAIN_MAXW / AM_CR)
 
 struct ChainIO {
   const uint32_t* reg_mask;   // [R][Wd]
@@ -57,6 +58,9 @@ struct ChainIO {
   int Cs;                     // stride of one component array in `planes` (>= C)
   double* planes;             // [count][4][Cs]: signed constraint planes, extras appended
   double* eq;                 // [count][4]: output plane with iso subtracted
+  const double* Wp[AM_MAX_LINEAR];  // li >= 1: W_li^T padded, [n_li][noutp[li]] (zero rows past n_out)
+  const double* bp[AM_MAX_LINEAR];  // li >= 1: b_li padded to noutp[li]
+  int noutp[AM_MAX_LINEAR];         // n_out rounded up to AM_CR
 };
 
 __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);

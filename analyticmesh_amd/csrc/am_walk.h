@@ -199,33 +199,62 @@ __device__ void full_sort(const T* Px, const T* Py, const T* Pz, const T* Pc, in
   }
 }
 
-// The first 64 positions of that order, in registers: lane l ends holding
-// position l (sentinel (-inf, INT_MAX) past C).  Each round sorts 64 new
-// constraints across the wave with a bitonic network of shuffles, then keeps
-// the better half of (best, reversed batch) -- a bitonic sequence holding the
-// 64 best of both -- and cleans it.  The comparisons are those of full_sort,
-// so the 64 entries are exactly its first 64.
+// One bitonic step across the wave: the lane keeps the first (keep_first) or
+// the second of its element and its partner's (lane ^ j) in the order.
 template <typename T>
-__device__ void wave_top64(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* p, T& bk, int& bi) {
+__device__ __forceinline__ void wave_cas(T& k, int& i, int j, bool keep_first) {
+  const T pk = __shfl_xor(k, j);
+  const int pi = __shfl_xor(i, j);
+  if (keep_first != key_before(k, i, pk, pi)) { k = pk; i = pi; }
+}
+
+// The first 64 positions of that order, in registers: lane l ends holding
+// position l (sentinel (-inf, INT_MAX) past C).  keys[0, C) and order[0, C)
+// are used as scratch.
+//  1. keys of all constraints into LDS; each lane's best; T0 = the worst of
+//     the 64 lane bests.  Those 64 distinct elements are all >= T0, so the
+//     64th of the whole order is >= T0 too: only elements >= T0 can be among
+//     the first 64 (about a quarter of them for independent keys).
+//  2. the survivors are compacted (ballots, in index order) into order[].
+//  3. rounds over 64 survivors at a time: a bitonic sort of the batch across
+//     the wave, then the better half of (best, reversed batch) -- a bitonic
+//     sequence holding the 64 best of both -- cleaned into order.
+// The comparisons are those of full_sort, so the result is exactly its first 64.
+template <typename T>
+__device__ void wave_top64(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* p, T* keys,
+                           int* order, T& bk, int& bi) {
   const int lane = threadIdx.x;
-  bk = T(-__builtin_inf());
-  bi = 0x7fffffff;
+  T mk = T(-__builtin_inf());
+  int mi = 0x7fffffff;
+  for (int g = lane; g < C; g += 64) {
+    const T k = distance_key<T>(Px, Py, Pz, Pc, g, p);
+    keys[g] = k;
+    if (key_before(k, g, mk, mi)) { mk = k; mi = g; }
+  }
+  // T0: the worst lane best
+  for (int j = 32; j > 0; j >>= 1) {
+    const T pk = __shfl_xor(mk, j);
+    const int pi = __shfl_xor(mi, j);
+    if (key_before(mk, mi, pk, pi)) { mk = pk; mi = pi; }
+  }
+  int S = 0;
   for (int base = 0; base < C; base += 64) {
     const int g = base + lane;
+    const bool keep = g < C && !key_before(mk, mi, keys[g], g);
+    const unsigned long long bal = __ballot(keep);
+    if (keep) order[S + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = g;
+    S += __popcll(bal);
+  }
+  __syncthreads();
+  bk = T(-__builtin_inf());
+  bi = 0x7fffffff;
+  for (int base = 0; base < S; base += 64) {
     T ek = T(-__builtin_inf());
     int ei = 0x7fffffff;
-    if (g < C) { ek = distance_key<T>(Px, Py, Pz, Pc, g, p); ei = g; }
-    // bitonic sort of the batch, descending across lanes
-    for (int k = 2; k <= 64; k <<= 1) {
-      for (int j = k >> 1; j > 0; j >>= 1) {
-        const T pk = __shfl_xor(ek, j);
-        const int pi = __shfl_xor(ei, j);
-        const bool lower = (lane & j) == 0;
-        const bool desc = (lane & k) == 0 || k == 64;
-        const bool mine_first = key_before(ek, ei, pk, pi);
-        if ((lower == desc) != mine_first) { ek = pk; ei = pi; }
-      }
-    }
+    if (base + lane < S) { ei = order[base + lane]; ek = keys[ei]; }
+    for (int k = 2; k <= 64; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1)
+        wave_cas(ek, ei, j, ((lane & j) == 0) == ((lane & k) == 0));
     if (base == 0) {
       bk = ek;
       bi = ei;
@@ -235,14 +264,9 @@ __device__ void wave_top64(const T* Px, const T* Py, const T* Pz, const T* Pc, i
     const T rk = __shfl(ek, 63 - lane);
     const int ri = __shfl(ei, 63 - lane);
     if (!key_before(bk, bi, rk, ri)) { bk = rk; bi = ri; }
-    for (int j = 32; j > 0; j >>= 1) {
-      const T pk = __shfl_xor(bk, j);
-      const int pi = __shfl_xor(bi, j);
-      const bool lower = (lane & j) == 0;
-      const bool mine_first = key_before(bk, bi, pk, pi);
-      if (lower != mine_first) { bk = pk; bi = pi; }
-    }
+    for (int j = 32; j > 0; j >>= 1) wave_cas(bk, bi, j, (lane & j) == 0);
   }
+  __syncthreads();
 }
 
 template <typename T, bool CHECKED>
@@ -267,7 +291,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
   {
     T bk;
     int bi;
-    wave_top64<T>(Px, Py, Pz, Pc, C, p, bk, bi);
+    wave_top64<T>(Px, Py, Pz, Pc, C, p, keys, order, bk, bi);
     keys[lane] = bk;
     order[lane] = bi;
   }
@@ -365,13 +389,17 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
           // distance order, a wave-wide chunk at a time, so that a rejected
           // candidate usually stops after the nearest 64 planes (the verdict
           // does not depend on the order of the tests)
-          // (first the nearest 64, then every constraint by index)
+          // (first the nearest 64, then every constraint by index; one ballot
+          // per four constraints per lane)
           bool viol = false;
-          for (int base = -64; base < C; base += 64) {
-            const int pos = base + lane;
+          for (int base = -64; base < C; base += 256) {
             bool vl = false;
-            const int g = base < 0 ? (lane < C ? order[lane] : C) : pos;
-            if (g < C && g != cur && g != r) vl = violates(Px[g], Py[g], Pz[g], Pc[g], vt);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int pos = base + 64 * u + lane;
+              const int g = pos < 0 ? (lane < C ? order[lane] : C) : pos;
+              if (g < C && g != cur && g != r) vl = vl || violates(Px[g], Py[g], Pz[g], Pc[g], vt);
+            }
             if (__any(vl)) { viol = true; break; }
           }
           if (viol) continue;

@@ -59,19 +59,36 @@ class Faces:
 
 
 class MarchResult:
-    def __init__(self, eng, spec_L, Wd, vm):
+    """Result of one ``Engine.run``.
+
+    The per-region records (``masks``, ``info``, ``tabs``, ``verts``,
+    ``vpairs``, ``points``) stay in device memory until first read; they
+    belong to the context's latest run, so reading them after the context has
+    run again raises RuntimeError."""
+
+    _REGION_FIELDS = ("masks", "info", "tabs", "verts", "vpairs", "points")
+
+    def __init__(self, eng, spec_L, Wd, vm, owner=None, generation=0):
         self._eng = eng
+        self._owner = owner
+        self._generation = generation
+        self._regions = None
         self.L = spec_L
         self.Wd = Wd
         self.vm = vm
-        r = eng.regions(Wd)
-        self.masks = r["masks"]
-        self.info = r["info"]
-        self.tabs = r["tabs"]
-        self.verts = r["verts"]
-        self.vpairs = r["vpairs"]
-        self.points = r["points"]
         self.stats = eng.stats()
+
+    def _check_current(self):
+        if self._owner is not None and self._owner._generation != self._generation:
+            raise RuntimeError("the engine has run again since this result; its region records are gone")
+
+    def __getattr__(self, name):
+        if name in MarchResult._REGION_FIELDS:
+            if self._regions is None:
+                self._check_current()
+                self._regions = self._eng.regions(self.Wd)
+            return self._regions[name]
+        raise AttributeError(name)
 
     def records(self):
         return RegionRecords(self.masks, self.info, self.tabs, self.verts, self.vpairs, L=self.L)
@@ -85,6 +102,7 @@ class MarchResult:
 
         Returns (vertices (V, 3) float64, Faces) where Faces is a list-like view
         over the CSR arrays ``offsets`` (F + 1) and ``indices``."""
+        self._check_current()
         self._eng.combine(float(scale), [float(c) for c in center])
         V, off, idx = self._eng.mesh()
         self.combine_ms = float(self._eng.stats()[10])
@@ -95,6 +113,7 @@ class MarchResult:
 
         grad_vertices: (V, 3) array in the order of ``combine()``'s vertices.
         layer_shapes: [(out, in), ...] per linear layer.  Returns (dW list, db list), float64."""
+        self._check_current()
         g = np.ascontiguousarray(grad_vertices, dtype=np.float64)
         dW = [np.zeros(s, dtype=np.float64) for s in layer_shapes]
         db = [np.zeros(s[0], dtype=np.float64) for s in layer_shapes]
@@ -102,6 +121,7 @@ class MarchResult:
         return dW, db
 
     def export_ply(self, path, is_polymesh=True, is_float32=True):
+        self._check_current()
         self._eng.export_ply(str(path), bool(is_polymesh), bool(is_float32))
 
 
@@ -131,6 +151,7 @@ class Engine:
         if table_log2 is not None:
             self._eng.set_option("table_log2", int(table_log2))
         self._eng.set_option("walk", {"ray": 0, "scan": 1}[walk])
+        self._generation = 0
 
     def set_option(self, key, value):
         self._eng.set_option(key, int(value))
@@ -177,7 +198,8 @@ class Engine:
         self._eng.run(wp, bp, tm_shapes, tp, dtype, _as_ptr(pts), _as_ptr(st), n,
                       _as_ptr(ew) if ew is not None else 0, _as_ptr(eb) if eb is not None else 0,
                       float(iso), bool(flip), int(stream))
-        return MarchResult(self._eng, self.L, self.Wd, self.vm)
+        self._generation += 1
+        return MarchResult(self._eng, self.L, self.Wd, self.vm, owner=self, generation=self._generation)
 
 
 def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip=False, max_regions=None,
