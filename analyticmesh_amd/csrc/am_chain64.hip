@@ -26,7 +26,6 @@
 
 namespace am {
 
-#define AM_CPF 4   // k steps of operands in flight
 
 // acc[j] = fma(w from lane j of this lane's 16-lane row, x, acc[j]), j < 16:
 // v_fmac_f64 with a row_newbcast DPP source is one fused multiply-add per
@@ -125,15 +124,19 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       for (int u = 0; u < AM_CPF; ++u) {
         wq[u] = wcol[(size_t)u * noutp];
         xq[u] = Xs[u * AM_CN + lane];
+        __builtin_amdgcn_sched_barrier(0);
       }
       for (int k = 0; k < ninp; k += AM_CPF) {
 #pragma unroll
         for (int u = 0; u < AM_CPF; ++u) {
           chain_step16(acc, wq[u], xq[u]);
-          // refill the slot just used, AM_CPF steps ahead
+          // refill the slot just used, AM_CPF steps ahead; the scheduling
+          // barrier keeps the loads in slot order, so each step waits only
+          // for its own (oldest) load
           const int kn = min(k + u + AM_CPF, ninp - 1);
           wq[u] = wcol[(size_t)kn * noutp];
           xq[u] = Xs[kn * AM_CN + lane];
+          __builtin_amdgcn_sched_barrier(0);
         }
       }
       if (li < net.D && col_live) {
@@ -164,8 +167,8 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   }
 }
 
-__global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int rid0, const double* planes,
-                                                  const double* eqb, int Cs_hbm) {
+__global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* slots,
+                                                  const double* planes, const double* eqb, int Cs_hbm) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
   const int Cs = io.Cs;
@@ -181,9 +184,45 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
 
   const int lane = threadIdx.x;
   if (blockIdx.x >= io.count) return;
-  const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
+  const int sl = slots ? slots[blockIdx.x] : blockIdx.x;     // planes slot of this region
+  const int rid = io.ids ? io.ids[sl] : rid0 + sl;
   for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
+  const double* src = planes + (size_t)sl * 4 * Cs_hbm;
+  for (int g = lane; g < C; g += 64) {
+    Px[g] = src[g];
+    Py[g] = src[Cs_hbm + g];
+    Pz[g] = src[2 * (size_t)Cs_hbm + g];
+    Pc[g] = src[3 * (size_t)Cs_hbm + g];
+  }
+  double p[3];
+  p[0] = io.reg_point[(size_t)rid * 3 + 0];
+  p[1] = io.reg_point[(size_t)rid * 3 + 1];
+  p[2] = io.reg_point[(size_t)rid * 3 + 2];
+  const double eq[4] = {eqb[(size_t)sl * 4 + 0], eqb[(size_t)sl * 4 + 1], eqb[(size_t)sl * 4 + 2],
+                        eqb[(size_t)sl * 4 + 3]};
+  __syncthreads();
+  const WalkParams wp{0.f, 0.f, 0.f};
+  walk_region<double, false>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
+}
+
+// Checked ray-shooting walk on the FP64 planes of k_chain_tile_fp64.  The
+// planes are the canonical FP64 planes, so the only uncertainty is the
+// rounding of the walk's own evaluations; regions it cannot certify are
+// flagged (io.flags, level-relative) for the candidate scan.
+__global__ __launch_bounds__(64) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
+                                                   const double* planes, const double* eqb, int Cs_hbm) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int C = net.C;
+  double* Px = reinterpret_cast<double*>(smem_raw);
+  double* Py = Px + C;
+  double* Pz = Py + C;
+  double* Pc = Pz + C;
+  uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + C);
+  const int lane = threadIdx.x;
+  if (blockIdx.x >= io.count) return;
+  const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
+  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)blockIdx.x * 4 * Cs_hbm;
   for (int g = lane; g < C; g += 64) {
     Px[g] = src[g];
@@ -198,8 +237,22 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
   const double eq[4] = {eqb[(size_t)blockIdx.x * 4 + 0], eqb[(size_t)blockIdx.x * 4 + 1],
                         eqb[(size_t)blockIdx.x * 4 + 2], eqb[(size_t)blockIdx.x * 4 + 3]};
   __syncthreads();
-  const WalkParams wp{0.f, 0.f, 0.f};
-  walk_region<double, false>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
+  walk_ray<double>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+}
+
+// Chunk-local slots of the regions k_walk_ray64 left undecided: region of
+// slot b is ids[b] or rid0 + b; its flag is level-relative.
+__global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
+                                int* slots) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int rid = ids ? ids[b] : rid0 + b;
+  const unsigned f = flags[rid - lvl_first];
+  if (f & AM_UNDECIDED) {
+    slots[atomicAdd(cnt, 1)] = b;
+    for (int w = 0; w < AM_WHY_COUNT; ++w)
+      if (f & AM_WHY_BIT(w)) atomicAdd(cnt + 1 + w, 1);
+  }
 }
 
 }  // namespace am

@@ -128,6 +128,7 @@ struct am_ctx {
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
   DevBuf<double> tplanes, teq;
+  DevBuf<int> tslots, tcnt;               // undecided slots of a chunk, their count (+ reasons)
   std::vector<DevBuf<double>> WTp, bpad;   // W_li^T and b_li with n_out padded to AM_CR (zeros)
 
   double stats[AM_STAT_COUNT] = {0};
@@ -273,33 +274,52 @@ size_t chain_lds_bytes(const am_ctx* c) {
   return (size_t)AM_CHAIN_MAXW * AM_CN * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
 }
 
+// Per-launch accounting of the FP64 stages.
+struct Fp64Times {
+  float chain_ms = 0, ray_ms = 0, scan_ms = 0;
+  long long scanned = 0, ray_undecided = 0;
+  long long why[AM_WHY_COUNT] = {0};
+};
+
+// Checked FP64 ray walk: planes are exact FP64, so eps only covers the walk's
+// own rounding (the vertex solve's share is added per system).
+const WalkParams kRay64{1e-13f, 1e-12f, 1e6f};
+
+size_t ray64_lds_bytes(const am_ctx* c) { return (size_t)4 * c->C * sizeof(double) + (size_t)c->Wd * 4 + 64; }
+
 // FP64 planes + walk of `count` regions (region ids first.., or ids[0..count)),
-// through the tiled chain and the walk kernel in chunks; candidate slots are
-// numbered from `lvl_first`.  Falls back to the one-region kernel where the
-// tiled chain does not apply.
-void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, float* chain_ms,
-                      float* walk_ms) {
-  *chain_ms = 0;
-  *walk_ms = 0;
+// through the tiled chain in chunks; candidate slots and flags are numbered
+// from `lvl_first`.  mode 0: the candidate scan on every region (the FP64
+// records of the CPU restatement, bit for bit); mode 1: the checked FP64 ray
+// walk, and the scan on the regions it leaves undecided, reusing their planes.
+// Falls back to the one-region kernel where the tiled chain does not apply.
+void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Times& t) {
   if (!c->tile_ok) {
-    // one region per workgroup: the chain and the walk in one kernel
+    // one region per workgroup: the chain and the candidate scan in one kernel
     float ms = 0;
     if (ids) {
       launch_region_fp64(c, lvl_first, count, &ms, ids);
     } else {
       launch_region_fp64(c, first, count, &ms);
     }
-    *walk_ms = ms;
+    t.scan_ms += ms;
+    t.scanned += count;
     return;
   }
+  if (mode == 1 && ray64_lds_bytes(c) > c->max_lds) mode = 0;
   int Cs, Csort;
   const size_t wlds = region_lds_bytes(c, Cs, Csort);
+  const size_t rlds = ray64_lds_bytes(c);
   const int Ch = (c->C + 63) & ~63;
   size_t chunk = ((size_t)1 << 30) / ((size_t)32 * Ch);
   chunk = std::max<size_t>(1024, std::min<size_t>(chunk, 65536)) & ~(size_t)(AM_CT - 1);
   const size_t cap = std::min<size_t>(chunk, (size_t)count);
   ensure(c->tplanes, cap * 4 * Ch, false, c->stream);
   ensure(c->teq, cap * 4, false, c->stream);
+  if (mode == 1) {
+    ensure(c->tslots, cap, false, c->stream);
+    ensure(c->tcnt, 1 + AM_WHY_COUNT, false, c->stream);
+  }
   const size_t clds = chain_lds_bytes(c);
   for (int off = 0; off < count; off += (int)chunk) {
     const int n = std::min<int>((int)chunk, count - off);
@@ -331,23 +351,59 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.count = n;
     io.Cs = Cs;
     io.Csort = Csort;
-    io.flags = nullptr;
+    io.flags = mode == 1 ? c->rflags.p : nullptr;
     io.ids = ids ? ids + off : nullptr;
     HIPCHK(hipEventRecord(c->ev0, c->stream));
     hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
                        c->net, cio);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev1, c->stream));
-    hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, c->tplanes.p,
-                       c->teq.p, Ch);
+    if (mode == 0) {
+      hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, (const int*)nullptr,
+                         (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->ev2, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev2));
+      float a = 0, b = 0;
+      HIPCHK(hipEventElapsedTime(&a, c->ev0, c->ev1));
+      HIPCHK(hipEventElapsedTime(&b, c->ev1, c->ev2));
+      t.chain_ms += a;
+      t.scan_ms += b;
+      t.scanned += n;
+      continue;
+    }
+    hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64), rlds, c->stream, c->net, io, first + off, kRay64,
+                       (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev2, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev2));
+    HIPCHK(hipMemsetAsync(c->tcnt.p, 0, sizeof(int) * (1 + AM_WHY_COUNT), c->stream));
+    hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const unsigned*)c->rflags.p,
+                       n, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p, c->tslots.p);
+    HIPCHK(hipGetLastError());
+    int hc[1 + AM_WHY_COUNT] = {0};
+    HIPCHK(hipMemcpyAsync(hc, c->tcnt.p, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
     float a = 0, b = 0;
     HIPCHK(hipEventElapsedTime(&a, c->ev0, c->ev1));
     HIPCHK(hipEventElapsedTime(&b, c->ev1, c->ev2));
-    *chain_ms += a;
-    *walk_ms += b;
+    t.chain_ms += a;
+    t.ray_ms += b;
+    t.ray_undecided += hc[0];
+    for (int w = 0; w < AM_WHY_COUNT; ++w) t.why[w] += hc[1 + w];
+    if (hc[0] > 0) {
+      io.count = hc[0];
+      io.flags = nullptr;
+      HIPCHK(hipEventRecord(c->ev0, c->stream));
+      hipLaunchKernelGGL(k_walk_fp64, dim3(hc[0]), dim3(64), wlds, c->stream, c->net, io, first + off,
+                         (const int*)c->tslots.p, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipEventRecord(c->ev1, c->stream));
+      HIPCHK(hipEventSynchronize(c->ev1));
+      float sm = 0;
+      HIPCHK(hipEventElapsedTime(&sm, c->ev0, c->ev1));
+      t.scan_ms += sm;
+      t.scanned += hc[0];
+    }
   }
 }
 
@@ -382,9 +438,9 @@ void upload_T(DevBuf<double>& dst, const void* src, int rows, int cols, int dtyp
   HIPCHK(hipMemcpy(dst.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
 }
 
-// W (rows x cols, row-major) -> W^T padded: [round_up(cols, 4)][round_up(rows, AM_CR)],
+// W (rows x cols, row-major) -> W^T padded: [round_up(cols, AM_CPF)][round_up(rows, AM_CR)],
 // zeros past `rows` and `cols`; b -> b padded to round_up(rows, AM_CR).  The tiled
-// chain reads whole AM_CR-row blocks and k in steps of 4.
+// chain reads whole AM_CR-row blocks and k in steps of AM_CPF.
 void upload_pad(DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const void* b, int rows, int cols,
                 int dtype, hipStream_t st) {
   const int rp = (rows + AM_CR - 1) / AM_CR * AM_CR;
@@ -400,7 +456,7 @@ void upload_pad(DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const
     for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
     for (int i = 0; i < rows; ++i) bh[i] = (double)fb[i];
   }
-  const int cp = (cols + 3) / 4 * 4;
+  const int cp = (cols + AM_CPF - 1) / AM_CPF * AM_CPF;
   std::vector<double> t((size_t)cp * rp, 0.0), bp(rp, 0.0);
   for (int r = 0; r < rows; ++r) {
     for (int q = 0; q < cols; ++q) t[(size_t)q * rp + r] = h[(size_t)r * cols + q];
@@ -460,7 +516,7 @@ am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_
     return AM_ERR_ARG;
   }
   if (precision != AM_PREC_FP32 && precision != AM_PREC_SPLIT_BF16X3 && precision != AM_PREC_FP64 &&
-      precision != AM_PREC_FP64_MFMA) {
+      precision != AM_PREC_FP64_RAY) {
     g_create_error = "unknown precision";
     return AM_ERR_ARG;
   }
@@ -661,7 +717,11 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)clds));
     }
-    if (c->precision != AM_PREC_FP64) fast_prepare(c->fast, c->fbuf, n, c->precision, weights, biases, dtype, st);
+    const bool fp64_form = c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY;
+    if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, weights, biases, dtype, st);
+    if (c->tile_ok && ray64_lds_bytes(c) > 65536 && ray64_lds_bytes(c) <= c->max_lds)
+      HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ray64_lds_bytes(c)));
 
     // ---- seeds -> first level (backend/inc/states.h:147-176)
     c->R = 0;
@@ -690,21 +750,29 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     }
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
-    double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0, t_chain64 = 0, t_walk64 = 0;
+    double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0, t_chain64 = 0, t_walk64 = 0, t_ray64 = 0;
+    long long scanned = 0, ray_undecided = 0;
     long long levels = 0, cands = 0, redo = 0, launches = 0;
+    long long why[AM_WHY_COUNT] = {0};
     while (lvl_count > 0) {
       if (c->max_regions >= 0 && lvl_first >= c->max_regions) break;
       const size_t ncand = (size_t)lvl_count * c->vm;
       grow_cands(c, ncand);
       float ms = 0;
-      if (c->precision == AM_PREC_FP64) {
-        float cm = 0, wm = 0;
-        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr, &cm, &wm);
-        t_region += cm + wm;
-        t_gemm += cm;
-        t_walk += wm;
-        t_chain64 += cm;
-        t_walk64 += wm;
+      ensure(c->rflags, (size_t)lvl_count, false, st);
+      if (fp64_form) {
+        Fp64Times ft;
+        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr,
+                         c->precision == AM_PREC_FP64 ? 0 : 1, ft);
+        t_region += ft.chain_ms + ft.ray_ms + ft.scan_ms;
+        t_gemm += ft.chain_ms;
+        t_walk += ft.ray_ms + ft.scan_ms;
+        t_chain64 += ft.chain_ms;
+        t_walk64 += ft.scan_ms;
+        t_ray64 += ft.ray_ms;
+        scanned += ft.scanned;
+        ray_undecided += ft.ray_undecided;
+        for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
         launches += 1;
       } else {
         FastLevel fl;
@@ -718,17 +786,22 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         int* ids = nullptr;
         const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->ev0, c->ev1, ft, &ids);
         if (nredo > 0) {
-          // FP64 re-decision of the undecided regions: rerun them through the
-          // FP64 kernel in place (their records and candidate slots are rewritten).
-          float cm = 0, wm = 0;
-          run_fp64_regions(c, (int)lvl_first, (int)lvl_first, nredo, ids, &cm, &wm);
-          t_redo += cm + wm;
-          t_chain64 += cm;
-          t_walk64 += wm;
+          // FP64 re-decision of the undecided regions, in place (their records
+          // and candidate slots are rewritten): FP64 planes, the checked FP64
+          // ray walk, and the candidate scan where that is undecided too.
+          Fp64Times f64;
+          run_fp64_regions(c, (int)lvl_first, (int)lvl_first, nredo, ids, 1, f64);
+          t_redo += f64.chain_ms + f64.ray_ms + f64.scan_ms;
+          t_chain64 += f64.chain_ms;
+          t_walk64 += f64.scan_ms;
+          t_ray64 += f64.ray_ms;
+          scanned += f64.scanned;
+          ray_undecided += f64.ray_undecided;
           redo += nredo;
         }
         t_gemm += ft.gemm_ms;
         t_walk += ft.walk_ms;
+        for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
         t_region += ft.gemm_ms + ft.walk_ms;
         launches += ft.launches;
       }
@@ -763,6 +836,10 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_REGION_LAUNCHES] = (double)launches;
     c->stats[AM_STAT_CHAIN64_MS] = t_chain64;
     c->stats[AM_STAT_WALK64_MS] = t_walk64;
+    c->stats[AM_STAT_RAY64_MS] = t_ray64;
+    c->stats[AM_STAT_SCAN64_REGIONS] = (double)scanned;
+    c->stats[AM_STAT_RAY64_UNDECIDED] = (double)ray_undecided;
+    for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;
     return AM_OK;
   });
@@ -887,7 +964,7 @@ am_status am_get_regions(am_ctx* c, uint32_t* masks, int* info, int* tabs, doubl
 
 am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, float* eq) {
   if (!c || !masks || !planes || !eq || n < 1) return AM_ERR_ARG;
-  if (!c->has_run || c->precision == AM_PREC_FP64)
+  if (!c->has_run || c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY)
     return fail(c, AM_ERR_STATE, "am_planes needs a reduced-precision context after am_run");
   return guarded(c, [&]() -> am_status {
     const hipStream_t st = c->stream;

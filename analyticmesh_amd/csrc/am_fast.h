@@ -27,15 +27,12 @@ struct FastNetDev {
   const float* Wp[AM_MAX_LINEAR];       // fp32 A operands, [nrb][nkb][64 lanes]
   const uint16_t* Wh[AM_MAX_LINEAR];    // bf16 hi A operands, [nrb][nkb][64 lanes][8]
   const uint16_t* Wl[AM_MAX_LINEAR];    // bf16 lo A operands
-  const double* Wp64[AM_MAX_LINEAR];    // fp64 A operands, [nrb][nkb][64 lanes]
-  const double* W0d;                    // (n1, 3) fp64
-  const double* biasd[AM_MAX_LINEAR];
   const float* ew;
   const float* eb;
   float iso;
 };
 
-// Forms: 0 FP32 MFMA, 1 split-BF16 MFMA, 3 FP64 MFMA (planes rounded to FP32 for the walk).
+// Forms: 0 FP32 MFMA, 1 split-BF16 MFMA.
 struct FastNet {
   int precision = 0;
   int walk_mode = 0;            // 0: ray-shooting walk, 1: the reference's candidate scan (checked)
@@ -66,6 +63,7 @@ struct FastLevel {
 struct FastTimes {
   float gemm_ms, walk_ms;
   int launches;
+  long long why[AM_WHY_COUNT];   // undecided regions per reason (AM_WHY_*), accumulated
 };
 
 // Upload weights in MFMA operand order; `weights`/`biases` as given to am_run.

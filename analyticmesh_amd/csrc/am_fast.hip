@@ -195,114 +195,6 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   }
 }
 
-typedef double f64x4 __attribute__((ext_vector_type(4)));
-#define AM_XS64 (AM_GN + 2)   // fp64 LDS row stride (doubles)
-
-// FP64 face kernel: the same chain on v_mfma_f64_16x16x4f64.  Accumulators
-// stay in registers through the layer, so one LDS buffer holds the layer's
-// input, then its output (staged for coalesced HBM writes), then the next
-// layer's masked input.  Planes are written as FP32 for the walk; the FP32
-// rounding of FP64-accurate planes is a uniform 2^-24 relative error.
-// Widths up to 256 (16 row blocks: four per wave).
-__global__ __launch_bounds__(256) void k_gemm_f64(FastNetDev f, GemmIO io) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  double* Xs = reinterpret_cast<double*>(smem_raw);                           // [Kpad][AM_XS64]
-  uint32_t* mk = reinterpret_cast<uint32_t*>(Xs + (size_t)f.Kpad * AM_XS64);
-  const int r0 = blockIdx.x * AM_GT;
-  const int nreg = min(AM_GT, io.count - r0);
-  for (int idx = tid; idx < AM_GT * f.Wd; idx += 256) {
-    const int t = idx / f.Wd, w = idx - t * f.Wd;
-    mk[idx] = t < nreg ? io.reg_mask[(size_t)(io.first + r0 + t) * f.Wd + w] : 0u;
-  }
-  for (int i = tid; i < f.Kpad * AM_XS64; i += 256) Xs[i] = 0.0;
-  __syncthreads();
-  const int n1 = f.nodes[1];
-  for (int idx = tid; idx < n1 * AM_GN; idx += 256) {
-    const int k = idx / AM_GN, col = idx - k * AM_GN, t = col >> 2, q = col & 3;
-    const double val = q < 3 ? f.W0d[k * 3 + q] : f.biasd[0][k];
-    Xs[k * AM_XS64 + col] = mbit(mk + t * f.Wd, k) ? val : 0.0;
-  }
-  for (int idx = tid; idx < AM_GN * n1; idx += 256) {
-    const int col = idx / n1, k = idx - col * n1, t = col >> 2, q = col & 3;
-    if (t < nreg) {
-      const double val = q < 3 ? f.W0d[k * 3 + q] : f.biasd[0][k];
-      io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + k] = (float)(mbit(mk + t * f.Wd, k) ? -val : val);
-    }
-  }
-  for (int idx = tid; idx < AM_GN * f.E; idx += 256) {
-    const int col = idx / f.E, e = idx - col * f.E, t = col >> 2, q = col & 3;
-    if (t < nreg) io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + f.L + e] = q < 3 ? f.ew[e * 3 + q] : f.eb[e];
-  }
-  __syncthreads();
-  for (int li = 1; li <= f.D; ++li) {
-    const int nout = f.nodes[li + 1];
-    const int nrb = f.nrb[li], nkb = f.nkb[li];
-    f64x4 acc[4][2];
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int c = 0; c < 2; ++c) acc[j][c] = f64x4{0.0, 0.0, 0.0, 0.0};
-    const double* xb = Xs + (lane >> 4) * AM_XS64 + (lane & 15);
-    for (int kb = 0; kb < nkb; ++kb) {
-      const double b0 = xb[(kb * 4) * AM_XS64];
-      const double b1 = xb[(kb * 4) * AM_XS64 + 16];
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int rb = wave + 4 * j;
-        if (rb < nrb) {
-          const double a = f.Wp64[li][((size_t)rb * nkb + kb) * 64 + lane];
-          acc[j][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b0, acc[j][0], 0, 0, 0);
-          acc[j][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b1, acc[j][1], 0, 0, 0);
-        }
-      }
-    }
-    __syncthreads();   // every wave is done reading this layer's input
-    const double* bias = f.biasd[li];
-    const int c0 = lane & 15;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int rb = wave + 4 * j;
-      if (rb >= nrb) continue;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        // f64 16x16x4 result layout: element i of lane l is D[(l >> 4) + 4 i][l & 15]
-        const int row = rb * 16 + (lane >> 4) + 4 * i;
-        if (row < nout) {
-          const double b = bias[row];
-          Xs[row * AM_XS64 + c0] = acc[j][0][i] + ((c0 & 3) == 3 ? b : 0.0);
-          Xs[row * AM_XS64 + 16 + c0] = acc[j][1][i] + (((16 + c0) & 3) == 3 ? b : 0.0);
-        }
-      }
-    }
-    __syncthreads();
-    if (li < f.D) {
-      const int out0 = f.off[li];
-      for (int idx = tid; idx < AM_GN * nout; idx += 256) {
-        const int col = idx / nout, j = idx - col * nout, t = col >> 2, q = col & 3;
-        if (t < nreg) {
-          const double y = Xs[j * AM_XS64 + col];
-          io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + out0 + j] = (float)(mbit(mk + t * f.Wd, out0 + j) ? -y : y);
-        }
-      }
-      __syncthreads();
-      const int rows = (nout + 3) & ~3;
-      for (int idx = tid; idx < rows * AM_GN; idx += 256) {
-        const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2;
-        if (!(j < nout && mbit(mk + t * f.Wd, out0 + j))) Xs[j * AM_XS64 + col] = 0.0;
-      }
-    } else if (tid < AM_GN) {
-      const int t = tid >> 2, q = tid & 3;
-      if (t < nreg) {
-        double v = Xs[tid];
-        if (q == 3) v = v - (double)f.iso;
-        io.eq[(size_t)(r0 + t) * 4 + q] = (float)v;
-      }
-    }
-    __syncthreads();
-  }
-}
-
 // Checked FP32 walk of one region whose planes the face kernel wrote.
 __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
                                                  const float* eqbuf, int Cs_hbm, int chunk_first) {
@@ -367,9 +259,15 @@ __global__ __launch_bounds__(64) void k_walk_ray_f32(NetDev net, RegionIO io, Wa
   walk_ray<float>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
 }
 
+// Compact the undecided regions' ids; cnt[0] = their number, cnt[1 + w] = how
+// many were left for reason w (a region can carry one reason per walk exit).
 __global__ void k_collect_undecided(const unsigned* flags, int n, int first, int* cnt, int* ids) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && (flags[i] & AM_UNDECIDED)) ids[atomicAdd(cnt, 1)] = first + i;
+  if (i < n && (flags[i] & AM_UNDECIDED)) {
+    ids[atomicAdd(cnt, 1)] = first + i;
+    for (int w = 0; w < AM_WHY_COUNT; ++w)
+      if (flags[i] & AM_WHY_BIT(w)) atomicAdd(cnt + 1 + w, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -428,7 +326,6 @@ float host_bf16_f(uint16_t h) {
 }
 
 size_t gemm_lds(const FastNetDev& d, int prec) {
-  if (prec == 2) return (size_t)d.Kpad * AM_XS64 * 8 + (size_t)AM_GT * d.Wd * 4 + 16;
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
   const size_t xs = prec == 0 ? (size_t)d.Kpad * AM_XS * 4 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;
   return ys + xs + (size_t)AM_GT * d.Wd * 4 + 16;
@@ -454,8 +351,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   for (int i = 0; i <= net.D; ++i) d.off[i] = net.off[i];
   d.Kpad = (maxw + 31) & ~31;
   d.iso = (float)net.iso;
-  const int prec = precision == 1 ? 1 : (precision == 3 ? 2 : 0);
-  if (prec == 2 && maxw > 256) throw std::runtime_error("the FP64-MFMA form supports widths up to 256");
+  const int prec = precision == 1 ? 1 : 0;
   // W0, biases, extras
   {
     auto w0 = to_float(weights[0], (size_t)net.nodes[1] * 3, dtype);
@@ -468,11 +364,6 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
     float* p = (float*)slot(b, S_BIAS0 + li, bb.size() * 4);
     FCHK(hipMemcpy(p, bb.data(), bb.size() * 4, hipMemcpyHostToDevice));
     d.bias[li] = p;
-  }
-  if (prec == 2) {
-    // FP64 copies of W0 and the biases: the engine's own FP64 buffers
-    d.W0d = net.W0;
-    for (int li = 0; li <= net.D; ++li) d.biasd[li] = net.b[li];
   }
   {
     std::vector<float> ew(std::max(1, net.E * 3)), eb(std::max(1, net.E));
@@ -497,22 +388,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
     auto w = to_float(weights[li], (size_t)nout * nin, dtype);
     const int nrb = (nout + 15) / 16;
     d.nrb[li] = nrb;
-    if (prec == 2) {
-      const int nkb = (nin + 3) / 4;
-      d.nkb[li] = nkb;
-      std::vector<double> wd((size_t)nout * nin);
-      FCHK(hipMemcpy(wd.data(), net.Wrow[li], wd.size() * 8, hipMemcpyDeviceToHost));
-      std::vector<double> a((size_t)nrb * nkb * 64, 0.0);
-      for (int rb = 0; rb < nrb; ++rb)
-        for (int kb = 0; kb < nkb; ++kb)
-          for (int l = 0; l < 64; ++l) {
-            const int row = rb * 16 + (l & 15), k = kb * 4 + (l >> 4);
-            if (row < nout && k < nin) a[((size_t)rb * nkb + kb) * 64 + l] = wd[(size_t)row * nin + k];
-          }
-      double* p = (double*)slot(b, S_WP0 + li, a.size() * 8);
-      FCHK(hipMemcpy(p, a.data(), a.size() * 8, hipMemcpyHostToDevice));
-      d.Wp64[li] = p;
-    } else if (prec == 0) {
+    if (prec == 0) {
       const int nkb = (nin + 3) / 4;
       d.nkb[li] = nkb;
       std::vector<float> a((size_t)nrb * nkb * 64, 0.f);
@@ -549,9 +425,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   }
   const size_t lds = gemm_lds(d, prec);
   if (lds > 65536) {
-    if (prec == 2)
-      FCHK(hipFuncSetAttribute((const void*)k_gemm_f64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    else if (prec == 0)
+    if (prec == 0)
       FCHK(hipFuncSetAttribute((const void*)k_gemm<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     else
       FCHK(hipFuncSetAttribute((const void*)k_gemm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -561,15 +435,14 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   // coefficient): about 3x the largest error measured against FP64 planes on
   // the 4x256 and 2x64 networks (FP32: 2.6e-5, split-BF16: 5.1e-4).
   if (prec == 0) f.wp = WalkParams{8e-5f, 1e-3f, 1e3f};
-  else if (prec == 1) f.wp = WalkParams{2e-3f, 2e-2f, 1e2f};
-  else f.wp = WalkParams{6e-7f, 1e-3f, 1e3f};   // FP32 rounding of FP64-accurate planes (2^-24) and FP32 evaluation
+  else f.wp = WalkParams{2e-3f, 2e-2f, 1e2f};
   f.ready = true;
 }
 
 int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
                hipEvent_t e0, hipEvent_t e1, FastTimes& t, int** ids) {
   const FastNetDev& d = f.dev;
-  const int prec = f.precision == 1 ? 1 : (f.precision == 3 ? 2 : 0);
+  const int prec = f.precision == 1 ? 1 : 0;
   t.gemm_ms = t.walk_ms = 0;
   t.launches = 0;
   const int CH = 1 << 16;
@@ -578,7 +451,7 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
   float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * 4 * 4);
   unsigned* flags = (unsigned*)slot(b, S_FLAGS, (size_t)lvl.count * 4);
   int* idl = (int*)slot(b, S_IDS, (size_t)lvl.count * 4);
-  int* cnt = (int*)slot(b, S_CNT, 16);
+  int* cnt = (int*)slot(b, S_CNT, 4 * (1 + AM_WHY_COUNT));
 
   int Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
   while (Csort < net.C) Csort <<= 1;
@@ -601,9 +474,7 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     g.eq = eqb;
     float ms = 0;
     FCHK(hipEventRecord(e0, st));
-    if (prec == 2)
-      hipLaunchKernelGGL(k_gemm_f64, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
-    else if (prec == 0)
+    if (prec == 0)
       hipLaunchKernelGGL(k_gemm<0>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
     else
       hipLaunchKernelGGL(k_gemm<1>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
@@ -644,19 +515,21 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     t.walk_ms += ms;
     t.launches += 1;
   }
-  FCHK(hipMemsetAsync(cnt, 0, 4, st));
+  FCHK(hipMemsetAsync(cnt, 0, 4 * (1 + AM_WHY_COUNT), st));
   hipLaunchKernelGGL(k_collect_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count,
                      lvl.first, cnt, idl);
   FCHK(hipGetLastError());
-  int h = 0;
-  FCHK(hipMemcpyAsync(&h, cnt, 4, hipMemcpyDeviceToHost, st));
+  int hc[1 + AM_WHY_COUNT] = {0};
+  FCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
   FCHK(hipStreamSynchronize(st));
+  for (int w = 0; w < AM_WHY_COUNT; ++w) t.why[w] += hc[1 + w];
+  const int h = hc[0];
   *ids = idl;
   return h;
 }
 
 void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st) {
-  const int prec = f.precision == 1 ? 1 : (f.precision == 3 ? 2 : 0);
+  const int prec = f.precision == 1 ? 1 : 0;
   GemmIO g;
   g.reg_mask = reg_mask;
   g.first = 0;
@@ -664,9 +537,7 @@ void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, flo
   g.planes = planes;
   g.eq = eq;
   const size_t glds = gemm_lds(f.dev, prec);
-  if (prec == 2)
-    hipLaunchKernelGGL(k_gemm_f64, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, f.dev, g);
-  else if (prec == 0)
+  if (prec == 0)
     hipLaunchKernelGGL(k_gemm<0>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, f.dev, g);
   else
     hipLaunchKernelGGL(k_gemm<1>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, f.dev, g);

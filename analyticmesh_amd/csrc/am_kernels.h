@@ -49,6 +49,7 @@ __global__ void k_region_fp64(NetDev net, RegionIO io);
 #define AM_CN (4 * AM_CT)       // chain columns (one per lane): x, y, z, c of each region
 #define AM_CR 16                // output rows per wave
 #define AM_CHAIN_MAXW 256       // widest layer the tiled chain takes
+#define AM_CPF 16               // k steps of chain operands in flight (k is padded to a multiple)
 #define AM_CHAIN_THREADS (64 * AM_CHAIN_MAXW / AM_CR)
 
 struct ChainIO {
@@ -64,9 +65,15 @@ struct ChainIO {
 };
 
 __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
-// The walk of k_region_fp64 on planes from k_chain_tile_fp64 (blockIdx.x indexes both).
-// Region of block b: io.ids[b] when ids are given, else rid0 + b; candidate slots
-// are numbered from io.first as in k_region_fp64.
-__global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const double* planes, const double* eq, int Cs_hbm);
+// The walk of k_region_fp64 on planes from k_chain_tile_fp64.  Block b takes
+// planes slot s = slots[b] (or b); its region is io.ids[s] when ids are given,
+// else rid0 + s; candidate slots are numbered from io.first as in k_region_fp64.
+__global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* slots, const double* planes,
+                            const double* eq, int Cs_hbm);
+struct WalkParams;
+__global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, const double* planes,
+                             const double* eq, int Cs_hbm);
+__global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
+                                int* slots);
 
 }  // namespace am

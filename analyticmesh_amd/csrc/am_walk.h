@@ -18,6 +18,21 @@
 namespace am {
 
 #define AM_UNDECIDED 0x1u
+// Why a reduced-precision walk left its region to FP64 (flags bit 1 + reason index).
+#define AM_WHY_SEED 0       // seed: no plane bounds the seed's shot
+#define AM_WHY_START 1      // start edge interval empty or not finite
+#define AM_WHY_TAB 2        // proposed plane already in the polygon
+#define AM_WHY_UNSURE 3     // a constraint's sign at the new vertex within its bound
+#define AM_WHY_VIOL 4       // a constraint certainly violated at the proposed vertex
+#define AM_WHY_HMAX 5       // vertex system too ill-conditioned to trust
+#define AM_WHY_TURN 6       // turn direction within its bound
+#define AM_WHY_TRUNC 7      // polygon reached the vertex cap
+#define AM_WHY_NEXT 8       // no plane bounds the next edge
+#define AM_WHY_ORIENT 9     // orientation within its bound
+#define AM_WHY_SCAN 10      // candidate-scan walk (start search / scan) undecided
+#define AM_WHY_SAME 11      // new vertex within the FP64 same-vertex distance of the last one
+#define AM_WHY_COUNT 12
+#define AM_WHY_BIT(w) (1u << (1 + (w)))
 
 struct WalkParams {
   float eps;        // relative error bound of the planes (CHECKED only)
@@ -34,11 +49,78 @@ __device__ __forceinline__ T hadamard_ratio(const T* p1, const T* p2, const T* p
   return (n1 * n2 * n3) / absT(d);
 }
 
+// Error bound of constraint signs at the vertex v of the system [p1; p2; p3]
+// when every plane's coefficients carry an error of at most eps times its
+// largest coefficient.  Perturbing the system by E moves the vertex by
+// dv = -A^-1 (E v + e) to first order, so constraint g's sign moves by
+// lambda_g . (E v + e) with lambda_g = A^-T a_g; its rows are the cross
+// products of the system's normals over the determinant.  The bound for g is
+//   eps m_g + kappa eps sum_i |lambda_g,i| m_i,   m_x = max|P_x| (1 + |v|_1),
+// where kappa = 1 / (1 - 2 rho), rho = eps (1 + H) (H the Hadamard ratio),
+// covers the second-order term; systems with rho > 1/4 are not trusted.
+// eps includes the working-precision rounding of the vertex solve.
+template <typename T>
+struct VertexBound {
+  T n1[3], n2[3], n3[3];   // rows of A^-T
+  T w1, w2, w3;            // kappa eps m_i
+  T e0;                    // eps (1 + |v|_1)
+  bool ok;
+};
+
+template <typename T>
+__device__ __forceinline__ T coef_max(const T* p) {
+  return fmax(fmax(absT(p[0]), absT(p[1])), fmax(absT(p[2]), absT(p[3])));
+}
+
+template <typename T>
+__device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2, const T* p3, const T* v, T eps,
+                                                       T h_max) {
+  VertexBound<T> b;
+  const T c1[3] = {p2[1] * p3[2] - p2[2] * p3[1], p2[2] * p3[0] - p2[0] * p3[2], p2[0] * p3[1] - p2[1] * p3[0]};
+  const T c2[3] = {p3[1] * p1[2] - p3[2] * p1[1], p3[2] * p1[0] - p3[0] * p1[2], p3[0] * p1[1] - p3[1] * p1[0]};
+  const T c3[3] = {p1[1] * p2[2] - p1[2] * p2[1], p1[2] * p2[0] - p1[0] * p2[2], p1[0] * p2[1] - p1[1] * p2[0]};
+  const T det = p1[0] * c1[0] + p1[1] * c1[1] + p1[2] * c1[2];
+  const T h = __builtin_sqrt(plane_nn(p1[0], p1[1], p1[2])) * __builtin_sqrt(plane_nn(p2[0], p2[1], p2[2])) *
+              __builtin_sqrt(plane_nn(p3[0], p3[1], p3[2])) / absT(det);
+  // Cramer's rule in working precision adds an error equivalent to perturbing
+  // the system by about 16 u H of its coefficients (each 3x3 expansion's
+  // absolute terms sum to at most 5.2 H |det|, with three roundings each)
+  const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);   // 2^-53 / 2^-24
+  eps = eps + T(32) * u * (T(1) + h);
+  const T rho = eps * (T(1) + h);
+  b.ok = h <= h_max && rho <= T(0.25);
+  const T vn = T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]);
+  // 1.001: slack for the rounding of these bound computations themselves
+  const T kappa = T(1.001) / (T(1) - T(2) * rho);
+  const T inv = T(1) / det;
+  for (int i = 0; i < 3; ++i) { b.n1[i] = c1[i] * inv; b.n2[i] = c2[i] * inv; b.n3[i] = c3[i] * inv; }
+  b.w1 = kappa * eps * coef_max(p1) * vn;
+  b.w2 = kappa * eps * coef_max(p2) * vn;
+  b.w3 = kappa * eps * coef_max(p3) * vn;
+  b.e0 = T(1.001) * eps * vn;
+  return b;
+}
+
+// Signed-distance verdict of constraint (ax, ay, az, c) at v under the bound:
+// 0 = satisfied, 1 = certainly violated, 2 = undecided.
+template <typename T>
+__device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, const T* v, const VertexBound<T>& b) {
+  const T s = plane_eval(ax, ay, az, c, v);
+  const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
+  const T l1 = b.n1[0] * ax + b.n1[1] * ay + b.n1[2] * az;
+  const T l2 = b.n2[0] * ax + b.n2[1] * ay + b.n2[2] * az;
+  const T l3 = b.n3[0] * ax + b.n3[1] * ay + b.n3[2] * az;
+  const T err = b.e0 * pm + b.w1 * absT(l1) + b.w2 * absT(l2) + b.w3 * absT(l3);
+  if (s > err) return 1;
+  if (s >= -err) return 2;
+  return 0;
+}
+
 // Verdict of one vertex against all constraints owned by this lane.
 // 0 = satisfied, 1 = certainly violated, 2 = undecided.
 template <typename T, bool CHECKED>
 __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int lane,
-                                          int ex1, int ex2, const T* v, T epsv) {
+                                          int ex1, int ex2, const T* v, const VertexBound<T>& vb) {
   int verdict = 0;
   for (int g = lane; g < C; g += 64) {
     if (g == ex1 || g == ex2) continue;
@@ -46,12 +128,9 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
     if (!CHECKED) {
       if (violates(ax, ay, az, c, v)) return 1;
     } else {
-      const T s = plane_eval(ax, ay, az, c, v);
-      // plane errors are bounded relative to the plane's largest coefficient
-      const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
-      const T err = epsv * pm * (T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
-      if (s > err) return 1;
-      if (s >= -err) verdict = 2;
+      const int vg = checked_verdict(ax, ay, az, c, v, vb);
+      if (vg == 1) return 1;
+      if (vg == 2) verdict = 2;
     }
   }
   return verdict;
@@ -63,7 +142,7 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
 template <typename T, bool CHECKED, typename V = T>
 __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                               const uint32_t* smask, const T* eq, int* s_tab, V* s_verts, int* s_vpair, int ntab,
-                              int nv, int start_in, int start, int status, bool undecided) {
+                              int nv, int start_in, int start, int status, bool undecided, unsigned why = 0u) {
   const int L = net.L, vm = net.vm;
   const int lane = threadIdx.x;
   // ---- orientation (backend/inc/kernel.h:1325-1371)
@@ -76,7 +155,7 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
       const V dir = cx * V(eq[0]) + cy * V(eq[1]) + cz * V(eq[2]);
       if (CHECKED) {
         const V mag = (absT(cx) + absT(cy) + absT(cz)) * (absT(V(eq[0])) + absT(V(eq[1])) + absT(V(eq[2])));
-        if (!(absT(dir) > V(wp.eps_turn) * mag)) undecided = true;
+        if (!(absT(dir) > V(wp.eps_turn) * mag)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_ORIENT); }
       }
       reversed = ((dir < V(0)) != (net.flip != 0)) ? 1 : 0;
       __syncthreads();
@@ -97,7 +176,7 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
   }
   if (CHECKED && (status != AM_ST_OK)) undecided = true;
   if (CHECKED && undecided) {
-    if (lane == 0) io.flags[slot_region] = AM_UNDECIDED;
+    if (lane == 0) io.flags[slot_region] = AM_UNDECIDED | why;
     return;   // the FP64 redo writes this region's record and candidate slots
   }
   if (CHECKED && lane == 0) io.flags[slot_region] = 0u;
@@ -307,7 +386,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
       int verdict = x <= xmax ? 0 : 1;
       int ri = 0, rj = 0;
       T v[3] = {T(0), T(0), T(0)};
-      T epsv = T(0);
+      VertexBound<T> vb{};
       if (x <= xmax) {
         int i, j;
         get_i_j(x, C, i, j);
@@ -317,9 +396,8 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
         const T pj4[4] = {Px[rj], Py[rj], Pz[rj], Pc[rj]};
         solve3(pi4, pj4, eq, v);
         if (CHECKED) {
-          const T h = hadamard_ratio(pi4, pj4, eq);
-          epsv = T(wp.eps) * (T(1) + h);
-          if (!(h <= T(wp.h_max))) verdict = 2;
+          vb = vertex_bound<T>(pi4, pj4, eq, v, T(wp.eps), T(wp.h_max));
+          if (!vb.ok) verdict = 2;
         }
       }
       for (int g = 0; g < C; ++g) {
@@ -328,11 +406,9 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
           if (!CHECKED) {
             if (violates(Px[g], Py[g], Pz[g], Pc[g], v)) verdict = 1;
           } else {
-            const T s = plane_eval(Px[g], Py[g], Pz[g], Pc[g], v);
-            const T pm = fmax(fmax(absT(Px[g]), absT(Py[g])), fmax(absT(Pz[g]), absT(Pc[g])));
-            const T m = pm * (T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
-            if (s > epsv * m) verdict = 1;
-            else if (s >= -epsv * m) verdict = 2;
+            const int vg = checked_verdict(Px[g], Py[g], Pz[g], Pc[g], v, vb);
+            if (vg == 1) verdict = 1;
+            else if (vg == 2) verdict = 2;
           }
         }
       }
@@ -363,7 +439,6 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
       if (cur < 0 || cur >= C) { status = AM_ST_BAD_START; break; }
       int win = -1;
       T v[3] = {T(0), T(0), T(0)};
-      T vepsv = T(0);
       const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
       const int first_entry = s_tab[0];
       for (int w = 0; w < C; ++w) {
@@ -377,13 +452,6 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
         const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
         T vt[3];
         solve3(pr4, pc4, eq, vt);
-        T epsv = T(0);
-        bool trusted = true;
-        if (CHECKED) {
-          const T h = hadamard_ratio(pr4, pc4, eq);
-          epsv = T(wp.eps) * (T(1) + h);
-          trusted = h <= T(wp.h_max);
-        }
         if (!CHECKED) {
           // infeasible as soon as one constraint is violated: test them in
           // distance order, a wave-wide chunk at a time, so that a rejected
@@ -407,12 +475,12 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
           v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
           break;
         }
-        const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, C, lane, cur, r, vt, epsv);
+        const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, vt, T(wp.eps), T(wp.h_max));
+        const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, C, lane, cur, r, vt, vb);
         if (__any(verdict == 1)) continue;          // certainly infeasible
-        if (CHECKED && (__any(verdict == 2) || !trusted)) { undecided = true; break; }
+        if (CHECKED && (__any(verdict == 2) || !vb.ok)) { undecided = true; break; }
         win = r;
         v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
-        vepsv = epsv;
         break;
       }
       if (undecided) break;
@@ -486,7 +554,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
     }
   }
   finish_region<T, CHECKED>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv, start_in,
-                            start, status, undecided);
+                            start, status, undecided, undecided ? AM_WHY_BIT(AM_WHY_SCAN) : 0u);
 }
 
 // Minimum of (t, idx) over the wave, ties to the smaller index.
@@ -537,6 +605,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   const int start_in = io.reg_start[rid];
   const int NONE = 0x7fffffff;
   bool undecided = false;
+  unsigned why = 0u;
   int start = start_in;
   T tdummy;
 
@@ -550,12 +619,12 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     const T d0[3] = {eq[1] * e[2] - eq[2] * e[1], eq[2] * e[0] - eq[0] * e[2], eq[0] * e[1] - eq[1] * e[0]};
     int g0;
     shoot(Px, Py, Pz, Pc, C, q, d0, -1, -1, tdummy, g0);
-    if (g0 == NONE) undecided = true;
+    if (g0 == NONE) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SEED); }
     start = g0;
   }
   int ntab = 0, nv = 0;
   int r = NONE;
-  if (!undecided && (start < 0 || start >= C)) undecided = true;
+  if (!undecided && (start < 0 || start >= C)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SEED); }
   if (!undecided) {
     // edge on the start plane: its line's interval inside every other half-space
     const T ps[4] = {Px[start], Py[start], Pz[start], Pc[start]};
@@ -568,7 +637,10 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     int rh, rl;
     shoot(Px, Py, Pz, Pc, C, x0, d, start, -1, th, rh);
     shoot(Px, Py, Pz, Pc, C, x0, dm, start, -1, tl, rl);
-    if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) undecided = true;
+    if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) {
+      undecided = true;
+      why |= AM_WHY_BIT(AM_WHY_START);
+    }
     r = rh;
   }
   if (!undecided) {
@@ -580,15 +652,20 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       // r is the proposed next edge plane
       bool in_tab = false;
       for (int i = 0; i < ntab; ++i) in_tab = in_tab || (s_tab[i] == r);
-      if (in_tab && !(r == s_tab[0] && ntab >= 3)) { undecided = true; break; }
+      if (in_tab && !(r == s_tab[0] && ntab >= 3)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TAB); break; }
       const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
       const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
       T v[3];
       solve3(pr4, pc4, eq, v);
-      const T h = hadamard_ratio(pr4, pc4, eq);
-      const T epsv = T(wp.eps) * (T(1) + h);
-      const int verdict = lane_check<T, true>(Px, Py, Pz, Pc, C, lane, cur, r, v, epsv);
-      if (__any(verdict != 0) || !(h <= T(wp.h_max))) { undecided = true; break; }
+      const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, v, T(wp.eps), T(wp.h_max));
+      const int verdict = lane_check<T, true>(Px, Py, Pz, Pc, C, lane, cur, r, v, vb);
+      if (__any(verdict != 0) || !vb.ok) {
+        undecided = true;
+        if (__any(verdict == 1)) why |= AM_WHY_BIT(AM_WHY_VIOL);
+        else if (__any(verdict == 2)) why |= AM_WHY_BIT(AM_WHY_UNSURE);
+        else why |= AM_WHY_BIT(AM_WHY_HMAX);
+        break;
+      }
       if (ntab >= 2) {
         const int a = s_tab[ntab - 2], b = s_tab[ntab - 1];
         const T n1[3] = {Px[a], Py[a], Pz[a]};
@@ -599,7 +676,11 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         const T val = (A12 * A23 - A22 * A13) - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3) / B;
         const T mag = absT(A12 * A23) + absT(A22 * A13) +
                       (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
-        if (!(val > T(wp.eps_turn) * mag)) { undecided = true; break; }   // must certainly turn the right way
+        if (!(val > T(wp.eps_turn) * mag)) {   // must certainly turn the right way
+          undecided = true;
+          why |= AM_WHY_BIT(AM_WHY_TURN);
+          break;
+        }
       }
       double vd[3];
       {
@@ -608,10 +689,15 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
         solve3(a4, b4, e4, vd);
       }
-      // The reference's "same vertex" test (L1 < 3e-14) needs no separate
-      // bound: the certified check above found the new vertex strictly inside
-      // the plane of the previous vertex by more than its error bound, so the
-      // two are farther apart than 3e-14.
+      // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
+      // planes the certified check above already puts the new vertex farther
+      // than that from the last one; with FP64 planes it may not, so vertices
+      // that close are left to the scan.
+      if (nv > 0) {
+        const double l1 = fabs(vd[0] - s_verts[(nv - 1) * 3 + 0]) + fabs(vd[1] - s_verts[(nv - 1) * 3 + 1]) +
+                          fabs(vd[2] - s_verts[(nv - 1) * 3 + 2]);
+        if (!(l1 > 4.0 * Tol<double>::SAME_EPS)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SAME); break; }
+      }
       __syncthreads();
       if (lane == 0) {
         s_verts[nv * 3 + 0] = vd[0];
@@ -625,7 +711,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       nv += 1;
       ntab += 1;
       if (ntab >= 4 && s_tab[0] == s_tab[ntab - 1]) break;          // closed
-      if (ntab >= vm + 1) { undecided = true; break; }               // truncated: FP64 decides
+      if (ntab >= vm + 1) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TRUNC); break; }   // truncated: FP64 decides
       // walk along the new edge line, away from the plane we came along
       const int prev = cur;
       cur = r;
@@ -634,12 +720,12 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       if (Px[prev] * d[0] + Py[prev] * d[1] + Pz[prev] * d[2] > T(0)) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
       int nx;
       shoot(Px, Py, Pz, Pc, C, v, d, cur, prev, tdummy, nx);
-      if (nx == NONE) { undecided = true; break; }
+      if (nx == NONE) { undecided = true; why |= AM_WHY_BIT(AM_WHY_NEXT); break; }
       r = nx;
     }
   }
   finish_region<T, true, double>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv,
-                                 start_in, start, AM_ST_OK, undecided);
+                                 start_in, start, AM_ST_OK, undecided, why);
 }
 
 }  // namespace am

@@ -147,7 +147,7 @@ PYBIND11_MODULE(_am, m) {
   m.attr("PREC_FP32") = AM_PREC_FP32;
   m.attr("PREC_SPLIT_BF16X3") = AM_PREC_SPLIT_BF16X3;
   m.attr("PREC_FP64") = AM_PREC_FP64;
-  m.attr("PREC_FP64_MFMA") = AM_PREC_FP64_MFMA;
+  m.attr("PREC_FP64_RAY") = AM_PREC_FP64_RAY;
   m.attr("DTYPE_F32") = AM_DTYPE_F32;
   m.attr("DTYPE_F64") = AM_DTYPE_F64;
   py::class_<Engine>(m, "Engine")

@@ -10,9 +10,9 @@ One process-wide context, as in the reference; create
 
 ``float_type`` selects the arithmetic: ``"float64"`` runs the FP64 form (the
 reference's default dtype); ``"float32"`` runs the FP32-MFMA form with FP64
-re-decision; ``"bf16x3"`` runs the split-BF16 MFMA form; ``"fp64mfma"`` builds
-the planes on FP64 MFMA and walks them in FP32.  The last two take float32
-tensors, like ``"float32"``.
+re-decision; ``"bf16x3"`` runs the split-BF16 MFMA form (float32 tensors);
+``"fp64ray"`` builds the FP64 planes and walks them with the checked FP64 ray
+walk, leaving only undecided regions to the candidate scan (float64 tensors).
 """
 import numpy as np
 import torch
@@ -21,7 +21,7 @@ from .engine import Engine, MarchResult
 
 _S = {"engine": None, "float_type": "", "nodes": None, "arc": None, "E": 0, "result": None}
 
-_PRECISION = {"float64": "fp64", "float32": "fp32", "bf16x3": "bf16x3", "fp64mfma": "fp64mfma"}
+_PRECISION = {"float64": "fp64", "float32": "fp32", "bf16x3": "bf16x3", "fp64ray": "fp64ray"}
 
 
 def _check(cond, msg):
@@ -67,7 +67,7 @@ def AnalyticMarching(weights, biases, states, points, arc_tm, w_extra_constraint
     if _S["engine"] is None:
         print("Environment must be initialized first!")
         return
-    ft = torch.float64 if _S["float_type"] == "float64" else torch.float32
+    ft = torch.float64 if _S["float_type"] in ("float64", "fp64ray") else torch.float32
     nodes = _S["nodes"]
     _check(len(weights) == len(biases), "weights.size() == biases.size()")
     _check(len(weights) == len(nodes) - 1, "fc_layers_num == nodesnum.size() - 1")

@@ -11,7 +11,7 @@ import numpy as np
 
 from .canonical import RegionRecords
 
-PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp64": 2, "fp64mfma": 3}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp64": 2, "fp64ray": 3}
 
 _mod = None
 

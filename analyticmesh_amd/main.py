@@ -219,7 +219,7 @@ def AnalyticMarching(model, save_ply_path="mesh.ply", iso=0.0, scale=1.0, center
 
     ``precision`` (optional) overrides the arithmetic chosen by ``dtype``:
     "fp64", "fp32" (FP32 MFMA + FP64 re-decision), "bf16x3" (split-BF16 MFMA) or
-    "fp64mfma" (FP64 MFMA planes, FP32 walk, FP64 re-decision).
+    "fp64ray" (FP64 planes, checked FP64 ray walk, candidate scan where undecided).
     Returns {"init_point_time", "init_cuda_time", "am_time", "export_time"} in seconds.
     """
     assert w_extra_constraints_.shape[0] == b_extra_constraints_.shape[0]
@@ -238,9 +238,9 @@ def AnalyticMarching(model, save_ply_path="mesh.ply", iso=0.0, scale=1.0, center
     out["init_point_time"] = time.time() - t
 
     info = model.get_info()
-    float_type = {"fp64": "float64", "fp32": "float32", "bf16x3": "bf16x3", "fp64mfma": "fp64mfma"}.get(
+    float_type = {"fp64": "float64", "fp32": "float32", "bf16x3": "bf16x3", "fp64ray": "fp64ray"}.get(
         precision, "float64" if dtype == torch.float64 else "float32")
-    tdt = torch.float64 if float_type == "float64" else torch.float32
+    tdt = torch.float64 if float_type in ("float64", "fp64ray") else torch.float32
     # states at the seeds, in the marching precision's own arithmetic
     mdt = copy.deepcopy(model).to(tdt)
     states = _states(mdt, points.to(tdt))

@@ -25,8 +25,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp64": 78.6, "fp64mfma": 78.6}
-GEMM_KERNEL = {"fp32": "k_gemm<0>", "bf16x3": "k_gemm<1>", "fp64mfma": "k_gemm_f64", "fp64": "k_chain_tile_fp64"}
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp64": 78.6, "fp64ray": 78.6}
+GEMM_KERNEL = {"fp32": "k_gemm<0>", "bf16x3": "k_gemm<1>", "fp64ray": "k_chain_tile_fp64", "fp64": "k_chain_tile_fp64"}
 
 
 def log(*a):
@@ -155,7 +155,7 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--cpu-regions", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,bf16x3,fp64mfma,fp64")
+    ap.add_argument("--forms", default="fp32,bf16x3,fp64ray,fp64")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
     ap.add_argument("--extras", action="store_true", help="also measure the 64-MLP batch and the adjoint")
     args = ap.parse_args()
@@ -187,7 +187,7 @@ def main():
     st = h["stats"]
     # the plane GEMM of the form (FLOPs per region x regions it built planes for)
     kern_ms, launches, kname = st[3], max(1.0, st[11]), GEMM_KERNEL[head]
-    flops = flops_region * (h["regions"] - (st[9] if head != "fp64" else 0))
+    flops = flops_region * h["regions"]        # every region's planes go through this kernel once
     achieved = flops / (kern_ms / 1e3) / 1e12 if kern_ms > 0 else 0.0
     roofline = {"bound": "mfma", "kernel": kname, "achieved": round(achieved, 3), "peak": PEAK_TFLOPS[head],
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_TFLOPS[head], 5),
@@ -216,7 +216,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64", "fp64mfma": "f64 planes, f32 walk"}[head],
+        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64", "fp64ray": "f64"}[head],
         "data": "synthetic (seeded geometric-init weights, bisection seeds)",
         "config": {"workload": "deep 3->256x4->1 ReLU MLP, r=0.6, sigma=0.05, seed 0, 1024 seeds",
                    "faces": h["faces"], "regions": h["regions"], "form": head},
@@ -231,7 +231,10 @@ def main():
             "fp64_redo_ms": round(float(out[f]["stats"][5]), 3),
             "fp64_redo_regions": int(out[f]["stats"][9]),
             "fp64_chain_ms": round(float(out[f]["stats"][13]), 3),
-            "fp64_walk_ms": round(float(out[f]["stats"][14]), 3),
+            "fp64_ray_walk_ms": round(float(out[f]["stats"][28]), 3),
+            "fp64_scan_walk_ms": round(float(out[f]["stats"][14]), 3),
+            "fp64_ray_undecided": int(out[f]["stats"][30]),
+            "fp64_scanned_regions": int(out[f]["stats"][29]),
             "insert_ms": round(float(out[f]["stats"][2]), 3),
             "combine_ms": round(float(getattr(out[f]["res"], "combine_ms", 0.0)), 3),
             "levels": int(out[f]["stats"][6]),

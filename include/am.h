@@ -32,7 +32,7 @@ typedef int am_status;
 #define AM_PREC_FP32 0            /* FP32 MFMA planes, FP64 re-decision of undecided regions */
 #define AM_PREC_SPLIT_BF16X3 1    /* split-BF16 (3 products) MFMA planes, FP64 re-decision */
 #define AM_PREC_FP64 2            /* FP64 everywhere (the reference's default dtype) */
-#define AM_PREC_FP64_MFMA 3       /* FP64 MFMA planes, FP32 walk, FP64 re-decision */
+#define AM_PREC_FP64_RAY 3        /* FP64 planes, checked FP64 ray walk, FP64 candidate scan where undecided */
 
 /* Element type of caller buffers. */
 #define AM_DTYPE_F32 0
@@ -127,7 +127,15 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_ADJOINT_MS 12   /* am_adjoint kernels (wall time) */
 #define AM_STAT_CHAIN64_MS 13   /* FP64 tiled plane chain, all launches (HIP events) */
 #define AM_STAT_WALK64_MS 14    /* FP64 walk after the tiled chain, all launches (HIP events) */
-#define AM_STAT_COUNT 16
+#define AM_STAT_WHY0 16         /* 16..27: regions the reduced-precision walk left to FP64, per reason:
+                                   seed shot, start edge, plane already in polygon, sign within bound,
+                                   proposed vertex infeasible, ill-conditioned system, turn within bound,
+                                   vertex cap, open edge, orientation within bound, candidate scan,
+                                   same vertex */
+#define AM_STAT_RAY64_MS 28     /* FP64 checked ray walk (HIP events) */
+#define AM_STAT_SCAN64_REGIONS 29 /* regions decided by the FP64 candidate scan */
+#define AM_STAT_RAY64_UNDECIDED 30 /* regions the FP64 ray walk left to the scan */
+#define AM_STAT_COUNT 32
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 
 #ifdef __cplusplus
