@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
 // planes are the canonical FP64 planes, so the only uncertainty is the
 // rounding of the walk's own evaluations; regions it cannot certify are
 // flagged (io.flags, level-relative) for the candidate scan.
-__global__ __launch_bounds__(64) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
+__global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
                                                    const double* planes, const double* eqb, int Cs_hbm) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
@@ -222,9 +222,9 @@ __global__ __launch_bounds__(64) void k_walk_ray64(NetDev net, RegionIO io, int 
   const int lane = threadIdx.x;
   if (blockIdx.x >= io.count) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
-  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)blockIdx.x * 4 * Cs_hbm;
-  for (int g = lane; g < C; g += 64) {
+  for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * (size_t)Cs_hbm + g];
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(64) void k_walk_ray64(NetDev net, RegionIO io, int 
   const double eq[4] = {eqb[(size_t)blockIdx.x * 4 + 0], eqb[(size_t)blockIdx.x * 4 + 1],
                         eqb[(size_t)blockIdx.x * 4 + 2], eqb[(size_t)blockIdx.x * 4 + 3]};
   __syncthreads();
-  walk_ray<double>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+  walk_ray<double, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
 }
 
 // Chunk-local slots of the regions k_walk_ray64 left undecided: region of

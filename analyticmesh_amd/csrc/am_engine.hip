@@ -372,7 +372,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
       t.scanned += n;
       continue;
     }
-    hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64), rlds, c->stream, c->net, io, first + off, kRay64,
+    hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, c->stream, c->net, io, first + off, kRay64,
                        (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(c->ev2, c->stream));

@@ -230,7 +230,7 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
 }
 
 // Ray-shooting FP32 walk: LDS holds only the planes and the mask.
-__global__ __launch_bounds__(64) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
+__global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
                                                      const float* eqbuf, int Cs_hbm, int chunk_first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
@@ -245,18 +245,18 @@ __global__ __launch_bounds__(64) void k_walk_ray_f32(NetDev net, RegionIO io, Wa
   if (b >= io.count) return;
   const int rid = chunk_first + b;
   const float* src = planes + (size_t)b * 4 * Cs_hbm;
-  for (int g = lane; g < C; g += 64) {
+  for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * Cs_hbm + g];
     Pc[g] = src[3 * Cs_hbm + g];
   }
-  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const float eq[4] = {eqbuf[b * 4 + 0], eqbuf[b * 4 + 1], eqbuf[b * 4 + 2], eqbuf[b * 4 + 3]};
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
-  walk_ray<float>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
 }
 
 // Compact the undecided regions' ids; cnt[0] = their number, cnt[1 + w] = how
@@ -503,7 +503,7 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     io.ids = nullptr;
     FCHK(hipEventRecord(e0, st));
     if (f.walk_mode == 0)
-      hipLaunchKernelGGL(k_walk_ray_f32, dim3(n), dim3(64), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
+      hipLaunchKernelGGL(k_walk_ray_f32, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
                          lvl.first + c0);
     else
       hipLaunchKernelGGL(k_walk_f32, dim3(n), dim3(64), wlds, st, net, io, f.wp, planes, eqb, d.Cs,

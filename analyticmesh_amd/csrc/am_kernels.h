@@ -49,6 +49,7 @@ __global__ void k_region_fp64(NetDev net, RegionIO io);
 #define AM_CN (4 * AM_CT)       // chain columns (one per lane): x, y, z, c of each region
 #define AM_CR 16                // output rows per wave
 #define AM_CHAIN_MAXW 256       // widest layer the tiled chain takes
+#define AM_RAY_WAVES 4          // waves per region in the ray-shooting walks
 #define AM_CPF 16               // k steps of chain operands in flight (k is padded to a multiple)
 #define AM_CHAIN_THREADS (64 * AM_CHAIN_MAXW / AM_CR)
 

@@ -122,6 +122,7 @@ template <typename T, bool CHECKED>
 __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int lane,
                                           int ex1, int ex2, const T* v, const VertexBound<T>& vb) {
   int verdict = 0;
+#pragma unroll 2
   for (int g = lane; g < C; g += 64) {
     if (g == ex1 || g == ex2) continue;
     const T ax = Px[g], ay = Py[g], az = Pz[g], c = Pc[g];
@@ -144,7 +145,8 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
                               const uint32_t* smask, const T* eq, int* s_tab, V* s_verts, int* s_vpair, int ntab,
                               int nv, int start_in, int start, int status, bool undecided, unsigned why = 0u) {
   const int L = net.L, vm = net.vm;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x;     // any block size: strides are blockDim.x
+  const int nth = blockDim.x;
   // ---- orientation (backend/inc/kernel.h:1325-1371)
   int reversed = 0;
   if (nv >= 3 && !undecided) {
@@ -187,10 +189,10 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
     inf[0] = start_in; inf[1] = start; inf[2] = nv; inf[3] = ntab;
     inf[4] = reversed; inf[5] = status; inf[6] = CHECKED ? 2 : (io.ids ? 1 : 0); inf[7] = 0;
   }
-  for (int i = lane; i < vm + 1; i += 64) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
-  for (int i = lane; i < vm * 3; i += 64)
+  for (int i = lane; i < vm + 1; i += nth) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
+  for (int i = lane; i < vm * 3; i += nth)
     io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[i] : 0.0;
-  for (int i = lane; i < vm * 2; i += 64) io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[i] : 0;
+  for (int i = lane; i < vm * 2; i += nth) io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[i] : 0;
 
   // ---- neighbour candidates (backend/inc/kernel.h:1390-1474)
   const size_t slot0 = (size_t)slot_region * vm;
@@ -218,7 +220,7 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
     io.cand_point[(slot0 + m) * 3 + 2] = (double)mid[2];
   }
   const int Wd = net.Wd;
-  for (int idx = lane; idx < vm * Wd; idx += 64) {
+  for (int idx = lane; idx < vm * Wd; idx += nth) {
     const int m = idx / Wd, w = idx - m * Wd;
     int t = -1;
     if (m + 1 < ntab) t = s_tab[m];
@@ -567,24 +569,79 @@ __device__ __forceinline__ void wave_argmin(T& t, int& idx) {
   }
 }
 
+// Block-wide reductions for walks run by NW waves (one region per block).
+// Each wave reduces with shuffles, then wave results meet in LDS; the combine
+// runs in wave order, so the result is the same on every thread.
+template <typename T, int NW>
+__device__ __forceinline__ void block_argmin(T& t, int& g) {
+  wave_argmin(t, g);
+  if (NW == 1) return;
+  __shared__ T red_t[NW];
+  __shared__ int red_g[NW];
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();                       // earlier readers of red_* are done
+  if ((threadIdx.x & 63) == 0) { red_t[wave] = t; red_g[wave] = g; }
+  __syncthreads();
+  t = red_t[0];
+  g = red_g[0];
+  for (int w = 1; w < NW; ++w)
+    if (red_t[w] < t || (red_t[w] == t && red_g[w] < g)) { t = red_t[w]; g = red_g[w]; }
+}
+
+// OR of per-thread bits (bit 0: violated, bit 1: undecided) over the block.
+template <int NW>
+__device__ __forceinline__ int block_or_bits(int bits) {
+  int b = (__any(bits & 1) ? 1 : 0) | (__any(bits & 2) ? 2 : 0);
+  if (NW == 1) return b;
+  __shared__ int red_b[NW];
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red_b[wave] = b;
+  __syncthreads();
+  b = 0;
+  for (int w = 0; w < NW; ++w) b |= red_b[w];
+  return b;
+}
+
 // Nearest plane crossed by the ray x + t d (t may be negative when x lies
 // outside a half-space), skipping ex1/ex2; INT_MAX when no plane bounds it.
-template <typename T>
+// Each thread keeps its best crossing t = num / den (den > 0) as a fraction
+// and compares by cross-multiplication, dividing once: the result is a
+// proposal that the checked test certifies, so ties may break either way.
+template <typename T, int NW>
 __device__ __forceinline__ void shoot(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* x,
                                       const T* d, int ex1, int ex2, T& tmin, int& gmin) {
-  T tb = T(__builtin_inff());
+  T nb = T(1), db = T(0);   // no crossing yet
   int gb = 0x7fffffff;
-  for (int g = threadIdx.x; g < C; g += 64) {
+#pragma unroll 2
+  for (int g = threadIdx.x; g < C; g += 64 * NW) {
     if (g == ex1 || g == ex2) continue;
     const T al = fmaT(Px[g], d[0], fmaT(Py[g], d[1], Pz[g] * d[2]));
     if (al > T(0)) {
-      const T tg = -plane_eval(Px[g], Py[g], Pz[g], Pc[g], x) / al;
-      if (tg < tb || (tg == tb && g < gb)) { tb = tg; gb = g; }
+      const T num = -plane_eval(Px[g], Py[g], Pz[g], Pc[g], x);
+      // num / al < nb / db  <=>  num db < nb al  (al, db > 0); db == 0: none yet
+      if (db == T(0) || num * db < nb * al) { nb = num; db = al; gb = g; }
     }
   }
-  wave_argmin(tb, gb);
+  T tb = db > T(0) ? nb / db : T(__builtin_inff());
+  block_argmin<T, NW>(tb, gb);
   tmin = tb;
   gmin = gb;
+}
+
+// Checked verdict bits of vertex v over all constraints but ex1/ex2, for the
+// whole block: bit 0 some constraint certainly violated, bit 1 some undecided.
+template <typename T, int NW>
+__device__ __forceinline__ int block_check(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int ex1,
+                                           int ex2, const T* v, const VertexBound<T>& vb) {
+  int bits = 0;
+#pragma unroll 2
+  for (int g = threadIdx.x; g < C; g += 64 * NW) {
+    if (g == ex1 || g == ex2) continue;
+    const int vg = checked_verdict(Px[g], Py[g], Pz[g], Pc[g], v, vb);
+    bits |= vg == 1 ? 1 : (vg == 2 ? 2 : 0);
+  }
+  return block_or_bits<NW>(bits);
 }
 
 // Reduced-precision walk by ray shooting.  From the start plane's edge each
@@ -593,7 +650,7 @@ __device__ __forceinline__ void shoot(const T* Px, const T* Py, const T* Pz, con
 // vertex with the checked feasibility test.  On generic input the proposal
 // is the unique feasible candidate the reference's scan would accept, so the
 // face is the same; anything the bounds cannot decide goes to FP64.
-template <typename T>
+template <typename T, int NW>
 __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                          const T* Px, const T* Py, const T* Pz, const T* Pc, const uint32_t* smask, const T* eq,
                          const T* p) {
@@ -601,7 +658,6 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
   __shared__ int s_vpair[AM_VM_MAX * 2];
   const int C = net.C, vm = net.vm;
-  const int lane = threadIdx.x;
   const int start_in = io.reg_start[rid];
   const int NONE = 0x7fffffff;
   bool undecided = false;
@@ -618,7 +674,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     const T e[3] = {T(ax <= ay && ax <= az ? 1 : 0), T(ay < ax && ay <= az ? 1 : 0), T(az < ax && az < ay ? 1 : 0)};
     const T d0[3] = {eq[1] * e[2] - eq[2] * e[1], eq[2] * e[0] - eq[0] * e[2], eq[0] * e[1] - eq[1] * e[0]};
     int g0;
-    shoot(Px, Py, Pz, Pc, C, q, d0, -1, -1, tdummy, g0);
+    shoot<T, NW>(Px, Py, Pz, Pc, C, q, d0, -1, -1, tdummy, g0);
     if (g0 == NONE) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SEED); }
     start = g0;
   }
@@ -635,8 +691,8 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     const T dm[3] = {-d[0], -d[1], -d[2]};
     T th, tl;
     int rh, rl;
-    shoot(Px, Py, Pz, Pc, C, x0, d, start, -1, th, rh);
-    shoot(Px, Py, Pz, Pc, C, x0, dm, start, -1, tl, rl);
+    shoot<T, NW>(Px, Py, Pz, Pc, C, x0, d, start, -1, th, rh);
+    shoot<T, NW>(Px, Py, Pz, Pc, C, x0, dm, start, -1, tl, rl);
     if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) {
       undecided = true;
       why |= AM_WHY_BIT(AM_WHY_START);
@@ -644,7 +700,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     r = rh;
   }
   if (!undecided) {
-    if (lane == 0) s_tab[0] = start;
+    if (threadIdx.x == 0) s_tab[0] = start;
     __syncthreads();
     ntab = 1;
     int cur = start;
@@ -658,11 +714,11 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       T v[3];
       solve3(pr4, pc4, eq, v);
       const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, v, T(wp.eps), T(wp.h_max));
-      const int verdict = lane_check<T, true>(Px, Py, Pz, Pc, C, lane, cur, r, v, vb);
-      if (__any(verdict != 0) || !vb.ok) {
+      const int bits = block_check<T, NW>(Px, Py, Pz, Pc, C, cur, r, v, vb);
+      if (bits != 0 || !vb.ok) {
         undecided = true;
-        if (__any(verdict == 1)) why |= AM_WHY_BIT(AM_WHY_VIOL);
-        else if (__any(verdict == 2)) why |= AM_WHY_BIT(AM_WHY_UNSURE);
+        if (bits & 1) why |= AM_WHY_BIT(AM_WHY_VIOL);
+        else if (bits & 2) why |= AM_WHY_BIT(AM_WHY_UNSURE);
         else why |= AM_WHY_BIT(AM_WHY_HMAX);
         break;
       }
@@ -683,7 +739,9 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         }
       }
       double vd[3];
-      {
+      if (sizeof(T) == 8) {
+        vd[0] = v[0]; vd[1] = v[1]; vd[2] = v[2];
+      } else {
         const double a4[4] = {pr4[0], pr4[1], pr4[2], pr4[3]};
         const double b4[4] = {pc4[0], pc4[1], pc4[2], pc4[3]};
         const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
@@ -699,7 +757,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         if (!(l1 > 4.0 * Tol<double>::SAME_EPS)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SAME); break; }
       }
       __syncthreads();
-      if (lane == 0) {
+      if (threadIdx.x == 0) {
         s_verts[nv * 3 + 0] = vd[0];
         s_verts[nv * 3 + 1] = vd[1];
         s_verts[nv * 3 + 2] = vd[2];
@@ -719,7 +777,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       T d[3] = {pc[1] * eq[2] - pc[2] * eq[1], pc[2] * eq[0] - pc[0] * eq[2], pc[0] * eq[1] - pc[1] * eq[0]};
       if (Px[prev] * d[0] + Py[prev] * d[1] + Pz[prev] * d[2] > T(0)) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
       int nx;
-      shoot(Px, Py, Pz, Pc, C, v, d, cur, prev, tdummy, nx);
+      shoot<T, NW>(Px, Py, Pz, Pc, C, v, d, cur, prev, tdummy, nx);
       if (nx == NONE) { undecided = true; why |= AM_WHY_BIT(AM_WHY_NEXT); break; }
       r = nx;
     }
