@@ -14,10 +14,11 @@
 // Operand delivery: X[k][lane] is a private LDS read per lane; the wave's 16
 // weights W[k][row0..row0+15] arrive as one coalesced vector load (lane l
 // holds row l & 15) and each fma takes its row's weight through a DPP row
-// broadcast, so a weight costs no extra instruction.  The next k's weights and
-// X are loaded while this k's fmas run.  After the k loop each wave writes its rows to HBM (signed,
-// a.x + c <= 0 inside), and, once every wave is done reading X, parks them in
-// LDS masked as the next layer's input.
+// broadcast, so a weight costs no extra instruction.  Weights and X of the next
+// AM_CPF k steps are in flight while a step's fmas run.  After the k loop, once every wave is
+// done reading X, the waves park their rows in LDS; the tile is then written
+// to HBM signed (a.x + c <= 0 inside) with consecutive threads on consecutive
+// constraints, and masked in place as the next layer's input.
 //
 // k_walk_fp64 is the polygon walk of the region kernel on those planes.
 #include "am_device.h"
@@ -60,8 +61,8 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  double* Xs = reinterpret_cast<double*>(smem_raw);                              // [AM_CHAIN_MAXW][AM_CN]
-  uint32_t* mk = reinterpret_cast<uint32_t*>(Xs + (size_t)AM_CHAIN_MAXW * AM_CN);  // [AM_CT][Wd]
+  double* Xs = reinterpret_cast<double*>(smem_raw);                              // [AM_CHAIN_MAXW][AM_CXS]
+  uint32_t* mk = reinterpret_cast<uint32_t*>(Xs + (size_t)AM_CHAIN_MAXW * AM_CXS);  // [AM_CT][Wd]
   const int r0 = blockIdx.x * AM_CT;
   const int nreg = min(AM_CT, io.count - r0);
   for (int idx = tid; idx < AM_CT * net.Wd; idx += AM_CHAIN_THREADS) {
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
       x = mbit(mk + t * net.Wd, k) ? val : 0.0;
     }
-    Xs[k * AM_CN + col] = x;
+    Xs[k * AM_CXS + col] = x;
   }
   for (int idx = tid; idx < AM_CN * n1; idx += AM_CHAIN_THREADS) {
     const int col = idx / n1, k = idx - col * n1, t = col >> 2, q = col & 3;
@@ -99,9 +100,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
     if (t < nreg) io.planes[((size_t)(r0 + t) * 4 + q) * Cs + net.L + e] = q < 3 ? net.ew[e * 3 + q] : net.eb[e];
   }
   __syncthreads();
-  const int t_l = lane >> 2, q_l = lane & 3;
-  const bool col_live = t_l < nreg;
-  const uint32_t* mk_l = mk + t_l * net.Wd;
+  const int q_l = lane & 3;           // this lane's column is x, y, z or c of region lane >> 2
   for (int li = 1; li <= net.D; ++li) {
     const int nin = net.nodes[li], nout = net.nodes[li + 1], noutp = io.noutp[li];
     const int row0 = wave * AM_CR;
@@ -123,7 +122,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
 #pragma unroll
       for (int u = 0; u < AM_CPF; ++u) {
         wq[u] = wcol[(size_t)u * noutp];
-        xq[u] = Xs[u * AM_CN + lane];
+        xq[u] = Xs[u * AM_CXS + lane];
         __builtin_amdgcn_sched_barrier(0);
       }
       for (int k = 0; k < ninp; k += AM_CPF) {
@@ -135,33 +134,43 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
           // for its own (oldest) load
           const int kn = min(k + u + AM_CPF, ninp - 1);
           wq[u] = wcol[(size_t)kn * noutp];
-          xq[u] = Xs[kn * AM_CN + lane];
+          xq[u] = Xs[kn * AM_CXS + lane];
           __builtin_amdgcn_sched_barrier(0);
         }
       }
-      if (li < net.D && col_live) {
-        const int out0 = net.off[li];
-        double* dst = io.planes + ((size_t)(r0 + t_l) * 4 + q_l) * Cs + out0 + row0;
-#pragma unroll
-        for (int j = 0; j < AM_CR; ++j)
-          if (row0 + j < nout) dst[j] = mbit(mk_l, out0 + row0 + j) ? -acc[j] : acc[j];
-      }
     }
     __syncthreads();                  // every wave is done reading this layer's input
-    if (li < net.D) {
-      // masked input of the next layer; rows nout..round_up(nout, AM_CPF) are +0
-      const int noutk = (nout + AM_CPF - 1) / AM_CPF * AM_CPF;
-      if (active) {
-        const int out0 = net.off[li];
+    // park the layer's outputs in LDS (unsigned, unmasked); rows
+    // nout..round_up(nout, AM_CPF) become +0 for the next layer's k padding
+    const int noutk = (nout + AM_CPF - 1) / AM_CPF * AM_CPF;
+    if (active) {
 #pragma unroll
-        for (int j = 0; j < AM_CR; ++j) {
-          const int row = row0 + j;
-          if (row < nout) Xs[row * AM_CN + lane] = mbit(mk_l, out0 + row) ? acc[j] : 0.0;
-          else if (row < noutk) Xs[row * AM_CN + lane] = 0.0;
+      for (int j = 0; j < AM_CR; ++j) {
+        const int row = row0 + j;
+        if (row < nout) Xs[row * AM_CXS + lane] = acc[j];
+        else if (row < noutk) Xs[row * AM_CXS + lane] = 0.0;
+      }
+    }
+    __syncthreads();
+    if (li < net.D) {
+      // signed planes to HBM, consecutive threads on consecutive constraints
+      const int out0 = net.off[li];
+      for (int idx = tid; idx < AM_CN * nout; idx += AM_CHAIN_THREADS) {
+        const int col = idx / nout, j = idx - col * nout, t = col >> 2, q = col & 3;
+        if (t < nreg) {
+          const double y = Xs[j * AM_CXS + col];
+          io.planes[((size_t)(r0 + t) * 4 + q) * Cs + out0 + j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
         }
       }
-    } else if (wave == 0 && col_live) {
-      io.eq[(size_t)(r0 + t_l) * 4 + q_l] = q_l == 3 ? acc[0] - net.iso : acc[0];
+      __syncthreads();
+      // masked input of the next layer, in place
+      for (int idx = tid; idx < nout * AM_CN; idx += AM_CHAIN_THREADS) {
+        const int j = idx / AM_CN, col = idx - j * AM_CN, t = col >> 2;
+        if (!mbit(mk + t * net.Wd, out0 + j)) Xs[j * AM_CXS + col] = 0.0;
+      }
+    } else if (tid < AM_CN && (tid >> 2) < nreg) {
+      const double y = Xs[tid];
+      io.eq[(size_t)(r0 + (tid >> 2)) * 4 + (tid & 3)] = (tid & 3) == 3 ? y - net.iso : y;
     }
     __syncthreads();
   }

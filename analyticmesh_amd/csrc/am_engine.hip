@@ -271,13 +271,13 @@ void launch_region_fp64(am_ctx* c, int first, int count, float* ms, const int* i
 }
 
 size_t chain_lds_bytes(const am_ctx* c) {
-  return (size_t)AM_CHAIN_MAXW * AM_CN * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
+  return (size_t)AM_CHAIN_MAXW * AM_CXS * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
 }
 
 // Per-launch accounting of the FP64 stages.
 struct Fp64Times {
   float chain_ms = 0, ray_ms = 0, scan_ms = 0;
-  long long scanned = 0, ray_undecided = 0;
+  long long scanned = 0, ray_undecided = 0, chain_launches = 0;
   long long why[AM_WHY_COUNT] = {0};
 };
 
@@ -357,6 +357,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
                        c->net, cio);
     HIPCHK(hipGetLastError());
+    t.chain_launches += 1;
     HIPCHK(hipEventRecord(c->ev1, c->stream));
     if (mode == 0) {
       hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, (const int*)nullptr,
@@ -751,7 +752,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
     double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0, t_chain64 = 0, t_walk64 = 0, t_ray64 = 0;
-    long long scanned = 0, ray_undecided = 0;
+    long long scanned = 0, ray_undecided = 0, chain_launches = 0;
     long long levels = 0, cands = 0, redo = 0, launches = 0;
     long long why[AM_WHY_COUNT] = {0};
     while (lvl_count > 0) {
@@ -772,6 +773,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         t_ray64 += ft.ray_ms;
         scanned += ft.scanned;
         ray_undecided += ft.ray_undecided;
+        chain_launches += ft.chain_launches;
         for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
         launches += 1;
       } else {
@@ -797,6 +799,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
           t_ray64 += f64.ray_ms;
           scanned += f64.scanned;
           ray_undecided += f64.ray_undecided;
+          chain_launches += f64.chain_launches;
           redo += nredo;
         }
         t_gemm += ft.gemm_ms;
@@ -839,6 +842,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_RAY64_MS] = t_ray64;
     c->stats[AM_STAT_SCAN64_REGIONS] = (double)scanned;
     c->stats[AM_STAT_RAY64_UNDECIDED] = (double)ray_undecided;
+    c->stats[AM_STAT_CHAIN64_LAUNCHES] = (double)chain_launches;
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;
     return AM_OK;

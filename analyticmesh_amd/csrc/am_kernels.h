@@ -47,6 +47,7 @@ __global__ void k_region_fp64(NetDev net, RegionIO io);
 // Tiled FP64 chain: the planes of a tile of regions in HBM, for the walk kernel.
 #define AM_CT 16                // regions per chain workgroup
 #define AM_CN (4 * AM_CT)       // chain columns (one per lane): x, y, z, c of each region
+#define AM_CXS (AM_CN + 1)      // LDS row stride (doubles) of the chain's layer buffer
 #define AM_CR 16                // output rows per wave
 #define AM_CHAIN_MAXW 256       // widest layer the tiled chain takes
 #define AM_RAY_WAVES 4          // waves per region in the ray-shooting walks
