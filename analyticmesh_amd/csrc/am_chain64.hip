@@ -57,12 +57,27 @@ __device__ __forceinline__ void chain_step16(double (&acc)[AM_CR], double w, dou
       : "v"(w), "v"(x));
 }
 
+// 16 consecutive k indices (uint16, 32-byte aligned in LDS) into 8 uniform words.
+__device__ __forceinline__ void load_k16(const uint16_t* src, uint32_t (&k)[AM_CPF / 2]) {
+  static_assert(AM_CPF == 16, "two 16-byte reads per group");
+  const uint4 a = reinterpret_cast<const uint4*>(src)[0];
+  const uint4 b = reinterpret_cast<const uint4*>(src)[1];
+  k[0] = __builtin_amdgcn_readfirstlane(a.x); k[1] = __builtin_amdgcn_readfirstlane(a.y);
+  k[2] = __builtin_amdgcn_readfirstlane(a.z); k[3] = __builtin_amdgcn_readfirstlane(a.w);
+  k[4] = __builtin_amdgcn_readfirstlane(b.x); k[5] = __builtin_amdgcn_readfirstlane(b.y);
+  k[6] = __builtin_amdgcn_readfirstlane(b.z); k[7] = __builtin_amdgcn_readfirstlane(b.w);
+}
+
 __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  double* Xs = reinterpret_cast<double*>(smem_raw);                              // [AM_CHAIN_MAXW][AM_CXS]
-  uint32_t* mk = reinterpret_cast<uint32_t*>(Xs + (size_t)AM_CHAIN_MAXW * AM_CXS);  // [AM_CT][Wd]
+  // LDS: X (one more row than any layer's input: row AM_CHAIN_MAXW stays +0),
+  // each layer's list of active inputs, the tile's masks
+  double* Xs = reinterpret_cast<double*>(smem_raw);                                   // [AM_CHAIN_MAXW + 1][AM_CXS]
+  uint16_t* klist = reinterpret_cast<uint16_t*>(Xs + (size_t)(AM_CHAIN_MAXW + 1) * AM_CXS + 1);  // [D][AM_KLS], 16-B aligned
+  uint32_t* mk = reinterpret_cast<uint32_t*>(klist + (size_t)net.D * AM_KLS);          // [AM_CT][Wd]
+  __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_CT;
   const int nreg = min(AM_CT, io.count - r0);
   for (int idx = tid; idx < AM_CT * net.Wd; idx += AM_CHAIN_THREADS) {
@@ -73,6 +88,31 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       v = io.reg_mask[(size_t)rid * net.Wd + w];
     }
     mk[idx] = v;
+  }
+  for (int i = tid; i < AM_CXS; i += AM_CHAIN_THREADS) Xs[(size_t)AM_CHAIN_MAXW * AM_CXS + i] = 0.0;
+  __syncthreads();
+  // Active inputs of each layer for the whole tile: input k of linear layer li
+  // is skipped when it is inactive in all AM_CT regions (its X row is +0 in
+  // every column, and fma(w, +0, acc) = acc).  Wave li-1 lists layer li's in
+  // ascending order and pads the list with the +0 row AM_CHAIN_MAXW up to a
+  // multiple of AM_CPF, plus AM_CPF more entries for the operand pipeline.
+  if (wave < net.D) {
+    const int li = wave + 1;
+    const int nin = net.nodes[li], in0 = li == 1 ? 0 : net.off[li - 1];
+    uint16_t* kl = klist + (size_t)wave * AM_KLS;
+    int base = 0;
+    for (int kb = 0; kb < nin; kb += 64) {
+      const int k = kb + lane;
+      bool act = false;
+      if (k < nin)
+        for (int t = 0; t < AM_CT; ++t) act = act || mbit(mk + t * net.Wd, in0 + k);
+      const unsigned long long bal = __ballot(act);
+      if (act) kl[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = (uint16_t)k;
+      base += __popcll(bal);
+    }
+    const int nkp = (base + AM_CPF - 1) / AM_CPF * AM_CPF;
+    for (int i = base + lane; i < nkp + AM_CPF; i += 64) kl[i] = (uint16_t)AM_CHAIN_MAXW;
+    if (lane == 0) s_nk[li] = nkp;
   }
   __syncthreads();
   const size_t Cs = (size_t)io.Cs;
@@ -113,28 +153,38 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       // lane l carries W[k][row0 + (l & 15)]; each fma takes row j's weight by a
       // DPP row broadcast of that register
       const double* wcol = io.Wp[li] + row0 + (lane & 15);
-      // operands of the next AM_CPF k steps are in flight while a step's fmas
-      // run (past the end the ring reloads row nin-1 and is not used)
-      // k runs over nin rounded up to AM_CPF: the padding rows of W^T and of X
-      // are +0, and fma(+0, +0, acc) = acc
-      const int ninp = (nin + AM_CPF - 1) / AM_CPF * AM_CPF;
+      // k runs over the tile's active inputs in ascending order (padding
+      // entries name the +0 row: fma(+0, +0, acc) = acc).  Operands of the
+      // next AM_CPF steps are in flight while a step's fmas run, and the k
+      // indices of the group after are read a group ahead.
+      const uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
+      const int nkp = s_nk[li];
       double wq[AM_CPF], xq[AM_CPF];
+      uint32_t kn[AM_CPF / 2];   // k indices of the group after next, two per word (uniform: SGPRs)
+      load_k16(kl, kn);
 #pragma unroll
       for (int u = 0; u < AM_CPF; ++u) {
-        wq[u] = wcol[(size_t)u * noutp];
-        xq[u] = Xs[u * AM_CXS + lane];
+        const int k = (int)((kn[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
+        wq[u] = wcol[(size_t)k * noutp];
+        xq[u] = Xs[k * AM_CXS + lane];
         __builtin_amdgcn_sched_barrier(0);
       }
-      for (int k = 0; k < ninp; k += AM_CPF) {
+      load_k16(kl + AM_CPF, kn);
+      for (int i = 0; i < nkp; i += AM_CPF) {
+        uint32_t kk[AM_CPF / 2];
+#pragma unroll
+        for (int u = 0; u < AM_CPF / 2; ++u) kk[u] = kn[u];
+        // indices of the group after next (past the end: unused entries of the list)
+        load_k16(kl + i + 2 * AM_CPF, kn);
 #pragma unroll
         for (int u = 0; u < AM_CPF; ++u) {
           chain_step16(acc, wq[u], xq[u]);
           // refill the slot just used, AM_CPF steps ahead; the scheduling
           // barrier keeps the loads in slot order, so each step waits only
           // for its own (oldest) load
-          const int kn = min(k + u + AM_CPF, ninp - 1);
-          wq[u] = wcol[(size_t)kn * noutp];
-          xq[u] = Xs[kn * AM_CXS + lane];
+          const int k = (int)((kk[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
+          wq[u] = wcol[(size_t)k * noutp];
+          xq[u] = Xs[k * AM_CXS + lane];
           __builtin_amdgcn_sched_barrier(0);
         }
       }
@@ -176,8 +226,9 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   }
 }
 
-__global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* slots,
-                                                  const double* planes, const double* eqb, int Cs_hbm) {
+__global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count,
+                                                  const int* slots, const double* planes, const double* eqb,
+                                                  int Cs_hbm) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
   const int Cs = io.Cs;
@@ -192,27 +243,31 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
   uint32_t* smask = used + ((C + 31) >> 5);
 
   const int lane = threadIdx.x;
-  if (blockIdx.x >= io.count) return;
-  const int sl = slots ? slots[blockIdx.x] : blockIdx.x;     // planes slot of this region
-  const int rid = io.ids ? io.ids[sl] : rid0 + sl;
-  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
-  for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
-  const double* src = planes + (size_t)sl * 4 * Cs_hbm;
-  for (int g = lane; g < C; g += 64) {
-    Px[g] = src[g];
-    Py[g] = src[Cs_hbm + g];
-    Pz[g] = src[2 * (size_t)Cs_hbm + g];
-    Pc[g] = src[3 * (size_t)Cs_hbm + g];
+  const int total = count ? min(*count, io.count) : io.count;
+  // block-stride over the regions (one iteration per block when the grid covers them)
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int sl = slots ? slots[b] : b;     // planes slot of this region
+    const int rid = io.ids ? io.ids[sl] : rid0 + sl;
+    for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+    for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
+    const double* src = planes + (size_t)sl * 4 * Cs_hbm;
+    for (int g = lane; g < C; g += 64) {
+      Px[g] = src[g];
+      Py[g] = src[Cs_hbm + g];
+      Pz[g] = src[2 * (size_t)Cs_hbm + g];
+      Pc[g] = src[3 * (size_t)Cs_hbm + g];
+    }
+    double p[3];
+    p[0] = io.reg_point[(size_t)rid * 3 + 0];
+    p[1] = io.reg_point[(size_t)rid * 3 + 1];
+    p[2] = io.reg_point[(size_t)rid * 3 + 2];
+    const double eq[4] = {eqb[(size_t)sl * 4 + 0], eqb[(size_t)sl * 4 + 1], eqb[(size_t)sl * 4 + 2],
+                          eqb[(size_t)sl * 4 + 3]};
+    __syncthreads();
+    const WalkParams wp{0.f, 0.f, 0.f};
+    walk_region<double, false>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
+    __syncthreads();   // this region's LDS is free for the next
   }
-  double p[3];
-  p[0] = io.reg_point[(size_t)rid * 3 + 0];
-  p[1] = io.reg_point[(size_t)rid * 3 + 1];
-  p[2] = io.reg_point[(size_t)rid * 3 + 2];
-  const double eq[4] = {eqb[(size_t)sl * 4 + 0], eqb[(size_t)sl * 4 + 1], eqb[(size_t)sl * 4 + 2],
-                        eqb[(size_t)sl * 4 + 3]};
-  __syncthreads();
-  const WalkParams wp{0.f, 0.f, 0.f};
-  walk_region<double, false>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
 }
 
 // Checked ray-shooting walk on the FP64 planes of k_chain_tile_fp64.  The
@@ -252,15 +307,16 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, Re
 // Chunk-local slots of the regions k_walk_ray64 left undecided: region of
 // slot b is ids[b] or rid0 + b; its flag is level-relative.
 __global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
-                                int* slots) {
+                                int* slots, int* run) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int rid = ids ? ids[b] : rid0 + b;
   const unsigned f = flags[rid - lvl_first];
   if (f & AM_UNDECIDED) {
     slots[atomicAdd(cnt, 1)] = b;
+    atomicAdd(run, 1);
     for (int w = 0; w < AM_WHY_COUNT; ++w)
-      if (f & AM_WHY_BIT(w)) atomicAdd(cnt + 1 + w, 1);
+      if (f & AM_WHY_BIT(w)) atomicAdd(run + 1 + w, 1);
   }
 }
 

@@ -128,11 +128,12 @@ struct am_ctx {
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
   DevBuf<double> tplanes, teq;
-  DevBuf<int> tslots, tcnt;               // undecided slots of a chunk, their count (+ reasons)
+  DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
+  DevBuf<int> runacc;                     // per run: FP64 ray walk undecided count, then per reason
+  StageTimer timer;
   std::vector<DevBuf<double>> WTp, bpad;   // W_li^T and b_li with n_out padded to AM_CR (zeros)
 
   double stats[AM_STAT_COUNT] = {0};
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 };
 
 namespace {
@@ -242,7 +243,7 @@ size_t region_lds_bytes(const am_ctx* c, int& Cs, int& Csort) {
          (size_t)((c->C + 31) / 32) * 4 + (size_t)c->Wd * 4 + 64;
 }
 
-void launch_region_fp64(am_ctx* c, int first, int count, float* ms, const int* ids = nullptr) {
+void launch_region_fp64(am_ctx* c, int first, int count, const int* ids = nullptr) {
   int Cs, Csort;
   const size_t lds = region_lds_bytes(c, Cs, Csort);
   RegionIO io;
@@ -262,23 +263,21 @@ void launch_region_fp64(am_ctx* c, int first, int count, float* ms, const int* i
   io.Csort = Csort;
   io.flags = nullptr;
   io.ids = ids;
-  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  const size_t m0 = c->timer.mark(c->stream);
   hipLaunchKernelGGL(k_region_fp64, dim3(count), dim3(64), lds, c->stream, c->net, io);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipEventRecord(c->ev1, c->stream));
-  HIPCHK(hipEventSynchronize(c->ev1));
-  HIPCHK(hipEventElapsedTime(ms, c->ev0, c->ev1));
+  c->timer.span(m0, c->timer.mark(c->stream), T_REGION64);
 }
 
 size_t chain_lds_bytes(const am_ctx* c) {
-  return (size_t)AM_CHAIN_MAXW * AM_CXS * sizeof(double) + (size_t)AM_CT * c->Wd * 4 + 16;
+  return ((size_t)(AM_CHAIN_MAXW + 1) * AM_CXS + 1) * sizeof(double) + (size_t)c->D * AM_KLS * sizeof(uint16_t) +
+         (size_t)AM_CT * c->Wd * 4 + 16;
 }
 
-// Per-launch accounting of the FP64 stages.
-struct Fp64Times {
-  float chain_ms = 0, ray_ms = 0, scan_ms = 0;
-  long long scanned = 0, ray_undecided = 0, chain_launches = 0;
-  long long why[AM_WHY_COUNT] = {0};
+// Per-run counters of the FP64 stages (kernel times go to the context's timer;
+// the ray walk's undecided count and reasons accumulate on the device).
+struct Fp64Counts {
+  long long scanned_host = 0, chain_launches = 0;
 };
 
 // Checked FP64 ray walk: planes are exact FP64, so eps only covers the walk's
@@ -291,19 +290,19 @@ size_t ray64_lds_bytes(const am_ctx* c) { return (size_t)4 * c->C * sizeof(doubl
 // through the tiled chain in chunks; candidate slots and flags are numbered
 // from `lvl_first`.  mode 0: the candidate scan on every region (the FP64
 // records of the CPU restatement, bit for bit); mode 1: the checked FP64 ray
-// walk, and the scan on the regions it leaves undecided, reusing their planes.
-// Falls back to the one-region kernel where the tiled chain does not apply.
-void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Times& t) {
+// walk, and the scan on the regions it leaves undecided, reusing their planes
+// (the scan runs a block-stride loop up to the device-side count).
+// Nothing here waits for the device.  Falls back to the one-region kernel
+// where the tiled chain does not apply.
+void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t) {
   if (!c->tile_ok) {
     // one region per workgroup: the chain and the candidate scan in one kernel
-    float ms = 0;
     if (ids) {
-      launch_region_fp64(c, lvl_first, count, &ms, ids);
+      launch_region_fp64(c, lvl_first, count, ids);
     } else {
-      launch_region_fp64(c, first, count, &ms);
+      launch_region_fp64(c, first, count);
     }
-    t.scan_ms += ms;
-    t.scanned += count;
+    t.scanned_host += count;
     return;
   }
   if (mode == 1 && ray64_lds_bytes(c) > c->max_lds) mode = 0;
@@ -318,9 +317,10 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   ensure(c->teq, cap * 4, false, c->stream);
   if (mode == 1) {
     ensure(c->tslots, cap, false, c->stream);
-    ensure(c->tcnt, 1 + AM_WHY_COUNT, false, c->stream);
+    ensure(c->tcnt, 1, false, c->stream);
   }
   const size_t clds = chain_lds_bytes(c);
+  StageTimer& tm = c->timer;
   for (int off = 0; off < count; off += (int)chunk) {
     const int n = std::min<int>((int)chunk, count - off);
     ChainIO cio;
@@ -353,58 +353,38 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.Csort = Csort;
     io.flags = mode == 1 ? c->rflags.p : nullptr;
     io.ids = ids ? ids + off : nullptr;
-    HIPCHK(hipEventRecord(c->ev0, c->stream));
+    const size_t m0 = tm.mark(c->stream);
     hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
                        c->net, cio);
     HIPCHK(hipGetLastError());
     t.chain_launches += 1;
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
+    const size_t m1 = tm.mark(c->stream);
+    tm.span(m0, m1, T_CHAIN64);
     if (mode == 0) {
       hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, (const int*)nullptr,
-                         (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+                         (const int*)nullptr, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c->ev2, c->stream));
-      HIPCHK(hipEventSynchronize(c->ev2));
-      float a = 0, b = 0;
-      HIPCHK(hipEventElapsedTime(&a, c->ev0, c->ev1));
-      HIPCHK(hipEventElapsedTime(&b, c->ev1, c->ev2));
-      t.chain_ms += a;
-      t.scan_ms += b;
-      t.scanned += n;
+      tm.span(m1, tm.mark(c->stream), T_SCAN64);
+      t.scanned_host += n;
       continue;
     }
-    hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, c->stream, c->net, io, first + off, kRay64,
-                       (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+    hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, c->stream, c->net, io, first + off,
+                       kRay64, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipEventRecord(c->ev2, c->stream));
-    HIPCHK(hipMemsetAsync(c->tcnt.p, 0, sizeof(int) * (1 + AM_WHY_COUNT), c->stream));
+    const size_t m2 = tm.mark(c->stream);
+    tm.span(m1, m2, T_RAY64);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->tcnt.p, (size_t)1, 0);
     hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const unsigned*)c->rflags.p,
-                       n, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p, c->tslots.p);
+                       n, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p, c->tslots.p,
+                       c->runacc.p);
     HIPCHK(hipGetLastError());
-    int hc[1 + AM_WHY_COUNT] = {0};
-    HIPCHK(hipMemcpyAsync(hc, c->tcnt.p, sizeof(hc), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    float a = 0, b = 0;
-    HIPCHK(hipEventElapsedTime(&a, c->ev0, c->ev1));
-    HIPCHK(hipEventElapsedTime(&b, c->ev1, c->ev2));
-    t.chain_ms += a;
-    t.ray_ms += b;
-    t.ray_undecided += hc[0];
-    for (int w = 0; w < AM_WHY_COUNT; ++w) t.why[w] += hc[1 + w];
-    if (hc[0] > 0) {
-      io.count = hc[0];
-      io.flags = nullptr;
-      HIPCHK(hipEventRecord(c->ev0, c->stream));
-      hipLaunchKernelGGL(k_walk_fp64, dim3(hc[0]), dim3(64), wlds, c->stream, c->net, io, first + off,
-                         (const int*)c->tslots.p, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
-      HIPCHK(hipGetLastError());
-      HIPCHK(hipEventRecord(c->ev1, c->stream));
-      HIPCHK(hipEventSynchronize(c->ev1));
-      float sm = 0;
-      HIPCHK(hipEventElapsedTime(&sm, c->ev0, c->ev1));
-      t.scan_ms += sm;
-      t.scanned += hc[0];
-    }
+    io.flags = nullptr;
+    const size_t m3 = tm.mark(c->stream);
+    hipLaunchKernelGGL(k_walk_fp64, dim3(std::min(n, 2048)), dim3(64), wlds, c->stream, c->net, io, first + off,
+                       (const int*)c->tcnt.p, (const int*)c->tslots.p, (const double*)c->tplanes.p,
+                       (const double*)c->teq.p, Ch);
+    HIPCHK(hipGetLastError());
+    tm.span(m3, tm.mark(c->stream), T_SCAN64);
   }
 }
 
@@ -439,9 +419,9 @@ void upload_T(DevBuf<double>& dst, const void* src, int rows, int cols, int dtyp
   HIPCHK(hipMemcpy(dst.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
 }
 
-// W (rows x cols, row-major) -> W^T padded: [round_up(cols, AM_CPF)][round_up(rows, AM_CR)],
+// W (rows x cols, row-major) -> W^T padded: [AM_CHAIN_MAXW + 1][round_up(rows, AM_CR)],
 // zeros past `rows` and `cols`; b -> b padded to round_up(rows, AM_CR).  The tiled
-// chain reads whole AM_CR-row blocks and k in steps of AM_CPF.
+// chain reads whole AM_CR-row blocks, and row AM_CHAIN_MAXW (+0) for its padding.
 void upload_pad(DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const void* b, int rows, int cols,
                 int dtype, hipStream_t st) {
   const int rp = (rows + AM_CR - 1) / AM_CR * AM_CR;
@@ -457,7 +437,8 @@ void upload_pad(DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const
     for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
     for (int i = 0; i < rows; ++i) bh[i] = (double)fb[i];
   }
-  const int cp = (cols + AM_CPF - 1) / AM_CPF * AM_CPF;
+  // row AM_CHAIN_MAXW: the +0 row the chain's padding names (wider nets do not use the tiled chain)
+  const int cp = std::max(cols, AM_CHAIN_MAXW) + 1;
   std::vector<double> t((size_t)cp * rp, 0.0), bp(rp, 0.0);
   for (int r = 0; r < rows; ++r) {
     for (int q = 0; q < cols; ++q) t[(size_t)q * rp + r] = h[(size_t)r * cols + q];
@@ -566,9 +547,6 @@ am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_
     int optin = 0;
     if (hipDeviceGetAttribute(&optin, hipDeviceAttributeSharedMemPerBlockOptin, c->dev) == hipSuccess && optin > 0)
       c->max_lds = std::max<size_t>(c->max_lds, (size_t)optin);
-    HIPCHK(hipEventCreate(&c->ev0));
-    HIPCHK(hipEventCreate(&c->ev1));
-    HIPCHK(hipEventCreate(&c->ev2));
     HIPCHK(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
     c->stream = c->own_stream;
     return AM_OK;
@@ -724,6 +702,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)ray64_lds_bytes(c)));
 
+    const auto t_setup = std::chrono::steady_clock::now();
     // ---- seeds -> first level (backend/inc/states.h:147-176)
     c->R = 0;
     c->table_cap = 0;
@@ -751,30 +730,20 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     }
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
-    double t_region = 0, t_insert = 0, t_gemm = 0, t_walk = 0, t_redo = 0, t_chain64 = 0, t_walk64 = 0, t_ray64 = 0;
-    long long scanned = 0, ray_undecided = 0, chain_launches = 0;
     long long levels = 0, cands = 0, redo = 0, launches = 0;
     long long why[AM_WHY_COUNT] = {0};
+    Fp64Counts f64;
+    c->timer.reset();
+    ensure(c->runacc, 1 + AM_WHY_COUNT, false, st);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->runacc.p, (size_t)(1 + AM_WHY_COUNT), 0);
     while (lvl_count > 0) {
       if (c->max_regions >= 0 && lvl_first >= c->max_regions) break;
       const size_t ncand = (size_t)lvl_count * c->vm;
       grow_cands(c, ncand);
-      float ms = 0;
       ensure(c->rflags, (size_t)lvl_count, false, st);
       if (fp64_form) {
-        Fp64Times ft;
         run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr,
-                         c->precision == AM_PREC_FP64 ? 0 : 1, ft);
-        t_region += ft.chain_ms + ft.ray_ms + ft.scan_ms;
-        t_gemm += ft.chain_ms;
-        t_walk += ft.ray_ms + ft.scan_ms;
-        t_chain64 += ft.chain_ms;
-        t_walk64 += ft.scan_ms;
-        t_ray64 += ft.ray_ms;
-        scanned += ft.scanned;
-        ray_undecided += ft.ray_undecided;
-        chain_launches += ft.chain_launches;
-        for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
+                         c->precision == AM_PREC_FP64 ? 0 : 1, f64);
         launches += 1;
       } else {
         FastLevel fl;
@@ -786,35 +755,21 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.cand_point = c->cand_point.p;
         FastTimes ft{};
         int* ids = nullptr;
-        const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->ev0, c->ev1, ft, &ids);
+        const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, &ids);
         if (nredo > 0) {
           // FP64 re-decision of the undecided regions, in place (their records
           // and candidate slots are rewritten): FP64 planes, the checked FP64
           // ray walk, and the candidate scan where that is undecided too.
-          Fp64Times f64;
           run_fp64_regions(c, (int)lvl_first, (int)lvl_first, nredo, ids, 1, f64);
-          t_redo += f64.chain_ms + f64.ray_ms + f64.scan_ms;
-          t_chain64 += f64.chain_ms;
-          t_walk64 += f64.scan_ms;
-          t_ray64 += f64.ray_ms;
-          scanned += f64.scanned;
-          ray_undecided += f64.ray_undecided;
-          chain_launches += f64.chain_launches;
           redo += nredo;
         }
-        t_gemm += ft.gemm_ms;
-        t_walk += ft.walk_ms;
         for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
-        t_region += ft.gemm_ms + ft.walk_ms;
         launches += ft.launches;
       }
-      HIPCHK(hipEventRecord(c->ev0, st));
+      const size_t mi = c->timer.mark(st);
       const long long before = c->R;
-      const int added = insert_level(c, ncand);
-      HIPCHK(hipEventRecord(c->ev1, st));
-      HIPCHK(hipEventSynchronize(c->ev1));
-      HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
-      t_insert += ms;
+      const int added = insert_level(c, ncand);   // waits once: the next level's size
+      c->timer.span(mi, c->timer.mark(st), T_INSERT);
       cands += (long long)ncand;
       levels += 1;
       lvl_first = before;
@@ -824,25 +779,39 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       // regions found but not processed: drop them from the store
       c->R = lvl_first;
     }
+    int acc[1 + AM_WHY_COUNT] = {0};
+    HIPCHK(hipMemcpyAsync(acc, c->runacc.p, sizeof(acc), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    c->timer.resolve();
     const auto t1 = std::chrono::steady_clock::now();
+    const double* T = c->timer.acc;
+    const double fp64_ms = T[T_CHAIN64] + T[T_RAY64] + T[T_SCAN64] + T[T_REGION64];
     c->stats[AM_STAT_RUN_MS] = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    c->stats[AM_STAT_REGION_MS] = t_region;
-    c->stats[AM_STAT_INSERT_MS] = t_insert;
-    c->stats[AM_STAT_GEMM_MS] = t_gemm;
-    c->stats[AM_STAT_WALK_MS] = t_walk;
-    c->stats[AM_STAT_REDO_MS] = t_redo;
+    c->stats[AM_STAT_INSERT_MS] = T[T_INSERT];
+    if (fp64_form) {
+      c->stats[AM_STAT_REGION_MS] = fp64_ms;
+      c->stats[AM_STAT_GEMM_MS] = T[T_CHAIN64];
+      c->stats[AM_STAT_WALK_MS] = T[T_RAY64] + T[T_SCAN64] + T[T_REGION64];
+      c->stats[AM_STAT_REDO_MS] = 0;
+      for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] = acc[1 + w];
+    } else {
+      c->stats[AM_STAT_REGION_MS] = T[T_GEMM] + T[T_WALK];
+      c->stats[AM_STAT_GEMM_MS] = T[T_GEMM];
+      c->stats[AM_STAT_WALK_MS] = T[T_WALK];
+      c->stats[AM_STAT_REDO_MS] = fp64_ms;
+    }
     c->stats[AM_STAT_LEVELS] = (double)levels;
     c->stats[AM_STAT_REGIONS] = (double)c->R;
     c->stats[AM_STAT_CANDIDATES] = (double)cands;
     c->stats[AM_STAT_REDO_REGIONS] = (double)redo;
     c->stats[AM_STAT_REGION_LAUNCHES] = (double)launches;
-    c->stats[AM_STAT_CHAIN64_MS] = t_chain64;
-    c->stats[AM_STAT_WALK64_MS] = t_walk64;
-    c->stats[AM_STAT_RAY64_MS] = t_ray64;
-    c->stats[AM_STAT_SCAN64_REGIONS] = (double)scanned;
-    c->stats[AM_STAT_RAY64_UNDECIDED] = (double)ray_undecided;
-    c->stats[AM_STAT_CHAIN64_LAUNCHES] = (double)chain_launches;
+    c->stats[AM_STAT_CHAIN64_MS] = T[T_CHAIN64];
+    c->stats[AM_STAT_WALK64_MS] = T[T_SCAN64] + T[T_REGION64];
+    c->stats[AM_STAT_RAY64_MS] = T[T_RAY64];
+    c->stats[AM_STAT_SCAN64_REGIONS] = (double)(f64.scanned_host + acc[0]);
+    c->stats[AM_STAT_RAY64_UNDECIDED] = (double)acc[0];
+    c->stats[AM_STAT_CHAIN64_LAUNCHES] = (double)f64.chain_launches;
+    c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;
     return AM_OK;
@@ -1139,9 +1108,7 @@ am_status am_export_ply(am_ctx* c, const char* path, int is_polymesh, int is_flo
 
 am_status am_destroy(am_ctx* c) {
   if (!c) return AM_ERR_ARG;
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
-  if (c->ev2) (void)hipEventDestroy(c->ev2);
+  c->timer.release();
   fast_release(c->fbuf);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   hipStream_t own = c->own_stream;

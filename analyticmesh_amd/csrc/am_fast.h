@@ -7,6 +7,7 @@
 #include <stdint.h>
 
 #include "am_kernels.h"
+#include "am_timer.h"
 #include "am_walk.h"
 
 namespace am {
@@ -61,7 +62,6 @@ struct FastLevel {
 };
 
 struct FastTimes {
-  float gemm_ms, walk_ms;
   int launches;
   long long why[AM_WHY_COUNT];   // undecided regions per reason (AM_WHY_*), accumulated
 };
@@ -71,8 +71,9 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
                   const void* const* biases, int dtype, hipStream_t st);
 // Planes + checked walk for one level; returns the number of undecided
 // regions and their ids in *ids (device memory owned by `b`).
+// Kernel times go to `timer` (T_GEMM, T_WALK); the count read back is the only wait.
 int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-               hipEvent_t e0, hipEvent_t e1, FastTimes& t, int** ids);
+               StageTimer& timer, FastTimes& t, int** ids);
 // Planes of n given regions (their masks in `reg_mask`) with the prepared
 // reduced-precision form: planes [n][4][Cs] f32 (signed), eq [n][4] f32.
 void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st);

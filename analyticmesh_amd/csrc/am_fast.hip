@@ -440,10 +440,9 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
 }
 
 int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-               hipEvent_t e0, hipEvent_t e1, FastTimes& t, int** ids) {
+               StageTimer& timer, FastTimes& t, int** ids) {
   const FastNetDev& d = f.dev;
   const int prec = f.precision == 1 ? 1 : 0;
-  t.gemm_ms = t.walk_ms = 0;
   t.launches = 0;
   const int CH = 1 << 16;
   const int chunk = std::min(lvl.count, CH);
@@ -472,17 +471,14 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     g.count = n;
     g.planes = planes;
     g.eq = eqb;
-    float ms = 0;
-    FCHK(hipEventRecord(e0, st));
+    const size_t m0 = timer.mark(st);
     if (prec == 0)
       hipLaunchKernelGGL(k_gemm<0>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
     else
       hipLaunchKernelGGL(k_gemm<1>, dim3((n + AM_GT - 1) / AM_GT), dim3(256), glds, st, d, g);
     FCHK(hipGetLastError());
-    FCHK(hipEventRecord(e1, st));
-    FCHK(hipEventSynchronize(e1));
-    FCHK(hipEventElapsedTime(&ms, e0, e1));
-    t.gemm_ms += ms;
+    const size_t m1 = timer.mark(st);
+    timer.span(m0, m1, T_GEMM);
 
     RegionIO io;
     io.reg_mask = lvl.reg_mask;
@@ -501,7 +497,6 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     io.Csort = Csort;
     io.flags = flags;
     io.ids = nullptr;
-    FCHK(hipEventRecord(e0, st));
     if (f.walk_mode == 0)
       hipLaunchKernelGGL(k_walk_ray_f32, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
                          lvl.first + c0);
@@ -509,10 +504,7 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
       hipLaunchKernelGGL(k_walk_f32, dim3(n), dim3(64), wlds, st, net, io, f.wp, planes, eqb, d.Cs,
                          lvl.first + c0);
     FCHK(hipGetLastError());
-    FCHK(hipEventRecord(e1, st));
-    FCHK(hipEventSynchronize(e1));
-    FCHK(hipEventElapsedTime(&ms, e0, e1));
-    t.walk_ms += ms;
+    timer.span(m1, timer.mark(st), T_WALK);
     t.launches += 1;
   }
   FCHK(hipMemsetAsync(cnt, 0, 4 * (1 + AM_WHY_COUNT), st));

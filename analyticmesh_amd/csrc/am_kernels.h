@@ -51,7 +51,8 @@ __global__ void k_region_fp64(NetDev net, RegionIO io);
 #define AM_CR 16                // output rows per wave
 #define AM_CHAIN_MAXW 256       // widest layer the tiled chain takes
 #define AM_RAY_WAVES 4          // waves per region in the ray-shooting walks
-#define AM_CPF 16               // k steps of chain operands in flight (k is padded to a multiple)
+#define AM_CPF 16               // k steps of chain operands in flight (k lists are padded to a multiple)
+#define AM_KLS (AM_CHAIN_MAXW + 2 * AM_CPF)   // entries of one layer's active-input list
 #define AM_CHAIN_THREADS (64 * AM_CHAIN_MAXW / AM_CR)
 
 struct ChainIO {
@@ -70,12 +71,16 @@ __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
 // The walk of k_region_fp64 on planes from k_chain_tile_fp64.  Block b takes
 // planes slot s = slots[b] (or b); its region is io.ids[s] when ids are given,
 // else rid0 + s; candidate slots are numbered from io.first as in k_region_fp64.
-__global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* slots, const double* planes,
-                            const double* eq, int Cs_hbm);
+// With `count`, blocks at or past *count exit (a grid sized for the worst case).
+__global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count, const int* slots,
+                            const double* planes, const double* eq, int Cs_hbm);
 struct WalkParams;
 __global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, const double* planes,
                              const double* eq, int Cs_hbm);
+// Chunk-local slots of the regions k_walk_ray64 left undecided (cnt[0] counts
+// them); run[0] accumulates the count and run[1 + w] the reasons, per run.
 __global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
-                                int* slots);
+                                int* slots, int* run);
+__global__ void k_fill_i32(int* p, size_t n, int v);
 
 }  // namespace am

@@ -127,6 +127,7 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_ADJOINT_MS 12   /* am_adjoint kernels (wall time) */
 #define AM_STAT_CHAIN64_MS 13   /* FP64 tiled plane chain, all launches (HIP events) */
 #define AM_STAT_WALK64_MS 14    /* FP64 walk after the tiled chain, all launches (HIP events) */
+#define AM_STAT_SETUP_MS 15     /* am_run set-up: weight uploads and operand layouts (wall time) */
 #define AM_STAT_WHY0 16         /* 16..27: regions the reduced-precision walk left to FP64, per reason:
                                    seed shot, start edge, plane already in polygon, sign within bound,
                                    proposed vertex infeasible, ill-conditioned system, turn within bound,
