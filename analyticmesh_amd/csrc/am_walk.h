@@ -49,7 +49,7 @@ __device__ __forceinline__ T hadamard_ratio(const T* p1, const T* p2, const T* p
   return (n1 * n2 * n3) / absT(d);
 }
 
-// Error bound of constraint signs at the vertex v of the system [p1; p2; p3]
+// Error bound of constraint signs at the vertex v of the system A = [p1; p2; p3]
 // when every plane's coefficients carry an error of at most eps times its
 // largest coefficient.  Perturbing the system by E moves the vertex by
 // dv = -A^-1 (E v + e) to first order, so constraint g's sign moves by
@@ -59,11 +59,15 @@ __device__ __forceinline__ T hadamard_ratio(const T* p1, const T* p2, const T* p
 // where kappa = 1 / (1 - 2 rho), rho = eps (1 + H) (H the Hadamard ratio),
 // covers the second-order term; systems with rho > 1/4 are not trusted.
 // eps includes the working-precision rounding of the vertex solve.
+// Since |lambda_g,i| <= |n_i|_1 max|a_g| (n_i the rows of A^-T), the bound is
+// at most max|P_g| E with the per-vertex constant E = eps (1 + |v|_1) +
+// sum_i kappa eps m_i |n_i|_1.  FP64 planes (eps ~ 1e-13) use that cheap form;
+// coarse planes keep the per-constraint lambda form, which certifies more.
 template <typename T>
 struct VertexBound {
-  T n1[3], n2[3], n3[3];   // rows of A^-T
-  T w1, w2, w3;            // kappa eps m_i
-  T e0;                    // eps (1 + |v|_1)
+  T E;                     // FP64: err_g = E max|P_g|
+  T n1[3], n2[3], n3[3];   // coarse planes: rows of A^-T
+  T w1, w2, w3, e0;        // coarse planes: kappa eps m_i, eps (1 + |v|_1)
   bool ok;
 };
 
@@ -80,8 +84,10 @@ __device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2,
   const T c2[3] = {p3[1] * p1[2] - p3[2] * p1[1], p3[2] * p1[0] - p3[0] * p1[2], p3[0] * p1[1] - p3[1] * p1[0]};
   const T c3[3] = {p1[1] * p2[2] - p1[2] * p2[1], p1[2] * p2[0] - p1[0] * p2[2], p1[0] * p2[1] - p1[1] * p2[0]};
   const T det = p1[0] * c1[0] + p1[1] * c1[1] + p1[2] * c1[2];
-  const T h = __builtin_sqrt(plane_nn(p1[0], p1[1], p1[2])) * __builtin_sqrt(plane_nn(p2[0], p2[1], p2[2])) *
-              __builtin_sqrt(plane_nn(p3[0], p3[1], p3[2])) / absT(det);
+  const T adet = absT(det);
+  const T iadet = T(1) / adet;
+  const T h = __builtin_sqrt(plane_nn(p1[0], p1[1], p1[2]) * plane_nn(p2[0], p2[1], p2[2]) *
+                             plane_nn(p3[0], p3[1], p3[2])) * iadet;
   // Cramer's rule in working precision adds an error equivalent to perturbing
   // the system by about 16 u H of its coefficients (each 3x3 expansion's
   // absolute terms sum to at most 5.2 H |det|, with three roundings each)
@@ -92,12 +98,21 @@ __device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2,
   const T vn = T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]);
   // 1.001: slack for the rounding of these bound computations themselves
   const T kappa = T(1.001) / (T(1) - T(2) * rho);
-  const T inv = T(1) / det;
-  for (int i = 0; i < 3; ++i) { b.n1[i] = c1[i] * inv; b.n2[i] = c2[i] * inv; b.n3[i] = c3[i] * inv; }
-  b.w1 = kappa * eps * coef_max(p1) * vn;
-  b.w2 = kappa * eps * coef_max(p2) * vn;
-  b.w3 = kappa * eps * coef_max(p3) * vn;
-  b.e0 = T(1.001) * eps * vn;
+  // |n_i|_1 = |c_i|_1 / |det|
+  const T l1 = absT(c1[0]) + absT(c1[1]) + absT(c1[2]);
+  const T l2 = absT(c2[0]) + absT(c2[1]) + absT(c2[2]);
+  const T l3 = absT(c3[0]) + absT(c3[1]) + absT(c3[2]);
+  const T m1 = coef_max(p1) * vn, m2 = coef_max(p2) * vn, m3 = coef_max(p3) * vn;
+  const T w = kappa * eps * (m1 * l1 + m2 * l2 + m3 * l3) * iadet;
+  b.E = T(1.001) * (eps * vn + w);
+  if (sizeof(T) == 4) {
+    const T inv = T(1) / det;
+    for (int i = 0; i < 3; ++i) { b.n1[i] = c1[i] * inv; b.n2[i] = c2[i] * inv; b.n3[i] = c3[i] * inv; }
+    b.w1 = kappa * eps * m1;
+    b.w2 = kappa * eps * m2;
+    b.w3 = kappa * eps * m3;
+    b.e0 = T(1.001) * eps * vn;
+  }
   return b;
 }
 
@@ -107,10 +122,15 @@ template <typename T>
 __device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, const T* v, const VertexBound<T>& b) {
   const T s = plane_eval(ax, ay, az, c, v);
   const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
-  const T l1 = b.n1[0] * ax + b.n1[1] * ay + b.n1[2] * az;
-  const T l2 = b.n2[0] * ax + b.n2[1] * ay + b.n2[2] * az;
-  const T l3 = b.n3[0] * ax + b.n3[1] * ay + b.n3[2] * az;
-  const T err = b.e0 * pm + b.w1 * absT(l1) + b.w2 * absT(l2) + b.w3 * absT(l3);
+  T err;
+  if (sizeof(T) == 4) {
+    const T l1 = b.n1[0] * ax + b.n1[1] * ay + b.n1[2] * az;
+    const T l2 = b.n2[0] * ax + b.n2[1] * ay + b.n2[2] * az;
+    const T l3 = b.n3[0] * ax + b.n3[1] * ay + b.n3[2] * az;
+    err = b.e0 * pm + b.w1 * absT(l1) + b.w2 * absT(l2) + b.w3 * absT(l3);
+  } else {
+    err = b.E * pm;
+  }
   if (s > err) return 1;
   if (s >= -err) return 2;
   return 0;
@@ -569,6 +589,19 @@ __device__ __forceinline__ void wave_argmin(T& t, int& idx) {
   }
 }
 
+// Cramer's rule with one reciprocal (the checked walks' vertex; its extra
+// rounding is inside the bound's 32 u (1 + H) term).
+template <typename T>
+__device__ __forceinline__ void solve3_rcp(const T* p1, const T* p2, const T* p3, T* x) {
+  const T A0 = p1[0], A1 = p1[1], A2 = p1[2], A3 = p2[0], A4 = p2[1], A5 = p2[2], A6 = p3[0], A7 = p3[1],
+          A8 = p3[2];
+  const T b0 = -p1[3], b1 = -p2[3], b2 = -p3[3];
+  const T r = T(1) / AM_DET3(A0, A1, A2, A3, A4, A5, A6, A7, A8);
+  x[0] = AM_DET3(b0, A1, A2, b1, A4, A5, b2, A7, A8) * r;
+  x[1] = AM_DET3(A0, b0, A2, A3, b1, A5, A6, b2, A8) * r;
+  x[2] = AM_DET3(A0, A1, b0, A3, A4, b1, A6, A7, b2) * r;
+}
+
 // Block-wide reductions for walks run by NW waves (one region per block).
 // Each wave reduces with shuffles, then wave results meet in LDS; the combine
 // runs in wave order, so the result is the same on every thread.
@@ -627,6 +660,91 @@ __device__ __forceinline__ void shoot(const T* Px, const T* Py, const T* Pz, con
   block_argmin<T, NW>(tb, gb);
   tmin = tb;
   gmin = gb;
+}
+
+// argmin (t, g) and OR of verdict bits over the block, with one LDS exchange.
+template <typename T, int NW>
+__device__ __forceinline__ void block_argmin_or(T& t, int& g, int& bits) {
+  wave_argmin(t, g);
+  int b = (__any(bits & 1) ? 1 : 0) | (__any(bits & 2) ? 2 : 0);
+  if (NW == 1) { bits = b; return; }
+  __shared__ T red_t[NW];
+  __shared__ int red_g[NW];
+  __shared__ int red_b[NW];
+  const int wave = threadIdx.x >> 6;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) { red_t[wave] = t; red_g[wave] = g; red_b[wave] = b; }
+  __syncthreads();
+  t = red_t[0];
+  g = red_g[0];
+  b = red_b[0];
+  for (int w = 1; w < NW; ++w) {
+    if (red_t[w] < t || (red_t[w] == t && red_g[w] < g)) { t = red_t[w]; g = red_g[w]; }
+    b |= red_b[w];
+  }
+  bits = b;
+}
+
+// One pass over the constraints for a proposed vertex v: its checked verdict
+// bits over all constraints but c1/c2 (bit 0 certainly violated, bit 1
+// undecided), and the nearest plane crossed by the ray v + t d, skipping
+// s1/s2 (the next step's proposal; meaningful only when the bits are 0).
+template <typename T, int NW>
+__device__ __forceinline__ int check_and_shoot(const T* Px, const T* Py, const T* Pz, const T* Pc, int C,
+                                               const T* v, const VertexBound<T>& vb, int c1, int c2, const T* d,
+                                               int s1, int s2, T& tmin, int& gmin) {
+  int bits = 0;
+  T nb = T(1), db = T(0);
+  int gb = 0x7fffffff;
+#pragma unroll 2
+  for (int g = threadIdx.x; g < C; g += 64 * NW) {
+    const T ax = Px[g], ay = Py[g], az = Pz[g], c = Pc[g];
+    if (g != c1 && g != c2) {
+      const int vg = checked_verdict(ax, ay, az, c, v, vb);
+      bits |= vg == 1 ? 1 : (vg == 2 ? 2 : 0);
+    }
+    if (g != s1 && g != s2) {
+      const T al = fmaT(ax, d[0], fmaT(ay, d[1], az * d[2]));
+      if (al > T(0)) {
+        const T num = -plane_eval(ax, ay, az, c, v);
+        if (db == T(0) || num * db < nb * al) { nb = num; db = al; gb = g; }
+      }
+    }
+  }
+  T tb = db > T(0) ? nb / db : T(__builtin_inff());
+  block_argmin_or<T, NW>(tb, gb, bits);
+  tmin = tb;
+  gmin = gb;
+  return bits;
+}
+
+// Nearest planes crossed by x + t d for t > 0 and for t < 0 (the ray along
+// -d), skipping ex1, in one pass: (tp, gp) along +d, (tm, gm) along -d.
+template <typename T, int NW>
+__device__ __forceinline__ void shoot_both(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* x,
+                                           const T* d, int ex1, T& tp, int& gp, T& tm, int& gm) {
+  T np = T(1), dp = T(0), nm = T(1), dmn = T(0);
+  int bp = 0x7fffffff, bm = 0x7fffffff;
+#pragma unroll 2
+  for (int g = threadIdx.x; g < C; g += 64 * NW) {
+    if (g == ex1) continue;
+    const T al = fmaT(Px[g], d[0], fmaT(Py[g], d[1], Pz[g] * d[2]));
+    if (al == T(0)) continue;
+    const T num = -plane_eval(Px[g], Py[g], Pz[g], Pc[g], x);
+    if (al > T(0)) {
+      if (dp == T(0) || num * dp < np * al) { np = num; dp = al; bp = g; }
+    } else {
+      // along -d the crossing parameter is num / (-al)
+      const T a2 = -al;
+      if (dmn == T(0) || num * dmn < nm * a2) { nm = num; dmn = a2; bm = g; }
+    }
+  }
+  T t1 = dp > T(0) ? np / dp : T(__builtin_inff());
+  T t2 = dmn > T(0) ? nm / dmn : T(__builtin_inff());
+  int z = 0;
+  block_argmin_or<T, NW>(t1, bp, z);
+  block_argmin_or<T, NW>(t2, bm, z);
+  tp = t1; gp = bp; tm = t2; gm = bm;
 }
 
 // Checked verdict bits of vertex v over all constraints but ex1/ex2, for the
@@ -688,11 +806,9 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     const T pd[4] = {d[0], d[1], d[2], -(d[0] * p[0] + d[1] * p[1] + d[2] * p[2])};
     T x0[3];
     solve3(ps, eq, pd, x0);
-    const T dm[3] = {-d[0], -d[1], -d[2]};
     T th, tl;
     int rh, rl;
-    shoot<T, NW>(Px, Py, Pz, Pc, C, x0, d, start, -1, th, rh);
-    shoot<T, NW>(Px, Py, Pz, Pc, C, x0, dm, start, -1, tl, rl);
+    shoot_both<T, NW>(Px, Py, Pz, Pc, C, x0, d, start, th, rh, tl, rl);
     if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) {
       undecided = true;
       why |= AM_WHY_BIT(AM_WHY_START);
@@ -712,9 +828,14 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
       const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
       T v[3];
-      solve3(pr4, pc4, eq, v);
+      solve3_rcp(pr4, pc4, eq, v);
       const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, v, T(wp.eps), T(wp.h_max));
-      const int bits = block_check<T, NW>(Px, Py, Pz, Pc, C, cur, r, v, vb);
+      // the next edge runs along plane r, away from the plane we came along;
+      // its nearest crossing is found in the same pass as this vertex's check
+      T dn[3] = {pr4[1] * eq[2] - pr4[2] * eq[1], pr4[2] * eq[0] - pr4[0] * eq[2], pr4[0] * eq[1] - pr4[1] * eq[0]};
+      if (pc4[0] * dn[0] + pc4[1] * dn[1] + pc4[2] * dn[2] > T(0)) { dn[0] = -dn[0]; dn[1] = -dn[1]; dn[2] = -dn[2]; }
+      int nx;
+      const int bits = check_and_shoot<T, NW>(Px, Py, Pz, Pc, C, v, vb, cur, r, dn, r, cur, tdummy, nx);
       if (bits != 0 || !vb.ok) {
         undecided = true;
         if (bits & 1) why |= AM_WHY_BIT(AM_WHY_VIOL);
@@ -770,14 +891,8 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       ntab += 1;
       if (ntab >= 4 && s_tab[0] == s_tab[ntab - 1]) break;          // closed
       if (ntab >= vm + 1) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TRUNC); break; }   // truncated: FP64 decides
-      // walk along the new edge line, away from the plane we came along
-      const int prev = cur;
+      // walk along the new edge line (found by the fused pass above)
       cur = r;
-      const T pc[3] = {Px[cur], Py[cur], Pz[cur]};
-      T d[3] = {pc[1] * eq[2] - pc[2] * eq[1], pc[2] * eq[0] - pc[0] * eq[2], pc[0] * eq[1] - pc[1] * eq[0]};
-      if (Px[prev] * d[0] + Py[prev] * d[1] + Pz[prev] * d[2] > T(0)) { d[0] = -d[0]; d[1] = -d[1]; d[2] = -d[2]; }
-      int nx;
-      shoot<T, NW>(Px, Py, Pz, Pc, C, v, d, cur, prev, tdummy, nx);
       if (nx == NONE) { undecided = true; why |= AM_WHY_BIT(AM_WHY_NEXT); break; }
       r = nx;
     }
