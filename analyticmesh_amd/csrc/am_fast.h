@@ -25,7 +25,7 @@ struct FastNetDev {
   int nkb[AM_MAX_LINEAR];       // 4-wide (fp32) or 32-wide (bf16) k blocks of layer li
   const float* W0;              // (n1, 3)
   const float* bias[AM_MAX_LINEAR];
-  const float* Wp[AM_MAX_LINEAR];       // fp32 A operands, [nrb][nkb][64 lanes]
+  const float* Wp[AM_MAX_LINEAR];       // fp32 W^T, k-major [n_in + 1][nrb * 16], row n_in = +0
   const uint16_t* Wh[AM_MAX_LINEAR];    // bf16 hi A operands, [nrb][nkb][64 lanes][8]
   const uint16_t* Wl[AM_MAX_LINEAR];    // bf16 lo A operands
   const float* ew;

@@ -5,7 +5,9 @@
 // is one GEMM per layer, W_l[n_out x n_in] x X[n_in x 4*AM_GT], on MFMA with
 // X staged in LDS (backend/inc/process.h:18-193 does the same chain as
 // per-region cuBLAS calls).  Two arithmetic forms:
-//   * FP32: v_mfma_f32_16x16x4f32;
+//   * FP32: v_mfma_f32_16x16x4f32 over the inputs active in some region of the
+//     tile only (the others are +0 in every column); A rows are gathered from
+//     a k-major copy of W^T, X is staged already compacted;
 //   * split-BF16: w = wh + wl, x = xh + xl (bf16 parts), y = wh.xh + wh.xl + wl.xh
 //     with v_mfma_f32_16x16x32_bf16 and FP32 accumulation.
 // The signed planes go to HBM, one SoA record per region, and the checked FP32
@@ -40,18 +42,27 @@ __device__ __forceinline__ uint16_t bf16_rn(float x) {
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 // Stage layer-1 planes: X = masked rows of W0 (and b0); signed planes to HBM.
+// FP32: X holds only the tile's active inputs (kl1, nkp1 entries).
 template <int PREC>
 __device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32_t* mk, int r0, int nreg,
-                             float* Xs, uint16_t* Xh, uint16_t* Xl, int KS) {
+                             float* Xs, uint16_t* Xh, uint16_t* Xl, int KS, const uint16_t* kl1, int nkp1) {
   const int tid = threadIdx.x;
   const int n1 = f.nodes[1];
-  for (int idx = tid; idx < n1 * AM_GN; idx += 256) {
+  if (PREC == 0) {
+    // X rows = the tile's active inputs of layer 1, in list order (+0 past the list)
+    for (int idx = tid; idx < nkp1 * AM_GN; idx += 256) {
+      const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2, q = col & 3;
+      const int k = kl1[j];
+      float x = 0.f;
+      if (k < n1 && mbit(mk + t * f.Wd, k)) x = q < 3 ? f.W0[k * 3 + q] : f.bias[0][k];
+      Xs[j * AM_XS + col] = x;
+    }
+  }
+  for (int idx = tid; PREC != 0 && idx < n1 * AM_GN; idx += 256) {
     const int k = idx / AM_GN, col = idx - k * AM_GN, t = col >> 2, q = col & 3;
     const float val = q < 3 ? f.W0[k * 3 + q] : f.bias[0][k];
     const float x = mbit(mk + t * f.Wd, k) ? val : 0.f;
-    if (PREC == 0) {
-      Xs[k * AM_XS + col] = x;
-    } else {
+    {
       const uint16_t h = bf16_rn(x);
       Xh[col * KS + k] = h;
       Xl[col * KS + k] = bf16_rn(x - bf16_to_f(h));
@@ -81,6 +92,9 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   uint16_t* Xl = Xh + (size_t)AM_GN * KS;
   uint32_t* mk = reinterpret_cast<uint32_t*>(PREC == 0 ? (unsigned char*)(Xs + (size_t)f.Kpad * AM_XS)
                                                        : (unsigned char*)(Xl + (size_t)AM_GN * KS));
+  const int KLF = f.Kpad + 4;                                        // entries of one layer's input list
+  uint16_t* klist = reinterpret_cast<uint16_t*>(mk + AM_GT * f.Wd);  // fp32: [D][KLF]
+  __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_GT;
   const int nreg = min(AM_GT, io.count - r0);
 
@@ -94,7 +108,31 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
     for (int i = tid; i < 2 * AM_GN * KS; i += 256) Xh[i] = 0;
   }
   __syncthreads();
-  stage_layer1<PREC>(f, io, mk, r0, nreg, Xs, Xh, Xl, KS);
+  if (PREC == 0) {
+    // Inputs of linear layer li that are active in some region of the tile,
+    // ascending, padded to a multiple of 4 with n_li (the +0 row of W^T and,
+    // through the staging below, of X).  Inactive inputs are +0 in every
+    // column, so the GEMM over the list equals the full one.
+    for (int li = wave + 1; li <= f.D; li += 4) {
+      const int nin = f.nodes[li], in0 = li == 1 ? 0 : f.off[li - 1];
+      uint16_t* kl = klist + (size_t)(li - 1) * KLF;
+      int base = 0;
+      for (int kb = 0; kb < nin; kb += 64) {
+        const int k = kb + lane;
+        bool act = false;
+        if (k < nin)
+          for (int t = 0; t < AM_GT; ++t) act = act || mbit(mk + t * f.Wd, in0 + k);
+        const unsigned long long bal = __ballot(act);
+        if (act) kl[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = (uint16_t)k;
+        base += __popcll(bal);
+      }
+      const int nkp = (base + 3) & ~3;
+      for (int i = base + lane; i < nkp; i += 64) kl[i] = (uint16_t)nin;
+      if (lane == 0) s_nk[li] = nkp;
+    }
+    __syncthreads();
+  }
+  stage_layer1<PREC>(f, io, mk, r0, nreg, Xs, Xh, Xl, KS, klist, PREC == 0 ? s_nk[1] : 0);
   __syncthreads();
 
   for (int li = 1; li <= f.D; ++li) {
@@ -104,11 +142,16 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
     for (int rb = wave; rb < nrb; rb += 4) {
       f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
       if (PREC == 0) {
-        const float* Wp = f.Wp[li] + (size_t)rb * nkb * 64 + lane;
+        // A[row][kb*4 + (lane >> 4)] = W^T[list entry][row] (k-major, rows padded to 16)
+        const int nkl = s_nk[li] >> 2;
+        const uint16_t* kl = klist + (size_t)(li - 1) * KLF + (lane >> 4);
+        const float* Wc = f.Wp[li] + rb * 16 + (lane & 15);
+        const int ld = nrb * 16;
         const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
         int kb = 0;
-        for (; kb + 4 <= nkb; kb += 4) {
-          const float a0 = Wp[(kb + 0) * 64], a1 = Wp[(kb + 1) * 64], a2 = Wp[(kb + 2) * 64], a3 = Wp[(kb + 3) * 64];
+        for (; kb + 4 <= nkl; kb += 4) {
+          const float a0 = Wc[(size_t)kl[(kb + 0) * 4] * ld], a1 = Wc[(size_t)kl[(kb + 1) * 4] * ld];
+          const float a2 = Wc[(size_t)kl[(kb + 2) * 4] * ld], a3 = Wc[(size_t)kl[(kb + 3) * 4] * ld];
           const float* x0 = xb + (kb * 4) * AM_XS;
           acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[0], acc0, 0, 0, 0);
           acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[16], acc1, 0, 0, 0);
@@ -119,8 +162,8 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
           acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS], acc0, 0, 0, 0);
           acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS + 16], acc1, 0, 0, 0);
         }
-        for (; kb < nkb; ++kb) {
-          const float a = Wp[kb * 64];
+        for (; kb < nkl; ++kb) {
+          const float a = Wc[(size_t)kl[kb * 4] * ld];
           const float* x0 = xb + (kb * 4) * AM_XS;
           acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[0], acc0, 0, 0, 0);
           acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[16], acc1, 0, 0, 0);
@@ -171,13 +214,21 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
           io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + out0 + j] = mbit(mk + t * f.Wd, out0 + j) ? -y : y;
         }
       }
-      const int rows = PREC == 0 ? ((nout + 3) & ~3) : ((nout + 31) & ~31);
+      if (PREC == 0) {
+        // next layer's X: its list's rows of the masked outputs (+0 for the padding entries)
+        const uint16_t* kl2 = klist + (size_t)li * KLF;
+        const int nkp2 = s_nk[li + 1];
+        for (int idx = tid; idx < nkp2 * AM_GN; idx += 256) {
+          const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2;
+          const int k = kl2[j];
+          Xs[j * AM_XS + col] = (k < nout && mbit(mk + t * f.Wd, out0 + k)) ? Ys[k * AM_XS + col] : 0.f;
+        }
+      }
+      const int rows = PREC == 0 ? 0 : ((nout + 31) & ~31);
       for (int idx = tid; idx < rows * AM_GN; idx += 256) {
         const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2;
         const float x = (j < nout && mbit(mk + t * f.Wd, out0 + j)) ? Ys[j * AM_XS + col] : 0.f;
-        if (PREC == 0) {
-          Xs[j * AM_XS + col] = x;
-        } else {
+        {
           const uint16_t h = bf16_rn(x);
           Xh[col * KS + j] = h;
           Xl[col * KS + j] = bf16_rn(x - bf16_to_f(h));
@@ -328,7 +379,8 @@ float host_bf16_f(uint16_t h) {
 size_t gemm_lds(const FastNetDev& d, int prec) {
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
   const size_t xs = prec == 0 ? (size_t)d.Kpad * AM_XS * 4 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;
-  return ys + xs + (size_t)AM_GT * d.Wd * 4 + 16;
+  const size_t kls = prec == 0 ? (size_t)d.D * (d.Kpad + 4) * 2 : 0;
+  return ys + xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
 }
 
 }  // namespace
@@ -389,15 +441,12 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
     const int nrb = (nout + 15) / 16;
     d.nrb[li] = nrb;
     if (prec == 0) {
-      const int nkb = (nin + 3) / 4;
-      d.nkb[li] = nkb;
-      std::vector<float> a((size_t)nrb * nkb * 64, 0.f);
-      for (int rb = 0; rb < nrb; ++rb)
-        for (int kb = 0; kb < nkb; ++kb)
-          for (int l = 0; l < 64; ++l) {
-            const int row = rb * 16 + (l & 15), k = kb * 4 + (l >> 4);
-            if (row < nout && k < nin) a[((size_t)rb * nkb + kb) * 64 + l] = w[(size_t)row * nin + k];
-          }
+      // W^T, k-major: [nin + 1][nrb * 16]; row nin (and rows past nout) are +0
+      d.nkb[li] = (nin + 3) / 4;
+      const int ld = nrb * 16;
+      std::vector<float> a((size_t)(nin + 1) * ld, 0.f);
+      for (int k = 0; k < nin; ++k)
+        for (int row = 0; row < nout; ++row) a[(size_t)k * ld + row] = w[(size_t)row * nin + k];
       float* p = (float*)slot(b, S_WP0 + li, a.size() * 4);
       FCHK(hipMemcpy(p, a.data(), a.size() * 4, hipMemcpyHostToDevice));
       d.Wp[li] = p;
@@ -430,7 +479,6 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
     else
       FCHK(hipFuncSetAttribute((const void*)k_gemm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  // error bounds of the planes, relative to each plane's largest coefficient
   // Relative error bounds of the planes (relative to each plane's largest
   // coefficient): about 3x the largest error measured against FP64 planes on
   // the 4x256 and 2x64 networks (FP32: 2.6e-5, split-BF16: 5.1e-4).

@@ -64,8 +64,57 @@ def run_form(spec, pts, states, precision, steps, warmup):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     st = np.mean(stats, axis=0)
+    V, _ = res.combine()                      # outside the timed region: the mesh for the checks below
     faces = int((res.info[:, 2] >= 3).sum())
-    return dict(sec_per_step=dt, faces=faces, regions=int(st[7]), stats=st, res=res, eng=eng)
+    return dict(sec_per_step=dt, faces=faces, regions=int(st[7]), stats=st, res=res, eng=eng, V=V)
+
+
+def residual_fp64(spec, V):
+    """max |f(v)| over the mesh vertices, f evaluated in FP64 (torch on the GPU)."""
+    import torch
+    x = torch.from_numpy(np.ascontiguousarray(V, dtype=np.float64)).cuda()
+    h = x
+    for i, (w, b) in enumerate(zip(spec.weights, spec.biases)):
+        h = h @ torch.from_numpy(w.astype(np.float64)).cuda().T + torch.from_numpy(b.astype(np.float64)).cuda()
+        if i + 1 < len(spec.weights):
+            h = torch.relu(h)
+    return float(h.abs().max().item())
+
+
+def vertex_error_vs(res, ref, L):
+    """Largest vertex difference (relative to |v|) between two runs with the same regions, matched by
+    region mask and, within a region, by the vertex's pair of tight constraints."""
+    idx = {m.tobytes(): i for i, m in enumerate(ref.masks)}
+    perm = np.fromiter((idx[m.tobytes()] for m in res.masks), dtype=np.int64, count=len(res.masks))
+    big = np.int64(1) << 40
+
+    def keyed(verts, pairs, nv):
+        p = np.sort(pairs.astype(np.int64), axis=2)
+        key = p[:, :, 0] * big + p[:, :, 1]
+        slot = np.arange(key.shape[1])[None, :]
+        key = np.where(slot < nv[:, None], key, np.iinfo(np.int64).max)
+        o = np.argsort(key, axis=1)
+        return np.take_along_axis(verts, o[:, :, None], axis=1), np.take_along_axis(key, o, axis=1)
+
+    nv = res.info[:, 2].astype(np.int64)
+    va, ka = keyed(res.verts, res.vpairs, nv)
+    vb, kb = keyed(ref.verts[perm], ref.vpairs[perm], ref.info[perm, 2].astype(np.int64))
+    valid = (np.arange(va.shape[1])[None, :] < nv[:, None]) & (nv[:, None] >= 3)
+    if not np.array_equal(np.where(valid, ka, 0), np.where(valid, kb, 0)):
+        return None
+    d = np.abs(va - vb).max(axis=2) / np.maximum(np.abs(vb).max(axis=2), 1e-12)
+    return float(np.where(valid, d, 0.0).max())
+
+
+def traversal_bytes(res, Wd, L):
+    """Algorithmic HBM bytes of the visited-set stage: R*4W (pop mask) + E*(4W+4) (one probe per emitted
+    hidden-plane edge) + R*(8W+16) (insert key + frontier record)."""
+    R = len(res.masks)
+    ntab = res.info[:, 3].astype(np.int64)
+    pos = np.arange(res.tabs.shape[1])[None, :]
+    t = res.tabs
+    E = int(((pos < (ntab - 1)[:, None]) & (t >= 0) & (t < L)).sum())
+    return R * 4 * Wd + E * (4 * Wd + 4) + R * (8 * Wd + 16), E
 
 
 def copy_calibration(mb):
@@ -212,14 +261,28 @@ def main():
     roofline = lines[0]
 
     topo = {}
+    vert_err = {}
     if out.get("fp64"):
-        ref = out["fp64"]["res"].records()
+        log("[bench] topology and vertex error against FP64")
+        ref_res = out["fp64"]["res"]
+        ref = ref_res.records()
         for f in forms:
             if f != "fp64" and out.get(f):
                 r = out[f]["res"].records()
                 same = (r.visited() == ref.visited() and r.faces() == ref.faces()
                         and r.vertex_keys() == ref.vertex_keys())
                 topo[f] = "identical" if same else "DIFFERENT"
+                if same:
+                    vert_err[f] = vertex_error_vs(out[f]["res"], ref_res, spec.L)
+    resid = {f: residual_fp64(spec, out[f]["V"]) for f in forms if out.get(f)}
+    # traversal (visited set) stage against HBM bandwidth
+    tbytes, n_edges = traversal_bytes(h["res"], (spec.L + 31) // 32, spec.L)
+    ins_ms = float(st[2])
+    traversal = {"bound": "hbm", "stage": "visited-set insert + compaction", "bytes": int(tbytes),
+                 "emitted_edges": n_edges, "ms": round(ins_ms, 3),
+                 "achieved": round(tbytes / (ins_ms / 1e3) / 1e9, 1) if ins_ms > 0 else None,
+                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": round(tbytes / (ins_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if ins_ms > 0 else None}
 
     line = {
         "metric": "mesh faces/s, 4x256 ReLU MLP: plain FP32 vs split-BF16 MFMA, same topology",
@@ -238,6 +301,10 @@ def main():
                    "faces": h["faces"], "regions": h["regions"], "form": head},
         "forms": {f: (None if not out.get(f) else {
             "faces_per_s": round(out[f]["faces"] / out[f]["sec_per_step"], 1),
+            "vertices": int(len(out[f]["V"])),
+            "vertices_per_s": round(len(out[f]["V"]) / out[f]["sec_per_step"], 1),
+            "max_abs_f_at_vertices_fp64": resid.get(f),
+            "max_vertex_err_vs_fp64": vert_err.get(f),
             "ms_per_step": round(out[f]["sec_per_step"] * 1e3, 3),
             "faces": out[f]["faces"],
             "run_ms": round(float(out[f]["stats"][0]), 3),
@@ -258,6 +325,7 @@ def main():
         "topology_vs_fp64": topo,
         "roofline": roofline,
         "roofline_other": lines[1:],
+        "traversal": traversal,
     }
     if args.extras:
         log("[bench] extras: 64-MLP batch and adjoint")
