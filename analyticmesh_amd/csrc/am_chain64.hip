@@ -275,7 +275,8 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
 // rounding of the walk's own evaluations; regions it cannot certify are
 // flagged (io.flags, level-relative) for the candidate scan.
 __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
-                                                   const double* planes, const double* eqb, int Cs_hbm) {
+                                                   const double* planes, const double* eqb, int Cs_hbm,
+                                                   int only_undecided) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
   double* Px = reinterpret_cast<double*>(smem_raw);
@@ -286,6 +287,8 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, Re
   const int lane = threadIdx.x;
   if (blockIdx.x >= io.count) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
+  // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
+  if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
   for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)blockIdx.x * 4 * Cs_hbm;
   for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
@@ -302,6 +305,37 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, Re
                         eqb[(size_t)blockIdx.x * 4 + 2], eqb[(size_t)blockIdx.x * 4 + 3]};
   __syncthreads();
   walk_ray<double, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+}
+
+// Vertices of a region whose polygon (the tight-constraint pair of every
+// vertex, its order and orientation) a reduced-precision walk certified: each
+// is the intersection of the FP64 planes of its pair with the face plane,
+// solved as the FP64 ray walk solves it (new constraint, constraint walked
+// along, face plane), so the coordinates carry FP64 rounding only.
+__global__ __launch_bounds__(64) void k_vertex_fp64(NetDev net, RegionIO io, int rid0, const double* planes,
+                                                    const double* eqb, int Cs_hbm) {
+  const int b = blockIdx.x;
+  if (b >= io.count) return;
+  const int rid = io.ids ? io.ids[b] : rid0 + b;
+  if (io.flags && (io.flags[rid - io.first] & AM_UNDECIDED)) return;   // left to the FP64 walk
+  const int vm = net.vm;
+  const int nv = io.reg_info[(size_t)rid * AM_INFO_STRIDE + 2];
+  const double* src = planes + (size_t)b * 4 * Cs_hbm;
+  const double eq[4] = {eqb[(size_t)b * 4 + 0], eqb[(size_t)b * 4 + 1], eqb[(size_t)b * 4 + 2],
+                        eqb[(size_t)b * 4 + 3]};
+  for (int i = threadIdx.x; i < nv && i < vm; i += 64) {
+    const int r = io.reg_vpair[((size_t)rid * vm + i) * 2 + 0];
+    const int s = io.reg_vpair[((size_t)rid * vm + i) * 2 + 1];
+    if (r < 0 || r >= net.C || s < 0 || s >= net.C) continue;
+    const double pr[4] = {src[r], src[Cs_hbm + r], src[2 * (size_t)Cs_hbm + r], src[3 * (size_t)Cs_hbm + r]};
+    const double ps[4] = {src[s], src[Cs_hbm + s], src[2 * (size_t)Cs_hbm + s], src[3 * (size_t)Cs_hbm + s]};
+    double v[3];
+    solve3_rcp(pr, ps, eq, v);
+    double* out = io.reg_verts + ((size_t)rid * vm + i) * 3;
+    out[0] = v[0];
+    out[1] = v[1];
+    out[2] = v[2];
+  }
 }
 
 // Chunk-local slots of the regions k_walk_ray64 left undecided: region of

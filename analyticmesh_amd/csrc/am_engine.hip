@@ -291,10 +291,15 @@ size_t ray64_lds_bytes(const am_ctx* c) { return (size_t)4 * c->C * sizeof(doubl
 // from `lvl_first`.  mode 0: the candidate scan on every region (the FP64
 // records of the CPU restatement, bit for bit); mode 1: the checked FP64 ray
 // walk, and the scan on the regions it leaves undecided, reusing their planes
-// (the scan runs a block-stride loop up to the device-side count).
+// (the scan runs a block-stride loop up to the device-side count); mode 2:
+// after a reduced-precision walk that flagged the level in rflags
+// (`undecided` of them): the regions it decided get FP64 vertices
+// (k_vertex_fp64, their other records stay as written), the others go
+// through mode 1.
 // Nothing here waits for the device.  Falls back to the one-region kernel
 // where the tiled chain does not apply.
-void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t) {
+void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t,
+                      long long undecided = -1) {
   if (!c->tile_ok) {
     // one region per workgroup: the chain and the candidate scan in one kernel
     if (ids) {
@@ -305,6 +310,8 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     t.scanned_host += count;
     return;
   }
+  const bool after_walk = mode == 2;
+  if (after_walk) mode = 1;
   if (mode == 1 && ray64_lds_bytes(c) > c->max_lds) mode = 0;
   int Cs, Csort;
   const size_t wlds = region_lds_bytes(c, Cs, Csort);
@@ -315,7 +322,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   const size_t cap = std::min<size_t>(chunk, (size_t)count);
   ensure(c->tplanes, cap * 4 * Ch, false, c->stream);
   ensure(c->teq, cap * 4, false, c->stream);
-  if (mode == 1) {
+  if (mode == 1 || after_walk) {
     ensure(c->tslots, cap, false, c->stream);
     ensure(c->tcnt, 1, false, c->stream);
   }
@@ -351,7 +358,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.count = n;
     io.Cs = Cs;
     io.Csort = Csort;
-    io.flags = mode == 1 ? c->rflags.p : nullptr;
+    io.flags = (mode == 1 || after_walk) ? c->rflags.p : nullptr;
     io.ids = ids ? ids + off : nullptr;
     const size_t m0 = tm.mark(c->stream);
     hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
@@ -360,7 +367,16 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     t.chain_launches += 1;
     const size_t m1 = tm.mark(c->stream);
     tm.span(m0, m1, T_CHAIN64);
-    if (mode == 0) {
+    size_t m1v = m1;
+    if (after_walk && undecided != count) {
+      hipLaunchKernelGGL(k_vertex_fp64, dim3(n), dim3(64), 0, c->stream, c->net, io, first + off,
+                         (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+      HIPCHK(hipGetLastError());
+      m1v = tm.mark(c->stream);
+      tm.span(m1, m1v, T_VERT64);
+    }
+    if (after_walk && undecided == 0) continue;
+    if (mode == 0 && !after_walk) {
       hipLaunchKernelGGL(k_walk_fp64, dim3(n), dim3(64), wlds, c->stream, c->net, io, first + off, (const int*)nullptr,
                          (const int*)nullptr, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
       HIPCHK(hipGetLastError());
@@ -368,11 +384,13 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
       t.scanned_host += n;
       continue;
     }
-    hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, c->stream, c->net, io, first + off,
-                       kRay64, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
-    HIPCHK(hipGetLastError());
-    const size_t m2 = tm.mark(c->stream);
-    tm.span(m1, m2, T_RAY64);
+    if (mode == 1) {
+      hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, c->stream, c->net, io, first + off,
+                         kRay64, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch, after_walk ? 1 : 0);
+      HIPCHK(hipGetLastError());
+      const size_t m2 = tm.mark(c->stream);
+      tm.span(m1v, m2, T_RAY64);
+    }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->tcnt.p, (size_t)1, 0);
     hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const unsigned*)c->rflags.p,
                        n, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p, c->tslots.p,
@@ -730,7 +748,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     }
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
-    long long levels = 0, cands = 0, redo = 0, launches = 0;
+    long long levels = 0, cands = 0, redo = 0, launches = 0, vert64 = 0;
     long long why[AM_WHY_COUNT] = {0};
     Fp64Counts f64;
     c->timer.reset();
@@ -754,15 +772,14 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.reg_vpair = c->reg_vpair.p; fl.cand_mask = c->cand_mask.p; fl.cand_start = c->cand_start.p;
         fl.cand_point = c->cand_point.p;
         FastTimes ft{};
-        int* ids = nullptr;
-        const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, &ids);
-        if (nredo > 0) {
-          // FP64 re-decision of the undecided regions, in place (their records
-          // and candidate slots are rewritten): FP64 planes, the checked FP64
-          // ray walk, and the candidate scan where that is undecided too.
-          run_fp64_regions(c, (int)lvl_first, (int)lvl_first, nredo, ids, 1, f64);
-          redo += nredo;
-        }
+        const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p);
+        // FP64 planes of the whole level: the regions the walk decided get
+        // their vertices re-solved from them; the undecided ones are re-decided
+        // in place (records and candidate slots rewritten) by the checked FP64
+        // ray walk, and by the candidate scan where that is undecided too.
+        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr, 2, f64, nredo);
+        redo += nredo;
+        vert64 += lvl_count - nredo;
         for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
         launches += ft.launches;
       }
@@ -785,7 +802,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->timer.resolve();
     const auto t1 = std::chrono::steady_clock::now();
     const double* T = c->timer.acc;
-    const double fp64_ms = T[T_CHAIN64] + T[T_RAY64] + T[T_SCAN64] + T[T_REGION64];
+    const double fp64_ms = T[T_CHAIN64] + T[T_RAY64] + T[T_SCAN64] + T[T_REGION64] + T[T_VERT64];
     c->stats[AM_STAT_RUN_MS] = std::chrono::duration<double, std::milli>(t1 - t0).count();
     c->stats[AM_STAT_INSERT_MS] = T[T_INSERT];
     if (fp64_form) {
@@ -811,6 +828,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_SCAN64_REGIONS] = (double)(f64.scanned_host + acc[0]);
     c->stats[AM_STAT_RAY64_UNDECIDED] = (double)acc[0];
     c->stats[AM_STAT_CHAIN64_LAUNCHES] = (double)f64.chain_launches;
+    c->stats[AM_STAT_VERT64_MS] = T[T_VERT64];
+    c->stats[AM_STAT_VERT64_REGIONS] = (double)vert64;
     c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;

@@ -312,10 +312,11 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
 
 // Compact the undecided regions' ids; cnt[0] = their number, cnt[1 + w] = how
 // many were left for reason w (a region can carry one reason per walk exit).
-__global__ void k_collect_undecided(const unsigned* flags, int n, int first, int* cnt, int* ids) {
+// Regions the walk left undecided (cnt[0]) and their reasons (cnt[1 + w]).
+__global__ void k_count_undecided(const unsigned* flags, int n, int* cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && (flags[i] & AM_UNDECIDED)) {
-    ids[atomicAdd(cnt, 1)] = first + i;
+    atomicAdd(cnt, 1);
     for (int w = 0; w < AM_WHY_COUNT; ++w)
       if (flags[i] & AM_WHY_BIT(w)) atomicAdd(cnt + 1 + w, 1);
   }
@@ -488,7 +489,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
 }
 
 int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-               StageTimer& timer, FastTimes& t, int** ids) {
+               StageTimer& timer, FastTimes& t, unsigned* flags) {
   const FastNetDev& d = f.dev;
   const int prec = f.precision == 1 ? 1 : 0;
   t.launches = 0;
@@ -496,8 +497,6 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
   const int chunk = std::min(lvl.count, CH);
   float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * 4 * d.Cs * 4);
   float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * 4 * 4);
-  unsigned* flags = (unsigned*)slot(b, S_FLAGS, (size_t)lvl.count * 4);
-  int* idl = (int*)slot(b, S_IDS, (size_t)lvl.count * 4);
   int* cnt = (int*)slot(b, S_CNT, 4 * (1 + AM_WHY_COUNT));
 
   int Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
@@ -556,16 +555,13 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     t.launches += 1;
   }
   FCHK(hipMemsetAsync(cnt, 0, 4 * (1 + AM_WHY_COUNT), st));
-  hipLaunchKernelGGL(k_collect_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count,
-                     lvl.first, cnt, idl);
+  hipLaunchKernelGGL(k_count_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count, cnt);
   FCHK(hipGetLastError());
   int hc[1 + AM_WHY_COUNT] = {0};
   FCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
   FCHK(hipStreamSynchronize(st));
   for (int w = 0; w < AM_WHY_COUNT; ++w) t.why[w] += hc[1 + w];
-  const int h = hc[0];
-  *ids = idl;
-  return h;
+  return hc[0];
 }
 
 void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st) {

@@ -75,12 +75,19 @@ __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
 __global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count, const int* slots,
                             const double* planes, const double* eq, int Cs_hbm);
 struct WalkParams;
+// only_undecided: blocks whose region is not flagged AM_UNDECIDED in io.flags exit.
 __global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, const double* planes,
-                             const double* eq, int Cs_hbm);
+                             const double* eq, int Cs_hbm, int only_undecided);
 // Chunk-local slots of the regions k_walk_ray64 left undecided (cnt[0] counts
 // them); run[0] accumulates the count and run[1 + w] the reasons, per run.
 __global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
                                 int* slots, int* run);
+// FP64 vertices of regions whose topology a reduced-precision walk decided:
+// block b re-solves the face vertices of region io.ids[b] (or rid0 + b) from
+// the FP64 planes of slot b (its two tight constraints and the face plane);
+// with io.flags, regions flagged AM_UNDECIDED are skipped.
+__global__ void k_vertex_fp64(NetDev net, RegionIO io, int rid0, const double* planes, const double* eq,
+                              int Cs_hbm);
 __global__ void k_fill_i32(int* p, size_t n, int v);
 
 }  // namespace am

@@ -313,6 +313,8 @@ def main():
             "walk_ms": round(float(out[f]["stats"][4]), 3),
             "fp64_redo_ms": round(float(out[f]["stats"][5]), 3),
             "fp64_redo_regions": int(out[f]["stats"][9]),
+            "fp64_vertex_ms": round(float(out[f]["stats"][32]), 3),
+            "fp64_vertex_regions": int(out[f]["stats"][33]),
             "fp64_chain_ms": round(float(out[f]["stats"][13]), 3),
             "fp64_ray_walk_ms": round(float(out[f]["stats"][28]), 3),
             "fp64_scan_walk_ms": round(float(out[f]["stats"][14]), 3),

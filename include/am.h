@@ -137,7 +137,9 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_SCAN64_REGIONS 29 /* regions decided by the FP64 candidate scan */
 #define AM_STAT_RAY64_UNDECIDED 30 /* regions the FP64 ray walk left to the scan */
 #define AM_STAT_CHAIN64_LAUNCHES 31 /* launches of the FP64 tiled chain */
-#define AM_STAT_COUNT 32
+#define AM_STAT_VERT64_MS 32    /* FP64 vertex re-solve of regions the reduced-precision walk decided (HIP events) */
+#define AM_STAT_VERT64_REGIONS 33 /* regions whose vertices were re-solved from FP64 planes */
+#define AM_STAT_COUNT 34
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 
 #ifdef __cplusplus

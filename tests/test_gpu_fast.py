@@ -2,11 +2,12 @@
 ray walk) against FP64.
 
 Every form must give the FP64 topology exactly (same visited regions, faces
-and vertex keys).  Vertices (VERT_TOL): the FP64-plane form solves each vertex
-from the FP64 planes, so only the order of the 3x3 solve's rows can differ
-from the scan's; the FP32 and split-BF16 forms' decided vertices are FP64
-re-solves of their own planes, whose error reaches 2.6e-5 and 5.1e-4 of the
-largest coefficient on the 4x256 net."""
+and vertex keys).  Vertices (VERT_TOL): every form solves each vertex from the
+FP64 planes (the FP32 and split-BF16 forms re-solve the vertices of the
+regions their own walk decided), so only the order of the 3x3 solve's rows
+can differ from the scan's.  Solving from the reduced-precision planes
+instead gives errors up to 3.2e-5 on the 4x256 net (FP32 plane error 2.6e-5
+of the largest coefficient), over the 1e-5 the forms are held to."""
 import sys
 
 import numpy as np
@@ -16,7 +17,7 @@ from _common import assert_same_topology, max_vertex_error, workload
 
 pytestmark = pytest.mark.gpu
 
-VERT_TOL = {"fp64ray": 1e-12, "fp32": 1e-4, "bf16x3": 1e-3}
+VERT_TOL = {"fp64ray": 1e-12, "fp32": 1e-12, "bf16x3": 1e-12}
 
 
 def _say(*a):
@@ -38,7 +39,7 @@ def test_fast_topology_vs_oracle(name, prec, walk):
     err = max_vertex_error(a, b)
     _say(f"  redo {int(g.stats[9])}/{len(g.masks)} max vertex rel err {err:.2e}")
     # the polytope's planes come in near-parallel pairs (1e-7 noise): FP64 itself is only good to 1e-7 there
-    assert err < (VERT_TOL[prec] if name != "polytope" else max(VERT_TOL[prec], 1e-4))
+    assert err < (VERT_TOL[prec] if name != "polytope" else 1e-4)
 
 
 @pytest.mark.parametrize("prec", ["fp64ray", "fp32", "bf16x3"])
@@ -52,5 +53,7 @@ def test_fast_deep_vs_fp64(prec):
     # both stop after the same number of BFS levels; compare what both finished
     assert_same_topology(a, b)
     err = max_vertex_error(a, b)
-    _say(f"  regions {len(a)} redo {int(g.stats[9])} max vertex rel err {err:.2e}")
+    _say(f"  regions {len(a)} redo {int(g.stats[9])} re-solved {int(g.stats[33])} max vertex rel err {err:.2e}")
+    if prec != "fp64ray":
+        assert int(g.stats[9]) + int(g.stats[33]) == len(a)
     assert err < VERT_TOL[prec]
