@@ -768,6 +768,10 @@ __device__ __forceinline__ int block_check(const T* Px, const T* Py, const T* Pz
 // vertex with the checked feasibility test.  On generic input the proposal
 // is the unique feasible candidate the reference's scan would accept, so the
 // face is the same; anything the bounds cannot decide goes to FP64.
+// The per-vertex serial work (vertex solve, its bound, the next direction, the
+// turn and same-vertex tests) is done by one wave, a different one for each
+// vertex so that the SIMDs share it, and handed to the block through LDS;
+// only the passes over the constraints use every wave.
 template <typename T, int NW>
 __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                          const T* Px, const T* Py, const T* Pz, const T* Pc, const uint32_t* smask, const T* eq,
@@ -775,7 +779,11 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   __shared__ int s_tab[AM_VM_MAX + 2];
   __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
   __shared__ int s_vpair[AM_VM_MAX * 2];
+  __shared__ T s_v[6];                 // the vertex and the next edge direction
+  __shared__ VertexBound<T> s_vb;
+  __shared__ unsigned s_stat;          // reason bits of the lead wave's tests (0: passed)
   const int C = net.C, vm = net.vm;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int start_in = io.reg_start[rid];
   const int NONE = 0x7fffffff;
   bool undecided = false;
@@ -825,15 +833,27 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       bool in_tab = false;
       for (int i = 0; i < ntab; ++i) in_tab = in_tab || (s_tab[i] == r);
       if (in_tab && !(r == s_tab[0] && ntab >= 3)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TAB); break; }
-      const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
-      const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
-      T v[3];
-      solve3_rcp(pr4, pc4, eq, v);
-      const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, v, T(wp.eps), T(wp.h_max));
-      // the next edge runs along plane r, away from the plane we came along;
-      // its nearest crossing is found in the same pass as this vertex's check
-      T dn[3] = {pr4[1] * eq[2] - pr4[2] * eq[1], pr4[2] * eq[0] - pr4[0] * eq[2], pr4[0] * eq[1] - pr4[1] * eq[0]};
-      if (pc4[0] * dn[0] + pc4[1] * dn[1] + pc4[2] * dn[2] > T(0)) { dn[0] = -dn[0]; dn[1] = -dn[1]; dn[2] = -dn[2]; }
+      const int lead = nv % NW;   // the wave that does this vertex's serial work
+      if (wave == lead) {
+        const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
+        const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
+        T v[3];
+        solve3_rcp(pr4, pc4, eq, v);
+        const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, v, T(wp.eps), T(wp.h_max));
+        // the next edge runs along plane r, away from the plane we came along;
+        // its nearest crossing is found in the same pass as this vertex's check
+        T dn[3] = {pr4[1] * eq[2] - pr4[2] * eq[1], pr4[2] * eq[0] - pr4[0] * eq[2], pr4[0] * eq[1] - pr4[1] * eq[0]};
+        if (pc4[0] * dn[0] + pc4[1] * dn[1] + pc4[2] * dn[2] > T(0)) { dn[0] = -dn[0]; dn[1] = -dn[1]; dn[2] = -dn[2]; }
+        if (lane == 0) {
+          s_v[0] = v[0]; s_v[1] = v[1]; s_v[2] = v[2];
+          s_v[3] = dn[0]; s_v[4] = dn[1]; s_v[5] = dn[2];
+          s_vb = vb;
+        }
+      }
+      __syncthreads();
+      const T v[3] = {s_v[0], s_v[1], s_v[2]};
+      const T dn[3] = {s_v[3], s_v[4], s_v[5]};
+      const VertexBound<T> vb = s_vb;
       int nx;
       const int bits = check_and_shoot<T, NW>(Px, Py, Pz, Pc, C, v, vb, cur, r, dn, r, cur, tdummy, nx);
       if (bits != 0 || !vb.ok) {
@@ -843,50 +863,53 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         else why |= AM_WHY_BIT(AM_WHY_HMAX);
         break;
       }
-      if (ntab >= 2) {
-        const int a = s_tab[ntab - 2], b = s_tab[ntab - 1];
-        const T n1[3] = {Px[a], Py[a], Pz[a]};
-        const T n2[3] = {Px[b], Py[b], Pz[b]};
-        const T n3[3] = {pr4[0], pr4[1], pr4[2]};
-        const T A12 = AM_DOT3(n1, n2), A23 = AM_DOT3(n2, n3), A13 = AM_DOT3(n1, n3), A22 = AM_DOT3(n2, n2);
-        const T C1 = AM_DOT3(n1, eq), C2 = AM_DOT3(n2, eq), C3 = AM_DOT3(n3, eq), B = AM_DOT3(eq, eq);
-        const T val = (A12 * A23 - A22 * A13) - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3) / B;
-        const T mag = absT(A12 * A23) + absT(A22 * A13) +
-                      (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
-        if (!(val > T(wp.eps_turn) * mag)) {   // must certainly turn the right way
-          undecided = true;
-          why |= AM_WHY_BIT(AM_WHY_TURN);
-          break;
+      if (wave == lead) {
+        unsigned w = 0u;
+        if (ntab >= 2) {
+          const int a = s_tab[ntab - 2], b = s_tab[ntab - 1];
+          const T n1[3] = {Px[a], Py[a], Pz[a]};
+          const T n2[3] = {Px[b], Py[b], Pz[b]};
+          const T n3[3] = {Px[r], Py[r], Pz[r]};
+          const T A12 = AM_DOT3(n1, n2), A23 = AM_DOT3(n2, n3), A13 = AM_DOT3(n1, n3), A22 = AM_DOT3(n2, n2);
+          const T C1 = AM_DOT3(n1, eq), C2 = AM_DOT3(n2, eq), C3 = AM_DOT3(n3, eq), B = AM_DOT3(eq, eq);
+          const T val = (A12 * A23 - A22 * A13) - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3) / B;
+          const T mag = absT(A12 * A23) + absT(A22 * A13) +
+                        (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
+          if (!(val > T(wp.eps_turn) * mag)) w = AM_WHY_BIT(AM_WHY_TURN);   // must certainly turn the right way
+        }
+        double vd[3];
+        if (sizeof(T) == 8) {
+          vd[0] = v[0]; vd[1] = v[1]; vd[2] = v[2];
+        } else {
+          const double a4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
+          const double b4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
+          const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
+          solve3(a4, b4, e4, vd);
+        }
+        // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
+        // planes the certified check above already puts the new vertex farther
+        // than that from the last one; with FP64 planes it may not, so vertices
+        // that close are left to the scan.
+        if (w == 0u && nv > 0) {
+          const double l1 = fabs(vd[0] - s_verts[(nv - 1) * 3 + 0]) + fabs(vd[1] - s_verts[(nv - 1) * 3 + 1]) +
+                            fabs(vd[2] - s_verts[(nv - 1) * 3 + 2]);
+          if (!(l1 > 4.0 * Tol<double>::SAME_EPS)) w = AM_WHY_BIT(AM_WHY_SAME);
+        }
+        if (lane == 0) {
+          s_stat = w;
+          if (w == 0u) {
+            s_verts[nv * 3 + 0] = vd[0];
+            s_verts[nv * 3 + 1] = vd[1];
+            s_verts[nv * 3 + 2] = vd[2];
+            s_vpair[nv * 2 + 0] = r;
+            s_vpair[nv * 2 + 1] = cur;
+            s_tab[ntab] = r;
+          }
         }
       }
-      double vd[3];
-      if (sizeof(T) == 8) {
-        vd[0] = v[0]; vd[1] = v[1]; vd[2] = v[2];
-      } else {
-        const double a4[4] = {pr4[0], pr4[1], pr4[2], pr4[3]};
-        const double b4[4] = {pc4[0], pc4[1], pc4[2], pc4[3]};
-        const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
-        solve3(a4, b4, e4, vd);
-      }
-      // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
-      // planes the certified check above already puts the new vertex farther
-      // than that from the last one; with FP64 planes it may not, so vertices
-      // that close are left to the scan.
-      if (nv > 0) {
-        const double l1 = fabs(vd[0] - s_verts[(nv - 1) * 3 + 0]) + fabs(vd[1] - s_verts[(nv - 1) * 3 + 1]) +
-                          fabs(vd[2] - s_verts[(nv - 1) * 3 + 2]);
-        if (!(l1 > 4.0 * Tol<double>::SAME_EPS)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SAME); break; }
-      }
       __syncthreads();
-      if (threadIdx.x == 0) {
-        s_verts[nv * 3 + 0] = vd[0];
-        s_verts[nv * 3 + 1] = vd[1];
-        s_verts[nv * 3 + 2] = vd[2];
-        s_vpair[nv * 2 + 0] = r;
-        s_vpair[nv * 2 + 1] = cur;
-        s_tab[ntab] = r;
-      }
-      __syncthreads();
+      const unsigned st = s_stat;
+      if (st != 0u) { undecided = true; why |= st; break; }
       nv += 1;
       ntab += 1;
       if (ntab >= 4 && s_tab[0] == s_tab[ntab - 1]) break;          // closed
