@@ -235,16 +235,16 @@ def main():
     h = out[head]
     st = h["stats"]
     # plane kernels of the head form: its own GEMM (reduced forms) and the FP64
-    # chain (every region for the FP64 forms, the re-decided ones otherwise);
-    # the roofline line is the one that took longer
-    redo = int(st[9])
+    # chain (every region in every form: the reduced forms re-solve the
+    # vertices of the regions they decide from FP64 planes and re-decide the
+    # others on them); the roofline line is the one that took longer
     kernels = []
     if head in ("fp32", "bf16x3"):
         kernels.append(dict(kernel=GEMM_KERNEL[head], ms=float(st[3]), regions=h["regions"], peak=PEAK_TFLOPS[head],
                             launches=max(1.0, float(st[11]))))
-        if redo:
-            kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=redo, peak=PEAK_TFLOPS["fp64"],
-                                launches=max(1.0, float(st[31]))))
+        if float(st[13]) > 0:
+            kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=h["regions"],
+                                peak=PEAK_TFLOPS["fp64"], launches=max(1.0, float(st[31]))))
     else:
         kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=h["regions"],
                             peak=PEAK_TFLOPS["fp64"], launches=max(1.0, float(st[31]))))
