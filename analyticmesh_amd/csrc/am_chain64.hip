@@ -128,11 +128,13 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
     }
     Xs[k * AM_CXS + col] = x;
   }
-  for (int idx = tid; idx < AM_CN * n1; idx += AM_CHAIN_THREADS) {
-    const int col = idx / n1, k = idx - col * n1, t = col >> 2, q = col & 3;
-    if (t < nreg) {
+  for (int col = tid >> 8; col < AM_CN; col += AM_CHAIN_THREADS / 256) {
+    const int t = col >> 2, q = col & 3;
+    if (t >= nreg) break;
+    double* dst = io.planes + ((size_t)(r0 + t) * 4 + q) * Cs;
+    for (int k = tid & 255; k < n1; k += 256) {
       const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
-      io.planes[((size_t)(r0 + t) * 4 + q) * Cs + k] = mbit(mk + t * net.Wd, k) ? -val : val;
+      dst[k] = mbit(mk + t * net.Wd, k) ? -val : val;
     }
   }
   for (int idx = tid; idx < AM_CN * net.E; idx += AM_CHAIN_THREADS) {
@@ -152,7 +154,14 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       for (int j = 0; j < AM_CR; ++j) acc[j] = q_l == 3 ? bias[j] + 0.0 : 0.0;
       // lane l carries W[k][row0 + (l & 15)]; each fma takes row j's weight by a
       // DPP row broadcast of that register
-      const double* wcol = io.Wp[li] + row0 + (lane & 15);
+      // Weights through a buffer resource: row k's byte offset k * noutp * 8 is
+      // uniform (the scalar offset), the lane's element row0 + (lane & 15) the
+      // vector offset, so a weight load needs no vector address arithmetic.
+      // Wp holds rows 0..AM_CHAIN_MAXW (the last is the +0 padding row).
+      const __amdgpu_buffer_rsrc_t wres = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)io.Wp[li], (short)0, (AM_CHAIN_MAXW + 1) * noutp * 8, 0x00020000);
+      const int woff = (row0 + (lane & 15)) * 8;
+      const int wrow = noutp * 8;
       // k runs over the tile's active inputs in ascending order (padding
       // entries name the +0 row: fma(+0, +0, acc) = acc).  Operands of the
       // next AM_CPF steps are in flight while a step's fmas run, and the k
@@ -165,7 +174,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
 #pragma unroll
       for (int u = 0; u < AM_CPF; ++u) {
         const int k = (int)((kn[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
-        wq[u] = wcol[(size_t)k * noutp];
+        wq[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(wres, woff, k * wrow, 0));
         xq[u] = Xs[k * AM_CXS + lane];
         __builtin_amdgcn_sched_barrier(0);
       }
@@ -183,7 +192,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
           // barrier keeps the loads in slot order, so each step waits only
           // for its own (oldest) load
           const int k = (int)((kk[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
-          wq[u] = wcol[(size_t)k * noutp];
+          wq[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(wres, woff, k * wrow, 0));
           xq[u] = Xs[k * AM_CXS + lane];
           __builtin_amdgcn_sched_barrier(0);
         }
@@ -204,12 +213,16 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
     __syncthreads();
     if (li < net.D) {
       // signed planes to HBM, consecutive threads on consecutive constraints
+      // (thread: row jt, columns ct, ct + AM_CHAIN_THREADS / 256, ...)
       const int out0 = net.off[li];
-      for (int idx = tid; idx < AM_CN * nout; idx += AM_CHAIN_THREADS) {
-        const int col = idx / nout, j = idx - col * nout, t = col >> 2, q = col & 3;
-        if (t < nreg) {
+      const int jt = tid & 255, ct = tid >> 8;
+      for (int col = ct; col < AM_CN; col += AM_CHAIN_THREADS / 256) {
+        const int t = col >> 2, q = col & 3;
+        if (t >= nreg) break;
+        double* dst = io.planes + ((size_t)(r0 + t) * 4 + q) * Cs + out0;
+        for (int j = jt; j < nout; j += 256) {
           const double y = Xs[j * AM_CXS + col];
-          io.planes[((size_t)(r0 + t) * 4 + q) * Cs + out0 + j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
+          dst[j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
         }
       }
       __syncthreads();
