@@ -72,10 +72,11 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  // LDS: X (one more row than any layer's input: row AM_CHAIN_MAXW stays +0),
-  // each layer's list of active inputs, the tile's masks
-  double* Xs = reinterpret_cast<double*>(smem_raw);                                   // [AM_CHAIN_MAXW + 1][AM_CXS]
-  uint16_t* klist = reinterpret_cast<uint16_t*>(Xs + (size_t)(AM_CHAIN_MAXW + 1) * AM_CXS + 1);  // [D][AM_KLS], 16-B aligned
+  // LDS: X, one row per position of the layer's active-input list (rows past
+  // the list are read by the operand prefetch only), each layer's list of
+  // active inputs, the tile's masks
+  double* Xs = reinterpret_cast<double*>(smem_raw);                                   // [AM_CXR][AM_CXS]
+  uint16_t* klist = reinterpret_cast<uint16_t*>(Xs + (size_t)AM_CXR * AM_CXS + 1);     // [D][AM_KLS], 16-B aligned
   uint32_t* mk = reinterpret_cast<uint32_t*>(klist + (size_t)net.D * AM_KLS);          // [AM_CT][Wd]
   __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_CT;
@@ -89,7 +90,6 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
     }
     mk[idx] = v;
   }
-  for (int i = tid; i < AM_CXS; i += AM_CHAIN_THREADS) Xs[(size_t)AM_CHAIN_MAXW * AM_CXS + i] = 0.0;
   __syncthreads();
   // Active inputs of each layer for the whole tile: input k of linear layer li
   // is skipped when it is inactive in all AM_CT regions (its X row is +0 in
@@ -116,17 +116,15 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   }
   __syncthreads();
   const size_t Cs = (size_t)io.Cs;
-  // hidden node 1: the rows of W0 (unsigned into X, signed into HBM)
+  // hidden node 1: the rows of W0 (unsigned and masked into X at their list
+  // positions, signed into HBM)
   const int n1 = net.nodes[1];
-  const int n1p = (n1 + AM_CPF - 1) / AM_CPF * AM_CPF;
-  for (int idx = tid; idx < n1p * AM_CN; idx += AM_CHAIN_THREADS) {
-    const int k = idx / AM_CN, col = idx - k * AM_CN, t = col >> 2, q = col & 3;
+  for (int idx = tid; idx < s_nk[1] * AM_CN; idx += AM_CHAIN_THREADS) {
+    const int i = idx / AM_CN, col = idx - i * AM_CN, t = col >> 2, q = col & 3;
+    const int k = klist[i];
     double x = 0.0;
-    if (k < n1) {
-      const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
-      x = mbit(mk + t * net.Wd, k) ? val : 0.0;
-    }
-    Xs[k * AM_CXS + col] = x;
+    if (k < n1 && mbit(mk + t * net.Wd, k)) x = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
+    Xs[i * AM_CXS + col] = x;
   }
   for (int col = tid >> 8; col < AM_CN; col += AM_CHAIN_THREADS / 256) {
     const int t = col >> 2, q = col & 3;
@@ -162,10 +160,11 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
           (void*)io.Wp[li], (short)0, (AM_CHAIN_MAXW + 1) * noutp * 8, 0x00020000);
       const int woff = (row0 + (lane & 15)) * 8;
       const int wrow = noutp * 8;
-      // k runs over the tile's active inputs in ascending order (padding
-      // entries name the +0 row: fma(+0, +0, acc) = acc).  Operands of the
-      // next AM_CPF steps are in flight while a step's fmas run, and the k
-      // indices of the group after are read a group ahead.
+      // k runs over the tile's active inputs in ascending order; X row i holds
+      // list entry i (padding entries name the +0 weight row and have +0 X:
+      // fma(+0, +0, acc) = acc).  Operands of the next AM_CPF steps are in
+      // flight while a step's fmas run, and the k indices of the group after
+      // are read a group ahead.
       const uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
       const int nkp = s_nk[li];
       double wq[AM_CPF], xq[AM_CPF];
@@ -175,11 +174,12 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       for (int u = 0; u < AM_CPF; ++u) {
         const int k = (int)((kn[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
         wq[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(wres, woff, k * wrow, 0));
-        xq[u] = Xs[k * AM_CXS + lane];
+        xq[u] = Xs[u * AM_CXS + lane];
         __builtin_amdgcn_sched_barrier(0);
       }
       load_k16(kl + AM_CPF, kn);
       for (int i = 0; i < nkp; i += AM_CPF) {
+        const double* xnext = Xs + (size_t)(i + AM_CPF) * AM_CXS + lane;   // rows of the next group
         uint32_t kk[AM_CPF / 2];
 #pragma unroll
         for (int u = 0; u < AM_CPF / 2; ++u) kk[u] = kn[u];
@@ -193,21 +193,18 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
           // for its own (oldest) load
           const int k = (int)((kk[u >> 1] >> ((u & 1) * 16)) & 0xffffu);
           wq[u] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(wres, woff, k * wrow, 0));
-          xq[u] = Xs[k * AM_CXS + lane];
+          xq[u] = xnext[u * AM_CXS];
           __builtin_amdgcn_sched_barrier(0);
         }
       }
     }
     __syncthreads();                  // every wave is done reading this layer's input
-    // park the layer's outputs in LDS (unsigned, unmasked); rows
-    // nout..round_up(nout, AM_CPF) become +0 for the next layer's k padding
-    const int noutk = (nout + AM_CPF - 1) / AM_CPF * AM_CPF;
+    // park the layer's outputs in LDS (unsigned, unmasked), row = output index
     if (active) {
 #pragma unroll
       for (int j = 0; j < AM_CR; ++j) {
         const int row = row0 + j;
         if (row < nout) Xs[row * AM_CXS + lane] = acc[j];
-        else if (row < noutk) Xs[row * AM_CXS + lane] = 0.0;
       }
     }
     __syncthreads();
@@ -225,11 +222,39 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
           dst[j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
         }
       }
-      __syncthreads();
-      // masked input of the next layer, in place
-      for (int idx = tid; idx < nout * AM_CN; idx += AM_CHAIN_THREADS) {
-        const int j = idx / AM_CN, col = idx - j * AM_CN, t = col >> 2;
-        if (!mbit(mk + t * net.Wd, out0 + j)) Xs[j * AM_CXS + col] = 0.0;
+      // the next layer's input at its list positions: position i holds output
+      // kl2[i] masked (+0 for padding entries).  The list ascends, so position
+      // i reads row kl2[i] >= i: in two halves of positions, each half reads
+      // (rows at or past its first position) before a barrier and writes
+      // (its own positions) after it, and no half overwrites a row a later
+      // half still reads.
+      const uint16_t* kl2 = klist + (size_t)li * AM_KLS;
+      const int nkp2 = s_nk[li + 1];
+      constexpr int PPT = AM_CHAIN_THREADS / AM_CN;                 // positions per pass of the block
+      constexpr int XH = AM_CHAIN_MAXW / PPT / 2;                   // passes per half
+      // (an empty asm keeps the thread's offsets from being hoisted out of the
+      // layer loop, where they would stay live through the k loop)
+      int tq = tid;
+      asm volatile("" : "+v"(tq));
+      const int col = tq & (AM_CN - 1), t = col >> 2, p0 = tq / AM_CN;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double xv[XH];
+#pragma unroll
+        for (int c = 0; c < XH; ++c) {
+          const int i = (h * XH + c) * PPT + p0;
+          xv[c] = 0.0;
+          if (i < nkp2) {
+            const int k = kl2[i];
+            if (k < nout && mbit(mk + t * net.Wd, out0 + k)) xv[c] = Xs[k * AM_CXS + col];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < XH; ++c) {
+          const int i = (h * XH + c) * PPT + p0;
+          if (i < nkp2) Xs[i * AM_CXS + col] = xv[c];
+        }
       }
     } else if (tid < AM_CN && (tid >> 2) < nreg) {
       const double y = Xs[tid];

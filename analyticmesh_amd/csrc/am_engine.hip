@@ -270,7 +270,7 @@ void launch_region_fp64(am_ctx* c, int first, int count, const int* ids = nullpt
 }
 
 size_t chain_lds_bytes(const am_ctx* c) {
-  return ((size_t)(AM_CHAIN_MAXW + 1) * AM_CXS + 1) * sizeof(double) + (size_t)c->D * AM_KLS * sizeof(uint16_t) +
+  return ((size_t)AM_CXR * AM_CXS + 1) * sizeof(double) + (size_t)c->D * AM_KLS * sizeof(uint16_t) +
          (size_t)AM_CT * c->Wd * 4 + 16;
 }
 

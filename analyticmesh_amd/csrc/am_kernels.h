@@ -53,6 +53,7 @@ __global__ void k_region_fp64(NetDev net, RegionIO io);
 #define AM_RAY_WAVES 4          // waves per region in the ray-shooting walks
 #define AM_CPF 16               // k steps of chain operands in flight (k lists are padded to a multiple)
 #define AM_KLS (AM_CHAIN_MAXW + 2 * AM_CPF)   // entries of one layer's active-input list
+#define AM_CXR (AM_CHAIN_MAXW + AM_CPF)       // rows of the chain's layer buffer (list positions + prefetch)
 #define AM_CHAIN_THREADS (64 * AM_CHAIN_MAXW / AM_CR)
 
 struct ChainIO {
