@@ -68,11 +68,14 @@ __device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32
       Xl[col * KS + k] = bf16_rn(x - bf16_to_f(h));
     }
   }
-  for (int idx = tid; idx < AM_GN * n1; idx += 256) {
-    const int col = idx / n1, k = idx - col * n1, t = col >> 2, q = col & 3;
-    if (t < nreg) {
-      const float val = q < 3 ? f.W0[k * 3 + q] : f.bias[0][k];
-      io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + k] = mbit(mk + t * f.Wd, k) ? -val : val;
+  // signed planes to HBM: thread = constraint (consecutive threads, consecutive addresses)
+  for (int k = tid; k < n1; k += 256) {
+    const float v4[4] = {f.W0[k * 3 + 0], f.W0[k * 3 + 1], f.W0[k * 3 + 2], f.bias[0][k]};
+    for (int t = 0; t < nreg; ++t) {
+      const bool on = mbit(mk + t * f.Wd, k);
+      float* dst = io.planes + (size_t)(r0 + t) * 4 * f.Cs + k;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) dst[(size_t)q * f.Cs] = on ? -v4[q] : v4[q];
     }
   }
   for (int idx = tid; idx < AM_GN * f.E; idx += 256) {
@@ -143,15 +146,19 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
       f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
       if (PREC == 0) {
         // A[row][kb*4 + (lane >> 4)] = W^T[list entry][row] (k-major, rows padded to 16)
+        // (through a buffer resource: 32-bit byte offsets, one multiply-add each)
         const int nkl = s_nk[li] >> 2;
         const uint16_t* kl = klist + (size_t)(li - 1) * KLF + (lane >> 4);
-        const float* Wc = f.Wp[li] + rb * 16 + (lane & 15);
-        const int ld = nrb * 16;
+        const int ld4 = nrb * 16 * 4;
+        const __amdgpu_buffer_rsrc_t wres = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)f.Wp[li], (short)0, (f.nodes[li] + 1) * ld4, 0x00020000);
+        const int c4 = (rb * 16 + (lane & 15)) * 4;
+#define AM_GA(e) __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(wres, (int)kl[(e) * 4] * ld4 + c4, 0, 0))
         const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
         int kb = 0;
         for (; kb + 4 <= nkl; kb += 4) {
-          const float a0 = Wc[(size_t)kl[(kb + 0) * 4] * ld], a1 = Wc[(size_t)kl[(kb + 1) * 4] * ld];
-          const float a2 = Wc[(size_t)kl[(kb + 2) * 4] * ld], a3 = Wc[(size_t)kl[(kb + 3) * 4] * ld];
+          const float a0 = AM_GA(kb + 0), a1 = AM_GA(kb + 1);
+          const float a2 = AM_GA(kb + 2), a3 = AM_GA(kb + 3);
           const float* x0 = xb + (kb * 4) * AM_XS;
           acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[0], acc0, 0, 0, 0);
           acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[16], acc1, 0, 0, 0);
@@ -163,11 +170,12 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
           acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS + 16], acc1, 0, 0, 0);
         }
         for (; kb < nkl; ++kb) {
-          const float a = Wc[(size_t)kl[kb * 4] * ld];
+          const float a = AM_GA(kb);
           const float* x0 = xb + (kb * 4) * AM_XS;
           acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[0], acc0, 0, 0, 0);
           acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[16], acc1, 0, 0, 0);
         }
+#undef AM_GA
       } else {
         const uint4* Ah = reinterpret_cast<const uint4*>(f.Wh[li]) + (size_t)rb * nkb * 64 + lane;
         const uint4* Al = reinterpret_cast<const uint4*>(f.Wl[li]) + (size_t)rb * nkb * 64 + lane;
@@ -207,11 +215,16 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
     __syncthreads();
     if (li < f.D) {
       const int out0 = f.off[li];
-      for (int idx = tid; idx < AM_GN * nout; idx += 256) {
-        const int col = idx / nout, j = idx - col * nout, t = col >> 2, q = col & 3;
-        if (t < nreg) {
-          const float y = Ys[j * AM_XS + col];
-          io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + out0 + j] = mbit(mk + t * f.Wd, out0 + j) ? -y : y;
+      // signed planes to HBM: thread = constraint (consecutive threads, consecutive addresses)
+      for (int j = tid; j < nout; j += 256) {
+        for (int t = 0; t < nreg; ++t) {
+          const bool on = mbit(mk + t * f.Wd, out0 + j);
+          float* dst = io.planes + (size_t)(r0 + t) * 4 * f.Cs + out0 + j;
+          const float4 y = *reinterpret_cast<const float4*>(Ys + j * AM_XS + 4 * t);
+          dst[0] = on ? -y.x : y.x;
+          dst[f.Cs] = on ? -y.y : y.y;
+          dst[2 * (size_t)f.Cs] = on ? -y.z : y.z;
+          dst[3 * (size_t)f.Cs] = on ? -y.w : y.w;
         }
       }
       if (PREC == 0) {
