@@ -95,7 +95,7 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   uint16_t* Xl = Xh + (size_t)AM_GN * KS;
   uint32_t* mk = reinterpret_cast<uint32_t*>(PREC == 0 ? (unsigned char*)(Xs + (size_t)f.Kpad * AM_XS)
                                                        : (unsigned char*)(Xl + (size_t)AM_GN * KS));
-  const int KLF = f.Kpad + 4;                                        // entries of one layer's input list
+  const int KLF = f.Kpad + 32;                                       // entries of one layer's input list
   uint16_t* klist = reinterpret_cast<uint16_t*>(mk + AM_GT * f.Wd);  // fp32: [D][KLF]
   __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_GT;
@@ -113,8 +113,9 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   __syncthreads();
   if (PREC == 0) {
     // Inputs of linear layer li that are active in some region of the tile,
-    // ascending, padded to a multiple of 4 with n_li (the +0 row of W^T and,
-    // through the staging below, of X).  Inactive inputs are +0 in every
+    // ascending, padded to a multiple of 16 with n_li (the +0 row of W^T and,
+    // through the staging below, of X), plus 32 such entries that only the
+    // operand prefetch reads.  Inactive inputs are +0 in every
     // column, so the GEMM over the list equals the full one.
     for (int li = wave + 1; li <= f.D; li += 4) {
       const int nin = f.nodes[li], in0 = li == 1 ? 0 : f.off[li - 1];
@@ -129,8 +130,8 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
         if (act) kl[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = (uint16_t)k;
         base += __popcll(bal);
       }
-      const int nkp = (base + 3) & ~3;
-      for (int i = base + lane; i < nkp; i += 64) kl[i] = (uint16_t)nin;
+      const int nkp = (base + 15) & ~15;
+      for (int i = base + lane; i < nkp + 32; i += 64) kl[i] = (uint16_t)nin;
       if (lane == 0) s_nk[li] = nkp;
     }
     __syncthreads();
@@ -155,26 +156,31 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
         const int c4 = (rb * 16 + (lane & 15)) * 4;
 #define AM_GA(e) __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(wres, (int)kl[(e) * 4] * ld4 + c4, 0, 0))
         const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
-        int kb = 0;
-        for (; kb + 4 <= nkl; kb += 4) {
-          const float a0 = AM_GA(kb + 0), a1 = AM_GA(kb + 1);
-          const float a2 = AM_GA(kb + 2), a3 = AM_GA(kb + 3);
-          const float* x0 = xb + (kb * 4) * AM_XS;
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[0], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, x0[16], acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, x0[4 * AM_XS], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, x0[4 * AM_XS + 16], acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, x0[8 * AM_XS], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a2, x0[8 * AM_XS + 16], acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a3, x0[12 * AM_XS + 16], acc1, 0, 0, 0);
+        // nkl is a multiple of 4; the A operands of the next 4 k blocks are in
+        // flight while a block's MFMAs run (two register sets, alternating)
+#define AM_G4(r, kb)                                                               \
+  {                                                                                \
+    const float* x0 = xb + ((kb) * 4) * AM_XS;                                     \
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##0, x0[0], acc0, 0, 0, 0);       \
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##0, x0[16], acc1, 0, 0, 0);      \
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##1, x0[4 * AM_XS], acc0, 0, 0, 0);       \
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##1, x0[4 * AM_XS + 16], acc1, 0, 0, 0);  \
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##2, x0[8 * AM_XS], acc0, 0, 0, 0);       \
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##2, x0[8 * AM_XS + 16], acc1, 0, 0, 0);  \
+    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##3, x0[12 * AM_XS], acc0, 0, 0, 0);      \
+    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##3, x0[12 * AM_XS + 16], acc1, 0, 0, 0); \
+  }
+        float a0 = AM_GA(0), a1 = AM_GA(1), a2 = AM_GA(2), a3 = AM_GA(3);
+        for (int kb = 0; kb < nkl; kb += 8) {
+          const float b0 = AM_GA(kb + 4), b1 = AM_GA(kb + 5), b2 = AM_GA(kb + 6), b3 = AM_GA(kb + 7);
+          __builtin_amdgcn_sched_barrier(0);
+          AM_G4(a, kb)
+          if (kb + 4 >= nkl) break;
+          a0 = AM_GA(kb + 8); a1 = AM_GA(kb + 9); a2 = AM_GA(kb + 10); a3 = AM_GA(kb + 11);
+          __builtin_amdgcn_sched_barrier(0);
+          AM_G4(b, kb + 4)
         }
-        for (; kb < nkl; ++kb) {
-          const float a = AM_GA(kb);
-          const float* x0 = xb + (kb * 4) * AM_XS;
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[0], acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a, x0[16], acc1, 0, 0, 0);
-        }
+#undef AM_G4
 #undef AM_GA
       } else {
         const uint4* Ah = reinterpret_cast<const uint4*>(f.Wh[li]) + (size_t)rb * nkb * 64 + lane;
@@ -393,7 +399,7 @@ float host_bf16_f(uint16_t h) {
 size_t gemm_lds(const FastNetDev& d, int prec) {
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
   const size_t xs = prec == 0 ? (size_t)d.Kpad * AM_XS * 4 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;
-  const size_t kls = prec == 0 ? (size_t)d.D * (d.Kpad + 4) * 2 : 0;
+  const size_t kls = prec == 0 ? (size_t)d.D * (d.Kpad + 32) * 2 : 0;
   return ys + xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
 }
 
