@@ -68,6 +68,34 @@ __device__ __forceinline__ void load_k16(const uint16_t* src, uint32_t (&k)[AM_C
   k[6] = __builtin_amdgcn_readfirstlane(b.z); k[7] = __builtin_amdgcn_readfirstlane(b.w);
 }
 
+// Plane g of a region (x, y, z, c) as the chain defines it.  Hidden layer 1's
+// planes are the rows of (W0 | b0), negated where the neuron is active; the
+// chain does not store them, they are rebuilt here from the weights and the
+// region's mask.  The others come from the chain's record at src.
+__device__ __forceinline__ void plane64(const NetDev& net, const uint32_t* mask, const double* src, int Cs_hbm, int g,
+                                        double* p) {
+  if (g < net.nodes[1]) {
+    const bool on = mbit(mask, g);
+    const double v[4] = {net.W0[g * 3 + 0], net.W0[g * 3 + 1], net.W0[g * 3 + 2], net.b[0][g]};
+    for (int q = 0; q < 4; ++q) p[q] = on ? -v[q] : v[q];
+  } else {
+    for (int q = 0; q < 4; ++q) p[q] = src[(size_t)q * Cs_hbm + g];
+  }
+}
+
+// One region's planes into LDS, threads t0, t0 + nt, ...
+__device__ __forceinline__ void load_planes64(const NetDev& net, const uint32_t* mask, const double* src, int Cs_hbm,
+                                              double* Px, double* Py, double* Pz, double* Pc, int t0, int nt) {
+  for (int g = t0; g < net.C; g += nt) {
+    double p[4];
+    plane64(net, mask, src, Cs_hbm, g, p);
+    Px[g] = p[0];
+    Py[g] = p[1];
+    Pz[g] = p[2];
+    Pc[g] = p[3];
+  }
+}
+
 __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -116,8 +144,8 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   }
   __syncthreads();
   const size_t Cs = (size_t)io.Cs;
-  // hidden node 1: the rows of W0 (unsigned and masked into X at their list
-  // positions, signed into HBM)
+  // hidden node 1: the rows of W0, unsigned and masked into X at their list
+  // positions (their signed planes are not stored: readers rebuild them, plane64)
   const int n1 = net.nodes[1];
   for (int idx = tid; idx < s_nk[1] * AM_CN; idx += AM_CHAIN_THREADS) {
     const int i = idx / AM_CN, col = idx - i * AM_CN, t = col >> 2, q = col & 3;
@@ -125,15 +153,6 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
     double x = 0.0;
     if (k < n1 && mbit(mk + t * net.Wd, k)) x = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
     Xs[i * AM_CXS + col] = x;
-  }
-  for (int col = tid >> 8; col < AM_CN; col += AM_CHAIN_THREADS / 256) {
-    const int t = col >> 2, q = col & 3;
-    if (t >= nreg) break;
-    double* dst = io.planes + ((size_t)(r0 + t) * 4 + q) * Cs;
-    for (int k = tid & 255; k < n1; k += 256) {
-      const double val = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
-      dst[k] = mbit(mk + t * net.Wd, k) ? -val : val;
-    }
   }
   for (int idx = tid; idx < AM_CN * net.E; idx += AM_CHAIN_THREADS) {
     const int col = idx / net.E, e = idx - col * net.E, t = col >> 2, q = col & 3;
@@ -288,13 +307,8 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
     const int rid = io.ids ? io.ids[sl] : rid0 + sl;
     for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
     for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
-    const double* src = planes + (size_t)sl * 4 * Cs_hbm;
-    for (int g = lane; g < C; g += 64) {
-      Px[g] = src[g];
-      Py[g] = src[Cs_hbm + g];
-      Pz[g] = src[2 * (size_t)Cs_hbm + g];
-      Pc[g] = src[3 * (size_t)Cs_hbm + g];
-    }
+    load_planes64(net, io.reg_mask + (size_t)rid * net.Wd, planes + (size_t)sl * 4 * Cs_hbm, Cs_hbm, Px, Py, Pz,
+                  Pc, lane, 64);
     double p[3];
     p[0] = io.reg_point[(size_t)rid * 3 + 0];
     p[1] = io.reg_point[(size_t)rid * 3 + 1];
@@ -328,13 +342,8 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, Re
   // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
   if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
   for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
-  const double* src = planes + (size_t)blockIdx.x * 4 * Cs_hbm;
-  for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
-    Px[g] = src[g];
-    Py[g] = src[Cs_hbm + g];
-    Pz[g] = src[2 * (size_t)Cs_hbm + g];
-    Pc[g] = src[3 * (size_t)Cs_hbm + g];
-  }
+  load_planes64(net, io.reg_mask + (size_t)rid * net.Wd, planes + (size_t)blockIdx.x * 4 * Cs_hbm, Cs_hbm, Px, Py,
+                Pz, Pc, lane, 64 * AM_RAY_WAVES);
   double p[3];
   p[0] = io.reg_point[(size_t)rid * 3 + 0];
   p[1] = io.reg_point[(size_t)rid * 3 + 1];
@@ -359,14 +368,16 @@ __global__ __launch_bounds__(64) void k_vertex_fp64(NetDev net, RegionIO io, int
   const int vm = net.vm;
   const int nv = io.reg_info[(size_t)rid * AM_INFO_STRIDE + 2];
   const double* src = planes + (size_t)b * 4 * Cs_hbm;
+  const uint32_t* mask = io.reg_mask + (size_t)rid * net.Wd;
   const double eq[4] = {eqb[(size_t)b * 4 + 0], eqb[(size_t)b * 4 + 1], eqb[(size_t)b * 4 + 2],
                         eqb[(size_t)b * 4 + 3]};
   for (int i = threadIdx.x; i < nv && i < vm; i += 64) {
     const int r = io.reg_vpair[((size_t)rid * vm + i) * 2 + 0];
     const int s = io.reg_vpair[((size_t)rid * vm + i) * 2 + 1];
     if (r < 0 || r >= net.C || s < 0 || s >= net.C) continue;
-    const double pr[4] = {src[r], src[Cs_hbm + r], src[2 * (size_t)Cs_hbm + r], src[3 * (size_t)Cs_hbm + r]};
-    const double ps[4] = {src[s], src[Cs_hbm + s], src[2 * (size_t)Cs_hbm + s], src[3 * (size_t)Cs_hbm + s]};
+    double pr[4], ps[4];
+    plane64(net, mask, src, Cs_hbm, r, pr);
+    plane64(net, mask, src, Cs_hbm, s, ps);
     double v[3];
     solve3_rcp(pr, ps, eq, v);
     double* out = io.reg_verts + ((size_t)rid * vm + i) * 3;

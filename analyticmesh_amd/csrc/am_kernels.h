@@ -61,7 +61,8 @@ struct ChainIO {
   const int* ids;             // optional region ids; else first + tile-local index
   int first, count;           // regions of this launch
   int Cs;                     // stride of one component array in `planes` (>= C)
-  double* planes;             // [count][4][Cs]: signed constraint planes, extras appended
+  double* planes;             // [count][4][Cs]: signed constraint planes, extras appended; entries of
+                              // hidden layer 1 are not written (readers rebuild them from W0, b0, mask)
   double* eq;                 // [count][4]: output plane with iso subtracted
   const double* Wp[AM_MAX_LINEAR];  // li >= 1: W_li^T padded, [n_li][noutp[li]] (zero rows past n_out)
   const double* bp[AM_MAX_LINEAR];  // li >= 1: b_li padded to noutp[li]
