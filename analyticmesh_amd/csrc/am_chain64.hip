@@ -326,24 +326,32 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
 // planes are the canonical FP64 planes, so the only uncertainty is the
 // rounding of the walk's own evaluations; regions it cannot certify are
 // flagged (io.flags, level-relative) for the candidate scan.
-__global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
+__global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
                                                    const double* planes, const double* eqb, int Cs_hbm,
                                                    int only_undecided) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int C = net.C;
+  // LDS holds the planes past hidden layer 1 (index g - n1); layer 1's are
+  // rebuilt from W0 and b0 where read (PlaneView), which leaves LDS for more
+  // regions per CU
+  const int n1 = net.nodes[1], Cl = net.C - n1;
   double* Px = reinterpret_cast<double*>(smem_raw);
-  double* Py = Px + C;
-  double* Pz = Py + C;
-  double* Pc = Pz + C;
-  uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + C);
+  double* Py = Px + Cl;
+  double* Pz = Py + Cl;
+  double* Pc = Pz + Cl;
+  uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cl);
   const int lane = threadIdx.x;
   if (blockIdx.x >= io.count) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
   // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
   if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
   for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
-  load_planes64(net, io.reg_mask + (size_t)rid * net.Wd, planes + (size_t)blockIdx.x * 4 * Cs_hbm, Cs_hbm, Px, Py,
-                Pz, Pc, lane, 64 * AM_RAY_WAVES);
+  const double* src = planes + (size_t)blockIdx.x * 4 * Cs_hbm + n1;
+  for (int i = lane; i < Cl; i += 64 * AM_RAY_WAVES) {
+    Px[i] = src[i];
+    Py[i] = src[Cs_hbm + i];
+    Pz[i] = src[2 * (size_t)Cs_hbm + i];
+    Pc[i] = src[3 * (size_t)Cs_hbm + i];
+  }
   double p[3];
   p[0] = io.reg_point[(size_t)rid * 3 + 0];
   p[1] = io.reg_point[(size_t)rid * 3 + 1];
@@ -351,7 +359,8 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray64(NetDev net, Re
   const double eq[4] = {eqb[(size_t)blockIdx.x * 4 + 0], eqb[(size_t)blockIdx.x * 4 + 1],
                         eqb[(size_t)blockIdx.x * 4 + 2], eqb[(size_t)blockIdx.x * 4 + 3]};
   __syncthreads();
-  walk_ray<double, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+  const PlaneView<double> P{Px, Py, Pz, Pc, n1, net.W0, net.b[0], smask};
+  walk_ray<double, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
 }
 
 // Vertices of a region whose polygon (the tight-constraint pair of every

@@ -284,7 +284,10 @@ struct Fp64Counts {
 // own rounding (the vertex solve's share is added per system).
 const WalkParams kRay64{1e-13f, 1e-12f, 1e6f};
 
-size_t ray64_lds_bytes(const am_ctx* c) { return (size_t)4 * c->C * sizeof(double) + (size_t)c->Wd * 4 + 64; }
+// planes past hidden layer 1 (the walk rebuilds layer 1's) and the mask
+size_t ray64_lds_bytes(const am_ctx* c) {
+  return (size_t)4 * (c->C - c->nodes[1]) * sizeof(double) + (size_t)c->Wd * 4 + 64;
+}
 
 // FP64 planes + walk of `count` regions (region ids first.., or ids[0..count)),
 // through the tiled chain in chunks; candidate slots and flags are numbered

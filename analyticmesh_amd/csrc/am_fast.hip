@@ -326,7 +326,8 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
-  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, smask, eq, p);
+  const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask};
+  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
 }
 
 // Compact the undecided regions' ids; cnt[0] = their number, cnt[1 + w] = how

@@ -76,9 +76,10 @@ __device__ __forceinline__ T coef_max(const T* p) {
   return fmax(fmax(absT(p[0]), absT(p[1])), fmax(absT(p[2]), absT(p[3])));
 }
 
+// The bound is linear in (1 + |v|_1): vertex_bound_unit computes it for
+// 1 + |v|_1 = 1 (it does not need the vertex), scale_bound applies the factor.
 template <typename T>
-__device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2, const T* p3, const T* v, T eps,
-                                                       T h_max) {
+__device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T* p2, const T* p3, T eps, T h_max) {
   VertexBound<T> b;
   const T c1[3] = {p2[1] * p3[2] - p2[2] * p3[1], p2[2] * p3[0] - p2[0] * p3[2], p2[0] * p3[1] - p2[1] * p3[0]};
   const T c2[3] = {p3[1] * p1[2] - p3[2] * p1[1], p3[2] * p1[0] - p3[0] * p1[2], p3[0] * p1[1] - p3[1] * p1[0]};
@@ -95,7 +96,7 @@ __device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2,
   eps = eps + T(32) * u * (T(1) + h);
   const T rho = eps * (T(1) + h);
   b.ok = h <= h_max && rho <= T(0.25);
-  const T vn = T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]);
+  const T vn = T(1);
   // 1.001: slack for the rounding of these bound computations themselves
   const T kappa = T(1.001) / (T(1) - T(2) * rho);
   // |n_i|_1 = |c_i|_1 / |det|
@@ -113,6 +114,23 @@ __device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2,
     b.w3 = kappa * eps * m3;
     b.e0 = T(1.001) * eps * vn;
   }
+  return b;
+}
+
+template <typename T>
+__device__ __forceinline__ void scale_bound(VertexBound<T>& b, T vn) {
+  b.E *= vn;
+  b.w1 *= vn;
+  b.w2 *= vn;
+  b.w3 *= vn;
+  b.e0 *= vn;
+}
+
+template <typename T>
+__device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2, const T* p3, const T* v, T eps,
+                                                       T h_max) {
+  VertexBound<T> b = vertex_bound_unit<T>(p1, p2, p3, eps, h_max);
+  scale_bound<T>(b, T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
   return b;
 }
 
@@ -636,22 +654,53 @@ __device__ __forceinline__ int block_or_bits(int bits) {
   return b;
 }
 
+// Constraint planes of one region as the ray walk reads them: constraints
+// g >= g0 from LDS (arrays indexed g - g0); constraints g < g0, which are
+// hidden layer 1's, rebuilt from its weights (row g of (W0 | b0), negated where
+// the region's neuron g is active), so their LDS is not needed.
+template <typename T>
+struct PlaneView {
+  const T* Px;
+  const T* Py;
+  const T* Pz;
+  const T* Pc;
+  int g0;
+  const double* W0;
+  const double* b0;
+  const uint32_t* mask;
+  __device__ __forceinline__ void get(int g, T* p) const {
+    if (g < g0) {
+      const bool on = mbit(mask, g);
+      const T v[4] = {T(W0[g * 3 + 0]), T(W0[g * 3 + 1]), T(W0[g * 3 + 2]), T(b0[g])};
+      for (int q = 0; q < 4; ++q) p[q] = on ? -v[q] : v[q];
+    } else {
+      const int i = g - g0;
+      p[0] = Px[i];
+      p[1] = Py[i];
+      p[2] = Pz[i];
+      p[3] = Pc[i];
+    }
+  }
+};
+
 // Nearest plane crossed by the ray x + t d (t may be negative when x lies
 // outside a half-space), skipping ex1/ex2; INT_MAX when no plane bounds it.
 // Each thread keeps its best crossing t = num / den (den > 0) as a fraction
 // and compares by cross-multiplication, dividing once: the result is a
 // proposal that the checked test certifies, so ties may break either way.
 template <typename T, int NW>
-__device__ __forceinline__ void shoot(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* x,
-                                      const T* d, int ex1, int ex2, T& tmin, int& gmin) {
+__device__ __forceinline__ void shoot(const PlaneView<T>& P, int C, const T* x, const T* d, int ex1, int ex2,
+                                      T& tmin, int& gmin) {
   T nb = T(1), db = T(0);   // no crossing yet
   int gb = 0x7fffffff;
 #pragma unroll 2
   for (int g = threadIdx.x; g < C; g += 64 * NW) {
     if (g == ex1 || g == ex2) continue;
-    const T al = fmaT(Px[g], d[0], fmaT(Py[g], d[1], Pz[g] * d[2]));
+    T pg[4];
+    P.get(g, pg);
+    const T al = fmaT(pg[0], d[0], fmaT(pg[1], d[1], pg[2] * d[2]));
     if (al > T(0)) {
-      const T num = -plane_eval(Px[g], Py[g], Pz[g], Pc[g], x);
+      const T num = -plane_eval(pg[0], pg[1], pg[2], pg[3], x);
       // num / al < nb / db  <=>  num db < nb al  (al, db > 0); db == 0: none yet
       if (db == T(0) || num * db < nb * al) { nb = num; db = al; gb = g; }
     }
@@ -690,15 +739,16 @@ __device__ __forceinline__ void block_argmin_or(T& t, int& g, int& bits) {
 // undecided), and the nearest plane crossed by the ray v + t d, skipping
 // s1/s2 (the next step's proposal; meaningful only when the bits are 0).
 template <typename T, int NW>
-__device__ __forceinline__ int check_and_shoot(const T* Px, const T* Py, const T* Pz, const T* Pc, int C,
-                                               const T* v, const VertexBound<T>& vb, int c1, int c2, const T* d,
-                                               int s1, int s2, T& tmin, int& gmin) {
+__device__ __forceinline__ int check_and_shoot(const PlaneView<T>& P, int C, const T* v, const VertexBound<T>& vb,
+                                               int c1, int c2, const T* d, int s1, int s2, T& tmin, int& gmin) {
   int bits = 0;
   T nb = T(1), db = T(0);
   int gb = 0x7fffffff;
 #pragma unroll 2
   for (int g = threadIdx.x; g < C; g += 64 * NW) {
-    const T ax = Px[g], ay = Py[g], az = Pz[g], c = Pc[g];
+    T pg[4];
+    P.get(g, pg);
+    const T ax = pg[0], ay = pg[1], az = pg[2], c = pg[3];
     if (g != c1 && g != c2) {
       const int vg = checked_verdict(ax, ay, az, c, v, vb);
       bits |= vg == 1 ? 1 : (vg == 2 ? 2 : 0);
@@ -721,16 +771,18 @@ __device__ __forceinline__ int check_and_shoot(const T* Px, const T* Py, const T
 // Nearest planes crossed by x + t d for t > 0 and for t < 0 (the ray along
 // -d), skipping ex1, in one pass: (tp, gp) along +d, (tm, gm) along -d.
 template <typename T, int NW>
-__device__ __forceinline__ void shoot_both(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, const T* x,
-                                           const T* d, int ex1, T& tp, int& gp, T& tm, int& gm) {
+__device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T* x, const T* d, int ex1, T& tp,
+                                           int& gp, T& tm, int& gm) {
   T np = T(1), dp = T(0), nm = T(1), dmn = T(0);
   int bp = 0x7fffffff, bm = 0x7fffffff;
 #pragma unroll 2
   for (int g = threadIdx.x; g < C; g += 64 * NW) {
     if (g == ex1) continue;
-    const T al = fmaT(Px[g], d[0], fmaT(Py[g], d[1], Pz[g] * d[2]));
+    T pg[4];
+    P.get(g, pg);
+    const T al = fmaT(pg[0], d[0], fmaT(pg[1], d[1], pg[2] * d[2]));
     if (al == T(0)) continue;
-    const T num = -plane_eval(Px[g], Py[g], Pz[g], Pc[g], x);
+    const T num = -plane_eval(pg[0], pg[1], pg[2], pg[3], x);
     if (al > T(0)) {
       if (dp == T(0) || num * dp < np * al) { np = num; dp = al; bp = g; }
     } else {
@@ -747,41 +799,27 @@ __device__ __forceinline__ void shoot_both(const T* Px, const T* Py, const T* Pz
   tp = t1; gp = bp; tm = t2; gm = bm;
 }
 
-// Checked verdict bits of vertex v over all constraints but ex1/ex2, for the
-// whole block: bit 0 some constraint certainly violated, bit 1 some undecided.
-template <typename T, int NW>
-__device__ __forceinline__ int block_check(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int ex1,
-                                           int ex2, const T* v, const VertexBound<T>& vb) {
-  int bits = 0;
-#pragma unroll 2
-  for (int g = threadIdx.x; g < C; g += 64 * NW) {
-    if (g == ex1 || g == ex2) continue;
-    const int vg = checked_verdict(Px[g], Py[g], Pz[g], Pc[g], v, vb);
-    bits |= vg == 1 ? 1 : (vg == 2 ? 2 : 0);
-  }
-  return block_or_bits<NW>(bits);
-}
-
 // Reduced-precision walk by ray shooting.  From the start plane's edge each
 // step proposes the next edge plane as the nearest plane crossed along the
 // current edge line (one pass over the constraints), then certifies the new
 // vertex with the checked feasibility test.  On generic input the proposal
 // is the unique feasible candidate the reference's scan would accept, so the
 // face is the same; anything the bounds cannot decide goes to FP64.
-// The per-vertex serial work (vertex solve, its bound, the next direction, the
-// turn and same-vertex tests) is done by one wave, a different one for each
-// vertex so that the SIMDs share it, and handed to the block through LDS;
-// only the passes over the constraints use every wave.
+// The per-vertex serial work is split over the waves, none of it depending on
+// another's: the vertex solve, its error bound per unit of 1 + |v|_1, the next
+// edge direction, and the turn test (with, for coarse planes, the FP64 vertex)
+// run side by side and meet in LDS at one barrier; the passes over the
+// constraints use every wave.
 template <typename T, int NW>
 __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
-                         const T* Px, const T* Py, const T* Pz, const T* Pc, const uint32_t* smask, const T* eq,
-                         const T* p) {
+                         const PlaneView<T>& P, const uint32_t* smask, const T* eq, const T* p) {
   __shared__ int s_tab[AM_VM_MAX + 2];
   __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
   __shared__ int s_vpair[AM_VM_MAX * 2];
   __shared__ T s_v[6];                 // the vertex and the next edge direction
-  __shared__ VertexBound<T> s_vb;
-  __shared__ unsigned s_stat;          // reason bits of the lead wave's tests (0: passed)
+  __shared__ VertexBound<T> s_vb;      // its bound per unit of 1 + |v|_1
+  __shared__ unsigned s_turn;          // turn test: 0 passed, else its reason bit
+  __shared__ double s_vd[3];           // coarse planes: the vertex solved in FP64
   const int C = net.C, vm = net.vm;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int start_in = io.reg_start[rid];
@@ -800,7 +838,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     const T e[3] = {T(ax <= ay && ax <= az ? 1 : 0), T(ay < ax && ay <= az ? 1 : 0), T(az < ax && az < ay ? 1 : 0)};
     const T d0[3] = {eq[1] * e[2] - eq[2] * e[1], eq[2] * e[0] - eq[0] * e[2], eq[0] * e[1] - eq[1] * e[0]};
     int g0;
-    shoot<T, NW>(Px, Py, Pz, Pc, C, q, d0, -1, -1, tdummy, g0);
+    shoot<T, NW>(P, C, q, d0, -1, -1, tdummy, g0);
     if (g0 == NONE) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SEED); }
     start = g0;
   }
@@ -809,14 +847,15 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   if (!undecided && (start < 0 || start >= C)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SEED); }
   if (!undecided) {
     // edge on the start plane: its line's interval inside every other half-space
-    const T ps[4] = {Px[start], Py[start], Pz[start], Pc[start]};
+    T ps[4];
+    P.get(start, ps);
     const T d[3] = {ps[1] * eq[2] - ps[2] * eq[1], ps[2] * eq[0] - ps[0] * eq[2], ps[0] * eq[1] - ps[1] * eq[0]};
     const T pd[4] = {d[0], d[1], d[2], -(d[0] * p[0] + d[1] * p[1] + d[2] * p[2])};
     T x0[3];
     solve3(ps, eq, pd, x0);
     T th, tl;
     int rh, rl;
-    shoot_both<T, NW>(Px, Py, Pz, Pc, C, x0, d, start, th, rh, tl, rl);
+    shoot_both<T, NW>(P, C, x0, d, start, th, rh, tl, rl);
     if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) {
       undecided = true;
       why |= AM_WHY_BIT(AM_WHY_START);
@@ -825,37 +864,83 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   }
   if (!undecided) {
     if (threadIdx.x == 0) s_tab[0] = start;
-    __syncthreads();
     ntab = 1;
     int cur = start;
+    int tab1 = start, tab2 = -1;   // the last two tab entries
     while (true) {
-      // r is the proposed next edge plane
-      bool in_tab = false;
-      for (int i = 0; i < ntab; ++i) in_tab = in_tab || (s_tab[i] == r);
-      if (in_tab && !(r == s_tab[0] && ntab >= 3)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TAB); break; }
-      const int lead = nv % NW;   // the wave that does this vertex's serial work
-      if (wave == lead) {
-        const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
-        const T pc4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
+      // r is the proposed next edge plane.  The serial work of its vertex,
+      // one task per wave (all on wave 0 when the block has one):
+      if (wave == 0) {   // the vertex
+        T pr4[4], pc4[4];
+        P.get(r, pr4);
+        P.get(cur, pc4);
         T v[3];
         solve3_rcp(pr4, pc4, eq, v);
-        const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, v, T(wp.eps), T(wp.h_max));
-        // the next edge runs along plane r, away from the plane we came along;
-        // its nearest crossing is found in the same pass as this vertex's check
+        if (lane == 0) { s_v[0] = v[0]; s_v[1] = v[1]; s_v[2] = v[2]; }
+      }
+      if (wave == 1 % NW) {   // its error bound
+        T pr4[4], pc4[4];
+        P.get(r, pr4);
+        P.get(cur, pc4);
+        const VertexBound<T> vb = vertex_bound_unit<T>(pr4, pc4, eq, T(wp.eps), T(wp.h_max));
+        if (lane == 0) s_vb = vb;
+      }
+      if (wave == 2 % NW) {   // the next edge runs along plane r, away from the plane we came along
+        T pr4[4], pc4[4];
+        P.get(r, pr4);
+        P.get(cur, pc4);
         T dn[3] = {pr4[1] * eq[2] - pr4[2] * eq[1], pr4[2] * eq[0] - pr4[0] * eq[2], pr4[0] * eq[1] - pr4[1] * eq[0]};
         if (pc4[0] * dn[0] + pc4[1] * dn[1] + pc4[2] * dn[2] > T(0)) { dn[0] = -dn[0]; dn[1] = -dn[1]; dn[2] = -dn[2]; }
-        if (lane == 0) {
-          s_v[0] = v[0]; s_v[1] = v[1]; s_v[2] = v[2];
-          s_v[3] = dn[0]; s_v[4] = dn[1]; s_v[5] = dn[2];
-          s_vb = vb;
+        if (lane == 0) { s_v[3] = dn[0]; s_v[4] = dn[1]; s_v[5] = dn[2]; }
+      }
+      if (wave == 3 % NW) {   // the turn test (the walk must certainly turn the right way)
+        unsigned w = 0u;
+        if (ntab >= 2) {
+          T n1[4], n2[4], n3[4];
+          P.get(tab2, n1);
+          P.get(tab1, n2);
+          P.get(r, n3);
+          const T A12 = AM_DOT3(n1, n2), A23 = AM_DOT3(n2, n3), A13 = AM_DOT3(n1, n3), A22 = AM_DOT3(n2, n2);
+          const T C1 = AM_DOT3(n1, eq), C2 = AM_DOT3(n2, eq), C3 = AM_DOT3(n3, eq), B = AM_DOT3(eq, eq);
+          // (A12 A23 - A22 A13) - X / B > eps_turn (|A12 A23| + |A22 A13| + Y / B), times B > 0
+          const T val = (A12 * A23 - A22 * A13) * B - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3);
+          const T mag = (absT(A12 * A23) + absT(A22 * A13)) * B +
+                        (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3));
+          if (!(val > T(wp.eps_turn) * mag)) w = AM_WHY_BIT(AM_WHY_TURN);
         }
+        if (sizeof(T) == 4) {
+          T ar[4], ac[4];
+          P.get(r, ar);
+          P.get(cur, ac);
+          const double a4[4] = {ar[0], ar[1], ar[2], ar[3]};
+          const double b4[4] = {ac[0], ac[1], ac[2], ac[3]};
+          const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
+          double vd[3];
+          solve3(a4, b4, e4, vd);
+          if (lane == 0) { s_vd[0] = vd[0]; s_vd[1] = vd[1]; s_vd[2] = vd[2]; }
+        }
+        if (lane == 0) s_turn = w;
       }
       __syncthreads();
+      // (the tab, the turn verdict, the vertex and its bound; LDS written
+      // before this barrier is not rewritten until the pass's barriers below)
+      bool in_tab = false;
+      for (int i = 0; i < ntab; ++i) in_tab = in_tab || (s_tab[i] == r);
+      if (in_tab && !(r == start && ntab >= 3)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TAB); break; }
+      const unsigned turn = s_turn;
+      if (turn != 0u) { undecided = true; why |= turn; break; }
       const T v[3] = {s_v[0], s_v[1], s_v[2]};
       const T dn[3] = {s_v[3], s_v[4], s_v[5]};
-      const VertexBound<T> vb = s_vb;
+      VertexBound<T> vb = s_vb;
+      scale_bound<T>(vb, T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
+      double vd[3];
+      if (sizeof(T) == 8) {
+        vd[0] = v[0]; vd[1] = v[1]; vd[2] = v[2];
+      } else {
+        vd[0] = s_vd[0]; vd[1] = s_vd[1]; vd[2] = s_vd[2];
+      }
       int nx;
-      const int bits = check_and_shoot<T, NW>(Px, Py, Pz, Pc, C, v, vb, cur, r, dn, r, cur, tdummy, nx);
+      const int bits = check_and_shoot<T, NW>(P, C, v, vb, cur, r, dn, r, cur, tdummy, nx);
       if (bits != 0 || !vb.ok) {
         undecided = true;
         if (bits & 1) why |= AM_WHY_BIT(AM_WHY_VIOL);
@@ -863,56 +948,29 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         else why |= AM_WHY_BIT(AM_WHY_HMAX);
         break;
       }
-      if (wave == lead) {
-        unsigned w = 0u;
-        if (ntab >= 2) {
-          const int a = s_tab[ntab - 2], b = s_tab[ntab - 1];
-          const T n1[3] = {Px[a], Py[a], Pz[a]};
-          const T n2[3] = {Px[b], Py[b], Pz[b]};
-          const T n3[3] = {Px[r], Py[r], Pz[r]};
-          const T A12 = AM_DOT3(n1, n2), A23 = AM_DOT3(n2, n3), A13 = AM_DOT3(n1, n3), A22 = AM_DOT3(n2, n2);
-          const T C1 = AM_DOT3(n1, eq), C2 = AM_DOT3(n2, eq), C3 = AM_DOT3(n3, eq), B = AM_DOT3(eq, eq);
-          const T val = (A12 * A23 - A22 * A13) - (A12 * C2 * C3 + A23 * C1 * C2 - A13 * C2 * C2 - A22 * C1 * C3) / B;
-          const T mag = absT(A12 * A23) + absT(A22 * A13) +
-                        (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3)) / B;
-          if (!(val > T(wp.eps_turn) * mag)) w = AM_WHY_BIT(AM_WHY_TURN);   // must certainly turn the right way
-        }
-        double vd[3];
-        if (sizeof(T) == 8) {
-          vd[0] = v[0]; vd[1] = v[1]; vd[2] = v[2];
-        } else {
-          const double a4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
-          const double b4[4] = {Px[cur], Py[cur], Pz[cur], Pc[cur]};
-          const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
-          solve3(a4, b4, e4, vd);
-        }
-        // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
-        // planes the certified check above already puts the new vertex farther
-        // than that from the last one; with FP64 planes it may not, so vertices
-        // that close are left to the scan.
-        if (w == 0u && nv > 0) {
-          const double l1 = fabs(vd[0] - s_verts[(nv - 1) * 3 + 0]) + fabs(vd[1] - s_verts[(nv - 1) * 3 + 1]) +
-                            fabs(vd[2] - s_verts[(nv - 1) * 3 + 2]);
-          if (!(l1 > 4.0 * Tol<double>::SAME_EPS)) w = AM_WHY_BIT(AM_WHY_SAME);
-        }
-        if (lane == 0) {
-          s_stat = w;
-          if (w == 0u) {
-            s_verts[nv * 3 + 0] = vd[0];
-            s_verts[nv * 3 + 1] = vd[1];
-            s_verts[nv * 3 + 2] = vd[2];
-            s_vpair[nv * 2 + 0] = r;
-            s_vpair[nv * 2 + 1] = cur;
-            s_tab[ntab] = r;
-          }
-        }
+      // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
+      // planes the certified check above already puts the new vertex farther
+      // than that from the last one; with FP64 planes it may not, so vertices
+      // that close are left to the scan.
+      if (nv > 0) {
+        const double l1 = fabs(vd[0] - s_verts[(nv - 1) * 3 + 0]) + fabs(vd[1] - s_verts[(nv - 1) * 3 + 1]) +
+                          fabs(vd[2] - s_verts[(nv - 1) * 3 + 2]);
+        if (!(l1 > 4.0 * Tol<double>::SAME_EPS)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_SAME); break; }
       }
-      __syncthreads();
-      const unsigned st = s_stat;
-      if (st != 0u) { undecided = true; why |= st; break; }
+      // record the vertex (read after the next barrier)
+      if (threadIdx.x == 0) {
+        s_verts[nv * 3 + 0] = vd[0];
+        s_verts[nv * 3 + 1] = vd[1];
+        s_verts[nv * 3 + 2] = vd[2];
+        s_vpair[nv * 2 + 0] = r;
+        s_vpair[nv * 2 + 1] = cur;
+        s_tab[ntab] = r;
+      }
       nv += 1;
       ntab += 1;
-      if (ntab >= 4 && s_tab[0] == s_tab[ntab - 1]) break;          // closed
+      tab2 = tab1;
+      tab1 = r;
+      if (ntab >= 4 && r == start) break;          // closed
       if (ntab >= vm + 1) { undecided = true; why |= AM_WHY_BIT(AM_WHY_TRUNC); break; }   // truncated: FP64 decides
       // walk along the new edge line (found by the fused pass above)
       cur = r;
@@ -920,6 +978,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
       r = nx;
     }
   }
+  __syncthreads();   // the last records are visible to every thread
   finish_region<T, true, double>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv,
                                  start_in, start, AM_ST_OK, undecided, why);
 }
