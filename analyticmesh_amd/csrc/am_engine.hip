@@ -122,6 +122,13 @@ struct am_ctx {
   // mesh
   bool has_run = false, has_mesh = false;
   DevBuf<int> m_vfirst, m_vid;   // per (region, vertex slot): first occurrence flag, vertex index
+  // am_combine's work buffers (kept between calls) and the mesh in device memory
+  DevBuf<uint32_t> m_vt;
+  DevBuf<int> m_vstat, m_vwin, m_vown, m_vrank, m_fflag, m_fnv, m_frank, m_foff, m_face_off, m_face_idx;
+  DevBuf<double> m_xyz;
+  long long mesh_V = 0, mesh_F = 0, mesh_NI = 0;
+  // host copy of the mesh, made on first need (am_export_ply)
+  bool mesh_on_host = false;
   std::vector<double> mesh_v;
   std::vector<int> mesh_off, mesh_idx;
 
@@ -852,17 +859,22 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
     c->mesh_v.clear();
     c->mesh_off.assign(1, 0);
     c->mesh_idx.clear();
+    c->mesh_V = c->mesh_F = c->mesh_NI = 0;
+    c->mesh_on_host = false;
     if (R == 0) {
       c->has_mesh = true;
+      c->mesh_on_host = true;
       return AM_OK;
     }
     size_t cap = 1024;
     while (cap < 2 * n) cap *= 2;
-    DevBuf<uint32_t> vt;
-    DevBuf<int> vstat, vwin, vown, vrank, fflag, fnv, frank, foff, face_off, face_idx;
+    DevBuf<uint32_t>& vt = c->m_vt;
+    DevBuf<int>&vstat = c->m_vstat, &vwin = c->m_vwin, &vown = c->m_vown, &vrank = c->m_vrank, &fflag = c->m_fflag,
+       &fnv = c->m_fnv, &frank = c->m_frank, &foff = c->m_foff, &face_off = c->m_face_off,
+       &face_idx = c->m_face_idx;
     DevBuf<int>& vfirst = c->m_vfirst;
     DevBuf<int>& vid = c->m_vid;
-    DevBuf<double> xyz;
+    DevBuf<double>& xyz = c->m_xyz;
     ensure(vt, cap, false, st);
     ensure(vstat, n, false, st);
     ensure(vwin, n, false, st);
@@ -907,14 +919,10 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
     hipLaunchKernelGGL(k_face_write, dim3(blocks(R, 256)), dim3(256), 0, st, c->reg_info.p, fflag.p, frank.p, foff.p,
                        vid.p, (int)R, c->vm, face_off.p, face_idx.p);
     HIPCHK(hipGetLastError());
-    c->mesh_v.resize((size_t)V * 3);
-    c->mesh_off.resize((size_t)F + 1);
-    c->mesh_idx.resize((size_t)NI);
-    if (V) HIPCHK(hipMemcpyAsync(c->mesh_v.data(), xyz.p, (size_t)V * 3 * 8, hipMemcpyDeviceToHost, st));
-    if (F) HIPCHK(hipMemcpyAsync(c->mesh_off.data(), face_off.p, (size_t)F * 4, hipMemcpyDeviceToHost, st));
-    if (NI) HIPCHK(hipMemcpyAsync(c->mesh_idx.data(), face_idx.p, (size_t)NI * 4, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    c->mesh_off[F] = NI;
+    c->mesh_V = V;
+    c->mesh_F = F;
+    c->mesh_NI = NI;
     c->has_mesh = true;
     c->stats[AM_STAT_COMBINE_MS] =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -925,19 +933,48 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
 am_status am_get_counts(am_ctx* c, long long* nr, long long* nf, long long* nv, long long* ni) {
   if (!c) return AM_ERR_ARG;
   if (nr) *nr = c->R;
-  if (nf) *nf = c->has_mesh ? (long long)c->mesh_off.size() - 1 : 0;
-  if (nv) *nv = c->has_mesh ? (long long)c->mesh_v.size() / 3 : 0;
-  if (ni) *ni = c->has_mesh ? (long long)c->mesh_idx.size() : 0;
+  if (nf) *nf = c->has_mesh ? c->mesh_F : 0;
+  if (nv) *nv = c->has_mesh ? c->mesh_V : 0;
+  if (ni) *ni = c->has_mesh ? c->mesh_NI : 0;
   return AM_OK;
+}
+
+// Host copy of the combined mesh (for writers that walk it on the host).
+static void mesh_to_host(am_ctx* c) {
+  if (c->mesh_on_host) return;
+  c->mesh_v.resize((size_t)c->mesh_V * 3);
+  c->mesh_off.resize((size_t)c->mesh_F + 1);
+  c->mesh_idx.resize((size_t)c->mesh_NI);
+  if (c->mesh_V) HIPCHK(hipMemcpy(c->mesh_v.data(), c->m_xyz.p, (size_t)c->mesh_V * 3 * 8, hipMemcpyDeviceToHost));
+  if (c->mesh_F) HIPCHK(hipMemcpy(c->mesh_off.data(), c->m_face_off.p, (size_t)c->mesh_F * 4, hipMemcpyDeviceToHost));
+  if (c->mesh_NI)
+    HIPCHK(hipMemcpy(c->mesh_idx.data(), c->m_face_idx.p, (size_t)c->mesh_NI * 4, hipMemcpyDeviceToHost));
+  c->mesh_off[c->mesh_F] = (int)c->mesh_NI;
+  c->mesh_on_host = true;
 }
 
 am_status am_get_mesh(am_ctx* c, double* vertices, int* face_offsets, int* face_indices) {
   if (!c) return AM_ERR_ARG;
   if (!c->has_mesh) return fail(c, AM_ERR_STATE, "am_get_mesh called before am_combine");
-  if (vertices) std::memcpy(vertices, c->mesh_v.data(), c->mesh_v.size() * 8);
-  if (face_offsets) std::memcpy(face_offsets, c->mesh_off.data(), c->mesh_off.size() * 4);
-  if (face_indices) std::memcpy(face_indices, c->mesh_idx.data(), c->mesh_idx.size() * 4);
-  return AM_OK;
+  return guarded(c, [&]() -> am_status {
+    if (c->mesh_on_host) {
+      if (vertices) std::memcpy(vertices, c->mesh_v.data(), c->mesh_v.size() * 8);
+      if (face_offsets) std::memcpy(face_offsets, c->mesh_off.data(), c->mesh_off.size() * 4);
+      if (face_indices) std::memcpy(face_indices, c->mesh_idx.data(), c->mesh_idx.size() * 4);
+      return AM_OK;
+    }
+    // straight from device memory into the caller's arrays (host or device)
+    if (vertices && c->mesh_V)
+      HIPCHK(hipMemcpy(vertices, c->m_xyz.p, (size_t)c->mesh_V * 3 * 8, hipMemcpyDefault));
+    if (face_offsets) {
+      if (c->mesh_F) HIPCHK(hipMemcpy(face_offsets, c->m_face_off.p, (size_t)c->mesh_F * 4, hipMemcpyDefault));
+      const int ni = (int)c->mesh_NI;
+      HIPCHK(hipMemcpy(face_offsets + c->mesh_F, &ni, 4, hipMemcpyDefault));
+    }
+    if (face_indices && c->mesh_NI)
+      HIPCHK(hipMemcpy(face_indices, c->m_face_idx.p, (size_t)c->mesh_NI * 4, hipMemcpyDefault));
+    return AM_OK;
+  });
 }
 
 am_status am_get_regions(am_ctx* c, uint32_t* masks, int* info, int* tabs, double* verts, int* vpairs,
@@ -997,7 +1034,7 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
     for (auto& row : c->skips)
       if (!row.empty()) throw ArgErr{"the adjoint does not support skip connections"};
     const hipStream_t st = c->stream;
-    const size_t V = c->mesh_v.size() / 3;
+    const size_t V = (size_t)c->mesh_V;
     const int nl = c->D + 1;
     DevBuf<double> gv;
     upload(gv, grad_vertices, V * 3, dtype, st);
@@ -1078,6 +1115,11 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
 am_status am_export_ply(am_ctx* c, const char* path, int is_polymesh, int is_float32) {
   if (!c || !path) return AM_ERR_ARG;
   if (!c->has_mesh) return fail(c, AM_ERR_STATE, "am_export_ply called before am_combine");
+  try {
+    mesh_to_host(c);
+  } catch (const std::exception& e) {
+    return fail(c, AM_ERR_HIP, e.what());
+  }
   const long long V = (long long)c->mesh_v.size() / 3;
   const long long F = (long long)c->mesh_off.size() - 1;
   long long T = 0;
