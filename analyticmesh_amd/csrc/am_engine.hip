@@ -17,6 +17,7 @@
 #include <cstring>
 #include <fstream>
 #include <string>
+#include <deque>
 #include <vector>
 
 #include "../../include/am.h"
@@ -131,6 +132,10 @@ struct am_ctx {
   bool mesh_on_host = false;
   std::vector<double> mesh_v;
   std::vector<int> mesh_off, mesh_idx;
+
+  // staging for callers' arrays in host memory (dev_view)
+  std::deque<DevBuf<char>> stage;
+  size_t stage_used = 0;
 
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
@@ -416,66 +421,59 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   }
 }
 
-void upload(DevBuf<double>& dst, const void* src, size_t n, int dtype, hipStream_t st) {
+// A device-readable view of a caller's array: the pointer itself when it is
+// device memory, else a copy in one of the context's staging buffers (the
+// copy is ordered on st before the kernels that read it; the buffers are
+// reused from run to run, c->stage_used counts this run's).
+const void* dev_view(am_ctx* c, const void* src, size_t bytes, hipStream_t st) {
+  if (!src || bytes == 0) return src;
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, src) == hipSuccess &&
+      (a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged))
+    return src;
+  (void)hipGetLastError();   // plain host memory: the query's error is not a failure
+  if (c->stage_used == c->stage.size()) c->stage.emplace_back();
+  DevBuf<char>& b = c->stage[c->stage_used++];
+  ensure(b, bytes, false, st);
+  HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, st));
+  return b.p;
+}
+
+// n elements (float or double) -> FP64 copy
+void upload(am_ctx* c, DevBuf<double>& dst, const void* src, size_t n, int dtype, hipStream_t st) {
   ensure(dst, std::max<size_t>(n, 1), false, st);
   if (n == 0) return;
-  std::vector<double> h(n);
-  if (dtype == AM_DTYPE_F64) {
-    HIPCHK(hipMemcpy(h.data(), src, n * sizeof(double), hipMemcpyDefault));
-  } else {
-    std::vector<float> f(n);
-    HIPCHK(hipMemcpy(f.data(), src, n * sizeof(float), hipMemcpyDefault));
-    for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
-  }
-  HIPCHK(hipMemcpy(dst.p, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  const int f64 = dtype == AM_DTYPE_F64;
+  const void* v = dev_view(c, src, n * (f64 ? 8 : 4), st);
+  launch_transpose_pad(v, f64, (int)n, 1, dst.p, (int)n, 1, st);
+  HIPCHK(hipGetLastError());
 }
 
-void upload_T(DevBuf<double>& dst, const void* src, int rows, int cols, int dtype, hipStream_t st) {
-  // (rows, cols) row-major -> (cols, rows) row-major
+// (rows, cols) row-major -> FP64 (cols, rows) row-major
+void upload_T(am_ctx* c, DevBuf<double>& dst, const void* src, int rows, int cols, int dtype, hipStream_t st) {
   const size_t n = (size_t)rows * cols;
-  std::vector<double> h(n), t(n);
-  if (dtype == AM_DTYPE_F64) {
-    HIPCHK(hipMemcpy(h.data(), src, n * sizeof(double), hipMemcpyDefault));
-  } else {
-    std::vector<float> f(n);
-    HIPCHK(hipMemcpy(f.data(), src, n * sizeof(float), hipMemcpyDefault));
-    for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
-  }
-  for (int r = 0; r < rows; ++r)
-    for (int q = 0; q < cols; ++q) t[(size_t)q * rows + r] = h[(size_t)r * cols + q];
   ensure(dst, std::max<size_t>(n, 1), false, st);
-  HIPCHK(hipMemcpy(dst.p, t.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  if (n == 0) return;
+  const int f64 = dtype == AM_DTYPE_F64;
+  const void* v = dev_view(c, src, n * (f64 ? 8 : 4), st);
+  launch_transpose_pad(v, f64, rows, cols, dst.p, rows, cols, st);
+  HIPCHK(hipGetLastError());
 }
 
-// W (rows x cols, row-major) -> W^T padded: [AM_CHAIN_MAXW + 1][round_up(rows, AM_CR)],
+// W (rows x cols, row-major) -> W^T padded: [max(cols, AM_CHAIN_MAXW) + 1][round_up(rows, AM_CR)],
 // zeros past `rows` and `cols`; b -> b padded to round_up(rows, AM_CR).  The tiled
 // chain reads whole AM_CR-row blocks, and row AM_CHAIN_MAXW (+0) for its padding.
-void upload_pad(DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const void* b, int rows, int cols,
-                int dtype, hipStream_t st) {
+void upload_pad(am_ctx* c, DevBuf<double>& wdst, DevBuf<double>& bdst, const void* w, const void* b, int rows,
+                int cols, int dtype, hipStream_t st) {
   const int rp = (rows + AM_CR - 1) / AM_CR * AM_CR;
-  const size_t n = (size_t)rows * cols;
-  std::vector<double> h(n), bh(rows);
-  if (dtype == AM_DTYPE_F64) {
-    HIPCHK(hipMemcpy(h.data(), w, n * sizeof(double), hipMemcpyDefault));
-    HIPCHK(hipMemcpy(bh.data(), b, rows * sizeof(double), hipMemcpyDefault));
-  } else {
-    std::vector<float> f(n), fb(rows);
-    HIPCHK(hipMemcpy(f.data(), w, n * sizeof(float), hipMemcpyDefault));
-    HIPCHK(hipMemcpy(fb.data(), b, rows * sizeof(float), hipMemcpyDefault));
-    for (size_t i = 0; i < n; ++i) h[i] = (double)f[i];
-    for (int i = 0; i < rows; ++i) bh[i] = (double)fb[i];
-  }
-  // row AM_CHAIN_MAXW: the +0 row the chain's padding names (wider nets do not use the tiled chain)
   const int cp = std::max(cols, AM_CHAIN_MAXW) + 1;
-  std::vector<double> t((size_t)cp * rp, 0.0), bp(rp, 0.0);
-  for (int r = 0; r < rows; ++r) {
-    for (int q = 0; q < cols; ++q) t[(size_t)q * rp + r] = h[(size_t)r * cols + q];
-    bp[r] = bh[r];
-  }
-  ensure(wdst, t.size(), false, st);
-  ensure(bdst, bp.size(), false, st);
-  HIPCHK(hipMemcpy(wdst.p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(bdst.p, bp.data(), bp.size() * sizeof(double), hipMemcpyHostToDevice));
+  const int f64 = dtype == AM_DTYPE_F64;
+  const size_t esz = f64 ? 8 : 4;
+  ensure(wdst, (size_t)cp * rp, false, st);
+  ensure(bdst, (size_t)rp, false, st);
+  launch_transpose_pad(dev_view(c, w, (size_t)rows * cols * esz, st), f64, rows, cols, wdst.p, rp, cp, st);
+  launch_transpose_pad(dev_view(c, b, (size_t)rows * esz, st), f64, rows, 1, bdst.p, rp, 1, st);
+  HIPCHK(hipGetLastError());
 }
 
 am_status fail(am_ctx* c, am_status s, const std::string& m) {
@@ -637,16 +635,24 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->WT.resize(nl);
     c->Wrow.resize(nl);
     c->b.resize(nl);
-    upload(c->W0, weights[0], (size_t)c->nodes[1] * 3, dtype, st);
+    // device-readable views of the weights (host arrays are staged once)
+    c->stage_used = 0;
+    const size_t esz = dtype == AM_DTYPE_F64 ? 8 : 4;
+    std::vector<const void*> wv(nl), bv(nl);
+    wv[0] = dev_view(c, weights[0], (size_t)c->nodes[1] * 3 * esz, st);
+    for (int li = 1; li < nl; ++li)
+      wv[li] = dev_view(c, weights[li], (size_t)c->nodes[li + 1] * c->nodes[li] * esz, st);
+    for (int li = 0; li < nl; ++li) bv[li] = dev_view(c, biases[li], (size_t)c->nodes[li + 1] * esz, st);
+    upload(c, c->W0, wv[0], (size_t)c->nodes[1] * 3, dtype, st);
     for (int li = 0; li < nl; ++li) {
-      upload(c->b[li], biases[li], (size_t)c->nodes[li + 1], dtype, st);
-      if (li >= 1) upload_T(c->WT[li], weights[li], c->nodes[li + 1], c->nodes[li], dtype, st);
-      if (li >= 1) upload(c->Wrow[li], weights[li], (size_t)c->nodes[li + 1] * c->nodes[li], dtype, st);
+      upload(c, c->b[li], bv[li], (size_t)c->nodes[li + 1], dtype, st);
+      if (li >= 1) upload_T(c, c->WT[li], wv[li], c->nodes[li + 1], c->nodes[li], dtype, st);
+      if (li >= 1) upload(c, c->Wrow[li], wv[li], (size_t)c->nodes[li + 1] * c->nodes[li], dtype, st);
     }
     c->WTp.resize(nl);
     c->bpad.resize(nl);
     for (int li = 1; li < nl; ++li)
-      upload_pad(c->WTp[li], c->bpad[li], weights[li], biases[li], c->nodes[li + 1], c->nodes[li], dtype, st);
+      upload_pad(c, c->WTp[li], c->bpad[li], wv[li], bv[li], c->nodes[li + 1], c->nodes[li], dtype, st);
     c->tm.resize(n_tm);
     c->tmT.resize(n_tm);
     c->tm_h.assign(n_tm, 0);
@@ -656,8 +662,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       c->tm_h[t] = h;
       c->tm_w[t] = w;
       if (h > 0 && w > 0) {
-        upload(c->tm[t], tms[t], (size_t)h * w, dtype, st);
-        upload_T(c->tmT[t], tms[t], h, w, dtype, st);
+        upload(c, c->tm[t], tms[t], (size_t)h * w, dtype, st);
+        upload_T(c, c->tmT[t], tms[t], h, w, dtype, st);
       }
     }
     // skip shape checks (backend/src/cuam.cpp:160-172)
@@ -672,8 +678,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
           if (w != c->nodes[s.first]) throw ArgErr{"transform matrix columns must match the source width"};
         }
       }
-    upload(c->ew, extra_w, (size_t)c->E * 3, dtype, st);
-    upload(c->eb, extra_b, (size_t)c->E, dtype, st);
+    upload(c, c->ew, extra_w, (size_t)c->E * 3, dtype, st);
+    upload(c, c->eb, extra_b, (size_t)c->E, dtype, st);
 
     NetDev& n = c->net;
     std::memset(&n, 0, sizeof(n));
@@ -725,7 +731,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
                                    (int)clds));
     }
     const bool fp64_form = c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY;
-    if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, weights, biases, dtype, st);
+    if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, wv.data(), bv.data(), dtype, st);
     if (c->tile_ok && ray64_lds_bytes(c) > 65536 && ray64_lds_bytes(c) <= c->max_lds)
       HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)ray64_lds_bytes(c)));
@@ -1037,7 +1043,8 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
     const size_t V = (size_t)c->mesh_V;
     const int nl = c->D + 1;
     DevBuf<double> gv;
-    upload(gv, grad_vertices, V * 3, dtype, st);
+    c->stage_used = 0;
+    upload(c, gv, grad_vertices, V * 3, dtype, st);
     std::vector<DevBuf<double>> dW(nl), db(nl), Gst(nl), Pst(nl);
     for (int li = 0; li < nl; ++li) {
       const size_t nw = (size_t)c->nodes[li + 1] * c->nodes[li];

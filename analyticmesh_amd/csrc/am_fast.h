@@ -66,7 +66,8 @@ struct FastTimes {
   long long why[AM_WHY_COUNT];   // undecided regions per reason (AM_WHY_*), accumulated
 };
 
-// Upload weights in MFMA operand order; `weights`/`biases` as given to am_run.
+// Weights in MFMA operand order; `weights`/`biases` are device-readable
+// (float or double as `dtype`), the layouts are built on stream st.
 void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, const void* const* weights,
                   const void* const* biases, int dtype, hipStream_t st);
 // Planes + checked walk for one level; the walk marks each region in

@@ -371,15 +371,18 @@ void* slot(FastBuffers& b, int s, size_t bytes) {
   return b.mem[s];
 }
 
-std::vector<float> to_float(const void* src, size_t n, int dtype) {
+// host copy as float of a device-readable array (ordered after earlier work on st)
+std::vector<float> to_float(const void* src, size_t n, int dtype, hipStream_t st) {
   std::vector<float> out(n);
   if (n == 0) return out;
   if (dtype == 1) {
     std::vector<double> d(n);
-    FCHK(hipMemcpy(d.data(), src, n * 8, hipMemcpyDefault));
+    FCHK(hipMemcpyAsync(d.data(), src, n * 8, hipMemcpyDefault, st));
+    FCHK(hipStreamSynchronize(st));
     for (size_t i = 0; i < n; ++i) out[i] = (float)d[i];
   } else {
-    FCHK(hipMemcpy(out.data(), src, n * 4, hipMemcpyDefault));
+    FCHK(hipMemcpyAsync(out.data(), src, n * 4, hipMemcpyDefault, st));
+    FCHK(hipStreamSynchronize(st));
   }
   return out;
 }
@@ -408,7 +411,6 @@ size_t gemm_lds(const FastNetDev& d, int prec) {
 
 void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, const void* const* weights,
                   const void* const* biases, int dtype, hipStream_t st) {
-  (void)st;
   FastNetDev& d = f.dev;
   std::memset(&d, 0, sizeof(d));
   f.precision = precision;
@@ -425,53 +427,47 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   d.Kpad = (maxw + 31) & ~31;
   d.iso = (float)net.iso;
   const int prec = precision == 1 ? 1 : 0;
-  // W0, biases, extras
+  // W0, biases, extras: FP32 copies built on the device (weights and biases
+  // are device-readable here; the extras are the engine's FP64 copies)
+  const int f64 = dtype == 1;
   {
-    auto w0 = to_float(weights[0], (size_t)net.nodes[1] * 3, dtype);
-    float* p = (float*)slot(b, S_W0, w0.size() * 4);
-    FCHK(hipMemcpy(p, w0.data(), w0.size() * 4, hipMemcpyHostToDevice));
+    const int n0 = net.nodes[1] * 3;
+    float* p = (float*)slot(b, S_W0, (size_t)n0 * 4);
+    launch_transpose_pad(weights[0], f64, n0, 1, p, n0, 1, st);
     d.W0 = p;
   }
   for (int li = 0; li <= net.D; ++li) {
-    auto bb = to_float(biases[li], (size_t)net.nodes[li + 1], dtype);
-    float* p = (float*)slot(b, S_BIAS0 + li, bb.size() * 4);
-    FCHK(hipMemcpy(p, bb.data(), bb.size() * 4, hipMemcpyHostToDevice));
+    const int nb = net.nodes[li + 1];
+    float* p = (float*)slot(b, S_BIAS0 + li, (size_t)nb * 4);
+    launch_transpose_pad(biases[li], f64, nb, 1, p, nb, 1, st);
     d.bias[li] = p;
   }
   {
-    std::vector<float> ew(std::max(1, net.E * 3)), eb(std::max(1, net.E));
+    float* pw = (float*)slot(b, S_EW, (size_t)std::max(1, net.E * 3) * 4);
+    float* pb = (float*)slot(b, S_EB, (size_t)std::max(1, net.E) * 4);
     if (net.E) {
-      std::vector<double> t(net.E * 3);
-      FCHK(hipMemcpy(t.data(), net.ew, t.size() * 8, hipMemcpyDeviceToHost));
-      for (size_t i = 0; i < t.size(); ++i) ew[i] = (float)t[i];
-      std::vector<double> u(net.E);
-      FCHK(hipMemcpy(u.data(), net.eb, u.size() * 8, hipMemcpyDeviceToHost));
-      for (size_t i = 0; i < u.size(); ++i) eb[i] = (float)u[i];
+      launch_transpose_pad(net.ew, 1, net.E * 3, 1, pw, net.E * 3, 1, st);
+      launch_transpose_pad(net.eb, 1, net.E, 1, pb, net.E, 1, st);
     }
-    float* pw = (float*)slot(b, S_EW, ew.size() * 4);
-    float* pb = (float*)slot(b, S_EB, eb.size() * 4);
-    FCHK(hipMemcpy(pw, ew.data(), ew.size() * 4, hipMemcpyHostToDevice));
-    FCHK(hipMemcpy(pb, eb.data(), eb.size() * 4, hipMemcpyHostToDevice));
     d.ew = pw;
     d.eb = pb;
   }
+  FCHK(hipGetLastError());
   // MFMA A operands of every layer after the first
   for (int li = 1; li <= net.D; ++li) {
     const int nin = net.nodes[li], nout = net.nodes[li + 1];
-    auto w = to_float(weights[li], (size_t)nout * nin, dtype);
     const int nrb = (nout + 15) / 16;
     d.nrb[li] = nrb;
     if (prec == 0) {
       // W^T, k-major: [nin + 1][nrb * 16]; row nin (and rows past nout) are +0
       d.nkb[li] = (nin + 3) / 4;
       const int ld = nrb * 16;
-      std::vector<float> a((size_t)(nin + 1) * ld, 0.f);
-      for (int k = 0; k < nin; ++k)
-        for (int row = 0; row < nout; ++row) a[(size_t)k * ld + row] = w[(size_t)row * nin + k];
-      float* p = (float*)slot(b, S_WP0 + li, a.size() * 4);
-      FCHK(hipMemcpy(p, a.data(), a.size() * 4, hipMemcpyHostToDevice));
+      float* p = (float*)slot(b, S_WP0 + li, (size_t)(nin + 1) * ld * 4);
+      launch_transpose_pad(weights[li], f64, nout, nin, p, ld, nin + 1, st);
+      FCHK(hipGetLastError());
       d.Wp[li] = p;
     } else {
+      auto w = to_float(weights[li], (size_t)nout * nin, dtype, st);
       const int nkb = (nin + 31) / 32;
       d.nkb[li] = nkb;
       const size_t n = (size_t)nrb * nkb * 64 * 8;

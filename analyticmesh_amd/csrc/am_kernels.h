@@ -92,4 +92,10 @@ __global__ void k_vertex_fp64(NetDev net, RegionIO io, int rid0, const double* p
                               int Cs_hbm);
 __global__ void k_fill_i32(int* p, size_t n, int v);
 
+// Weight layouts on the device (am_layout.hip): dst[j * ni + i] = src[i * cols + j]
+// for i < rows, j < cols, +0 elsewhere (i < ni, j < nj); src float or double
+// (f64), device-readable.  A copy with padding is cols = nj = 1.
+void launch_transpose_pad(const void* src, int f64, int rows, int cols, double* dst, int ni, int nj, hipStream_t st);
+void launch_transpose_pad(const void* src, int f64, int rows, int cols, float* dst, int ni, int nj, hipStream_t st);
+
 }  // namespace am
