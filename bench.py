@@ -136,7 +136,8 @@ def traffic_from_profile(kernel):
         return None
     for line in open(path):
         parts = line.split()
-        if len(parts) >= 2 and parts[0] == kernel:
+        # machine-read lines are exactly "kernel bytes"; table rows have more fields
+        if len(parts) == 2 and parts[0] == kernel:
             try:
                 return float(parts[1])
             except ValueError:
