@@ -308,7 +308,8 @@ size_t ray64_lds_bytes(const am_ctx* c) {
 // walk, and the scan on the regions it leaves undecided, reusing their planes
 // (the scan runs a block-stride loop up to the device-side count); mode 2:
 // after a reduced-precision walk that flagged the level in rflags
-// (`undecided` of them): the regions it decided get FP64 vertices
+// (`undecided` of them, -1 when the count is not known on the host): the
+// regions it decided get FP64 vertices
 // (k_vertex_fp64, their other records stay as written), the others go
 // through mode 1.
 // Nothing here waits for the device.  Falls back to the one-region kernel
@@ -764,12 +765,15 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     }
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
-    long long levels = 0, cands = 0, redo = 0, launches = 0, vert64 = 0;
+    long long levels = 0, cands = 0, launches = 0, processed = 0;
     long long why[AM_WHY_COUNT] = {0};
     Fp64Counts f64;
     c->timer.reset();
-    ensure(c->runacc, 1 + AM_WHY_COUNT, false, st);
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->runacc.p, (size_t)(1 + AM_WHY_COUNT), 0);
+    // run counters on the device: [0, 1 + W) the FP64 ray walk's undecided
+    // regions and reasons, [1 + W, 2 + 2W) the reduced-precision walk's
+    constexpr int NACC = 2 * (1 + AM_WHY_COUNT);
+    ensure(c->runacc, NACC, false, st);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->runacc.p, (size_t)NACC, 0);
     while (lvl_count > 0) {
       if (c->max_regions >= 0 && lvl_first >= c->max_regions) break;
       const size_t ncand = (size_t)lvl_count * c->vm;
@@ -788,15 +792,15 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.reg_vpair = c->reg_vpair.p; fl.cand_mask = c->cand_mask.p; fl.cand_start = c->cand_start.p;
         fl.cand_point = c->cand_point.p;
         FastTimes ft{};
-        const int nredo = fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p);
+        fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT);
         // FP64 planes of the whole level: the regions the walk decided get
         // their vertices re-solved from them; the undecided ones are re-decided
         // in place (records and candidate slots rewritten) by the checked FP64
         // ray walk, and by the candidate scan where that is undecided too.
-        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr, 2, f64, nredo);
-        redo += nredo;
-        vert64 += lvl_count - nredo;
-        for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += ft.why[w];
+        // (The split is known on the device only: both kernels cover the
+        // level and skip the other kind's regions.)
+        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr, 2, f64);
+        processed += lvl_count;
         launches += ft.launches;
       }
       const size_t mi = c->timer.mark(st);
@@ -812,8 +816,10 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       // regions found but not processed: drop them from the store
       c->R = lvl_first;
     }
-    int acc[1 + AM_WHY_COUNT] = {0};
+    int acc[NACC] = {0};
     HIPCHK(hipMemcpyAsync(acc, c->runacc.p, sizeof(acc), hipMemcpyDeviceToHost, st));
+    const long long redo = fp64_form ? 0 : acc[1 + AM_WHY_COUNT];
+    const long long vert64 = fp64_form ? 0 : processed - redo;
     HIPCHK(hipStreamSynchronize(st));
     c->timer.resolve();
     const auto t1 = std::chrono::steady_clock::now();
@@ -832,6 +838,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       c->stats[AM_STAT_GEMM_MS] = T[T_GEMM];
       c->stats[AM_STAT_WALK_MS] = T[T_WALK];
       c->stats[AM_STAT_REDO_MS] = fp64_ms;
+      for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] = acc[2 + AM_WHY_COUNT + w];
     }
     c->stats[AM_STAT_LEVELS] = (double)levels;
     c->stats[AM_STAT_REGIONS] = (double)c->R;

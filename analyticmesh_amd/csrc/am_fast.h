@@ -63,7 +63,6 @@ struct FastLevel {
 
 struct FastTimes {
   int launches;
-  long long why[AM_WHY_COUNT];   // undecided regions per reason (AM_WHY_*), accumulated
 };
 
 // Weights in MFMA operand order; `weights`/`biases` are device-readable
@@ -72,10 +71,11 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
                   const void* const* biases, int dtype, hipStream_t st);
 // Planes + checked walk for one level; the walk marks each region in
 // flags[0, lvl.count) (AM_UNDECIDED plus reason bits, or 0 when it wrote the
-// region's records).  Returns the number of undecided regions.
-// Kernel times go to `timer` (T_GEMM, T_WALK); the count read back is the only wait.
-int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-               StageTimer& timer, FastTimes& t, unsigned* flags);
+// region's records).  The undecided regions are added to acc[0] and their
+// reasons to acc[1 + w] on the device; nothing here waits.
+// Kernel times go to `timer` (T_GEMM, T_WALK).
+void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
+                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc);
 // Planes of n given regions (their masks in `reg_mask`) with the prepared
 // reduced-precision form: planes [n][4][Cs] f32 (signed), eq [n][4] f32.
 void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st);

@@ -330,15 +330,17 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
   walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
 }
 
-// Compact the undecided regions' ids; cnt[0] = their number, cnt[1 + w] = how
-// many were left for reason w (a region can carry one reason per walk exit).
-// Regions the walk left undecided (cnt[0]) and their reasons (cnt[1 + w]).
+// Regions the walk left undecided, added to cnt[0], and their reasons, added
+// to cnt[1 + w] (counted per wave, one atomic per wave and counter).
 __global__ void k_count_undecided(const unsigned* flags, int n, int* cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && (flags[i] & AM_UNDECIDED)) {
-    atomicAdd(cnt, 1);
-    for (int w = 0; w < AM_WHY_COUNT; ++w)
-      if (flags[i] & AM_WHY_BIT(w)) atomicAdd(cnt + 1 + w, 1);
+  const unsigned f = i < n ? flags[i] : 0u;
+  const bool lead = (threadIdx.x & 63) == 0;
+  const int u = __popcll(__ballot((f & AM_UNDECIDED) != 0u));
+  if (lead && u) atomicAdd(cnt, u);
+  for (int w = 0; w < AM_WHY_COUNT; ++w) {
+    const int k = __popcll(__ballot((f & AM_UNDECIDED) && (f & AM_WHY_BIT(w))));
+    if (lead && k) atomicAdd(cnt + 1 + w, k);
   }
 }
 
@@ -504,8 +506,8 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   f.ready = true;
 }
 
-int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-               StageTimer& timer, FastTimes& t, unsigned* flags) {
+void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
+                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc) {
   const FastNetDev& d = f.dev;
   const int prec = f.precision == 1 ? 1 : 0;
   t.launches = 0;
@@ -513,7 +515,6 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
   const int chunk = std::min(lvl.count, CH);
   float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * 4 * d.Cs * 4);
   float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * 4 * 4);
-  int* cnt = (int*)slot(b, S_CNT, 4 * (1 + AM_WHY_COUNT));
 
   int Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
   while (Csort < net.C) Csort <<= 1;
@@ -570,14 +571,8 @@ int fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& l
     timer.span(m1, timer.mark(st), T_WALK);
     t.launches += 1;
   }
-  FCHK(hipMemsetAsync(cnt, 0, 4 * (1 + AM_WHY_COUNT), st));
-  hipLaunchKernelGGL(k_count_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count, cnt);
+  hipLaunchKernelGGL(k_count_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count, acc);
   FCHK(hipGetLastError());
-  int hc[1 + AM_WHY_COUNT] = {0};
-  FCHK(hipMemcpyAsync(hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, st));
-  FCHK(hipStreamSynchronize(st));
-  for (int w = 0; w < AM_WHY_COUNT; ++w) t.why[w] += hc[1 + w];
-  return hc[0];
 }
 
 void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st) {
