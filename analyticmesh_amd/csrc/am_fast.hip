@@ -25,6 +25,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 #define AM_XS (AM_GN + 4)   // fp32 LDS row stride (floats)
+#define AM_GRBW 4           // 16-row output blocks per wave (layers up to 256 outputs)
 
 struct GemmIO {
   const uint32_t* reg_mask;
@@ -89,8 +90,10 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int KS = f.Kpad + 8;   // bf16 column stride (halves)
+  // fp32: one [Kpad][AM_XS] buffer, X at list positions during the k loop,
+  // then the layer's outputs parked over it (Ys = Xs); bf16: Ys, then Xh/Xl
   float* Ys = reinterpret_cast<float*>(smem_raw);                   // [Kpad][AM_XS]
-  float* Xs = Ys + (size_t)f.Kpad * AM_XS;                           // fp32: [Kpad][AM_XS]
+  float* Xs = PREC == 0 ? Ys : Ys + (size_t)f.Kpad * AM_XS;
   uint16_t* Xh = reinterpret_cast<uint16_t*>(Xs);                    // bf16: [AM_GN][KS]
   uint16_t* Xl = Xh + (size_t)AM_GN * KS;
   uint32_t* mk = reinterpret_cast<uint32_t*>(PREC == 0 ? (unsigned char*)(Xs + (size_t)f.Kpad * AM_XS)
@@ -143,9 +146,14 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
     const int nout = f.nodes[li + 1];
     const int nrb = f.nrb[li], nkb = f.nkb[li];
     const float* bias = f.bias[li];
-    for (int rb = wave; rb < nrb; rb += 4) {
+    // row blocks rb = wave + 4 u of this wave (n_out <= 64 * AM_GRBW)
+    f32x4 accs[AM_GRBW][2];
+#pragma unroll
+    for (int u = 0; u < AM_GRBW; ++u) {
+      const int rb = wave + 4 * u;
       f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-      if (PREC == 0) {
+      if (rb >= nrb) {
+      } else if (PREC == 0) {
         // A[row][kb*4 + (lane >> 4)] = W^T[list entry][row] (k-major, rows padded to 16)
         // (through a buffer resource: 32-bit byte offsets, one multiply-add each)
         const int nkl = s_nk[li] >> 2;
@@ -207,6 +215,16 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
           acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh1, acc1, 0, 0, 0);
         }
       }
+      accs[u][0] = acc0;
+      accs[u][1] = acc1;
+    }
+    // fp32: the outputs go over X, so every wave must be done reading it
+    if (PREC == 0) __syncthreads();
+#pragma unroll
+    for (int u = 0; u < AM_GRBW; ++u) {
+      const int rb = wave + 4 * u;
+      if (rb >= nrb) continue;
+      const f32x4 acc0 = accs[u][0], acc1 = accs[u][1];
       const int c0 = lane & 15;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
@@ -234,13 +252,35 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
         }
       }
       if (PREC == 0) {
-        // next layer's X: its list's rows of the masked outputs (+0 for the padding entries)
+        // next layer's X, in place: position i holds output kl2[i] masked (+0
+        // for the padding entries).  The list ascends, so position i reads row
+        // kl2[i] >= i: in chunks of 128 positions, each chunk reads (rows at or
+        // past its first position) before a barrier and writes after it, and
+        // no chunk overwrites a row a later chunk reads.
         const uint16_t* kl2 = klist + (size_t)li * KLF;
         const int nkp2 = s_nk[li + 1];
-        for (int idx = tid; idx < nkp2 * AM_GN; idx += 256) {
-          const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2;
-          const int k = kl2[j];
-          Xs[j * AM_XS + col] = (k < nout && mbit(mk + t * f.Wd, out0 + k)) ? Ys[k * AM_XS + col] : 0.f;
+        constexpr int PPT = 256 / AM_GN;   // positions per pass of the block
+        constexpr int CH = 128 / PPT;      // passes per chunk
+        int tq = tid;
+        asm volatile("" : "+v"(tq));       // keeps these offsets from being hoisted out of the layer loop
+        const int col = tq & (AM_GN - 1), t = col >> 2, p0 = tq / AM_GN;
+        for (int c0 = 0; c0 < nkp2; c0 += 128) {
+          float xv[CH];
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            const int i = c0 + c * PPT + p0;
+            xv[c] = 0.f;
+            if (i < nkp2) {
+              const int k = kl2[i];
+              if (k < nout && mbit(mk + t * f.Wd, out0 + k)) xv[c] = Ys[k * AM_XS + col];
+            }
+          }
+          __syncthreads();
+#pragma unroll
+          for (int c = 0; c < CH; ++c) {
+            const int i = c0 + c * PPT + p0;
+            if (i < nkp2) Xs[i * AM_XS + col] = xv[c];
+          }
         }
       }
       const int rows = PREC == 0 ? 0 : ((nout + 31) & ~31);
@@ -404,7 +444,7 @@ float host_bf16_f(uint16_t h) {
 
 size_t gemm_lds(const FastNetDev& d, int prec) {
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
-  const size_t xs = prec == 0 ? (size_t)d.Kpad * AM_XS * 4 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;
+  const size_t xs = prec == 0 ? 0 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;   // fp32: X shares Ys
   const size_t kls = prec == 0 ? (size_t)d.D * (d.Kpad + 32) * 2 : 0;
   return ys + xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
 }
@@ -418,6 +458,10 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   f.precision = precision;
   for (int li = 1; li < net.D; ++li)
     if (net.nskip[li] > 0) throw std::runtime_error("skip connections run on the FP64 path only");
+  for (int li = 1; li <= net.D; ++li)
+    if (net.nodes[li + 1] > 64 * AM_GRBW)
+      throw std::runtime_error("the reduced-precision forms take layers up to " + std::to_string(64 * AM_GRBW) +
+                               " wide; use fp64 or fp64ray");
   d.D = net.D; d.L = net.L; d.E = net.E; d.C = net.C; d.Wd = net.Wd;
   d.Cs = (net.C + 63) & ~63;
   int maxw = 0;

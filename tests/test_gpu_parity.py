@@ -84,3 +84,32 @@ def test_multi_seed_dedup(oracle_mod):
     o = oracle_mod.run_oracle(spec, pts, st, traversal="bfs")
     np.testing.assert_array_equal(g.masks, o.masks)
     np.testing.assert_array_equal(g.tabs, o.tabs)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_device_inputs_match_host_inputs(prec, dtype):
+    """Weights, seeds and states given as device tensors (no host staging) give the same
+    records and mesh as the same arrays given in host memory."""
+    import torch
+    from analyticmesh_amd.engine import Engine
+    spec, pts, st = workload("sphere", n_seeds=64)
+    _say(f"[device inputs] {prec} {dtype}")
+    W = [np.ascontiguousarray(w, dtype) for w in spec.weights]
+    B = [np.ascontiguousarray(b, dtype) for b in spec.biases]
+    P = np.ascontiguousarray(pts, dtype)
+    out = []
+    for on_device in (False, True):
+        eng = Engine(spec.nodes, precision=prec)
+        if on_device:
+            dev = torch.device("cuda:0")
+            args = ([torch.from_numpy(w).to(dev) for w in W], [torch.from_numpy(b).to(dev) for b in B],
+                    torch.from_numpy(P).to(dev), torch.from_numpy(np.ascontiguousarray(st, np.uint8)).to(dev))
+        else:
+            args = (W, B, P, st)
+        res = eng.run(*args)
+        V, faces = res.combine()
+        out.append((res.masks.copy(), res.info.copy(), res.verts.copy(), V.copy(), faces.offsets.copy(),
+                    faces.indices.copy()))
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
