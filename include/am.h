@@ -80,7 +80,8 @@ const char* am_last_error(const am_ctx* ctx);
 am_status am_get_counts(am_ctx* ctx, long long* n_regions, long long* n_faces, long long* n_vertices,
                         long long* n_face_indices);
 
-/* Combined mesh: vertices (V x 3 double), face_offsets (F + 1), face_indices. */
+/* Combined mesh: vertices (V x 3 double), face_offsets (F + 1), face_indices.  The arrays may be
+ * host or device memory; the mesh is copied from device memory straight into them. */
 am_status am_get_mesh(am_ctx* ctx, double* vertices, int* face_offsets, int* face_indices);
 
 /* Per-region records in visiting order (parity view): masks [R][W] u32,
