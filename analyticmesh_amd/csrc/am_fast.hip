@@ -90,14 +90,16 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int KS = f.Kpad + 8;   // bf16 column stride (halves)
-  // fp32: one [Kpad][AM_XS] buffer, X at list positions during the k loop,
-  // then the layer's outputs parked over it (Ys = Xs); bf16: Ys, then Xh/Xl
-  float* Ys = reinterpret_cast<float*>(smem_raw);                   // [Kpad][AM_XS]
-  float* Xs = PREC == 0 ? Ys : Ys + (size_t)f.Kpad * AM_XS;
-  uint16_t* Xh = reinterpret_cast<uint16_t*>(Xs);                    // bf16: [AM_GN][KS]
+  // One buffer holds a layer's input during its k loop and, after a barrier,
+  // its outputs (Ys, [Kpad][AM_XS] floats) that become the next input:
+  // fp32 X at list positions ([Kpad][AM_XS]); bf16 Xh/Xl ([AM_GN][KS] halves each).
+  float* Ys = reinterpret_cast<float*>(smem_raw);
+  float* Xs = Ys;
+  uint16_t* Xh = reinterpret_cast<uint16_t*>(smem_raw);
   uint16_t* Xl = Xh + (size_t)AM_GN * KS;
-  uint32_t* mk = reinterpret_cast<uint32_t*>(PREC == 0 ? (unsigned char*)(Xs + (size_t)f.Kpad * AM_XS)
-                                                       : (unsigned char*)(Xl + (size_t)AM_GN * KS));
+  const size_t xbytes = PREC == 0 ? (size_t)f.Kpad * AM_XS * 4
+                                  : std::max((size_t)f.Kpad * AM_XS * 4, (size_t)2 * AM_GN * KS * 2);
+  uint32_t* mk = reinterpret_cast<uint32_t*>(smem_raw + xbytes);
   const int KLF = f.Kpad + 32;                                       // entries of one layer's input list
   uint16_t* klist = reinterpret_cast<uint16_t*>(mk + AM_GT * f.Wd);  // fp32: [D][KLF]
   __shared__ int s_nk[AM_MAX_LINEAR];
@@ -218,8 +220,8 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
       accs[u][0] = acc0;
       accs[u][1] = acc1;
     }
-    // fp32: the outputs go over X, so every wave must be done reading it
-    if (PREC == 0) __syncthreads();
+    // the outputs go over X, so every wave must be done reading it
+    __syncthreads();
 #pragma unroll
     for (int u = 0; u < AM_GRBW; ++u) {
       const int rb = wave + 4 * u;
@@ -283,14 +285,30 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
           }
         }
       }
-      const int rows = PREC == 0 ? 0 : ((nout + 31) & ~31);
-      for (int idx = tid; idx < rows * AM_GN; idx += 256) {
-        const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2;
-        const float x = (j < nout && mbit(mk + t * f.Wd, out0 + j)) ? Ys[j * AM_XS + col] : 0.f;
-        {
-          const uint16_t h = bf16_rn(x);
-          Xh[col * KS + j] = h;
-          Xl[col * KS + j] = bf16_rn(x - bf16_to_f(h));
+      if (PREC != 0) {
+        // next layer's Xh/Xl (masked outputs split into bf16 parts, rows up to
+        // a multiple of 32), over the outputs: every thread reads its values,
+        // then a barrier, then the writes
+        const int rows = (nout + 31) & ~31;
+        constexpr int XPT = 256 * AM_GN / 256;   // values per thread for rows <= 256
+        int tq = tid;
+        asm volatile("" : "+v"(tq));             // keeps these offsets from being hoisted out of the layer loop
+        const int col = tq & (AM_GN - 1), t = col >> 2, j0 = tq / AM_GN;
+        float xv[XPT];
+#pragma unroll
+        for (int c = 0; c < XPT; ++c) {
+          const int j = j0 + c * (256 / AM_GN);
+          xv[c] = (j < nout && mbit(mk + t * f.Wd, out0 + j)) ? Ys[j * AM_XS + col] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < XPT; ++c) {
+          const int j = j0 + c * (256 / AM_GN);
+          if (j < rows) {
+            const uint16_t h = bf16_rn(xv[c]);
+            Xh[col * KS + j] = h;
+            Xl[col * KS + j] = bf16_rn(xv[c] - bf16_to_f(h));
+          }
         }
       }
     } else if (tid < AM_GN) {
@@ -443,10 +461,11 @@ float host_bf16_f(uint16_t h) {
 }
 
 size_t gemm_lds(const FastNetDev& d, int prec) {
+  // one buffer for a layer's input and its outputs (k_gemm), the masks, the fp32 input lists
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
-  const size_t xs = prec == 0 ? 0 : (size_t)2 * AM_GN * (d.Kpad + 8) * 2;   // fp32: X shares Ys
+  const size_t xs = prec == 0 ? ys : std::max(ys, (size_t)2 * AM_GN * (d.Kpad + 8) * 2);
   const size_t kls = prec == 0 ? (size_t)d.D * (d.Kpad + 32) * 2 : 0;
-  return ys + xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
+  return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
 }
 
 }  // namespace
