@@ -148,77 +148,81 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
     const int nout = f.nodes[li + 1];
     const int nrb = f.nrb[li], nkb = f.nkb[li];
     const float* bias = f.bias[li];
-    // row blocks rb = wave + 4 u of this wave (n_out <= 64 * AM_GRBW)
+    // Row blocks rb = wave + 4 u of this wave (n_out <= 64 * AM_GRBW).  The k
+    // loop is outermost, so each B fragment read from LDS feeds every row block.
     f32x4 accs[AM_GRBW][2];
 #pragma unroll
     for (int u = 0; u < AM_GRBW; ++u) {
-      const int rb = wave + 4 * u;
-      f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-      if (rb >= nrb) {
-      } else if (PREC == 0) {
-        // A[row][kb*4 + (lane >> 4)] = W^T[list entry][row] (k-major, rows padded to 16)
-        // (through a buffer resource: 32-bit byte offsets, one multiply-add each)
-        const int nkl = s_nk[li] >> 2;
-        const uint16_t* kl = klist + (size_t)(li - 1) * KLF + (lane >> 4);
-        const int ld4 = nrb * 16 * 4;
-        const __amdgpu_buffer_rsrc_t wres = __builtin_amdgcn_make_buffer_rsrc(
-            (void*)f.Wp[li], (short)0, (f.nodes[li] + 1) * ld4, 0x00020000);
-        const int c4 = (rb * 16 + (lane & 15)) * 4;
-#define AM_GA(e) __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(wres, (int)kl[(e) * 4] * ld4 + c4, 0, 0))
-        const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
-        // nkl is a multiple of 4; the A operands of the next 4 k blocks are in
-        // flight while a block's MFMAs run (two register sets, alternating)
-#define AM_G4(r, kb)                                                               \
-  {                                                                                \
-    const float* x0 = xb + ((kb) * 4) * AM_XS;                                     \
-    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##0, x0[0], acc0, 0, 0, 0);       \
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##0, x0[16], acc1, 0, 0, 0);      \
-    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##1, x0[4 * AM_XS], acc0, 0, 0, 0);       \
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##1, x0[4 * AM_XS + 16], acc1, 0, 0, 0);  \
-    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##2, x0[8 * AM_XS], acc0, 0, 0, 0);       \
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##2, x0[8 * AM_XS + 16], acc1, 0, 0, 0);  \
-    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##3, x0[12 * AM_XS], acc0, 0, 0, 0);      \
-    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(r##3, x0[12 * AM_XS + 16], acc1, 0, 0, 0); \
-  }
-        float a0 = AM_GA(0), a1 = AM_GA(1), a2 = AM_GA(2), a3 = AM_GA(3);
-        for (int kb = 0; kb < nkl; kb += 8) {
-          const float b0 = AM_GA(kb + 4), b1 = AM_GA(kb + 5), b2 = AM_GA(kb + 6), b3 = AM_GA(kb + 7);
-          __builtin_amdgcn_sched_barrier(0);
-          AM_G4(a, kb)
-          if (kb + 4 >= nkl) break;
-          a0 = AM_GA(kb + 8); a1 = AM_GA(kb + 9); a2 = AM_GA(kb + 10); a3 = AM_GA(kb + 11);
-          __builtin_amdgcn_sched_barrier(0);
-          AM_G4(b, kb + 4)
+      accs[u][0] = f32x4{0.f, 0.f, 0.f, 0.f};
+      accs[u][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+    const int nu = (nrb - wave + 3) / 4;   // row blocks of this wave (uniform; <= 0: none)
+    if (nu <= 0) {
+    } else if (PREC == 0) {
+      // A[row][kb*4 + (lane >> 4)] = W^T[list entry][row] (k-major, rows padded to
+      // 16), through a buffer resource (32-bit byte offsets); row block u's
+      // column is 64 u floats past row block 0's
+      const int nkl = s_nk[li] >> 2;   // a multiple of 4
+      const uint16_t* kl = klist + (size_t)(li - 1) * KLF + (lane >> 4);
+      const int ld4 = nrb * 16 * 4;
+      const __amdgpu_buffer_rsrc_t wres = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)f.Wp[li], (short)0, (f.nodes[li] + 1) * ld4, 0x00020000);
+      const int cw = (wave * 16 + (lane & 15)) * 4;
+      const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
+      for (int kb = 0; kb < nkl; kb += 4) {
+        int ko[4];
+        float bx[4][2];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          ko[e] = (int)kl[(kb + e) * 4] * ld4 + cw;
+          bx[e][0] = xb[(kb + e) * 4 * AM_XS];
+          bx[e][1] = xb[(kb + e) * 4 * AM_XS + 16];
         }
-#undef AM_G4
-#undef AM_GA
-      } else {
-        const uint4* Ah = reinterpret_cast<const uint4*>(f.Wh[li]) + (size_t)rb * nkb * 64 + lane;
-        const uint4* Al = reinterpret_cast<const uint4*>(f.Wl[li]) + (size_t)rb * nkb * 64 + lane;
-        const int kq = 8 * (lane >> 4);
-        const uint16_t* xh0 = Xh + (lane & 15) * KS + kq;
-        const uint16_t* xl0 = Xl + (lane & 15) * KS + kq;
-        const uint16_t* xh1 = xh0 + 16 * KS;
-        const uint16_t* xl1 = xl0 + 16 * KS;
-        for (int kb = 0; kb < nkb; ++kb) {
-          const uint4 ah = Ah[kb * 64], al = Al[kb * 64];
-          const uint4 bh0 = *reinterpret_cast<const uint4*>(xh0 + kb * 32);
-          const uint4 bl0 = *reinterpret_cast<const uint4*>(xl0 + kb * 32);
-          const uint4 bh1 = *reinterpret_cast<const uint4*>(xh1 + kb * 32);
-          const uint4 bl1 = *reinterpret_cast<const uint4*>(xl1 + kb * 32);
-          const bf16x8 vah = __builtin_bit_cast(bf16x8, ah), val_ = __builtin_bit_cast(bf16x8, al);
-          const bf16x8 vbh0 = __builtin_bit_cast(bf16x8, bh0), vbl0 = __builtin_bit_cast(bf16x8, bl0);
-          const bf16x8 vbh1 = __builtin_bit_cast(bf16x8, bh1), vbl1 = __builtin_bit_cast(bf16x8, bl1);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbh0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbh1, acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbl0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbl1, acc1, 0, 0, 0);
-          acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh0, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh1, acc1, 0, 0, 0);
+#pragma unroll
+        for (int u = 0; u < AM_GRBW; ++u) {
+          if (u < nu) {
+            float av[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              av[e] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(wres, ko[e] + 256 * u, 0, 0));
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              accs[u][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bx[e][0], accs[u][0], 0, 0, 0);
+              accs[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bx[e][1], accs[u][1], 0, 0, 0);
+            }
+          }
         }
       }
-      accs[u][0] = acc0;
-      accs[u][1] = acc1;
+    } else {
+      const int kq = 8 * (lane >> 4);
+      const uint16_t* xh0 = Xh + (lane & 15) * KS + kq;
+      const uint16_t* xl0 = Xl + (lane & 15) * KS + kq;
+      const uint16_t* xh1 = xh0 + 16 * KS;
+      const uint16_t* xl1 = xl0 + 16 * KS;
+      const uint4* Ah = reinterpret_cast<const uint4*>(f.Wh[li]) + lane;
+      const uint4* Al = reinterpret_cast<const uint4*>(f.Wl[li]) + lane;
+      for (int kb = 0; kb < nkb; ++kb) {
+        const bf16x8 vbh0 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xh0 + kb * 32));
+        const bf16x8 vbl0 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xl0 + kb * 32));
+        const bf16x8 vbh1 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xh1 + kb * 32));
+        const bf16x8 vbl1 = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xl1 + kb * 32));
+#pragma unroll
+        for (int u = 0; u < AM_GRBW; ++u) {
+          if (u < nu) {
+            const size_t at = ((size_t)(wave + 4 * u) * nkb + kb) * 64;
+            const bf16x8 vah = __builtin_bit_cast(bf16x8, Ah[at]), val_ = __builtin_bit_cast(bf16x8, Al[at]);
+            f32x4 acc0 = accs[u][0], acc1 = accs[u][1];
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbh0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbh1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbl0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(vah, vbl1, acc1, 0, 0, 0);
+            acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh0, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(val_, vbh1, acc1, 0, 0, 0);
+            accs[u][0] = acc0;
+            accs[u][1] = acc1;
+          }
+        }
+      }
     }
     // the outputs go over X, so every wave must be done reading it
     __syncthreads();
