@@ -163,8 +163,35 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   for (int li = 1; li <= net.D; ++li) {
     const int nin = net.nodes[li], nout = net.nodes[li + 1], noutp = io.noutp[li];
     const int row0 = wave * AM_CR;
-    const bool active = row0 < nout;   // wave-uniform
+    // A layer narrower than one row block (the output layer): wave j < nout
+    // computes row j alone, the same fma sequence with the weight as a scalar
+    // operand.  Lane l first loads the weights of list positions l, l + 64, ...
+    const bool narrow = nout < AM_CR;
+    const bool active = !narrow && row0 < nout;   // wave-uniform
     double acc[AM_CR];
+    double acc_n = 0.0;
+    if (narrow && wave < nout) {
+      const uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
+      const int nkp = s_nk[li];
+      const double* wr = io.Wp[li] + wave;
+      double wv[AM_CHAIN_MAXW / 64];
+#pragma unroll
+      for (int c = 0; c < AM_CHAIN_MAXW / 64; ++c) {
+        const int i = lane + 64 * c;
+        wv[c] = i < nkp ? wr[(size_t)kl[i] * noutp] : 0.0;
+      }
+      double a = q_l == 3 ? io.bp[li][wave] + 0.0 : 0.0;
+#pragma unroll
+      for (int c = 0; c < AM_CHAIN_MAXW / 64; ++c) {
+        const int lo = __double2loint(wv[c]), hi = __double2hiint(wv[c]);
+        const int n = min(64, nkp - 64 * c);
+        for (int t = 0; t < n; ++t) {
+          const double w = __hiloint2double(__builtin_amdgcn_readlane(hi, t), __builtin_amdgcn_readlane(lo, t));
+          a = __builtin_fma(w, Xs[(64 * c + t) * AM_CXS + lane], a);
+        }
+      }
+      acc_n = a;
+    }
     if (active) {
       const double* bias = io.bp[li] + row0;
 #pragma unroll
@@ -226,6 +253,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
         if (row < nout) Xs[row * AM_CXS + lane] = acc[j];
       }
     }
+    if (narrow && wave < nout) Xs[wave * AM_CXS + lane] = acc_n;
     __syncthreads();
     if (li < net.D) {
       // signed planes to HBM, consecutive threads on consecutive constraints
