@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   const size_t xbytes = PREC == 0 ? (size_t)f.Kpad * AM_XS * 4
                                   : std::max((size_t)f.Kpad * AM_XS * 4, (size_t)2 * AM_GN * KS * 2);
   uint32_t* mk = reinterpret_cast<uint32_t*>(smem_raw + xbytes);
-  const int KLF = f.Kpad + 32;                                       // entries of one layer's input list
+  const int KLF = f.Kpad;                                            // entries of one layer's input list
   uint16_t* klist = reinterpret_cast<uint16_t*>(mk + AM_GT * f.Wd);  // fp32: [D][KLF]
   __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_GT;
@@ -119,9 +119,8 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   if (PREC == 0) {
     // Inputs of linear layer li that are active in some region of the tile,
     // ascending, padded to a multiple of 16 with n_li (the +0 row of W^T and,
-    // through the staging below, of X), plus 32 such entries that only the
-    // operand prefetch reads.  Inactive inputs are +0 in every
-    // column, so the GEMM over the list equals the full one.
+    // through the staging below, of X; at most Kpad entries).  Inactive inputs
+    // are +0 in every column, so the GEMM over the list equals the full one.
     for (int li = wave + 1; li <= f.D; li += 4) {
       const int nin = f.nodes[li], in0 = li == 1 ? 0 : f.off[li - 1];
       uint16_t* kl = klist + (size_t)(li - 1) * KLF;
@@ -136,7 +135,7 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
         base += __popcll(bal);
       }
       const int nkp = (base + 15) & ~15;
-      for (int i = base + lane; i < nkp + 32; i += 64) kl[i] = (uint16_t)nin;
+      for (int i = base + lane; i < nkp; i += 64) kl[i] = (uint16_t)nin;
       if (lane == 0) s_nk[li] = nkp;
     }
     __syncthreads();
@@ -419,18 +418,20 @@ namespace {
   } while (0)
 
 enum Slot {
-  S_W0 = 0, S_EW, S_EB, S_PLANES, S_EQ, S_FLAGS, S_IDS, S_CNT,
+  S_W0 = 0, S_EW, S_EB, S_PLANES, S_EQ,
   S_BIAS0 = 8,      // 8..23: biases per linear layer
   S_WP0 = 24,       // 24..39: operand buffers per linear layer (fp32, or hi+lo packed)
 };
 
+// Buffer s with at least `bytes`; grows geometrically (levels grow run by run).
 void* slot(FastBuffers& b, int s, size_t bytes) {
   if (b.bytes[s] < bytes) {
+    const size_t cap = std::max<size_t>({bytes, 2 * b.bytes[s], 256});
     if (b.mem[s]) (void)hipFree(b.mem[s]);
     b.mem[s] = nullptr;
     b.bytes[s] = 0;
-    FCHK(hipMalloc(&b.mem[s], std::max<size_t>(bytes, 256)));
-    b.bytes[s] = std::max<size_t>(bytes, 256);
+    FCHK(hipMalloc(&b.mem[s], cap));
+    b.bytes[s] = cap;
   }
   return b.mem[s];
 }
@@ -468,7 +469,7 @@ size_t gemm_lds(const FastNetDev& d, int prec) {
   // one buffer for a layer's input and its outputs (k_gemm), the masks, the fp32 input lists
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
   const size_t xs = prec == 0 ? ys : std::max(ys, (size_t)2 * AM_GN * (d.Kpad + 8) * 2);
-  const size_t kls = prec == 0 ? (size_t)d.D * (d.Kpad + 32) * 2 : 0;
+  const size_t kls = prec == 0 ? (size_t)d.D * d.Kpad * 2 : 0;
   return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
 }
 
