@@ -598,12 +598,45 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
 }
 
 // Minimum of (t, idx) over the wave, ties to the smaller index.
+// One DPP step of wave_argmin: the pair from the lane DPP control CTRL names
+// (own pair where that lane is outside the row or row mask), kept if it is
+// lexicographically smaller.
+template <typename T, int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ void dpp_argmin_step(T& t, int& idx) {
+  T t2;
+  if constexpr (sizeof(T) == 8) {
+    const int lo = __double2loint(t), hi = __double2hiint(t);
+    t2 = __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWS, BANKS, false),
+                          __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWS, BANKS, false));
+  } else {
+    const int b = __float_as_int(t);
+    t2 = __int_as_float(__builtin_amdgcn_update_dpp(b, b, CTRL, ROWS, BANKS, false));
+  }
+  const int i2 = __builtin_amdgcn_update_dpp(idx, idx, CTRL, ROWS, BANKS, false);
+  if (t2 < t || (t2 == t && i2 < idx)) { t = t2; idx = i2; }
+}
+
 template <typename T>
 __device__ __forceinline__ void wave_argmin(T& t, int& idx) {
-  for (int off = 32; off > 0; off >>= 1) {
-    const T t2 = __shfl_xor(t, off);
-    const int i2 = __shfl_xor(idx, off);
-    if (t2 < t || (t2 == t && i2 < idx)) { t = t2; idx = i2; }
+  // Lexicographic min of (t, idx) over the wave through DPP moves (no LDS
+  // round trips): lanes gather rows of 16 (row_shr 1, 2, 4, 8), then rows
+  // (row_bcast 15, 31), so lane 63 ends with the minimum; it is then read
+  // into every lane.  A lane whose DPP source is out of its row keeps its own
+  // pair (bound_ctrl off, `old` = its own value).
+  static_assert(sizeof(T) == 8 || sizeof(T) == 4, "float or double");
+  dpp_argmin_step<T, 0x111, 0xf, 0xf>(t, idx);   // row_shr:1
+  dpp_argmin_step<T, 0x112, 0xf, 0xf>(t, idx);   // row_shr:2
+  dpp_argmin_step<T, 0x114, 0xf, 0xf>(t, idx);   // row_shr:4
+  dpp_argmin_step<T, 0x118, 0xf, 0xf>(t, idx);   // row_shr:8
+  dpp_argmin_step<T, 0x142, 0xa, 0xf>(t, idx);   // row_bcast:15 into rows 1 and 3
+  dpp_argmin_step<T, 0x143, 0xc, 0xf>(t, idx);   // row_bcast:31 into rows 2 and 3
+  idx = __builtin_amdgcn_readlane(idx, 63);
+  if constexpr (sizeof(T) == 8) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(t), 63);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(t), 63);
+    t = __hiloint2double(hi, lo);
+  } else {
+    t = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(t), 63));
   }
 }
 
