@@ -672,21 +672,6 @@ __device__ __forceinline__ void block_argmin(T& t, int& g) {
     if (red_t[w] < t || (red_t[w] == t && red_g[w] < g)) { t = red_t[w]; g = red_g[w]; }
 }
 
-// OR of per-thread bits (bit 0: violated, bit 1: undecided) over the block.
-template <int NW>
-__device__ __forceinline__ int block_or_bits(int bits) {
-  int b = (__any(bits & 1) ? 1 : 0) | (__any(bits & 2) ? 2 : 0);
-  if (NW == 1) return b;
-  __shared__ int red_b[NW];
-  const int wave = threadIdx.x >> 6;
-  __syncthreads();
-  if ((threadIdx.x & 63) == 0) red_b[wave] = b;
-  __syncthreads();
-  b = 0;
-  for (int w = 0; w < NW; ++w) b |= red_b[w];
-  return b;
-}
-
 // Constraint planes of one region as the ray walk reads them: constraints
 // g >= g0 from LDS (arrays indexed g - g0); constraints g < g0, which are
 // hidden layer 1's, rebuilt from its weights (row g of (W0 | b0), negated where
@@ -744,25 +729,29 @@ __device__ __forceinline__ void shoot(const PlaneView<T>& P, int C, const T* x, 
   gmin = gb;
 }
 
-// argmin (t, g) and OR of verdict bits over the block, with one LDS exchange.
+// argmin (t, g) and OR of verdict bits over the block, with one LDS exchange
+// and one barrier.  The exchange slots alternate between calls (`phase`, the
+// same count on every thread): a slot is written again two calls later, after
+// every wave has passed the barrier of the call in between, so its last
+// readers are done.
 template <typename T, int NW>
-__device__ __forceinline__ void block_argmin_or(T& t, int& g, int& bits) {
+__device__ __forceinline__ void block_argmin_or(T& t, int& g, int& bits, int& phase) {
   wave_argmin(t, g);
   int b = (__any(bits & 1) ? 1 : 0) | (__any(bits & 2) ? 2 : 0);
   if (NW == 1) { bits = b; return; }
-  __shared__ T red_t[NW];
-  __shared__ int red_g[NW];
-  __shared__ int red_b[NW];
-  const int wave = threadIdx.x >> 6;
+  __shared__ T red_t[2][NW];
+  __shared__ int red_g[2][NW];
+  __shared__ int red_b[2][NW];
+  const int wave = threadIdx.x >> 6, sl = phase & 1;
+  ++phase;
+  if ((threadIdx.x & 63) == 0) { red_t[sl][wave] = t; red_g[sl][wave] = g; red_b[sl][wave] = b; }
   __syncthreads();
-  if ((threadIdx.x & 63) == 0) { red_t[wave] = t; red_g[wave] = g; red_b[wave] = b; }
-  __syncthreads();
-  t = red_t[0];
-  g = red_g[0];
-  b = red_b[0];
+  t = red_t[sl][0];
+  g = red_g[sl][0];
+  b = red_b[sl][0];
   for (int w = 1; w < NW; ++w) {
-    if (red_t[w] < t || (red_t[w] == t && red_g[w] < g)) { t = red_t[w]; g = red_g[w]; }
-    b |= red_b[w];
+    if (red_t[sl][w] < t || (red_t[sl][w] == t && red_g[sl][w] < g)) { t = red_t[sl][w]; g = red_g[sl][w]; }
+    b |= red_b[sl][w];
   }
   bits = b;
 }
@@ -773,7 +762,8 @@ __device__ __forceinline__ void block_argmin_or(T& t, int& g, int& bits) {
 // s1/s2 (the next step's proposal; meaningful only when the bits are 0).
 template <typename T, int NW>
 __device__ __forceinline__ int check_and_shoot(const PlaneView<T>& P, int C, const T* v, const VertexBound<T>& vb,
-                                               int c1, int c2, const T* d, int s1, int s2, T& tmin, int& gmin) {
+                                               int c1, int c2, const T* d, int s1, int s2, T& tmin, int& gmin,
+                                               int& phase) {
   int bits = 0;
   T nb = T(1), db = T(0);
   int gb = 0x7fffffff;
@@ -795,7 +785,7 @@ __device__ __forceinline__ int check_and_shoot(const PlaneView<T>& P, int C, con
     }
   }
   T tb = db > T(0) ? nb / db : T(__builtin_inff());
-  block_argmin_or<T, NW>(tb, gb, bits);
+  block_argmin_or<T, NW>(tb, gb, bits, phase);
   tmin = tb;
   gmin = gb;
   return bits;
@@ -805,7 +795,7 @@ __device__ __forceinline__ int check_and_shoot(const PlaneView<T>& P, int C, con
 // -d), skipping ex1, in one pass: (tp, gp) along +d, (tm, gm) along -d.
 template <typename T, int NW>
 __device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T* x, const T* d, int ex1, T& tp,
-                                           int& gp, T& tm, int& gm) {
+                                           int& gp, T& tm, int& gm, int& phase) {
   T np = T(1), dp = T(0), nm = T(1), dmn = T(0);
   int bp = 0x7fffffff, bm = 0x7fffffff;
 #pragma unroll 2
@@ -827,8 +817,8 @@ __device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T
   T t1 = dp > T(0) ? np / dp : T(__builtin_inff());
   T t2 = dmn > T(0) ? nm / dmn : T(__builtin_inff());
   int z = 0;
-  block_argmin_or<T, NW>(t1, bp, z);
-  block_argmin_or<T, NW>(t2, bm, z);
+  block_argmin_or<T, NW>(t1, bp, z, phase);
+  block_argmin_or<T, NW>(t2, bm, z, phase);
   tp = t1; gp = bp; tm = t2; gm = bm;
 }
 
@@ -861,6 +851,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   unsigned why = 0u;
   int start = start_in;
   T tdummy;
+  int rph = 0;   // block_argmin_or's exchange slot count
 
   if (start == -1) {
     // seed: project p onto the f = iso plane and shoot inside it
@@ -888,7 +879,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
     solve3(ps, eq, pd, x0);
     T th, tl;
     int rh, rl;
-    shoot_both<T, NW>(P, C, x0, d, start, th, rh, tl, rl);
+    shoot_both<T, NW>(P, C, x0, d, start, th, rh, tl, rl, rph);
     if (rh == NONE || rl == NONE || !(th + tl > T(0)) || !(x0[0] == x0[0])) {
       undecided = true;
       why |= AM_WHY_BIT(AM_WHY_START);
@@ -973,7 +964,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         vd[0] = s_vd[0]; vd[1] = s_vd[1]; vd[2] = s_vd[2];
       }
       int nx;
-      const int bits = check_and_shoot<T, NW>(P, C, v, vb, cur, r, dn, r, cur, tdummy, nx);
+      const int bits = check_and_shoot<T, NW>(P, C, v, vb, cur, r, dn, r, cur, tdummy, nx, rph);
       if (bits != 0 || !vb.ok) {
         undecided = true;
         if (bits & 1) why |= AM_WHY_BIT(AM_WHY_VIOL);
