@@ -137,6 +137,10 @@ struct am_ctx {
   std::deque<DevBuf<char>> stage;
   size_t stage_used = 0;
 
+  // pinned, coherent host word the device writes scan totals into (scan_total)
+  int* host_total = nullptr;
+  int* host_total_dev = nullptr;
+
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
   DevBuf<double> tplanes, teq;
@@ -185,14 +189,22 @@ void exclusive_scan(am_ctx* c, const int* in, int* out, size_t n) {
   HIPCHK(rocprim::exclusive_scan(c->scan_tmp.p, bytes, in, out, 0, n, rocprim::plus<int>(), c->stream));
 }
 
+// total = rank[n - 1] + flags[n - 1], written straight into host memory
+__global__ void k_scan_total(const int* rank, const int* flags, size_t n, int* out) {
+  if (threadIdx.x == 0) *out = rank[n - 1] + flags[n - 1];
+}
+
 int scan_total(am_ctx* c, const int* flags, int* rank, size_t n) {
   if (n == 0) return 0;
   exclusive_scan(c, flags, rank, n);
-  int a = 0, b = 0;
-  HIPCHK(hipMemcpyAsync(&a, rank + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipMemcpyAsync(&b, flags + n - 1, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  if (!c->host_total) {
+    HIPCHK(hipHostMalloc((void**)&c->host_total, 256, hipHostMallocMapped | hipHostMallocCoherent));
+    HIPCHK(hipHostGetDevicePointer((void**)&c->host_total_dev, c->host_total, 0));
+  }
+  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->stream, rank, flags, n, c->host_total_dev);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  return a + b;
+  return *reinterpret_cast<volatile int*>(c->host_total);
 }
 
 void ensure_table(am_ctx* c, size_t entries) {
@@ -1189,6 +1201,7 @@ am_status am_destroy(am_ctx* c) {
   c->timer.release();
   fast_release(c->fbuf);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+  if (c->host_total) (void)hipHostFree(c->host_total);   // every scan_total waited for its write
   hipStream_t own = c->own_stream;
   delete c;
   if (own) (void)hipStreamDestroy(own);

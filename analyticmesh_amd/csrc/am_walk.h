@@ -97,8 +97,11 @@ __device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T
   const T rho = eps * (T(1) + h);
   b.ok = h <= h_max && rho <= T(0.25);
   const T vn = T(1);
-  // 1.001: slack for the rounding of these bound computations themselves
-  const T kappa = T(1.001) / (T(1) - T(2) * rho);
+  // 1.001: slack for the rounding of these bound computations themselves.
+  // kappa >= 1.001 / (1 - x) with x = 2 rho <= 1/2 (systems with larger rho
+  // are not trusted): 1 / (1 - x) = 1 + x + x^2 / (1 - x) <= 1 + x + 2 x^2
+  const T x2 = T(2) * rho;
+  const T kappa = T(1.001) * (T(1) + x2 + T(2) * x2 * x2);
   // |n_i|_1 = |c_i|_1 / |det|
   const T l1 = absT(c1[0]) + absT(c1[1]) + absT(c1[2]);
   const T l2 = absT(c2[0]) + absT(c2[1]) + absT(c2[2]);
@@ -107,7 +110,7 @@ __device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T
   const T w = kappa * eps * (m1 * l1 + m2 * l2 + m3 * l3) * iadet;
   b.E = T(1.001) * (eps * vn + w);
   if (sizeof(T) == 4) {
-    const T inv = T(1) / det;
+    const T inv = det < T(0) ? -iadet : iadet;   // 1 / det
     for (int i = 0; i < 3; ++i) { b.n1[i] = c1[i] * inv; b.n2[i] = c2[i] * inv; b.n3[i] = c3[i] * inv; }
     b.w1 = kappa * eps * m1;
     b.w2 = kappa * eps * m2;
