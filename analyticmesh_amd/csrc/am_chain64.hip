@@ -161,7 +161,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   __syncthreads();
   const int q_l = lane & 3;           // this lane's column is x, y, z or c of region lane >> 2
   for (int li = 1; li <= net.D; ++li) {
-    const int nin = net.nodes[li], nout = net.nodes[li + 1], noutp = io.noutp[li];
+    const int nout = net.nodes[li + 1], noutp = io.noutp[li];
     const int row0 = wave * AM_CR;
     // A layer narrower than one row block (the output layer): wave j < nout
     // computes row j alone, the same fma sequence with the weight as a scalar
@@ -368,7 +368,7 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net,
   double* Pc = Pz + Cl;
   uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cl);
   const int lane = threadIdx.x;
-  if (blockIdx.x >= io.count) return;
+  if ((int)blockIdx.x >= io.count) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
   // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
   if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;

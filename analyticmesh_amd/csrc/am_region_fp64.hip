@@ -22,7 +22,7 @@ namespace am {
 
 __global__ __launch_bounds__(64) void k_region_fp64(NetDev net, RegionIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int C = net.C, L = net.L, vm = net.vm;
+  const int C = net.C, L = net.L;
   const int Cs = io.Cs;        // stride of one SoA plane array (>= C)
   const int N2 = io.Csort;     // power of two >= max(C, 64)
   double* Px = reinterpret_cast<double*>(smem_raw);
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(64) void k_region_fp64(NetDev net, RegionIO io) {
   __shared__ double s_eq[4];
 
   const int lane = threadIdx.x;
-  if (blockIdx.x >= io.count) return;
+  if ((int)blockIdx.x >= io.count) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : io.first + blockIdx.x;
 
   for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];

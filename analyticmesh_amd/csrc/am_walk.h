@@ -399,7 +399,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
   __shared__ T s_verts[AM_VM_MAX * 3];
   __shared__ int s_vpair[AM_VM_MAX * 2];
 
-  const int C = net.C, L = net.L, vm = net.vm, N2 = io.Csort;
+  const int C = net.C, vm = net.vm, N2 = io.Csort;
   const int lane = threadIdx.x;
   const T eq[4] = {eq_in[0], eq_in[1], eq_in[2], eq_in[3]};
   const int start_in = io.reg_start[rid];
