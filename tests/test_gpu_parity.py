@@ -113,3 +113,29 @@ def test_device_inputs_match_host_inputs(prec, dtype):
                     faces.indices.copy()))
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64ray", "fp32"])
+def test_iso_flip_vm_vs_oracle(prec, oracle_mod):
+    """A non-zero level (iso), inside-out orientation (flip) and a raised polygon cap (vm): every
+    form gives the oracle's topology; the FP64 form its records bit for bit."""
+    from analyticmesh_amd import configs
+    from analyticmesh_amd.engine import march
+    spec = configs.sphere(0)
+    iso, vm = 0.05, 32
+    pts = configs.seeds_fibonacci(spec, 8, iso=iso)
+    st = configs.mask_of(spec, pts)
+    _say(f"[iso/flip/vm] {prec}")
+    g = march(spec, pts, st, precision=prec, vm=vm, iso=iso, flip=True)
+    traversal = "bfs" if prec == "fp64" else "lifo"
+    o = oracle_mod.run_oracle(spec, pts, st, traversal=traversal, vm=vm, iso=iso, flip=True)
+    assert len(g.masks) > 100
+    if prec == "fp64":
+        np.testing.assert_array_equal(g.masks, o.masks)
+        np.testing.assert_array_equal(g.info[:, :6], o.info[:, :6])
+        np.testing.assert_array_equal(g.tabs, o.tabs)
+        assert np.array_equal(g.verts.view(np.uint64), o.verts.view(np.uint64))
+    else:
+        a, b = g.records(), o.records()
+        assert_same_topology(a, b)
+        assert max_vertex_error(a, b) < 1e-10
