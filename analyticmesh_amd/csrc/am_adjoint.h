@@ -20,7 +20,11 @@ struct AdjointIO {
 };
 
 __global__ void k_adjoint_fp64(NetDev net, AdjointIO io);
-__global__ void k_dw_gemm(const double* G, const double* P, int K, int M, int N, double* dW, int kchunk);
-__global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out, int bchunk);
+// planes / eq: the tiled chain's store of regions r0 .. r0 + count - 1
+__global__ void k_adjoint_direct(NetDev net, AdjointIO io, const double* planes, const double* eq, int Cs);
+template <bool TA, int MODE>
+__global__ void k_adj_gemm(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
+                           int kchunk, const uint32_t* mask, int Wd, int n0);
+__global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out);
 
 }  // namespace am

@@ -140,40 +140,214 @@ __global__ __launch_bounds__(64) void k_adjoint_fp64(NetDev net, AdjointIO io) {
   }
 }
 
-// dW[j][k] += sum over kk < K of G[kk][j] * P[kk][k]  (G: [K][M], P: [K][N]),
-// 16 x 16 output tile per workgroup, K split over blockIdx.z, FP64.
-__global__ __launch_bounds__(256) void k_dw_gemm(const double* G, const double* P, int K, int M, int N,
-                                                 double* dW, int kchunk) {
-  __shared__ double sg[32][16];
-  __shared__ double sp[32][17];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
-  const int j = blockIdx.y * 16 + ty, k = blockIdx.x * 16 + tx;
-  const int k0 = blockIdx.z * kchunk, k1 = min(K, k0 + kchunk);
-  double acc = 0.0;
-  for (int kb = k0; kb < k1; kb += 32) {
-    for (int i = threadIdx.x; i < 32 * 16; i += 256) {
-      const int rr = i >> 4, cc = i & 15;
-      const int kk = kb + rr;
-      const int jj = blockIdx.y * 16 + cc, kk2 = blockIdx.x * 16 + cc;
-      sg[rr][cc] = (kk < k1 && jj < M) ? G[(size_t)kk * M + jj] : 0.0;
-      sp[rr][cc] = (kk < k1 && kk2 < N) ? P[(size_t)kk * N + kk2] : 0.0;
+// The raw (unsigned) plane of constraint g from the tiled chain's store of
+// one region: hidden layer 1 is (W0 | b0); deeper layers are stored negated
+// where the neuron is active; extra planes are (ew | eb).
+__device__ __forceinline__ void raw_plane64(const NetDev& net, const uint32_t* mask, const double* src, int Cs, int g,
+                                            double* p) {
+  if (g < net.nodes[1]) {
+    p[0] = net.W0[g * 3 + 0]; p[1] = net.W0[g * 3 + 1]; p[2] = net.W0[g * 3 + 2]; p[3] = net.b[0][g];
+  } else if (g < net.L) {
+    const bool on = mbit(mask, g);
+    for (int q = 0; q < 4; ++q) {
+      const double v = src[(size_t)q * Cs + g];
+      p[q] = on ? -v : v;
     }
-    __syncthreads();
-#pragma unroll 8
-    for (int rr = 0; rr < 32; ++rr) acc = fmaT(sg[rr][ty], sp[rr][tx], acc);
-    __syncthreads();
+  } else {
+    const int e = g - net.L;
+    p[0] = net.ew[e * 3 + 0]; p[1] = net.ew[e * 3 + 1]; p[2] = net.ew[e * 3 + 2]; p[3] = net.eb[e];
   }
-  if (j < M && k < N) atomicAdd(&dW[(size_t)j * N + k], acc);
 }
 
-// out[j] += sum over b of G[b][q][j] for q = qsel (bias / first-layer columns).
-__global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out, int bchunk) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= M) return;
-  const int b0 = blockIdx.y * bchunk, b1 = min(count, b0 + bchunk);
+// The direct terms of one region (one wave): lambda for each vertex the
+// region owns, one vertex per lane, the plane gradients -lambda_i [v; 1]
+// summed in vertex order by lane 0 (the order of k_adjoint_fp64), then
+// G of every layer and the masked inputs P written out.  The pull-back
+// through the weights is left to k_adj_gemm over all regions.
+__global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io, const double* planes,
+                                                       const double* eqb, int Cs) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int L = net.L;
+  double* G = reinterpret_cast<double*>(smem_raw);   // [4][L]
+  __shared__ double s_c[AM_VM_MAX][12];
+  __shared__ int s_lohi[AM_VM_MAX][2];
+  __shared__ int s_own[AM_VM_MAX];
+  __shared__ double g_eq[4];
+  const int lane = threadIdx.x;
+  const int b = blockIdx.x;
+  if (b >= io.count) return;
+  const int r = io.r0 + b;
+  const int vm = io.vm;
+  const int nv = io.info[(size_t)r * AM_INFO_STRIDE + 2];
+  const uint32_t* mask = io.reg_mask + (size_t)r * net.Wd;
+  const double* src = planes + (size_t)b * 4 * Cs;
+  for (int g = lane; g < 4 * L; g += 64) G[g] = 0.0;
+  const bool mine = lane < vm && lane < nv && nv >= 3 && io.vfirst[(size_t)r * vm + lane];
+  if (lane < vm) s_own[lane] = mine ? 1 : 0;
+  const bool active = __any(mine);
+  if (mine) {
+    const size_t q = (size_t)r * vm + lane;
+    const int lo = io.vpair[q * 2 + 0], hi = io.vpair[q * 2 + 1];
+    const int id = io.vid[q];
+    const double g3[3] = {io.gradv[(size_t)id * 3 + 0], io.gradv[(size_t)id * 3 + 1], io.gradv[(size_t)id * 3 + 2]};
+    double p1[4], p2[4];
+    raw_plane64(net, mask, src, Cs, lo, p1);
+    raw_plane64(net, mask, src, Cs, hi, p2);
+    const double eq[4] = {eqb[(size_t)b * 4 + 0], eqb[(size_t)b * 4 + 1], eqb[(size_t)b * 4 + 2],
+                          eqb[(size_t)b * 4 + 3]};
+    double v[3];
+    solve3(p1, p2, eq, v);
+    const double t1[4] = {p1[0], p2[0], eq[0], -g3[0]};
+    const double t2[4] = {p1[1], p2[1], eq[1], -g3[1]};
+    const double t3[4] = {p1[2], p2[2], eq[2], -g3[2]};
+    double lam[3];
+    solve3(t1, t2, t3, lam);
+    for (int i = 0; i < 3; ++i) {
+      s_c[lane][i * 4 + 0] = lam[i] * v[0];
+      s_c[lane][i * 4 + 1] = lam[i] * v[1];
+      s_c[lane][i * 4 + 2] = lam[i] * v[2];
+      s_c[lane][i * 4 + 3] = lam[i];
+    }
+    s_lohi[lane][0] = lo;
+    s_lohi[lane][1] = hi;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double ge[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < nv && k < vm; ++k) {
+      if (!s_own[k]) continue;
+      const int lo = s_lohi[k][0], hi = s_lohi[k][1];
+      if (lo < L)
+        for (int c = 0; c < 4; ++c) G[c * L + lo] -= s_c[k][c];
+      if (hi < L)
+        for (int c = 0; c < 4; ++c) G[c * L + hi] -= s_c[k][4 + c];
+      for (int c = 0; c < 4; ++c) ge[c] -= s_c[k][8 + c];
+    }
+    for (int c = 0; c < 4; ++c) g_eq[c] = ge[c];
+  }
+  __syncthreads();
+  // G of the output plane (one output) and of each hidden layer
+  if (lane < 4) io.Gst[net.D][(size_t)b * 4 + lane] = active ? g_eq[lane] : 0.0;
+  for (int li = 0; li < net.D; ++li) {
+    const int n = net.nodes[li + 1], o = net.off[li];
+    double* gs = io.Gst[li] + (size_t)b * 4 * n;
+    for (int j = lane; j < n; j += 64)
+      for (int c = 0; c < 4; ++c) gs[(size_t)c * n + j] = G[c * L + o + j];
+  }
+  // masked inputs of linear layers 1..D (zero for a region owning no vertex:
+  // its G rows are zero)
+  for (int li = 1; li <= net.D; ++li) {
+    const int n = net.nodes[li], in0 = net.off[li - 1];
+    double* ps = io.Pst[li] + (size_t)b * 4 * n;
+    for (int k = lane; k < n; k += 64) {
+      double p[4] = {0.0, 0.0, 0.0, 0.0};
+      if (active && mbit(mask, in0 + k)) raw_plane64(net, mask, src, Cs, in0 + k, p);
+      for (int c = 0; c < 4; ++c) ps[(size_t)c * n + k] = p[c];
+    }
+  }
+}
+
+// C (M x N) from A (M x K) and B (K x N), FP64: 64 x 64 outputs per block of
+// 256 threads, 4 x 4 per thread (rows 4 ty .. 4 ty + 3, columns 2 tx + {0, 1}
+// and 32 + 2 tx + {0, 1}), K in steps of 16 through LDS, summed in ascending k.
+//   TA: element (m, k) of A at A[k * lda + m], else at A[m * lda + k];
+//   B:  element (k, n) at B[k * ldb + n].
+//   MODE 0 (pull-back): rows are (region, component), region m / 4 of the
+//     tile's mask rows; C[m][n] += acc where that region's neuron n0 + n is
+//     active, else C is left as it is.  K whole (one block in z).
+//   MODE 1 (weight gradient): atomicAdd of acc into C[m][n]; K split over
+//     blockIdx.z in chunks of kchunk.
+template <bool TA, int MODE>
+__global__ __launch_bounds__(256) void k_adj_gemm(const double* __restrict__ A, int lda, const double* __restrict__ B,
+                                                  int ldb, double* C, int ldc, int M, int N, int K, int kchunk,
+                                                  const uint32_t* mask, int Wd, int n0) {
+  constexpr int SA = 66;   // row stride: column stores of A spread over the banks, rows stay 16-byte aligned
+  __shared__ __align__(16) double sa[16][SA];
+  __shared__ __align__(16) double sb[16][64];
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int m0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int k0 = blockIdx.z * kchunk, k1 = min(K, k0 + kchunk);
+  double acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+  // the next k block's operands are loaded into registers while this one is
+  // multiplied from LDS
+  double ra[4], rb[4];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = tid + 256 * e;
+      const int ka = TA ? (i >> 6) : (i & 15), ma = TA ? (i & 63) : (i >> 4);
+      const int gk = kb + ka, gm = m0 + ma;
+      ra[e] = (gk < k1 && gm < M) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
+      const int gk2 = kb + (i >> 6), gn = c0 + (i & 63);
+      rb[e] = (gk2 < k1 && gn < N) ? B[(size_t)gk2 * ldb + gn] : 0.0;
+    }
+  };
+  if (k0 < k1) fetch(k0);
+  for (int kb = k0; kb < k1; kb += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = tid + 256 * e;
+      const int ka = TA ? (i >> 6) : (i & 15), ma = TA ? (i & 63) : (i >> 4);
+      sa[ka][ma] = ra[e];
+      sb[i >> 6][i & 63] = rb[e];
+    }
+    __syncthreads();
+    if (kb + 16 < k1) fetch(kb + 16);
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      const double2 a01 = *reinterpret_cast<const double2*>(&sa[kk][4 * ty]);
+      const double2 a23 = *reinterpret_cast<const double2*>(&sa[kk][4 * ty + 2]);
+      const double2 b01 = *reinterpret_cast<const double2*>(&sb[kk][2 * tx]);
+      const double2 b23 = *reinterpret_cast<const double2*>(&sb[kk][32 + 2 * tx]);
+      const double a[4] = {a01.x, a01.y, a23.x, a23.y};
+      const double bv[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = fmaT(a[i], bv[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + 4 * ty + i;
+    if (m >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int n = c0 + (j >> 1) * 32 + 2 * tx + (j & 1);
+      if (n >= N) continue;
+      if (MODE == 0) {
+        if (mbit(mask + (size_t)(m >> 2) * Wd, n0 + n)) C[(size_t)m * ldc + n] += acc[i][j];
+      } else {
+        atomicAdd(&C[(size_t)m * ldc + n], acc[i][j]);
+      }
+    }
+  }
+}
+
+template __global__ void k_adj_gemm<false, 0>(const double*, int, const double*, int, double*, int, int, int, int, int,
+                                              const uint32_t*, int, int);
+template __global__ void k_adj_gemm<true, 1>(const double*, int, const double*, int, double*, int, int, int, int, int,
+                                             const uint32_t*, int, int);
+
+// out[j * stride_out] += sum over b < count of G[b][qsel][j]: 64 columns and
+// 256 regions per block, four partial sums per column added in LDS.
+__global__ __launch_bounds__(256) void k_col_sum(const double* G, int count, int M, int qsel, double* out,
+                                                 int stride_out) {
+  __shared__ double part[4][64];
+  const int jx = threadIdx.x & 63, by = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + jx;
+  const int b0 = blockIdx.y * 256, b1 = min(count, b0 + 256);
   double acc = 0.0;
-  for (int b = b0; b < b1; ++b) acc += G[((size_t)b * 4 + qsel) * M + j];
-  atomicAdd(&out[(size_t)j * stride_out], acc);
+  if (j < M)
+    for (int b = b0 + by; b < b1; b += 4) acc += G[((size_t)b * 4 + qsel) * M + j];
+  part[by][jx] = acc;
+  __syncthreads();
+  if (by == 0 && j < M) atomicAdd(&out[(size_t)j * stride_out], ((part[0][jx] + part[1][jx]) + part[2][jx]) + part[3][jx]);
 }
 
 }  // namespace am

@@ -92,6 +92,7 @@ struct am_ctx {
   int dev = 0;
   size_t max_lds = 65536;
   long long max_regions = -1;
+  bool adjoint_tiled = true;   // am_adjoint: tiled chain + GEMM pull-back where the tiled chain runs
   int table_log2 = 22;
 
   // network (device)
@@ -326,6 +327,25 @@ size_t ray64_lds_bytes(const am_ctx* c) {
 // through mode 1.
 // Nothing here waits for the device.  Falls back to the one-region kernel
 // where the tiled chain does not apply.
+// The tiled chain's arguments for regions first .. first + n - 1 (or ids),
+// planes into the context's tile buffers with component stride Ch.
+ChainIO chain_io(const am_ctx* c, const int* ids, int first, int n, int Ch) {
+  ChainIO cio;
+  cio.reg_mask = c->reg_mask.p;
+  cio.ids = ids;
+  cio.first = first;
+  cio.count = n;
+  cio.Cs = Ch;
+  cio.planes = c->tplanes.p;
+  cio.eq = c->teq.p;
+  for (int li = 0; li < AM_MAX_LINEAR; ++li) {
+    cio.Wp[li] = li >= 1 && li <= c->D ? c->WTp[li].p : nullptr;
+    cio.bp[li] = li >= 1 && li <= c->D ? c->bpad[li].p : nullptr;
+    cio.noutp[li] = li >= 1 && li <= c->D ? (c->nodes[li + 1] + AM_CR - 1) / AM_CR * AM_CR : 0;
+  }
+  return cio;
+}
+
 void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t,
                       long long undecided = -1) {
   if (!c->tile_ok) {
@@ -358,19 +378,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   StageTimer& tm = c->timer;
   for (int off = 0; off < count; off += (int)chunk) {
     const int n = std::min<int>((int)chunk, count - off);
-    ChainIO cio;
-    cio.reg_mask = c->reg_mask.p;
-    cio.ids = ids ? ids + off : nullptr;
-    cio.first = first + off;
-    cio.count = n;
-    cio.Cs = Ch;
-    cio.planes = c->tplanes.p;
-    cio.eq = c->teq.p;
-    for (int li = 0; li < AM_MAX_LINEAR; ++li) {
-      cio.Wp[li] = li >= 1 && li <= c->D ? c->WTp[li].p : nullptr;
-      cio.bp[li] = li >= 1 && li <= c->D ? c->bpad[li].p : nullptr;
-      cio.noutp[li] = li >= 1 && li <= c->D ? (c->nodes[li + 1] + AM_CR - 1) / AM_CR * AM_CR : 0;
-    }
+    const ChainIO cio = chain_io(c, ids ? ids + off : nullptr, first + off, n, Ch);
     RegionIO io;
     io.reg_mask = c->reg_mask.p;
     io.reg_start = c->reg_start.p;
@@ -614,6 +622,9 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
+  } else if (k == "adjoint_tiled") {
+    if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "adjoint_tiled must be 0 or 1");
+    c->adjoint_tiled = value != 0;
   } else {
     return fail(c, AM_ERR_ARG, "unknown option " + k);
   }
@@ -1073,16 +1084,32 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
       HIPCHK(hipMemsetAsync(db[li].p, 0, (size_t)c->nodes[li + 1] * 8, st));
     }
     const int R = (int)c->R;
-    const int CH = 16384;
+    // tiled: planes from the tiled chain, the pull-back as GEMMs over all
+    // regions of a chunk; else one kernel per region does chain and pull-back
+    const bool tiled = c->tile_ok && c->adjoint_tiled;
+    const int CH = 32768;
     const int chunk = std::min(R, CH);
     for (int li = 0; li < nl; ++li) {
       ensure(Gst[li], (size_t)chunk * 4 * c->nodes[li + 1], false, st);
       if (li >= 1) ensure(Pst[li], (size_t)chunk * 4 * c->nodes[li], false, st);
     }
-    const size_t lds = (size_t)8 * c->L * 8 + (size_t)c->Wd * 4 + 64;
-    if (lds > c->max_lds) throw CapErr{"network too wide for the adjoint kernel's LDS"};
-    if (lds > 65536)
-      HIPCHK(hipFuncSetAttribute((const void*)k_adjoint_fp64, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int Ch = (c->C + 63) & ~63;
+    size_t lds;
+    if (tiled) {
+      ensure(c->tplanes, (size_t)chunk * 4 * Ch, false, st);
+      ensure(c->teq, (size_t)chunk * 4, false, st);
+      lds = (size_t)4 * c->L * 8;
+      if (lds > c->max_lds) throw CapErr{"network too wide for the adjoint kernel's LDS"};
+      if (lds > 65536)
+        HIPCHK(hipFuncSetAttribute((const void*)k_adjoint_direct, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    } else {
+      lds = (size_t)8 * c->L * 8 + (size_t)c->Wd * 4 + 64;
+      if (lds > c->max_lds) throw CapErr{"network too wide for the adjoint kernel's LDS"};
+      if (lds > 65536)
+        HIPCHK(hipFuncSetAttribute((const void*)k_adjoint_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)lds));
+    }
     const auto t0 = std::chrono::steady_clock::now();
     for (int r0 = 0; r0 < R; r0 += CH) {
       const int cnt = std::min(CH, R - r0);
@@ -1100,22 +1127,41 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
         io.Gst[li] = Gst[li].p;
         io.Pst[li] = li >= 1 ? Pst[li].p : nullptr;
       }
-      hipLaunchKernelGGL(k_adjoint_fp64, dim3(cnt), dim3(64), lds, st, c->net, io);
-      HIPCHK(hipGetLastError());
-      const int K = cnt * 4, kchunk = 2048;
+      const uint32_t* mask0 = c->reg_mask.p + (size_t)r0 * c->Wd;
+      if (tiled) {
+        hipLaunchKernelGGL(k_chain_tile_fp64, dim3((cnt + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS),
+                           chain_lds_bytes(c), st, c->net, chain_io(c, nullptr, r0, cnt, Ch));
+        hipLaunchKernelGGL(k_adjoint_direct, dim3(cnt), dim3(64), lds, st, c->net, io, (const double*)c->tplanes.p,
+                           (const double*)c->teq.p, Ch);
+        HIPCHK(hipGetLastError());
+        // G_li += mask (G_li+1 W_li), li = D .. 1 (rows: region x component)
+        for (int li = nl - 1; li >= 1; --li) {
+          const int nout = c->nodes[li + 1], nin = c->nodes[li], M = cnt * 4;
+          hipLaunchKernelGGL((k_adj_gemm<false, 0>), dim3((nin + 63) / 64, (M + 63) / 64, 1), dim3(256), 0, st,
+                             (const double*)Gst[li].p, nout, c->net.Wrow[li], nin, Gst[li - 1].p, nin, M, nin, nout,
+                             nout, mask0, c->Wd, c->net.off[li - 1]);
+        }
+        HIPCHK(hipGetLastError());
+      } else {
+        hipLaunchKernelGGL(k_adjoint_fp64, dim3(cnt), dim3(64), lds, st, c->net, io);
+        HIPCHK(hipGetLastError());
+      }
+      // dW_li = sum over (region, component) of G_li+1^T P_li; db_li the c components
+      const int K = cnt * 4, kchunk = 512;
       for (int li = 1; li < nl; ++li) {
         const int M = c->nodes[li + 1], N = c->nodes[li];
-        hipLaunchKernelGGL(k_dw_gemm, dim3((N + 15) / 16, (M + 15) / 16, (K + kchunk - 1) / kchunk), dim3(256), 0,
-                           st, Gst[li].p, Pst[li].p, K, M, N, dW[li].p, kchunk);
-        hipLaunchKernelGGL(k_col_sum, dim3((M + 255) / 256, (cnt + 1023) / 1024), dim3(256), 0, st, Gst[li].p,
-                           cnt, M, 3, db[li].p, 1, 1024);
+        hipLaunchKernelGGL((k_adj_gemm<true, 1>), dim3((N + 63) / 64, (M + 63) / 64, (K + kchunk - 1) / kchunk),
+                           dim3(256), 0, st, (const double*)Gst[li].p, M, (const double*)Pst[li].p, N, dW[li].p, N, M,
+                           N, K, kchunk, (const uint32_t*)nullptr, 0, 0);
+        hipLaunchKernelGGL(k_col_sum, dim3((M + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
+                           (const double*)Gst[li].p, cnt, M, 3, db[li].p, 1);
       }
       const int n1 = c->nodes[1];
       for (int q = 0; q < 3; ++q)
-        hipLaunchKernelGGL(k_col_sum, dim3((n1 + 255) / 256, (cnt + 1023) / 1024), dim3(256), 0, st, Gst[0].p, cnt,
-                           n1, q, dW[0].p + q, 3, 1024);
-      hipLaunchKernelGGL(k_col_sum, dim3((n1 + 255) / 256, (cnt + 1023) / 1024), dim3(256), 0, st, Gst[0].p, cnt, n1,
-                         3, db[0].p, 1, 1024);
+        hipLaunchKernelGGL(k_col_sum, dim3((n1 + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
+                           (const double*)Gst[0].p, cnt, n1, q, dW[0].p + q, 3);
+      hipLaunchKernelGGL(k_col_sum, dim3((n1 + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
+                         (const double*)Gst[0].p, cnt, n1, 3, db[0].p, 1);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(st));

@@ -109,7 +109,9 @@ am_status am_planes(am_ctx* ctx, const uint32_t* masks, int n, float* planes, fl
 
 /* Tunables: "vm" (polygon vertex cap, 3..64, default 20 like the reference's
  * _VAR_VERT_MAX), "table_log2" (initial visited-table size), "max_regions"
- * (stop after this many regions, -1 = unlimited). */
+ * (stop after this many regions, -1 = unlimited), "walk" (0 ray walk, 1
+ * candidate scan, for the reduced forms), "adjoint_tiled" (1 default: the
+ * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region). */
 am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 
 /* Timings and counters of the last am_run/am_combine; see AM_STAT_* indices. */
