@@ -145,6 +145,9 @@ struct am_ctx {
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
   DevBuf<double> tplanes, teq;
+  // am_adjoint's buffers, kept from call to call: dL/dv, dW, db, G and P per layer
+  DevBuf<double> adj_gv;
+  std::vector<DevBuf<double>> adj_dW, adj_db, adj_G, adj_P;
   DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
   DevBuf<int> runacc;                     // per run: FP64 ray walk undecided count, then per reason
   StageTimer timer;
@@ -1072,10 +1075,14 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
     const hipStream_t st = c->stream;
     const size_t V = (size_t)c->mesh_V;
     const int nl = c->D + 1;
-    DevBuf<double> gv;
+    c->adj_dW.resize(nl);
+    c->adj_db.resize(nl);
+    c->adj_G.resize(nl);
+    c->adj_P.resize(nl);
+    DevBuf<double>& gv = c->adj_gv;
     c->stage_used = 0;
     upload(c, gv, grad_vertices, V * 3, dtype, st);
-    std::vector<DevBuf<double>> dW(nl), db(nl), Gst(nl), Pst(nl);
+    std::vector<DevBuf<double>>&dW = c->adj_dW, &db = c->adj_db, &Gst = c->adj_G, &Pst = c->adj_P;
     for (int li = 0; li < nl; ++li) {
       const size_t nw = (size_t)c->nodes[li + 1] * c->nodes[li];
       ensure(dW[li], nw, false, st);
