@@ -198,17 +198,40 @@ __global__ void k_scan_total(const int* rank, const int* flags, size_t n, int* o
   if (threadIdx.x == 0) *out = rank[n - 1] + flags[n - 1];
 }
 
-int scan_total(am_ctx* c, const int* flags, int* rank, size_t n) {
+// total = rank[n - 1] + flags[n - 1] of a finished exclusive scan.  The total
+// reaches the host through a coherent host word.  With `poll` the host spins
+// on that word (a stream sync after 100 ms, which also reports a failed
+// kernel) instead of waiting for the stream: the kernels ordered before
+// k_scan_total are then complete, k_scan_total itself may not be.  Callers
+// that read device memory with synchronous copies afterwards must not poll.
+int read_total(am_ctx* c, const int* flags, const int* rank, size_t n, bool poll) {
   if (n == 0) return 0;
-  exclusive_scan(c, flags, rank, n);
   if (!c->host_total) {
     HIPCHK(hipHostMalloc((void**)&c->host_total, 256, hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer((void**)&c->host_total_dev, c->host_total, 0));
   }
+  volatile int* h = reinterpret_cast<volatile int*>(c->host_total);
+  *h = -1;
   hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->stream, rank, flags, n, c->host_total_dev);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipStreamSynchronize(c->stream));
-  return *reinterpret_cast<volatile int*>(c->host_total);
+  if (poll) {
+    const auto t0 = std::chrono::steady_clock::now();
+    while (*h < 0) {
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100)) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        break;
+      }
+    }
+  } else {
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return *h;
+}
+
+int scan_total(am_ctx* c, const int* flags, int* rank, size_t n) {
+  if (n == 0) return 0;
+  exclusive_scan(c, flags, rank, n);
+  return read_total(c, flags, rank, n, false);
 }
 
 void ensure_table(am_ctx* c, size_t entries) {
@@ -233,6 +256,12 @@ int insert_level(am_ctx* c, size_t n) {
   const hipStream_t st = c->stream;
   if (n > (size_t)INT_MAX / 2) throw CapErr{"too many candidates in one level"};
   ensure_table(c, (size_t)c->R + n);
+  // Where the region store holds the worst case (every candidate new; grown
+  // to it for levels up to 4M candidates), the new regions are committed
+  // before the host learns how many there are: one wait per level, no host
+  // round trip between the count and the commit.
+  const bool early = (size_t)c->R + n <= c->reg_cap || n <= ((size_t)1 << 22);
+  if (early) grow_regions(c, (size_t)c->R + n);
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, c->winner.p, n, INT_MAX);
   hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, n, 0);
   InsertIO io;
@@ -251,14 +280,21 @@ int insert_level(am_ctx* c, size_t n) {
   hipLaunchKernelGGL(k_mark_first, dim3(blocks(n, 256)), dim3(256), 0, st, c->status.p, c->winner.p, c->first.p,
                      c->link.p, (int)n);
   HIPCHK(hipGetLastError());
-  const int total = scan_total(c, c->first.p, c->rank.p, n);
+  const auto commit = [&]() {
+    hipLaunchKernelGGL(k_commit, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, c->rank.p, c->link.p,
+                       c->slot.p, c->cand_mask.p, c->cand_start.p, c->cand_point.p, c->reg_mask.p, c->reg_start.p,
+                       c->reg_point.p, c->table.p, (int)n, c->Wd, (int)c->R);
+    HIPCHK(hipGetLastError());
+  };
+  exclusive_scan(c, c->first.p, c->rank.p, n);
+  if (early) commit();
+  const int total = read_total(c, c->first.p, c->rank.p, n, true);
   if (total == 0) return 0;
   if ((long long)c->R + total > (long long)INT_MAX / 2) throw CapErr{"region count exceeds 2^30"};
-  grow_regions(c, (size_t)c->R + total);
-  hipLaunchKernelGGL(k_commit, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, c->rank.p, c->link.p, c->slot.p,
-                     c->cand_mask.p, c->cand_start.p, c->cand_point.p, c->reg_mask.p, c->reg_start.p,
-                     c->reg_point.p, c->table.p, (int)n, c->Wd, (int)c->R);
-  HIPCHK(hipGetLastError());
+  if (!early) {
+    grow_regions(c, (size_t)c->R + total);
+    commit();
+  }
   c->R += total;
   return total;
 }
