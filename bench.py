@@ -211,7 +211,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,bf16x3,fp64ray,fp64")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
-    ap.add_argument("--extras", action="store_true", help="also measure the 64-MLP batch and the adjoint")
+    ap.add_argument("--extras", action="store_true", help="also measure the 64-MLP batch")
     ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -339,10 +339,11 @@ def main():
         "roofline_other": lines[1:],
         "traversal": traversal,
     }
+    log("[bench] adjoint (backward) on the diff 3->128x4->1 net")
+    line["adjoint"] = adjoint_extra()
     if args.extras:
-        log("[bench] extras: 64-MLP batch and adjoint")
+        log("[bench] extras: 64-MLP batch")
         line["batch64"] = batch64_extra(64, 16)
-        line["adjoint"] = adjoint_extra()
     if args.calib_copy_mb:
         line["copy_calibration"] = copy_calibration(args.calib_copy_mb)
     if not args.no_cpu:
