@@ -25,6 +25,9 @@ __global__ void k_adjoint_direct(NetDev net, AdjointIO io, const double* planes,
 template <bool TA, int MODE>
 __global__ void k_adj_gemm(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
                            int kchunk, const uint32_t* mask, int Wd, int n0);
+template <bool TA, int MODE>
+__global__ void k_adj_mfma(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
+                           int kchunk, const uint32_t* mask, int Wd, int n0);
 __global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out);
 
 }  // namespace am

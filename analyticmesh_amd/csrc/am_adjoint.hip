@@ -334,6 +334,89 @@ template __global__ void k_adj_gemm<false, 0>(const double*, int, const double*,
 template __global__ void k_adj_gemm<true, 1>(const double*, int, const double*, int, double*, int, int, int, int, int,
                                              const uint32_t*, int, int);
 
+// The same products on the FP64 matrix cores: each wave owns a 32 x 32
+// quarter of the 64 x 64 tile as 2 x 2 blocks of v_mfma_f64_16x16x4_f64
+// (A fragment: lane l holds A[l % 16][l / 16]; B: B[l / 16][l % 16]; result:
+// element v of lane l is row l / 16 + 4 v of column l % 16, as measured on
+// gfx950 with one-hot operands).  Operands
+// pass through LDS as in k_adj_gemm; TA, MODE and the arguments are the same.
+typedef double am_double4 __attribute__((ext_vector_type(4)));
+
+template <bool TA, int MODE>
+__global__ __launch_bounds__(256) void k_adj_mfma(const double* __restrict__ A, int lda, const double* __restrict__ B,
+                                                  int ldb, double* C, int ldc, int M, int N, int K, int kchunk,
+                                                  const uint32_t* mask, int Wd, int n0) {
+  constexpr int SA = 66;
+  __shared__ __align__(16) double sa[16][SA];
+  __shared__ __align__(16) double sb[16][64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int m0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int k0 = blockIdx.z * kchunk, k1 = min(K, k0 + kchunk);
+  am_double4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = am_double4{0.0, 0.0, 0.0, 0.0};
+  double ra[4], rb[4];
+  auto fetch = [&](int kb) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = tid + 256 * e;
+      const int ka = TA ? (i >> 6) : (i & 15), ma = TA ? (i & 63) : (i >> 4);
+      const int gk = kb + ka, gm = m0 + ma;
+      ra[e] = (gk < k1 && gm < M) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
+      const int gk2 = kb + (i >> 6), gn = c0 + (i & 63);
+      rb[e] = (gk2 < k1 && gn < N) ? B[(size_t)gk2 * ldb + gn] : 0.0;
+    }
+  };
+  const int l16 = lane & 15, l4 = lane >> 4;
+  if (k0 < k1) fetch(k0);
+  for (int kb = k0; kb < k1; kb += 16) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = tid + 256 * e;
+      const int ka = TA ? (i >> 6) : (i & 15), ma = TA ? (i & 63) : (i >> 4);
+      sa[ka][ma] = ra[e];
+      sb[i >> 6][i & 63] = rb[e];
+    }
+    __syncthreads();
+    if (kb + 16 < k1) fetch(kb + 16);
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const int kr = ks * 4 + l4;
+      const double a0 = sa[kr][wr + l16], a1 = sa[kr][wr + 16 + l16];
+      const double b0 = sb[kr][wc + l16], b1 = sb[kr][wc + 16 + l16];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = c0 + wc + 16 * j + l16;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int m = m0 + wr + 16 * i + l4 + 4 * v;
+        if (m >= M || n >= N) continue;
+        if (MODE == 0) {
+          if (mbit(mask + (size_t)(m >> 2) * Wd, n0 + n)) C[(size_t)m * ldc + n] += acc[i][j][v];
+        } else {
+          atomicAdd(&C[(size_t)m * ldc + n], acc[i][j][v]);
+        }
+      }
+    }
+}
+
+template __global__ void k_adj_mfma<false, 0>(const double*, int, const double*, int, double*, int, int, int, int, int,
+                                              const uint32_t*, int, int);
+template __global__ void k_adj_mfma<true, 1>(const double*, int, const double*, int, double*, int, int, int, int, int,
+                                             const uint32_t*, int, int);
+
 // out[j * stride_out] += sum over b < count of G[b][qsel][j]: 64 columns and
 // 256 regions per block, four partial sums per column added in LDS.
 __global__ __launch_bounds__(256) void k_col_sum(const double* G, int count, int M, int qsel, double* out,

@@ -93,6 +93,7 @@ struct am_ctx {
   size_t max_lds = 65536;
   long long max_regions = -1;
   bool adjoint_tiled = true;   // am_adjoint: tiled chain + GEMM pull-back where the tiled chain runs
+  bool adjoint_mfma = true;    // am_adjoint's GEMMs on the FP64 matrix cores (else VALU fmas)
   int table_log2 = 22;
 
   // network (device)
@@ -661,6 +662,9 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
+  } else if (k == "adjoint_mfma") {
+    if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "adjoint_mfma must be 0 or 1");
+    c->adjoint_mfma = value != 0;
   } else if (k == "adjoint_tiled") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "adjoint_tiled must be 0 or 1");
     c->adjoint_tiled = value != 0;
@@ -1180,9 +1184,10 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
         // G_li += mask (G_li+1 W_li), li = D .. 1 (rows: region x component)
         for (int li = nl - 1; li >= 1; --li) {
           const int nout = c->nodes[li + 1], nin = c->nodes[li], M = cnt * 4;
-          hipLaunchKernelGGL((k_adj_gemm<false, 0>), dim3((nin + 63) / 64, (M + 63) / 64, 1), dim3(256), 0, st,
-                             (const double*)Gst[li].p, nout, c->net.Wrow[li], nin, Gst[li - 1].p, nin, M, nin, nout,
-                             nout, mask0, c->Wd, c->net.off[li - 1]);
+          hipLaunchKernelGGL((c->adjoint_mfma ? k_adj_mfma<false, 0> : k_adj_gemm<false, 0>),
+                             dim3((nin + 63) / 64, (M + 63) / 64, 1), dim3(256), 0, st, (const double*)Gst[li].p,
+                             nout, c->net.Wrow[li], nin, Gst[li - 1].p, nin, M, nin, nout, nout, mask0, c->Wd,
+                             c->net.off[li - 1]);
         }
         HIPCHK(hipGetLastError());
       } else {
@@ -1193,9 +1198,10 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
       const int K = cnt * 4, kchunk = 512;
       for (int li = 1; li < nl; ++li) {
         const int M = c->nodes[li + 1], N = c->nodes[li];
-        hipLaunchKernelGGL((k_adj_gemm<true, 1>), dim3((N + 63) / 64, (M + 63) / 64, (K + kchunk - 1) / kchunk),
-                           dim3(256), 0, st, (const double*)Gst[li].p, M, (const double*)Pst[li].p, N, dW[li].p, N, M,
-                           N, K, kchunk, (const uint32_t*)nullptr, 0, 0);
+        hipLaunchKernelGGL((c->adjoint_mfma ? k_adj_mfma<true, 1> : k_adj_gemm<true, 1>),
+                           dim3((N + 63) / 64, (M + 63) / 64, (K + kchunk - 1) / kchunk), dim3(256), 0, st,
+                           (const double*)Gst[li].p, M, (const double*)Pst[li].p, N, dW[li].p, N, M, N, K, kchunk,
+                           (const uint32_t*)nullptr, 0, 0);
         hipLaunchKernelGGL(k_col_sum, dim3((M + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
                            (const double*)Gst[li].p, cnt, M, 3, db[li].p, 1);
       }

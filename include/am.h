@@ -111,7 +111,8 @@ am_status am_planes(am_ctx* ctx, const uint32_t* masks, int n, float* planes, fl
  * _VAR_VERT_MAX), "table_log2" (initial visited-table size), "max_regions"
  * (stop after this many regions, -1 = unlimited), "walk" (0 ray walk, 1
  * candidate scan, for the reduced forms), "adjoint_tiled" (1 default: the
- * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region). */
+ * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
+ * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU). */
 am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 
 /* Timings and counters of the last am_run/am_combine; see AM_STAT_* indices. */

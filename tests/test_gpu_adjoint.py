@@ -87,25 +87,32 @@ def test_autograd_function_backward():
 
 
 def test_adjoint_tiled_matches_per_region():
-    """The tiled path (planes from the tiled chain, pull-back as GEMMs over all
-    regions) against the one-kernel-per-region path (chain and pull-back in
-    one kernel).  Both sum each region's terms in the same order; only the
-    order of the atomic sums over regions differs.  64 seeds: 8 chunks."""
+    """The tiled path (planes from the tiled chain, pull-back and weight
+    gradients as GEMMs over all regions, on the FP64 matrix cores or as VALU
+    fmas) against the one-kernel-per-region path (chain and pull-back in one
+    kernel).  The sums run in different orders; the results agree to 1e-12 of
+    each array's largest entry.  64 seeds: 8 chunks."""
     from analyticmesh_amd.engine import march
     spec, pts, st = workload("diff", n_seeds=64)
-    _say("[adjoint] tiled vs per-region, diff seeds=64")
+    _say("[adjoint] tiled (MFMA, VALU) vs per-region, diff seeds=64")
     res = march(spec, pts, st, precision="fp64")
     V, _ = res.combine()
     g = np.random.Generator(np.random.PCG64(9)).normal(size=V.shape)
     shapes = [w.shape for w in spec.weights]
-    dW_t, db_t = res.adjoint(g, shapes)
-    res._eng.set_option("adjoint_tiled", 0)
+    runs = {}
     try:
-        dW_r, db_r = res.adjoint(g, shapes)
+        for name, tiled, mfma in (("mfma", 1, 1), ("valu", 1, 0), ("per-region", 0, 0)):
+            res._eng.set_option("adjoint_tiled", tiled)
+            res._eng.set_option("adjoint_mfma", mfma)
+            runs[name] = res.adjoint(g, shapes)
     finally:
         res._eng.set_option("adjoint_tiled", 1)
-    worst = 0.0
-    for a, b in list(zip(dW_t, dW_r)) + list(zip(db_t, db_r)):
-        worst = max(worst, np.abs(a - b).max() / max(1e-30, np.abs(b).max()))
-    _say(f"  V={len(V)} max rel diff {worst:.2e}")
-    assert worst < 1e-12
+        res._eng.set_option("adjoint_mfma", 1)
+    ref_W, ref_b = runs["per-region"]
+    for name in ("mfma", "valu"):
+        dW, db = runs[name]
+        worst = 0.0
+        for a, b in list(zip(dW, ref_W)) + list(zip(db, ref_b)):
+            worst = max(worst, np.abs(a - b).max() / max(1e-30, np.abs(b).max()))
+        _say(f"  {name}: V={len(V)} max rel diff vs per-region {worst:.2e}")
+        assert worst < 1e-12
