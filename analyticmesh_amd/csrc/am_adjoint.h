@@ -17,6 +17,13 @@ struct AdjointIO {
   int r0, count, vm;
   double* Gst[AM_MAX_LINEAR]; // li = 0..D: [count][4][n_{li+1}]
   double* Pst[AM_MAX_LINEAR]; // li = 1..D: [count][4][n_li]
+  // tiled path: each region's direct terms, one entry per distinct hidden
+  // neuron its vertices make tight: count [count], neuron [count][dcap],
+  // values [count][dcap][4] (x, y, z, c)
+  int* dn;
+  int* dg;
+  double* dv;
+  int dcap;
 };
 
 __global__ void k_adjoint_fp64(NetDev net, AdjointIO io);
@@ -28,6 +35,8 @@ __global__ void k_adj_gemm(const double* A, int lda, const double* B, int ldb, d
 template <bool TA, int MODE>
 __global__ void k_adj_mfma(const double* A, int lda, const double* B, int ldb, double* C, int ldc, int M, int N, int K,
                            int kchunk, const uint32_t* mask, int Wd, int n0);
+__global__ void k_adj_scatter(const int* dn, const int* dg, const double* dv, int count, int cap, int o, int n,
+                              double* G);
 __global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out);
 
 }  // namespace am

@@ -161,14 +161,18 @@ __device__ __forceinline__ void raw_plane64(const NetDev& net, const uint32_t* m
 
 // The direct terms of one region (one wave): lambda for each vertex the
 // region owns, one vertex per lane, the plane gradients -lambda_i [v; 1]
-// summed in vertex order by lane 0 (the order of k_adjoint_fp64), then
-// G of every layer and the masked inputs P written out.  The pull-back
-// through the weights is left to k_adj_gemm over all regions.
+// summed in vertex order by lane 0 (the order of k_adjoint_fp64).  Written
+// out: G of the output plane, the sums as a list per distinct tight hidden
+// neuron, and the masked inputs P.  The pull-back through the weights is left
+// to the GEMMs over all regions, and k_adj_scatter adds the lists to it.
 __global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io, const double* planes,
                                                        const double* eqb, int Cs) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int L = net.L;
   double* G = reinterpret_cast<double*>(smem_raw);   // [4][L]
+  int* mark = reinterpret_cast<int*>(G + 4 * L);      // [L]: neuron already in the list
+  __shared__ int s_list[2 * AM_VM_MAX];
+  __shared__ int s_nd;
   __shared__ double s_c[AM_VM_MAX][12];
   __shared__ int s_lohi[AM_VM_MAX][2];
   __shared__ int s_own[AM_VM_MAX];
@@ -182,6 +186,7 @@ __global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io,
   const uint32_t* mask = io.reg_mask + (size_t)r * net.Wd;
   const double* src = planes + (size_t)b * 4 * Cs;
   for (int g = lane; g < 4 * L; g += 64) G[g] = 0.0;
+  for (int g = lane; g < L; g += 64) mark[g] = 0;
   const bool mine = lane < vm && lane < nv && nv >= 3 && io.vfirst[(size_t)r * vm + lane];
   if (lane < vm) s_own[lane] = mine ? 1 : 0;
   const bool active = __any(mine);
@@ -214,25 +219,33 @@ __global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io,
   __syncthreads();
   if (lane == 0) {
     double ge[4] = {0.0, 0.0, 0.0, 0.0};
+    int nd = 0;
     for (int k = 0; k < nv && k < vm; ++k) {
       if (!s_own[k]) continue;
       const int lo = s_lohi[k][0], hi = s_lohi[k][1];
-      if (lo < L)
+      if (lo < L) {
         for (int c = 0; c < 4; ++c) G[c * L + lo] -= s_c[k][c];
-      if (hi < L)
+        if (!mark[lo]) { mark[lo] = 1; s_list[nd++] = lo; }
+      }
+      if (hi < L) {
         for (int c = 0; c < 4; ++c) G[c * L + hi] -= s_c[k][4 + c];
+        if (!mark[hi]) { mark[hi] = 1; s_list[nd++] = hi; }
+      }
       for (int c = 0; c < 4; ++c) ge[c] -= s_c[k][8 + c];
     }
     for (int c = 0; c < 4; ++c) g_eq[c] = ge[c];
+    s_nd = nd;
   }
   __syncthreads();
-  // G of the output plane (one output) and of each hidden layer
+  // G of the output plane (one output); the hidden layers' direct terms as a
+  // list, added to the pull-back's output by k_adj_scatter
   if (lane < 4) io.Gst[net.D][(size_t)b * 4 + lane] = active ? g_eq[lane] : 0.0;
-  for (int li = 0; li < net.D; ++li) {
-    const int n = net.nodes[li + 1], o = net.off[li];
-    double* gs = io.Gst[li] + (size_t)b * 4 * n;
-    for (int j = lane; j < n; j += 64)
-      for (int c = 0; c < 4; ++c) gs[(size_t)c * n + j] = G[c * L + o + j];
+  const int nd = s_nd;
+  if (lane == 0) io.dn[b] = nd;
+  for (int j = lane; j < nd; j += 64) {
+    const int g = s_list[j];
+    io.dg[(size_t)b * io.dcap + j] = g;
+    for (int c = 0; c < 4; ++c) io.dv[((size_t)b * io.dcap + j) * 4 + c] = G[c * L + g];
   }
   // masked inputs of linear layers 1..D (zero for a region owning no vertex:
   // its G rows are zero)
@@ -253,8 +266,8 @@ __global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io,
 //   TA: element (m, k) of A at A[k * lda + m], else at A[m * lda + k];
 //   B:  element (k, n) at B[k * ldb + n].
 //   MODE 0 (pull-back): rows are (region, component), region m / 4 of the
-//     tile's mask rows; C[m][n] += acc where that region's neuron n0 + n is
-//     active, else C is left as it is.  K whole (one block in z).
+//     tile's mask rows; C[m][n] = acc where that region's neuron n0 + n is
+//     active, else +0.  K whole (one block in z).
 //   MODE 1 (weight gradient): atomicAdd of acc into C[m][n]; K split over
 //     blockIdx.z in chunks of kchunk.
 template <bool TA, int MODE>
@@ -321,7 +334,7 @@ __global__ __launch_bounds__(256) void k_adj_gemm(const double* __restrict__ A, 
       const int n = c0 + (j >> 1) * 32 + 2 * tx + (j & 1);
       if (n >= N) continue;
       if (MODE == 0) {
-        if (mbit(mask + (size_t)(m >> 2) * Wd, n0 + n)) C[(size_t)m * ldc + n] += acc[i][j];
+        C[(size_t)m * ldc + n] = mbit(mask + (size_t)(m >> 2) * Wd, n0 + n) ? acc[i][j] : 0.0;
       } else {
         atomicAdd(&C[(size_t)m * ldc + n], acc[i][j]);
       }
@@ -404,7 +417,7 @@ __global__ __launch_bounds__(256) void k_adj_mfma(const double* __restrict__ A, 
         const int m = m0 + wr + 16 * i + l4 + 4 * v;
         if (m >= M || n >= N) continue;
         if (MODE == 0) {
-          if (mbit(mask + (size_t)(m >> 2) * Wd, n0 + n)) C[(size_t)m * ldc + n] += acc[i][j][v];
+          C[(size_t)m * ldc + n] = mbit(mask + (size_t)(m >> 2) * Wd, n0 + n) ? acc[i][j][v] : 0.0;
         } else {
           atomicAdd(&C[(size_t)m * ldc + n], acc[i][j][v]);
         }
@@ -416,6 +429,21 @@ template __global__ void k_adj_mfma<false, 0>(const double*, int, const double*,
                                               const uint32_t*, int, int);
 template __global__ void k_adj_mfma<true, 1>(const double*, int, const double*, int, double*, int, int, int, int, int,
                                              const uint32_t*, int, int);
+
+// Adds the direct terms of the hidden layer with neurons o .. o + n - 1 to
+// its pull-back output G [count][4][n] (k_adjoint_direct's lists, one thread
+// per entry; a region lists each neuron once, so no two threads meet).
+// With G = mask (G_next W) this is the per-region kernel's direct + pull-back,
+// as a + b and b + a are the same floating-point sum.
+__global__ void k_adj_scatter(const int* dn, const int* dg, const double* dv, int count, int cap, int o, int n,
+                              double* G) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = (int)(t / cap), j = (int)(t - (size_t)b * cap);
+  if (b >= count || j >= dn[b]) return;
+  const int g = dg[(size_t)b * cap + j] - o;
+  if (g < 0 || g >= n) return;
+  for (int c = 0; c < 4; ++c) G[((size_t)b * 4 + c) * n + g] += dv[((size_t)b * cap + j) * 4 + c];
+}
 
 // out[j * stride_out] += sum over b < count of G[b][qsel][j]: 64 columns and
 // 256 regions per block, four partial sums per column added in LDS.

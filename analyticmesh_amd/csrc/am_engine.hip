@@ -147,7 +147,8 @@ struct am_ctx {
   bool tile_ok = false;
   DevBuf<double> tplanes, teq;
   // am_adjoint's buffers, kept from call to call: dL/dv, dW, db, G and P per layer
-  DevBuf<double> adj_gv;
+  DevBuf<double> adj_gv, adj_dv;
+  DevBuf<int> adj_dn, adj_dg;
   std::vector<DevBuf<double>> adj_dW, adj_db, adj_G, adj_P;
   DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
   DevBuf<int> runacc;                     // per run: FP64 ray walk undecided count, then per reason
@@ -1142,10 +1143,14 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
     }
     const int Ch = (c->C + 63) & ~63;
     size_t lds;
+    const int dcap = 2 * c->vm;
     if (tiled) {
       ensure(c->tplanes, (size_t)chunk * 4 * Ch, false, st);
       ensure(c->teq, (size_t)chunk * 4, false, st);
-      lds = (size_t)4 * c->L * 8;
+      ensure(c->adj_dn, (size_t)chunk, false, st);
+      ensure(c->adj_dg, (size_t)chunk * dcap, false, st);
+      ensure(c->adj_dv, (size_t)chunk * dcap * 4, false, st);
+      lds = (size_t)4 * c->L * 8 + (size_t)c->L * 4;
       if (lds > c->max_lds) throw CapErr{"network too wide for the adjoint kernel's LDS"};
       if (lds > 65536)
         HIPCHK(hipFuncSetAttribute((const void*)k_adjoint_direct, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1174,6 +1179,10 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
         io.Gst[li] = Gst[li].p;
         io.Pst[li] = li >= 1 ? Pst[li].p : nullptr;
       }
+      io.dn = c->adj_dn.p;
+      io.dg = c->adj_dg.p;
+      io.dv = c->adj_dv.p;
+      io.dcap = dcap;
       const uint32_t* mask0 = c->reg_mask.p + (size_t)r0 * c->Wd;
       if (tiled) {
         hipLaunchKernelGGL(k_chain_tile_fp64, dim3((cnt + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS),
@@ -1181,13 +1190,17 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
         hipLaunchKernelGGL(k_adjoint_direct, dim3(cnt), dim3(64), lds, st, c->net, io, (const double*)c->tplanes.p,
                            (const double*)c->teq.p, Ch);
         HIPCHK(hipGetLastError());
-        // G_li += mask (G_li+1 W_li), li = D .. 1 (rows: region x component)
+        // G of hidden layer li = mask (G_li+1 W_li) + its direct terms, li = D .. 1
+        // (rows: region x component)
         for (int li = nl - 1; li >= 1; --li) {
           const int nout = c->nodes[li + 1], nin = c->nodes[li], M = cnt * 4;
           hipLaunchKernelGGL((c->adjoint_mfma ? k_adj_mfma<false, 0> : k_adj_gemm<false, 0>),
                              dim3((nin + 63) / 64, (M + 63) / 64, 1), dim3(256), 0, st, (const double*)Gst[li].p,
                              nout, c->net.Wrow[li], nin, Gst[li - 1].p, nin, M, nin, nout, nout, mask0, c->Wd,
                              c->net.off[li - 1]);
+          hipLaunchKernelGGL(k_adj_scatter, dim3((unsigned)(((size_t)cnt * dcap + 255) / 256)), dim3(256), 0, st,
+                             (const int*)c->adj_dn.p, (const int*)c->adj_dg.p, (const double*)c->adj_dv.p, cnt, dcap,
+                             c->net.off[li - 1], nin, Gst[li - 1].p);
         }
         HIPCHK(hipGetLastError());
       } else {
