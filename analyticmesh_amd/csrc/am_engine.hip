@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <thread>
 #include <climits>
 #include <cstdio>
 #include <cstring>
@@ -217,12 +218,16 @@ int read_total(am_ctx* c, const int* flags, const int* rank, size_t n, bool poll
   hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->stream, rank, flags, n, c->host_total_dev);
   HIPCHK(hipGetLastError());
   if (poll) {
+    // a short tight spin, then a yield between polls so that host threads
+    // driving other contexts (a batch of networks) keep their cores
     const auto t0 = std::chrono::steady_clock::now();
-    while (*h < 0) {
+    for (unsigned spins = 0; *h < 0; ++spins) {
+      if (spins < 4096) continue;
       if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100)) {
         HIPCHK(hipStreamSynchronize(c->stream));
         break;
       }
+      std::this_thread::yield();
     }
   } else {
     HIPCHK(hipStreamSynchronize(c->stream));
