@@ -38,5 +38,6 @@ __global__ void k_adj_mfma(const double* A, int lda, const double* B, int ldb, d
 __global__ void k_adj_scatter(const int* dn, const int* dg, const double* dv, int count, int cap, int o, int n,
                               double* G);
 __global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out);
+__global__ void k_col_sum4(const double* G, int count, int M, double* outw, double* outb);
 
 }  // namespace am

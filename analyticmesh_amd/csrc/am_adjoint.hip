@@ -461,4 +461,26 @@ __global__ __launch_bounds__(256) void k_col_sum(const double* G, int count, int
   if (by == 0 && j < M) atomicAdd(&out[(size_t)j * stride_out], ((part[0][jx] + part[1][jx]) + part[2][jx]) + part[3][jx]);
 }
 
+// All four components at once (the first layer): outw[j * 3 + q] += sum over
+// b of G[b][q][j] for q < 3, outb[j] += the same for q = 3.  64 columns and
+// 256 regions per block, four partial sums per column and component added in
+// LDS.
+__global__ __launch_bounds__(256) void k_col_sum4(const double* G, int count, int M, double* outw, double* outb) {
+  __shared__ double part[4][4][64];
+  const int jx = threadIdx.x & 63, by = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + jx;
+  const int b0 = blockIdx.y * 256, b1 = min(count, b0 + 256);
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  if (j < M)
+    for (int b = b0 + by; b < b1; b += 4)
+      for (int q = 0; q < 4; ++q) acc[q] += G[((size_t)b * 4 + q) * M + j];
+  for (int q = 0; q < 4; ++q) part[by][q][jx] = acc[q];
+  __syncthreads();
+  if (by == 0 && j < M)
+    for (int q = 0; q < 4; ++q) {
+      const double v = ((part[0][q][jx] + part[1][q][jx]) + part[2][q][jx]) + part[3][q][jx];
+      atomicAdd(q < 3 ? &outw[(size_t)j * 3 + q] : &outb[j], v);
+    }
+}
+
 }  // namespace am

@@ -1224,11 +1224,8 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
                            (const double*)Gst[li].p, cnt, M, 3, db[li].p, 1);
       }
       const int n1 = c->nodes[1];
-      for (int q = 0; q < 3; ++q)
-        hipLaunchKernelGGL(k_col_sum, dim3((n1 + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
-                           (const double*)Gst[0].p, cnt, n1, q, dW[0].p + q, 3);
-      hipLaunchKernelGGL(k_col_sum, dim3((n1 + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
-                         (const double*)Gst[0].p, cnt, n1, 3, db[0].p, 1);
+      hipLaunchKernelGGL(k_col_sum4, dim3((n1 + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
+                         (const double*)Gst[0].p, cnt, n1, dW[0].p, db[0].p);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(st));
