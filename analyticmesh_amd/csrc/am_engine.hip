@@ -1159,7 +1159,7 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
     // tiled: planes from the tiled chain, the pull-back as GEMMs over all
     // regions of a chunk; else one kernel per region does chain and pull-back
     const bool tiled = c->tile_ok && c->adjoint_tiled;
-    const int CH = 32768;
+    const int CH = 65536;
     const int chunk = std::min(R, CH);
     for (int li = 0; li < nl; ++li) {
       ensure(Gst[li], (size_t)chunk * 4 * c->nodes[li + 1], false, st);

@@ -91,7 +91,7 @@ def test_adjoint_tiled_matches_per_region():
     gradients as GEMMs over all regions, on the FP64 matrix cores or as VALU
     fmas) against the one-kernel-per-region path (chain and pull-back in one
     kernel).  The sums run in different orders; the results agree to 1e-12 of
-    each array's largest entry.  64 seeds: 8 chunks."""
+    each array's largest entry.  64 seeds: 4 chunks."""
     from analyticmesh_amd.engine import march
     spec, pts, st = workload("diff", n_seeds=64)
     _say("[adjoint] tiled (MFMA, VALU) vs per-region, diff seeds=64")
