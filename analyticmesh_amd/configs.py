@@ -13,18 +13,44 @@ import math
 
 import numpy as np
 
-__all__ = ["NetSpec", "polytope", "geometric", "sphere", "deep", "diff", "batch64",
-           "forward_fp64", "preact_fp64", "bisect_seed", "seeds_fibonacci", "mask_of"]
+__all__ = ["NetSpec", "polytope", "geometric", "sphere", "deep", "diff", "batch64", "skip_mlp2",
+           "skip_custom2", "wide", "forward_fp64", "preact_fp64", "bisect_seed", "seeds_fibonacci", "mask_of"]
 
 
 class NetSpec:
-    """Plain ReLU MLP: ``weights[i]`` is (out, in) float32, ``biases[i]`` (out,)."""
+    """ReLU MLP: ``weights[i]`` is (out, in) float32, ``biases[i]`` (out,).
 
-    def __init__(self, nodes, weights, biases, name=""):
+    Optional additive skips follow the reference's MLP (backend/model.py:23-31,
+    101-123): ``arc_table[i] = [count, from_0, tm_0, from_1, tm_1, ...]`` adds
+    ``tms[tm_j] @ node[from_j]`` (node 0 = the input, node k > 0 = hidden layer
+    k after its ReLU) to the output of linear layer i + 1 before its ReLU; a
+    ``tms`` entry of None is the identity.  Skips into the output layer are
+    ignored, as the reference's engine ignores them (backend/inc/process.h:42)."""
+
+    def __init__(self, nodes, weights, biases, name="", arc_table=None, tms=None):
         self.nodes = list(nodes)
         self.weights = [np.ascontiguousarray(w, dtype=np.float32) for w in weights]
         self.biases = [np.ascontiguousarray(b, dtype=np.float32) for b in biases]
         self.name = name
+        self.arc_table = None if arc_table is None else [list(map(int, r)) for r in arc_table]
+        self.tms = [None if t is None else np.ascontiguousarray(t, dtype=np.float32) for t in (tms or [])]
+
+    @property
+    def has_skips(self):
+        return self.arc_table is not None and any(r and r[0] > 0 for r in self.arc_table)
+
+    def skips_into(self, li):
+        """(from_node, transform or None) pairs added to linear layer li's output (li >= 1).
+
+        Skips into the output layer are ignored (backend/inc/process.h:42)."""
+        if self.arc_table is None or li < 1 or li >= len(self.nodes) - 2:
+            return []
+        row = self.arc_table[li - 1]
+        return [(row[1 + 2 * j], self.tms[row[2 + 2 * j]]) for j in range(row[0] if row else 0)]
+
+    def tm_arrays(self):
+        """Transform matrices as the engine takes them: (0, 0) arrays for identities."""
+        return [np.zeros((0, 0), dtype=np.float32) if t is None else t for t in self.tms]
 
     @property
     def L(self):
@@ -37,11 +63,16 @@ class NetSpec:
     def to_mlp(self):
         import torch
         from .model import MLP
-        m = MLP(self.nodes, [[0]] * (len(self.nodes) - 2), [], initialization=None, enable_print=False)
+        arc = self.arc_table if self.arc_table is not None else [[0]] * (len(self.nodes) - 2)
+        shapes = [[0, 0] if t is None else list(t.shape) for t in self.tms]
+        m = MLP(self.nodes, arc, shapes, initialization=None, enable_print=False)
         with torch.no_grad():
             for lin, w, b in zip(m.linears, self.weights, self.biases):
                 lin.weight.copy_(torch.from_numpy(w))
                 lin.bias.copy_(torch.from_numpy(b))
+            for tm, t in zip(m.tms, self.tms):
+                if t is not None:
+                    tm.weight.copy_(torch.from_numpy(t))
         return m
 
     def gemm_flops_per_region(self):
@@ -96,6 +127,41 @@ def diff(seed=0):
     return geometric([3, 128, 128, 128, 128, 1], 0.5, 0.03, seed, name=f"diff(seed={seed})")
 
 
+def _skipnet(nodes, arc_table, tm_shapes, radius, sigma, seed, tm_scale, name):
+    """Geometric-init net plus transform matrices U(-s, s)/sqrt(in) (torch's Linear default, scaled)."""
+    base = geometric(nodes, radius, sigma, seed)
+    rng = np.random.Generator(np.random.PCG64(seed + 7919))
+    tms = []
+    for h, w in tm_shapes:
+        if h == 0 and w == 0:
+            tms.append(None)
+        else:
+            bound = tm_scale / math.sqrt(w)
+            tms.append(rng.uniform(-bound, bound, (h, w)))
+    return NetSpec(nodes, base.weights, base.biases, name=name, arc_table=arc_table, tms=tms)
+
+
+def skip_mlp2(seed=0):
+    """The reference's test_mlp_2 architecture (backend/test/test_onnx_io.py:145-151): five hidden
+    layers of 80, a transform skip from the input, an identity skip from hidden 1, an 80x80
+    transform skip from hidden 3, and an output skip from the input (ignored, Q1)."""
+    return _skipnet([3, 80, 80, 80, 80, 80, 1], [[1, 0, 0], [1, 1, 1], [0], [1, 3, 2], [1, 0, 3]],
+                    [[80, 3], [0, 0], [80, 80], [1, 3]], 0.5, 0.02, seed, 0.5, name=f"skip_mlp2(seed={seed})")
+
+
+def skip_custom2(seed=0):
+    """The reference's test_custom_2 architecture (backend/test/test_onnx_io.py:99-141): widths
+    20, 50, 20, 80, 20; an identity skip from hidden 1 into hidden 3, an 80x20 transform skip from
+    hidden 1 into hidden 4, and an output skip from hidden 2 (ignored, Q1)."""
+    return _skipnet([3, 20, 50, 20, 80, 20, 1], [[0], [1, 1, 0], [1, 1, 1], [0], [1, 2, 2]],
+                    [[0, 0], [80, 20], [1, 50]], 0.5, 0.02, seed, 0.5, name=f"skip_custom2(seed={seed})")
+
+
+def wide(seed=0):
+    """3->320->64->1: a first layer wider than the tiled chain's 256 rows."""
+    return geometric([3, 320, 64, 1], 0.5, 0.02, seed, name=f"wide(seed={seed})")
+
+
 def batch64(seed=0, count=64):
     """64 nets: depth U{2..6}, width U{32,64,128,256}, r~U(0.3,0.8), perturbation 0.03."""
     rng = np.random.Generator(np.random.PCG64(seed))
@@ -114,16 +180,25 @@ def batch64(seed=0, count=64):
 # FP64 forward pass and seeds
 # ----------------------------------------------------------------------------
 def preact_fp64(spec, x):
-    """Hidden pre-activations (concatenated) and the output, in float64."""
+    """Hidden pre-activations (concatenated) and the output, in float64 (skips included,
+    output-layer skips ignored as the engine ignores them)."""
     x = np.atleast_2d(np.asarray(x, dtype=np.float64))
     pre = []
+    node = [x]          # node k's output: the input, then each hidden layer after its ReLU
     h = x
     n = len(spec.weights)
     for i in range(n):
         z = h @ spec.weights[i].astype(np.float64).T + spec.biases[i].astype(np.float64)
+        for frm, t in (spec.skips_into(i) if hasattr(spec, "skips_into") else []):
+            src = node[frm]
+            if t is None:
+                z = z + (src if src.shape[1] == z.shape[1] else np.pad(src, ((0, 0), (0, z.shape[1] - src.shape[1]))))
+            else:
+                z = z + src @ t.astype(np.float64).T
         if i != n - 1:
             pre.append(z)
             h = np.maximum(z, 0.0)
+            node.append(h)
         else:
             out = z[:, 0]
     return np.concatenate(pre, axis=1), out

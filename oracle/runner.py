@@ -72,6 +72,10 @@ class OracleRun:
         Wp = (ctypes.c_void_p * len(self._W))(*[w.ctypes.data for w in self._W])
         bp = (ctypes.c_void_p * len(self._b))(*[b.ctypes.data for b in self._b])
         D = len(spec.nodes) - 2
+        if arc_table is None and getattr(spec, "arc_table", None) is not None:
+            arc_table = spec.arc_table
+            if tms is None:
+                tms = spec.tm_arrays()
         if arc_table is None:
             arc = np.zeros((D, 1), dtype=np.int32)
         else:

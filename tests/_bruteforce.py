@@ -12,20 +12,35 @@ import numpy as np
 
 
 def frozen_preacts(spec, mask_bits, x):
-    """Pre-activations of the network with ReLUs replaced by the fixed mask."""
+    """Pre-activations of the network with ReLUs replaced by the fixed mask.
+
+    Additive skips (``spec.skips_into``) add ``T @ node_k`` or ``node_k`` to a
+    layer's pre-activation, node 0 being the input and node k > 0 hidden layer
+    k after its (frozen) ReLU; skips into the output layer are ignored, as the
+    reference's engine ignores them (backend/inc/process.h:42)."""
     x = np.atleast_2d(np.asarray(x, dtype=np.float64))
     h = x
     out, pre = None, []
+    node = [x]
     off = 0
     n = len(spec.weights)
+    skips = getattr(spec, "skips_into", lambda i: [])
     for i in range(n):
         z = h @ spec.weights[i].astype(np.float64).T + spec.biases[i].astype(np.float64)
+        for frm, t in skips(i):
+            src = node[frm]
+            if t is None:
+                k = min(src.shape[1], z.shape[1])
+                z[:, :k] += src[:, :k]
+            else:
+                z = z + src @ t.astype(np.float64).T
         if i != n - 1:
             w = z.shape[1]
             m = np.asarray(mask_bits[off:off + w], dtype=np.float64)
             off += w
             pre.append(z)
             h = z * m
+            node.append(h)
         else:
             out = z[:, 0]
     return np.concatenate(pre, axis=1), out

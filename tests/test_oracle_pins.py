@@ -222,8 +222,62 @@ def test_max_regions_truncates():
     assert r.truncated and 1000 <= len(r.masks) <= 1000 + 1024
 
 
-def test_oracle_fp32_mode_runs():
+def test_oracle_fp32_mode_matches_fp64():
+    """The FP32 restatement (the reference's dtype=float32 path, raw sign tests) gives the FP64
+    topology on the sphere, and its vertices agree with FP64's to FP32 accuracy: a wrong cast,
+    dropped term or sign in the FP32 instantiation breaks one of these."""
     spec, pts, st = workload("sphere")
-    r = oracle.run_oracle(spec, pts, st, precision="fp32", traversal="lifo")
-    rec = r.records()
+    a = oracle.run_oracle(spec, pts, st, precision="fp32", traversal="lifo").records()
+    b = oracle.run_oracle(spec, pts, st, precision="fp64", traversal="lifo").records()
+    assert a.face_count() > 10000
+    assert_same_topology(a, b)
+    ca, cb = a.vertex_coords(), b.vertex_coords()
+    worst = max(float(np.abs(np.asarray(v) - cb[k][0]).max()) for k, lst in ca.items() for v in lst)
+    assert 0.0 < worst < 1e-4
+
+
+@pytest.mark.parametrize("name", ["skip_mlp2", "skip_custom2"])
+def test_skip_planes_equal_frozen_network(name):
+    """Planes of nets with additive skips (identity and transform, from the input and from hidden
+    layers; backend/inc/process.h:73-146) == planes read off the frozen network with the same
+    skips at 0, e_x, e_y, e_z.  Dropping a skip, or not masking its source, fails this."""
+    spec = getattr(configs, name)(0)
+    run = oracle.OracleRun(spec)
+    rng = np.random.default_rng(2)
+    for _ in range(6):
+        bits = rng.integers(0, 2, spec.L).astype(np.uint8)
+        A, eq = run.planes(bf.pack_mask(bits))
+        Ab, eqb = bf.region_planes(spec, bits)
+        assert np.abs(A - Ab).max() <= 1e-12 * np.abs(Ab).max()
+        assert np.abs(eq - eqb).max() <= 1e-12 * max(1.0, np.abs(eqb).max())
+    # the skips matter: the same masks without them give other planes
+    plain = configs.NetSpec(spec.nodes, spec.weights, spec.biases)
+    bits = rng.integers(0, 2, spec.L).astype(np.uint8)
+    A, _ = run.planes(bf.pack_mask(bits))
+    Ap, _ = bf.region_planes(plain, bits)
+    assert np.abs(A - Ap).max() > 1e-3 * np.abs(Ap).max()
+
+
+def test_skip_net_mesh_is_closed_and_on_its_planes():
+    """A skip net (the reference's test_custom_2 architecture) meshes to a closed 2-manifold of
+    Euler characteristic 2 whose vertices zero their two tight neurons and f (frozen forward)."""
+    spec = configs.skip_custom2(0)
+    pts = configs.bisect_seed(spec, (1.0, 0.0, 0.0))[None]
+    st = configs.mask_of(spec, pts)
+    rec = oracle.run_oracle(spec, pts, st, traversal="bfs").records()
     assert rec.face_count() > 10000
+    V, F = rec.keyed_mesh()
+    E = edges_of_faces(F)
+    assert set(E.values()) == {2}
+    assert len(V) - len(E) + len(F) == 2
+    worst = 0.0
+    for i in range(0, len(rec), 41):
+        bits = bf.unpack_mask(rec.masks[i], spec.L)
+        nv = rec.info[i, 2]
+        pre, out = bf.frozen_preacts(spec, bits, rec.verts[i, :nv])
+        sgn = np.where(bits.astype(bool), -1.0, 1.0)
+        for k in range(nv):
+            a, b = rec.vpairs[i, k]
+            worst = max(worst, abs(pre[k, a]), abs(pre[k, b]), abs(out[k]))
+            assert (sgn * pre[k] <= 1e-9).all()
+    assert worst < 1e-11
