@@ -11,8 +11,11 @@ struct AdjointIO {
   const uint32_t* reg_mask;   // [R][Wd]
   const int* info;            // [R][AM_INFO_STRIDE]
   const int* vpair;           // [R][vm][2]
-  const int* vfirst;          // [R*vm] 1 where the item is its vertex's first occurrence
-  const int* vid;             // [R*vm] mesh vertex index of the item
+  // vertex uses ("items") of the combined mesh, numbered region by region:
+  // item of (region r, vertex k) = (int)item_off[r] + k
+  const unsigned long long* item_off;  // [R] low 32 bits: first item of the region
+  const unsigned* vfirst;     // [items] 1 where the item is its vertex's first occurrence
+  const int* vid;             // [items] mesh vertex index of the item
   const double* gradv;        // [V][3] dL/dv
   int r0, count, vm;
   double* Gst[AM_MAX_LINEAR]; // li = 0..D: [count][4][n_{li+1}]

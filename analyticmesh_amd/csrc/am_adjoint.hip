@@ -42,9 +42,10 @@ __global__ __launch_bounds__(64) void k_adjoint_fp64(NetDev net, AdjointIO io) {
   const int vm = io.vm;
   const int nv = io.info[(size_t)r * AM_INFO_STRIDE + 2];
 
+  const int item0 = (int)(io.item_off[r] & 0xffffffffu);
   if (lane == 0) s_any = 0;
   __syncthreads();
-  if (lane < vm && lane < nv && nv >= 3 && io.vfirst[(size_t)r * vm + lane]) s_any = 1;
+  if (lane < vm && lane < nv && nv >= 3 && io.vfirst[item0 + lane]) s_any = 1;
   __syncthreads();
   const bool active = s_any != 0;
 
@@ -60,9 +61,9 @@ __global__ __launch_bounds__(64) void k_adjoint_fp64(NetDev net, AdjointIO io) {
       const double eq[4] = {s_eq[0], s_eq[1], s_eq[2], s_eq[3]};
       for (int k = 0; k < nv; ++k) {
         const size_t q = (size_t)r * vm + k;
-        if (!io.vfirst[q]) continue;
+        if (!io.vfirst[item0 + k]) continue;
         const int lo = io.vpair[q * 2 + 0], hi = io.vpair[q * 2 + 1];
-        const int id = io.vid[q];
+        const int id = io.vid[item0 + k];
         const double g3[3] = {io.gradv[(size_t)id * 3 + 0], io.gradv[(size_t)id * 3 + 1],
                               io.gradv[(size_t)id * 3 + 2]};
         double p1[4], p2[4];
@@ -187,13 +188,14 @@ __global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io,
   const double* src = planes + (size_t)b * 4 * Cs;
   for (int g = lane; g < 4 * L; g += 64) G[g] = 0.0;
   for (int g = lane; g < L; g += 64) mark[g] = 0;
-  const bool mine = lane < vm && lane < nv && nv >= 3 && io.vfirst[(size_t)r * vm + lane];
+  const int item0 = (int)(io.item_off[r] & 0xffffffffu);
+  const bool mine = lane < vm && lane < nv && nv >= 3 && io.vfirst[item0 + lane];
   if (lane < vm) s_own[lane] = mine ? 1 : 0;
   const bool active = __any(mine);
   if (mine) {
     const size_t q = (size_t)r * vm + lane;
     const int lo = io.vpair[q * 2 + 0], hi = io.vpair[q * 2 + 1];
-    const int id = io.vid[q];
+    const int id = io.vid[item0 + lane];
     const double g3[3] = {io.gradv[(size_t)id * 3 + 0], io.gradv[(size_t)id * 3 + 1], io.gradv[(size_t)id * 3 + 2]};
     double p1[4], p2[4];
     raw_plane64(net, mask, src, Cs, lo, p1);

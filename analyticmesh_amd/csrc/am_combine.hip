@@ -1,0 +1,229 @@
+// Mesh assembly: vertex deduplication and face indexing of the region store
+// (replaces backend/inc/polymesh.h:144-346, the quantised-coordinate hash).
+//
+// A vertex use ("item") is one vertex of one face.  Items are numbered region
+// by region after an exclusive scan of the face sizes, so the item array is
+// exactly the mesh's face-index array and no slot is spent on regions without
+// a face or on the unused tail of the vertex cap.  A vertex is keyed by its two
+// tight constraints (lo, hi) and its region's mask with those two bits
+// cleared; every region around the vertex produces the same key.  The first
+// item of each key (lowest item index) gives the vertex its index and
+// coordinates, so vertex numbering follows (region, vertex) order.
+//
+// Passes:
+//   1. region scan: packed (face flag << 32 | face size) -> first item and face
+//      rank of every region; the down-sweep also writes each item's region.
+//   2. k_cinsert, one wave per region: the wave reads the region's mask once
+//      (coalesced); the key hash is a sum of per-word hashes, so each item's
+//      hash is the region's sum with the one or two changed words swapped out.
+//      Items claim a table slot by CAS; an item whose key is already present
+//      lowers the slot to the smallest item of that key (atomicMin).
+//   3. first-occurrence flags, scanned -> vertex ranks and the vertex count.
+//   4. k_cassign, one thread per item: vertex index into the face-index array,
+//      coordinates by the first item, face offsets by each face's first item.
+//
+// Scratch comes from a caller-provided workspace (CombineWork).
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+
+#include "am_combine.h"
+#include "am_device.h"
+#include "am_scan.h"
+
+namespace am {
+
+#define AM_CEMPTY 0xFFFFFFFFu
+
+__device__ __forceinline__ uint64_t word_hash(uint32_t x, int w) {
+  return mix64((uint64_t(x) << 32) ^ (0x9e3779b97f4a7c15ULL * uint64_t(w + 1)));
+}
+__device__ __forceinline__ uint64_t pair_hash(int lo, int hi) {
+  return mix64((uint64_t(uint32_t(lo)) << 32) | uint32_t(hi));
+}
+
+struct RegionCountSrc {
+  const int* info;
+  __device__ unsigned long long operator()(size_t r) const {
+    const int nv = info[r * AM_INFO_STRIDE + 2];
+    return nv >= 3 ? ((1ULL << 32) | (unsigned long long)nv) : 0ULL;
+  }
+};
+__global__ void k_citem_region(const unsigned long long* roff, const int* info, int R, int* item_region) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= R) return;
+  const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
+  if (nv < 3) return;
+  const int i0 = (int)(roff[r] & 0xffffffffu);
+  for (int k = 0; k < nv; ++k) item_region[i0 + k] = r;
+}
+struct FlagSrc {
+  const unsigned* p;
+  __device__ unsigned operator()(size_t i) const { return p[i]; }
+};
+
+// One wave per region, four regions per 256-thread block.
+__global__ __launch_bounds__(256) void k_cinsert(CombineIn in, CombineWork w) {
+  __shared__ uint32_t s_mask[4][AM_MAX_WORDS];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int r = blockIdx.x * 4 + wid;
+  const bool live = r < in.R;
+  const int nv = live ? in.info[(size_t)r * AM_INFO_STRIDE + 2] : 0;
+  uint64_t g = 0;
+  if (live && lane < in.Wd) {
+    const uint32_t x = in.reg_mask[(size_t)r * in.Wd + lane];
+    s_mask[wid][lane] = x;
+    g = word_hash(x, lane);
+  }
+  // wave sum of the per-word hashes (mod 2^64)
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)g, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(g >> 32), d, 64);
+    g += ((uint64_t)hi << 32) | lo;
+  }
+  __syncthreads();
+  if (!live || nv < 3 || lane >= nv) return;
+  const uint32_t* m = s_mask[wid];
+  const int item = (int)(w.roff[r] & 0xffffffffu) + lane;
+  const int a = in.vpair[((size_t)r * in.vm + lane) * 2 + 0], b = in.vpair[((size_t)r * in.vm + lane) * 2 + 1];
+  const int lo = a < b ? a : b, hi = a < b ? b : a;
+  // key hash: the region's word-hash sum with the words holding bits lo and hi cleared
+  uint64_t h = g;
+  const bool lo_in = lo >= 0 && lo < in.L, hi_in = hi >= 0 && hi < in.L;
+  const int wl = lo_in ? lo >> 5 : -1, wh = hi_in ? hi >> 5 : -1;
+  if (wl >= 0) {
+    uint32_t x = m[wl] & ~(1u << (lo & 31));
+    if (wh == wl) x &= ~(1u << (hi & 31));
+    h += word_hash(x, wl) - word_hash(m[wl], wl);
+  }
+  if (wh >= 0 && wh != wl) h += word_hash(m[wh] & ~(1u << (hi & 31)), wh) - word_hash(m[wh], wh);
+  h = mix64(h ^ pair_hash(lo, hi));
+  uint32_t s = (uint32_t)h & w.cap_mask;
+  for (uint32_t probes = 0; probes <= w.cap_mask; ++probes) {
+    const uint32_t prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
+    if (prev == AM_CEMPTY) break;
+    // compare keys with the slot's item (records are immutable here; any item
+    // the slot holds has the key of the item that claimed it)
+    const int r2 = w.item_region[prev];
+    const int k2 = (int)prev - (int)(w.roff[r2] & 0xffffffffu);
+    const int a2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 0], b2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 1];
+    bool eq = (a2 < b2 ? a2 : b2) == lo && (a2 < b2 ? b2 : a2) == hi;
+    if (eq) {
+      const uint32_t* m2 = in.reg_mask + (size_t)r2 * in.Wd;
+      for (int q = 0; q < in.Wd && eq; ++q) {
+        uint32_t x = m[q], y = m2[q];
+        if (q == wl) { x &= ~(1u << (lo & 31)); y &= ~(1u << (lo & 31)); }
+        if (q == wh) { x &= ~(1u << (hi & 31)); y &= ~(1u << (hi & 31)); }
+        eq = x == y;
+      }
+    }
+    if (eq) {
+      atomicMin(&w.table[s], (uint32_t)item);
+      break;
+    }
+    s = (s + 1) & w.cap_mask;
+  }
+  w.slot[item] = (int)s;
+}
+
+__global__ void k_cflag(CombineWork w, int N) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) w.vflag[i] = w.table[w.slot[i]] == (uint32_t)i ? 1u : 0u;
+}
+
+__global__ void k_cassign(CombineIn in, CombineWork w, CombineOut out, int N, double scale, double cx, double cy,
+                          double cz) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int r = w.item_region[i];
+  const unsigned long long ro = w.roff[r];
+  const int k = i - (int)(ro & 0xffffffffu);
+  const uint32_t owner = w.table[w.slot[i]];
+  const int id = (int)w.vrank[owner];
+  out.face_idx[i] = id;
+  if (owner == (uint32_t)i) {
+    const double* v = in.verts + ((size_t)r * in.vm + k) * 3;
+    out.xyz[(size_t)id * 3 + 0] = v[0] * scale + cx;
+    out.xyz[(size_t)id * 3 + 1] = v[1] * scale + cy;
+    out.xyz[(size_t)id * 3 + 2] = v[2] * scale + cz;
+  }
+  if (k == 0) out.face_off[(int)(ro >> 32)] = i;
+  if (i == 0) out.face_off[out.F] = N;
+}
+
+__global__ void k_cfill(uint32_t* p, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = AM_CEMPTY;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t combine_table_slots(long long N) {
+  size_t cap = 1024;
+  while (cap < (size_t)2 * (size_t)N) cap *= 2;
+  return cap;
+}
+
+size_t combine_region_bytes(long long R) {
+  return align256((size_t)R * 8) + align256((size_t)(scan_blocks((size_t)R) + 1) * 8) + 256;
+}
+
+size_t combine_item_bytes(long long N) {
+  const size_t n = (size_t)std::max<long long>(N, 1);
+  return align256(combine_table_slots(N) * 4) + 4 * align256(n * 4) + align256((scan_blocks(n) + 1) * 4) + 256;
+}
+
+void combine_carve_regions(char* base, long long R, CombineWork& w) {
+  size_t o = 0;
+  w.roff = (unsigned long long*)(base + o);
+  o += align256((size_t)R * 8);
+  w.rpart = (unsigned long long*)(base + o);
+  o += align256((size_t)(scan_blocks((size_t)R) + 1) * 8);
+  w.totals = (unsigned long long*)(base + o);   // [0]: packed (faces << 32 | items), [1]: vertices
+}
+
+void combine_carve_items(char* base, long long N, CombineWork& w) {
+  const size_t n = (size_t)std::max<long long>(N, 1);
+  const size_t cap = combine_table_slots(N);
+  size_t o = 0;
+  w.table = (uint32_t*)(base + o);
+  o += align256(cap * 4);
+  w.cap_mask = (uint32_t)(cap - 1);
+  w.item_region = (int*)(base + o);
+  o += align256(n * 4);
+  w.slot = (int*)(base + o);
+  o += align256(n * 4);
+  w.vflag = (unsigned*)(base + o);
+  o += align256(n * 4);
+  w.vrank = (unsigned*)(base + o);
+  o += align256(n * 4);
+  w.ipart = (unsigned*)(base + o);
+}
+
+void combine_count(const CombineIn& in, CombineWork& w, hipStream_t st) {
+  scan_launch<unsigned long long>(RegionCountSrc{in.info}, (size_t)in.R, ArraySink<unsigned long long>{w.roff},
+                                  w.rpart, w.totals, st);
+}
+
+void combine_dedup(const CombineIn& in, CombineWork& w, long long N, hipStream_t st) {
+  const size_t cap = (size_t)w.cap_mask + 1;
+  hipLaunchKernelGGL(k_citem_region, dim3((unsigned)((in.R + 255) / 256)), dim3(256), 0, st,
+                     (const unsigned long long*)w.roff, in.info, in.R, w.item_region);
+  hipLaunchKernelGGL(k_cfill, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, w.table, cap);
+  hipLaunchKernelGGL(k_cinsert, dim3((unsigned)((in.R + 3) / 4)), dim3(256), 0, st, in, w);
+  hipLaunchKernelGGL(k_cflag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, w, (int)N);
+  scan_launch<unsigned>(FlagSrc{w.vflag}, (size_t)N, ArraySink<unsigned>{w.vrank}, w.ipart,
+                        (unsigned*)(w.totals + 1), st);
+}
+
+void combine_assign(const CombineIn& in, const CombineWork& w, const CombineOut& out, long long N, double scale,
+                    const double* c3, hipStream_t st) {
+  hipLaunchKernelGGL(k_cassign, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, in, w, out, (int)N, scale,
+                     c3[0], c3[1], c3[2]);
+}
+
+}  // namespace am

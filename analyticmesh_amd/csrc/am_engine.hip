@@ -8,7 +8,6 @@
 // (backend/src/cuam_kernel.cu:53-98); both orders reach the same regions.
 #include <cstring>
 #include <hip/hip_runtime.h>
-#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -26,6 +25,8 @@
 #include "am_kernels.h"
 #include "am_fast.h"
 #include "am_adjoint.h"
+#include "am_combine.h"
+#include "am_scan.h"
 
 using namespace am;
 
@@ -112,6 +113,7 @@ struct am_ctx {
   int dev = 0;
   size_t max_lds = 65536;
   long long max_regions = -1;
+  long long region_limit = -1;   // AM_ERR_CAPACITY past this many regions (-1: the engine's own limit)
   bool adjoint_tiled = true;   // am_adjoint: tiled chain + GEMM pull-back where the tiled chain runs
   bool adjoint_mfma = true;    // am_adjoint's GEMMs on the FP64 matrix cores (else VALU fmas)
   int table_log2 = 22;
@@ -133,9 +135,9 @@ struct am_ctx {
 
   // level candidates
   DevBuf<uint32_t> cand_mask;
-  DevBuf<int> cand_start, status, winner, slot, owner, first, link, rank, scan_tmp_i;
+  DevBuf<int> cand_start, status, winner, slot, owner, first, link, rank;
   DevBuf<double> cand_point;
-  DevBuf<char> scan_tmp;
+  DevBuf<int> scan_part;
   DevBuf<unsigned> rflags;
 
   // visited table
@@ -144,10 +146,11 @@ struct am_ctx {
 
   // mesh
   bool has_run = false, has_mesh = false;
-  DevBuf<int> m_vfirst, m_vid;   // per (region, vertex slot): first occurrence flag, vertex index
-  // am_combine's work buffers (kept between calls) and the mesh in device memory
-  DevBuf<uint32_t> m_vt;
-  DevBuf<int> m_vstat, m_vwin, m_vown, m_vrank, m_fflag, m_fnv, m_frank, m_foff, m_face_off, m_face_idx;
+  // am_combine's workspace (kept between calls: the adjoint reads its item
+  // offsets and first-occurrence flags) and the mesh in device memory
+  DevBuf<char> cw_reg, cw_item;
+  CombineWork cwork;
+  DevBuf<int> m_face_off, m_face_idx;
   DevBuf<double> m_xyz;
   long long mesh_V = 0, mesh_F = 0, mesh_NI = 0;
   // host copy of the mesh, made on first need (am_export_ply)
@@ -159,7 +162,7 @@ struct am_ctx {
   std::deque<DevBuf<char>> stage;
   size_t stage_used = 0;
 
-  // pinned, coherent host word the device writes scan totals into (scan_total)
+  // pinned, coherent host word the device writes scan totals into (scan_flags_to_host)
   int* host_total = nullptr;
   int* host_total_dev = nullptr;
 
@@ -208,56 +211,36 @@ void grow_cands(am_ctx* c, size_t n) {
   ensure(c->rank, n, false, st);
 }
 
-void exclusive_scan(am_ctx* c, const int* in, int* out, size_t n) {
-  size_t bytes = 0;
-  HIPCHK(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, n, rocprim::plus<int>(), c->stream));
-  ensure(c->scan_tmp, bytes + 16, false, c->stream);
-  HIPCHK(rocprim::exclusive_scan(c->scan_tmp.p, bytes, in, out, 0, n, rocprim::plus<int>(), c->stream));
-}
-
-// total = rank[n - 1] + flags[n - 1], written straight into host memory
-__global__ void k_scan_total(const int* rank, const int* flags, size_t n, int* out) {
-  if (threadIdx.x == 0) *out = rank[n - 1] + flags[n - 1];
-}
-
-// total = rank[n - 1] + flags[n - 1] of a finished exclusive scan.  The total
-// reaches the host through a coherent host word.  With `poll` the host spins
-// on that word (a stream sync after 100 ms, which also reports a failed
-// kernel) instead of waiting for the stream: the kernels ordered before
-// k_scan_total are then complete, k_scan_total itself may not be.  Callers
-// that read device memory with synchronous copies afterwards must not poll.
-int read_total(am_ctx* c, const int* flags, const int* rank, size_t n, bool poll) {
-  if (n == 0) return 0;
+// Exclusive scan of n flags into rank (hand-written scan, am_scan.h).  The
+// total is written by the device straight into a pinned, coherent host word,
+// which the host then polls (wait_total): a short tight spin, then a yield
+// between polls so that host threads driving other contexts (a batch of
+// networks) keep their cores, and a stream sync after 100 ms, which also
+// reports a failed kernel.  Kernels enqueued after the scan may still run when
+// the word is read; later work on the stream stays ordered behind them.
+void scan_flags_to_host(am_ctx* c, const int* flags, int* rank, size_t n) {
   if (!c->host_total) {
     HIPCHK(hipHostMalloc((void**)&c->host_total, 256, hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer((void**)&c->host_total_dev, c->host_total, 0));
   }
-  volatile int* h = reinterpret_cast<volatile int*>(c->host_total);
-  *h = -1;
-  hipLaunchKernelGGL(k_scan_total, dim3(1), dim3(64), 0, c->stream, rank, flags, n, c->host_total_dev);
+  *reinterpret_cast<volatile int*>(c->host_total) = -1;
+  ensure(c->scan_part, scan_blocks(n) + 1, false, c->stream);
+  scan_launch<int>(ArraySrc<int>{flags}, n, ArraySink<int>{rank}, c->scan_part.p, c->host_total_dev, c->stream);
   HIPCHK(hipGetLastError());
-  if (poll) {
-    // a short tight spin, then a yield between polls so that host threads
-    // driving other contexts (a batch of networks) keep their cores
-    const auto t0 = std::chrono::steady_clock::now();
-    for (unsigned spins = 0; *h < 0; ++spins) {
-      if (spins < 4096) continue;
-      if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100)) {
-        HIPCHK(hipStreamSynchronize(c->stream));
-        break;
-      }
-      std::this_thread::yield();
-    }
-  } else {
-    HIPCHK(hipStreamSynchronize(c->stream));
-  }
-  return *h;
 }
 
-int scan_total(am_ctx* c, const int* flags, int* rank, size_t n) {
-  if (n == 0) return 0;
-  exclusive_scan(c, flags, rank, n);
-  return read_total(c, flags, rank, n, false);
+int wait_total(am_ctx* c) {
+  volatile int* h = reinterpret_cast<volatile int*>(c->host_total);
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned spins = 0; *h < 0; ++spins) {
+    if (spins < 4096) continue;
+    if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(100)) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      break;
+    }
+    std::this_thread::yield();
+  }
+  return *h;
 }
 
 void ensure_table(am_ctx* c, size_t entries) {
@@ -312,11 +295,15 @@ int insert_level(am_ctx* c, size_t n) {
                        c->reg_point.p, c->table.p, (int)n, c->Wd, (int)c->R);
     HIPCHK(hipGetLastError());
   };
-  exclusive_scan(c, c->first.p, c->rank.p, n);
+  scan_flags_to_host(c, c->first.p, c->rank.p, n);
   if (early) commit();
-  const int total = read_total(c, c->first.p, c->rank.p, n, true);
+  const int total = wait_total(c);
   if (total == 0) return 0;
-  if ((long long)c->R + total > (long long)INT_MAX / 2) throw CapErr{"region count exceeds 2^30"};
+  // am_combine indexes (region, vertex slot) items with 32-bit ints: R * vm must stay below 2^31
+  long long limit = std::min<long long>((long long)INT_MAX / 2, (long long)INT_MAX / c->vm);
+  if (c->region_limit >= 0) limit = std::min(limit, c->region_limit);
+  if ((long long)c->R + total > limit)
+    throw CapErr{"region count would exceed the limit of " + std::to_string(limit) + " regions"};
   if (!early) {
     grow_regions(c, (size_t)c->R + total);
     commit();
@@ -678,12 +665,22 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   if (k == "vm") {
     if (value < 3 || value > AM_VM_MAX) return fail(c, AM_ERR_ARG, "vm must be in [3, 64]");
     if (c->has_run) return fail(c, AM_ERR_STATE, "vm must be set before the first am_run");
+    if (value != c->vm) {
+      // the region store's tab, vertex and pair arrays are strided by vm: drop
+      // them (a failed am_run may have sized them already)
+      c->reg_mask.release(); c->reg_start.release(); c->reg_point.release(); c->reg_info.release();
+      c->reg_tab.release(); c->reg_verts.release(); c->reg_vpair.release();
+      c->reg_cap = 0;
+      c->R = 0;
+    }
     c->vm = (int)value;
   } else if (k == "table_log2") {
     if (value < 10 || value > 31) return fail(c, AM_ERR_ARG, "table_log2 must be in [10, 31]");
     c->table_log2 = (int)value;
   } else if (k == "max_regions") {
     c->max_regions = value;
+  } else if (k == "region_limit") {
+    c->region_limit = value;
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
@@ -822,7 +819,11 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)clds));
     }
-    const bool fp64_form = c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY;
+    // The reduced forms need the tiled FP64 chain for their vertices and
+    // re-decisions; nets it does not take (skips, layers over 256 wide) run
+    // those forms on the FP64 region kernel, with the same topology.
+    const bool reduced_asked = c->precision == AM_PREC_FP32 || c->precision == AM_PREC_SPLIT_BF16X3;
+    const bool fp64_form = !reduced_asked || !c->tile_ok;
     if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, wv.data(), bv.data(), dtype, st);
     if (c->tile_ok && ray64_lds_bytes(c) > 65536 && ray64_lds_bytes(c) <= c->max_lds)
       HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -945,10 +946,41 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_VERT64_MS] = T[T_VERT64];
     c->stats[AM_STAT_VERT64_REGIONS] = (double)vert64;
     c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
+    c->stats[AM_STAT_FP64_ONLY] = reduced_asked && fp64_form ? 1.0 : 0.0;
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;
     return AM_OK;
   });
+}
+
+// Region scan of the mesh assembly; returns (faces, items).
+static void combine_counts(am_ctx* c, const CombineIn& in, CombineWork& w, long long& F, long long& N) {
+  combine_count(in, w, c->stream);
+  unsigned long long tot = 0;
+  HIPCHK(hipMemcpyAsync(&tot, w.totals, sizeof(tot), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  F = (long long)(tot >> 32);
+  N = (long long)(tot & 0xffffffffu);
+}
+
+static long long combine_vertices(am_ctx* c, const CombineWork& w) {
+  unsigned v = 0;
+  HIPCHK(hipMemcpyAsync(&v, (const unsigned*)(w.totals + 1), sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return (long long)v;
+}
+
+static CombineIn combine_in(const am_ctx* c) {
+  CombineIn in;
+  in.reg_mask = c->reg_mask.p;
+  in.info = c->reg_info.p;
+  in.vpair = c->reg_vpair.p;
+  in.verts = c->reg_verts.p;
+  in.R = (int)c->R;
+  in.Wd = c->Wd;
+  in.vm = c->vm;
+  in.L = c->L;
+  return in;
 }
 
 am_status am_combine(am_ctx* c, double scale, const double* center3) {
@@ -957,76 +989,44 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
   return guarded(c, [&]() -> am_status {
     const auto t0 = std::chrono::steady_clock::now();
     const hipStream_t st = c->stream;
-    const double cx = center3 ? center3[0] : 0.0, cy = center3 ? center3[1] : 0.0, cz = center3 ? center3[2] : 0.0;
+    const double c3[3] = {center3 ? center3[0] : 0.0, center3 ? center3[1] : 0.0, center3 ? center3[2] : 0.0};
     const size_t R = (size_t)c->R;
-    const size_t n = R * c->vm;
+    if (R * c->vm > (size_t)INT_MAX / 2) throw CapErr{"too many (region, vertex) slots for am_combine"};
     c->mesh_v.clear();
     c->mesh_off.assign(1, 0);
     c->mesh_idx.clear();
     c->mesh_V = c->mesh_F = c->mesh_NI = 0;
     c->mesh_on_host = false;
+    c->has_mesh = false;
     if (R == 0) {
       c->has_mesh = true;
       c->mesh_on_host = true;
       return AM_OK;
     }
-    size_t cap = 1024;
-    while (cap < 2 * n) cap *= 2;
-    DevBuf<uint32_t>& vt = c->m_vt;
-    DevBuf<int>&vstat = c->m_vstat, &vwin = c->m_vwin, &vown = c->m_vown, &vrank = c->m_vrank, &fflag = c->m_fflag,
-       &fnv = c->m_fnv, &frank = c->m_frank, &foff = c->m_foff, &face_off = c->m_face_off,
-       &face_idx = c->m_face_idx;
-    DevBuf<int>& vfirst = c->m_vfirst;
-    DevBuf<int>& vid = c->m_vid;
-    DevBuf<double>& xyz = c->m_xyz;
-    ensure(vt, cap, false, st);
-    ensure(vstat, n, false, st);
-    ensure(vwin, n, false, st);
-    ensure(vown, n, false, st);
-    ensure(vfirst, n, false, st);
-    ensure(vrank, n, false, st);
-    ensure(vid, n, false, st);
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks(cap, 256)), dim3(256), 0, st, vt.p, cap, 0xFFFFFFFFu);
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, vwin.p, n, INT_MAX);
-    hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, vfirst.p, n, 0);
-    VertexIO io;
-    io.reg_mask = c->reg_mask.p;
-    io.info = c->reg_info.p;
-    io.vpair = c->reg_vpair.p;
-    io.verts = c->reg_verts.p;
-    io.table = vt.p;
-    io.status = vstat.p;
-    io.winner = vwin.p;
-    io.owner = vown.p;
-    io.n = (int)n;
-    io.vm = c->vm;
-    io.L = c->L;
-    io.Wd = c->Wd;
-    io.cap_mask = (uint32_t)(cap - 1);
-    hipLaunchKernelGGL(k_vinsert, dim3(blocks(n, 256)), dim3(256), 0, st, io);
-    hipLaunchKernelGGL(k_vmark_first, dim3(blocks(n, 256)), dim3(256), 0, st, vstat.p, vwin.p, vfirst.p, (int)n);
-    HIPCHK(hipGetLastError());
-    const int V = scan_total(c, vfirst.p, vrank.p, n);
-    ensure(xyz, std::max<size_t>((size_t)V * 3, 1), false, st);
-    hipLaunchKernelGGL(k_vassign, dim3(blocks(n, 256)), dim3(256), 0, st, io, vfirst.p, vrank.p, vid.p, xyz.p,
-                       scale, cx, cy, cz);
-    ensure(fflag, R, false, st);
-    ensure(fnv, R, false, st);
-    ensure(frank, R, false, st);
-    ensure(foff, R, false, st);
-    hipLaunchKernelGGL(k_face_flags, dim3(blocks(R, 256)), dim3(256), 0, st, c->reg_info.p, (int)R, fflag.p, fnv.p);
-    HIPCHK(hipGetLastError());
-    const int F = scan_total(c, fflag.p, frank.p, R);
-    const int NI = scan_total(c, fnv.p, foff.p, R);
-    ensure(face_off, std::max(F, 1), false, st);
-    ensure(face_idx, std::max(NI, 1), false, st);
-    hipLaunchKernelGGL(k_face_write, dim3(blocks(R, 256)), dim3(256), 0, st, c->reg_info.p, fflag.p, frank.p, foff.p,
-                       vid.p, (int)R, c->vm, face_off.p, face_idx.p);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(st));
+    const CombineIn in = combine_in(c);
+    CombineWork& w = c->cwork;
+    w = CombineWork{};
+    ensure(c->cw_reg, combine_region_bytes((long long)R), false, st);
+    combine_carve_regions(c->cw_reg.p, (long long)R, w);
+    long long F = 0, N = 0;
+    combine_counts(c, in, w, F, N);
+    long long V = 0;
+    if (N > 0) {
+      ensure(c->cw_item, combine_item_bytes(N), false, st);
+      combine_carve_items(c->cw_item.p, N, w);
+      combine_dedup(in, w, N, st);
+      V = combine_vertices(c, w);
+      ensure(c->m_xyz, (size_t)std::max<long long>(V * 3, 1), false, st);
+      ensure(c->m_face_off, (size_t)F + 1, false, st);
+      ensure(c->m_face_idx, (size_t)N, false, st);
+      CombineOut out{c->m_xyz.p, c->m_face_off.p, c->m_face_idx.p, (int)F};
+      combine_assign(in, w, out, N, scale, c3, st);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipStreamSynchronize(st));
+    }
     c->mesh_V = V;
     c->mesh_F = F;
-    c->mesh_NI = NI;
+    c->mesh_NI = N;
     c->has_mesh = true;
     c->stats[AM_STAT_COMBINE_MS] =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1100,8 +1100,9 @@ am_status am_get_regions(am_ctx* c, uint32_t* masks, int* info, int* tabs, doubl
 
 am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, float* eq) {
   if (!c || !masks || !planes || !eq || n < 1) return AM_ERR_ARG;
-  if (!c->has_run || c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY)
-    return fail(c, AM_ERR_STATE, "am_planes needs a reduced-precision context after am_run");
+  if (!c->has_run || c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY || !c->tile_ok)
+    return fail(c, AM_ERR_STATE, "am_planes needs a reduced-precision context after am_run (on a net the "
+                                 "reduced forms take: no skips, layers up to 256 wide)");
   return guarded(c, [&]() -> am_status {
     const hipStream_t st = c->stream;
     const int Cs = c->fast.dev.Cs;
@@ -1118,6 +1119,24 @@ am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, floa
       for (int q = 0; q < 4; ++q)
         std::memcpy(planes + ((size_t)r * 4 + q) * c->C, hp.data() + ((size_t)r * 4 + q) * Cs, (size_t)c->C * 4);
     HIPCHK(hipMemcpy(eq, e.p, (size_t)n * 4 * 4, hipMemcpyDeviceToHost));
+    return AM_OK;
+  });
+}
+
+am_status am_topology_fingerprint(am_ctx* c, unsigned long long* out5) {
+  if (!c || !out5) return AM_ERR_ARG;
+  if (!c->has_run) return fail(c, AM_ERR_STATE, "am_topology_fingerprint called before am_run");
+  return guarded(c, [&]() -> am_status {
+    const hipStream_t st = c->stream;
+    DevBuf<unsigned long long> acc;
+    ensure(acc, 5, false, st);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t*)acc.p, (size_t)10, 0u);
+    if (c->R > 0)
+      hipLaunchKernelGGL(k_fingerprint, dim3(blocks((size_t)c->R, 256)), dim3(256), 0, st, c->reg_mask.p,
+                         c->reg_info.p, c->reg_vpair.p, (int)c->R, c->Wd, c->vm, c->L, acc.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out5, acc.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
     return AM_OK;
   });
 }
@@ -1159,7 +1178,14 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
     // tiled: planes from the tiled chain, the pull-back as GEMMs over all
     // regions of a chunk; else one kernel per region does chain and pull-back
     const bool tiled = c->tile_ok && c->adjoint_tiled;
-    const int CH = 65536;
+    // regions per chunk: the chunk's planes (4 x Ch doubles per region) and
+    // its G and P arrays (4 x the layer widths, twice) each stay within 1 GiB
+    const int Chp = (c->C + 63) & ~63;
+    long long widths = 0;
+    for (int li = 0; li <= c->D + 1; ++li) widths += c->nodes[li];
+    long long chl = std::min<long long>(65536, ((long long)1 << 30) / (32LL * Chp));
+    chl = std::min<long long>(chl, ((long long)1 << 30) / (64LL * widths));
+    const int CH = (int)std::max<long long>(AM_CT, chl & ~(long long)(AM_CT - 1));
     const int chunk = std::min(R, CH);
     for (int li = 0; li < nl; ++li) {
       ensure(Gst[li], (size_t)chunk * 4 * c->nodes[li + 1], false, st);
@@ -1193,8 +1219,9 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
       io.reg_mask = c->reg_mask.p;
       io.info = c->reg_info.p;
       io.vpair = c->reg_vpair.p;
-      io.vfirst = c->m_vfirst.p;
-      io.vid = c->m_vid.p;
+      io.item_off = c->cwork.roff;
+      io.vfirst = c->cwork.vflag;
+      io.vid = c->m_face_idx.p;
       io.gradv = gv.p;
       io.r0 = r0;
       io.count = cnt;
@@ -1330,7 +1357,7 @@ am_status am_destroy(am_ctx* c) {
   c->timer.release();
   fast_release(c->fbuf);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
-  if (c->host_total) (void)hipHostFree(c->host_total);   // every scan_total waited for its write
+  if (c->host_total) (void)hipHostFree(c->host_total);   // every level waited for its total
   hipStream_t own = c->own_stream;
   delete c;
   if (own) (void)hipStreamDestroy(own);

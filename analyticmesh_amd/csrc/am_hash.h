@@ -38,4 +38,7 @@ __global__ void k_face_flags(const int* info, int R, int* fflag, int* fnv);
 __global__ void k_face_write(const int* info, const int* fflag, const int* frank, const int* foff, const int* vid,
                              int R, int vm, int* face_off, int* face_idx);
 
+__global__ void k_fingerprint(const uint32_t* reg_mask, const int* info, const int* vpair, int R, int Wd, int vm,
+                              int L, unsigned long long* out);
+
 }  // namespace am

@@ -213,3 +213,89 @@ __global__ void k_face_write(const int* info, const int* fflag, const int* frank
 }
 
 }  // namespace am
+
+namespace am {
+
+// Topology fingerprint of the region store: order-independent sums (mod 2^64)
+// of a per-row hash over three kinds of int64 rows -- the visited masks, the
+// canonical faces (mask words, vertex count, the vertex pair keys lo<<32|hi
+// rotated to start at the smallest, -1 past the count) and one vertex-key row
+// per vertex use (lo, hi, mask without bits lo and hi).  The same rows and hash
+// as analyticmesh_amd/canonical.py, so a GPU run can be compared with the
+// oracle's fingerprint at full size without copying its records to the host.
+struct RowHash {
+  uint64_t h = 0x243F6A8885A308D3ULL;
+  __device__ __forceinline__ void col(int64_t v) {
+    h = (h ^ (uint64_t)v) * 0x9E3779B97F4A7C15ULL;
+    h ^= h >> 31;
+  }
+  __device__ __forceinline__ uint64_t done() const {
+    uint64_t x = (h ^ (h >> 33)) * 0xFF51AFD7ED558CCDULL;
+    return x ^ (x >> 33);
+  }
+};
+
+// out[0..2]: visited, faces, vertex-key sums; out[3], out[4]: faces, vertex uses.
+__global__ __launch_bounds__(256) void k_fingerprint(const uint32_t* reg_mask, const int* info, const int* vpair,
+                                                     int R, int Wd, int vm, int L, unsigned long long* out) {
+  __shared__ unsigned long long red[5][256];
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t acc[5] = {0, 0, 0, 0, 0};
+  if (r < R) {
+    const uint32_t* m = reg_mask + (size_t)r * Wd;
+    RowHash hv;
+    for (int w = 0; w < Wd; ++w) hv.col((int64_t)m[w]);
+    acc[0] = hv.done();
+    const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
+    if (nv >= 3) {
+      const int* vp = vpair + (size_t)r * vm * 2;
+      int64_t best = INT64_MAX;
+      int i0 = 0;
+      for (int k = 0; k < nv; ++k) {
+        const int a = vp[2 * k], b = vp[2 * k + 1];
+        const int64_t key = ((int64_t)min(a, b) << 32) | (int64_t)max(a, b);
+        if (key < best) { best = key; i0 = k; }
+      }
+      RowHash hf;
+      for (int w = 0; w < Wd; ++w) hf.col((int64_t)m[w]);
+      hf.col(nv);
+      int k = i0;
+      for (int s = 0; s < vm; ++s) {
+        if (s < nv) {
+          const int a = vp[2 * k], b = vp[2 * k + 1];
+          hf.col(((int64_t)min(a, b) << 32) | (int64_t)max(a, b));
+          k = k + 1 == nv ? 0 : k + 1;
+        } else {
+          hf.col(-1);
+        }
+      }
+      acc[1] = hf.done();
+      acc[3] = 1;
+      for (int q = 0; q < nv; ++q) {
+        const int a = vp[2 * q], b = vp[2 * q + 1];
+        const int lo = min(a, b), hi = max(a, b);
+        RowHash hk;
+        hk.col(lo);
+        hk.col(hi);
+        for (int w = 0; w < Wd; ++w) {
+          uint32_t x = m[w];
+          if (lo >= 0 && lo < L && (lo >> 5) == w) x &= ~(1u << (lo & 31));
+          if (hi >= 0 && hi < L && (hi >> 5) == w) x &= ~(1u << (hi & 31));
+          hk.col((int64_t)x);
+        }
+        acc[2] += hk.done();
+      }
+      acc[4] = (uint64_t)nv;
+    }
+  }
+  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 5) atomicAdd(&out[threadIdx.x], red[threadIdx.x][0]);
+}
+
+}  // namespace am

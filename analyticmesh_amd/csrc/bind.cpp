@@ -5,6 +5,7 @@
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <cstdio>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,7 @@ void check(am_ctx* c, am_status s) {
 struct Engine {
   am_ctx* c = nullptr;
   int vm = 20;
+  int n_linear = 0;
 
   Engine(const std::vector<int>& nodes, py::array_t<int, py::array::c_style | py::array::forcecast> arc,
          int num_extra, int precision) {
@@ -29,6 +31,7 @@ struct Engine {
     const am_status s = am_create(&c, (int)nodes.size(), nodes.data(), arc.data(), (int)arc.shape(0),
                                   (int)arc.shape(1), num_extra, precision);
     if (s != AM_OK) throw std::runtime_error("am error " + std::to_string(s) + ": " + am_last_error(nullptr));
+    n_linear = (int)nodes.size() - 1;
   }
   ~Engine() {
     if (c) am_destroy(c);
@@ -114,6 +117,10 @@ struct Engine {
   }
 
   void adjoint(uintptr_t gradv, int dtype, const std::vector<uintptr_t>& gw, const std::vector<uintptr_t>& gb) {
+    // am_adjoint writes one array per linear layer into each list
+    if (gw.size() != (size_t)n_linear || gb.size() != (size_t)n_linear)
+      throw std::runtime_error("adjoint needs one weight and one bias gradient buffer per linear layer (" +
+                               std::to_string(n_linear) + ")");
     std::vector<void*> w(gw.size()), b(gb.size());
     for (size_t i = 0; i < w.size(); ++i) w[i] = (void*)gw[i];
     for (size_t i = 0; i < b.size(); ++i) b[i] = (void*)gb[i];
@@ -131,6 +138,23 @@ struct Engine {
     py::array_t<float> E({(py::ssize_t)n, (py::ssize_t)4});
     check(c, am_planes(c, masks.data(), n, P.mutable_data(), E.mutable_data()));
     return py::make_tuple(P, E);
+  }
+
+  py::dict fingerprint() {
+    unsigned long long f[5];
+    check(c, am_topology_fingerprint(c, f));
+    long long r, nf, nv, ni;
+    check(c, am_get_counts(c, &r, &nf, &nv, &ni));
+    char buf[3][17];
+    for (int i = 0; i < 3; ++i) std::snprintf(buf[i], sizeof(buf[i]), "%016llx", f[i]);
+    py::dict d;
+    d["regions"] = r;
+    d["faces"] = (long long)f[3];
+    d["vertex_uses"] = (long long)f[4];
+    d["visited"] = std::string(buf[0]);
+    d["faces_hash"] = std::string(buf[1]);
+    d["vertex_keys"] = std::string(buf[2]);
+    return d;
   }
 
   py::array_t<double> stats() {
@@ -164,5 +188,6 @@ PYBIND11_MODULE(_am, m) {
       .def("regions", &Engine::regions)
       .def("adjoint", &Engine::adjoint)
       .def("planes", &Engine::planes)
+      .def("fingerprint", &Engine::fingerprint)
       .def("stats", &Engine::stats);
 }

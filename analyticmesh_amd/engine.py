@@ -90,6 +90,11 @@ class MarchResult:
             return self._regions[name]
         raise AttributeError(name)
 
+    def fingerprint(self):
+        """Topology fingerprint computed on the device (see canonical.fingerprint)."""
+        self._check_current()
+        return dict(self._eng.fingerprint())
+
     def records(self):
         return RegionRecords(self.masks, self.info, self.tabs, self.verts, self.vpairs, L=self.L)
 
@@ -115,6 +120,15 @@ class MarchResult:
         layer_shapes: [(out, in), ...] per linear layer.  Returns (dW list, db list), float64."""
         self._check_current()
         g = np.ascontiguousarray(grad_vertices, dtype=np.float64)
+        nv = int(self._eng.counts()[2])
+        if g.shape != (nv, 3):
+            raise ValueError(f"grad_vertices must be ({nv}, 3), got {g.shape}")
+        layer_shapes = [tuple(int(x) for x in s) for s in layer_shapes]
+        if self._owner is not None:
+            n = self._owner.nodes
+            want = [(n[i + 1], n[i]) for i in range(len(n) - 1)]
+            if layer_shapes != want:
+                raise ValueError(f"layer_shapes must be {want}, got {layer_shapes}")
         dW = [np.zeros(s, dtype=np.float64) for s in layer_shapes]
         db = [np.zeros(s[0], dtype=np.float64) for s in layer_shapes]
         self._eng.adjoint(int(g.ctypes.data), 1, [int(a.ctypes.data) for a in dW], [int(a.ctypes.data) for a in db])
@@ -203,24 +217,31 @@ class Engine:
 
 
 def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip=False, max_regions=None,
-          table_log2=None, walk="ray"):
-    """Mesh one NetSpec from explicit seeds (points (N,3) float64, states (N,L) bool).
+          table_log2=None, walk="ray", extra_w=None, extra_b=None):
+    """Mesh one NetSpec (with its skips, if any) from explicit seeds (points (N,3) float64,
+    states (N,L) bool).
 
     walk: reduced-precision forms only; "ray" (ray shooting, default) or
-    "scan" (the reference's candidate scan with checked predicates)."""
-    eng = Engine(spec.nodes, None, 0, precision=precision, vm=vm, table_log2=table_log2, walk=walk)
+    "scan" (the reference's candidate scan with checked predicates).
+    extra_w (E,3), extra_b (E,): extra half-spaces a.x + c <= 0, used as given."""
+    E = 0 if extra_w is None else len(extra_w)
+    eng = Engine(spec.nodes, getattr(spec, "arc_table", None), E, precision=precision, vm=vm,
+                 table_log2=table_log2, walk=walk)
     if max_regions is not None:
         eng.set_option("max_regions", max_regions)
     pts = np.ascontiguousarray(seed_points, dtype=np.float64).reshape(-1, 3)
     st = np.ascontiguousarray(seed_states, dtype=np.uint8).reshape(len(pts), spec.L)
     W = [np.ascontiguousarray(w, dtype=np.float64) for w in spec.weights]
     B = [np.ascontiguousarray(b, dtype=np.float64) for b in spec.biases]
-    res = eng.run(W, B, pts, st, iso=iso, flip=flip)
+    tms = [np.ascontiguousarray(t, dtype=np.float64) for t in spec.tm_arrays()] if hasattr(spec, "tm_arrays") else []
+    ew = None if not E else np.ascontiguousarray(extra_w, dtype=np.float64).reshape(E, 3)
+    eb = None if not E else np.ascontiguousarray(extra_b, dtype=np.float64).reshape(E)
+    res = eng.run(W, B, pts, st, tms=tms, extra_w=ew, extra_b=eb, iso=iso, flip=flip)
     res.engine = eng
     return res
 
 
-def march_batch(specs, seeds, precision="fp32", workers=16, vm=20, walk="ray", combine=True):
+def march_batch(specs, seeds, precision="fp32", workers=16, vm=20, walk="ray", combine=True, max_regions=None):
     """Mesh many independent networks in one job.
 
     Every network gets its own engine context and HIP stream; a pool of host
@@ -234,7 +255,7 @@ def march_batch(specs, seeds, precision="fp32", workers=16, vm=20, walk="ray", c
     def one(i):
         spec = specs[i]
         pts, st = seeds[i]
-        r = march(spec, pts, st, precision=precision, vm=vm, walk=walk)
+        r = march(spec, pts, st, precision=precision, vm=vm, walk=walk, max_regions=max_regions)
         if combine:
             r.mesh = r.combine()
         return r

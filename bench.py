@@ -65,8 +65,24 @@ def run_form(spec, pts, states, precision, steps, warmup):
     dt = (time.perf_counter() - t0) / steps
     st = np.mean(stats, axis=0)
     V, _ = res.combine()                      # outside the timed region: the mesh for the checks below
-    faces = int((res.info[:, 2] >= 3).sum())
-    return dict(sec_per_step=dt, faces=faces, regions=int(st[7]), stats=st, res=res, eng=eng, V=V)
+    fp = res.fingerprint()
+    return dict(sec_per_step=dt, faces=int(fp["faces"]), regions=int(st[7]), stats=st, res=res, eng=eng, V=V,
+                fingerprint=fp)
+
+
+FP_KEYS = ("regions", "faces", "vertex_uses", "visited", "faces_hash", "vertex_keys")
+
+
+def same_fp(a, b):
+    return a is not None and b is not None and all(a[k] == b[k] for k in FP_KEYS)
+
+
+def golden_fingerprints():
+    path = os.path.join(ROOT, "tests", "golden", "topology_fingerprints.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def residual_fp64(spec, V):
@@ -145,7 +161,9 @@ def traffic_from_profile(kernel):
     return None
 
 
-def batch64_extra(n_nets, seeds_per_net):
+def batch64_extra(n_nets, seeds_per_net, golden):
+    """The 64-net batch workload meshed in one job per form; per-net topology against the oracle's
+    committed fingerprints and between the forms."""
     from analyticmesh_amd import configs
     from analyticmesh_amd.engine import march_batch
     specs = configs.batch64(0)[:n_nets]
@@ -153,18 +171,23 @@ def batch64_extra(n_nets, seeds_per_net):
     for s in specs:
         p = configs.seeds_fibonacci(s, seeds_per_net)
         seeds.append((p, configs.mask_of(s, p)))
-    out = {}
-    recs = {}
+    ref = (golden or {}).get("batch64")
+    out = {"workload": f"batch64(seed 0): {len(specs)} nets, depth U{{2..6}}, width U{{32,64,128,256}}, "
+                       f"{seeds_per_net} seeds each"}
+    fps = {}
     for f in ("fp32", "fp64"):
-        march_batch(specs[:2], seeds[:2], precision=f, workers=2)   # warm up
-        res, dt = march_batch(specs, seeds, precision=f, workers=16)
-        faces = sum(int((r.info[:, 2] >= 3).sum()) for r in res)
-        out[f] = {"faces": faces, "seconds": round(dt, 4), "faces_per_s": round(faces / dt, 1)}
-        recs[f] = [r.records() for r in res]
-    same = sum(1 for a, b in zip(recs["fp32"], recs["fp64"])
-               if a.visited() == b.visited() and a.faces() == b.faces())
+        march_batch(specs[:2], seeds[:2], precision=f, workers=2, combine=False)   # warm up
+        res, dt = march_batch(specs, seeds, precision=f, combine=True)
+        fps[f] = [r.fingerprint() for r in res]
+        faces = sum(int(x["faces"]) for x in fps[f])
+        out[f] = {"faces": faces, "seconds": round(dt, 4), "faces_per_s": round(faces / dt, 1),
+                  "levels": int(sum(float(r.stats[6]) for r in res)),
+                  "host_waits": int(sum(float(r.stats[6]) + 1 for r in res))}
+        if ref:
+            out[f]["nets_identical_to_oracle"] = sum(1 for a, b in zip(fps[f], ref) if same_fp(a, b))
+        del res
     out["nets"] = len(specs)
-    out["nets_with_identical_topology"] = same
+    out["nets_fp32_identical_to_fp64"] = sum(1 for a, b in zip(fps["fp32"], fps["fp64"]) if same_fp(a, b))
     return out
 
 
@@ -191,14 +214,18 @@ def adjoint_extra(reps=3):
 
 
 def cpu_baseline(spec, pts, states, max_regions):
+    """The oracle (FP64, the reference's LIFO order) timed on the host cores; returns the baseline
+    entry and the run's topology fingerprint (None when the run was cut short)."""
     import oracle
+    from analyticmesh_amd.canonical import fingerprint
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
     r = oracle.run_oracle(spec, pts, states, precision="fp64", traversal="lifo", max_regions=max_regions,
                           threads=threads)
     faces = int((r.info[:, 2] >= 3).sum())
+    fp = None if r.truncated else fingerprint(r.records())
     return dict(value=faces / r.seconds, unit="faces/s", cores=threads, kind="port",
                 sample=(f"{'first ' if r.truncated else 'all '}{len(r.masks)} regions of the LIFO march of the "
-                        f"same 4x256 net and seeds, FP64 (the reference default dtype), {r.seconds:.1f} s"))
+                        f"same 4x256 net and seeds, FP64 (the reference default dtype), {r.seconds:.1f} s")), fp
 
 
 def main():
@@ -211,7 +238,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--forms", default="fp32,bf16x3,fp64ray,fp64")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
-    ap.add_argument("--extras", action="store_true", help="also measure the 64-MLP batch")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-MLP batch")
     ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -275,15 +302,16 @@ def main():
     if out.get("fp64"):
         log("[bench] topology and vertex error against FP64")
         ref_res = out["fp64"]["res"]
-        ref = ref_res.records()
         for f in forms:
             if f != "fp64" and out.get(f):
-                r = out[f]["res"].records()
-                same = (r.visited() == ref.visited() and r.faces() == ref.faces()
-                        and r.vertex_keys() == ref.vertex_keys())
+                same = same_fp(out[f]["fingerprint"], out["fp64"]["fingerprint"])
                 topo[f] = "identical" if same else "DIFFERENT"
                 if same:
                     vert_err[f] = vertex_error_vs(out[f]["res"], ref_res, spec.L)
+    golden = golden_fingerprints()
+    gold = (golden or {}).get("deep_1024") if args.seeds == 1024 else None
+    topo_oracle = {f: ("identical" if same_fp(out[f]["fingerprint"], gold) else "DIFFERENT")
+                   for f in forms if out.get(f) and gold}
     resid = {f: residual_fp64(spec, out[f]["V"]) for f in forms if out.get(f)}
     # traversal (visited set) stage against HBM bandwidth
     tbytes, n_edges = traversal_bytes(h["res"], (spec.L + 31) // 32, spec.L)
@@ -335,20 +363,25 @@ def main():
             "levels": int(out[f]["stats"][6]),
         }) for f in forms},
         "topology_vs_fp64": topo,
+        "topology_vs_oracle_golden": topo_oracle,
         "roofline": roofline,
         "roofline_other": lines[1:],
         "traversal": traversal,
     }
     log("[bench] adjoint (backward) on the diff 3->128x4->1 net")
     line["adjoint"] = adjoint_extra()
-    if args.extras:
-        log("[bench] extras: 64-MLP batch")
-        line["batch64"] = batch64_extra(64, 16)
+    if not args.no_batch:
+        log("[bench] 64-MLP batch")
+        line["batch64"] = batch64_extra(64, 16, golden)
     if args.calib_copy_mb:
         line["copy_calibration"] = copy_calibration(args.calib_copy_mb)
     if not args.no_cpu:
         log(f"[bench] cpu baseline: oracle on the first {args.cpu_regions} regions")
-        line["cpu_baseline"] = cpu_baseline(spec, pts, states, args.cpu_regions)
+        line["cpu_baseline"], cpu_fp = cpu_baseline(spec, pts, states, args.cpu_regions)
+        if cpu_fp is not None:
+            line["topology_vs_cpu_baseline_run"] = {
+                f: ("identical" if same_fp(out[f]["fingerprint"], cpu_fp) else "DIFFERENT")
+                for f in forms if out.get(f)}
     print(json.dumps(line))
 
 

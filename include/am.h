@@ -107,9 +107,19 @@ am_status am_adjoint(am_ctx* ctx, const void* grad_vertices, int dtype, void* co
  * No counterpart in the reference (its planes stay internal). */
 am_status am_planes(am_ctx* ctx, const uint32_t* masks, int n, float* planes, float* eq);
 
+/* Order-independent topology fingerprint of the last am_run (no counterpart in
+ * the reference; a parity view for full-size runs): out5[0..2] are sums mod
+ * 2^64 of a per-row hash over the visited masks, the canonical faces and the
+ * vertex keys (one row per vertex use); out5[3] faces, out5[4] vertex uses.
+ * Rows and hash are those of analyticmesh_amd/canonical.py's fingerprint(). */
+am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
+
 /* Tunables: "vm" (polygon vertex cap, 3..64, default 20 like the reference's
  * _VAR_VERT_MAX), "table_log2" (initial visited-table size), "max_regions"
- * (stop after this many regions, -1 = unlimited), "walk" (0 ray walk, 1
+ * (stop after the breadth-first level during which this many regions were
+ * processed, -1 = unlimited), "region_limit" (am_run fails with
+ * AM_ERR_CAPACITY once more regions than this would be stored; -1 = the
+ * engine's own limit, min(2^30, (2^31 - 1) / vm)), "walk" (0 ray walk, 1
  * candidate scan, for the reduced forms), "adjoint_tiled" (1 default: the
  * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
  * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU). */
@@ -143,7 +153,8 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_CHAIN64_LAUNCHES 31 /* launches of the FP64 tiled chain */
 #define AM_STAT_VERT64_MS 32    /* FP64 vertex re-solve of regions the reduced-precision walk decided (HIP events) */
 #define AM_STAT_VERT64_REGIONS 33 /* regions whose vertices were re-solved from FP64 planes */
-#define AM_STAT_COUNT 34
+#define AM_STAT_FP64_ONLY 34    /* 1: a reduced form ran on the FP64 region kernel (skips or layers over 256 wide) */
+#define AM_STAT_COUNT 35
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 
 #ifdef __cplusplus

@@ -1,0 +1,208 @@
+"""Every workload against the oracle on the GPU: the 4x256 net, the batch of 64 nets, nets with
+skip connections, a layer wider than the tiled chain, extra planes and forced capacities.
+
+The FP64 form must equal the oracle's breadth-first records bit for bit (masks, tabs, vertex
+pairs, seed points, vertex bits); the other forms must give the oracle's topology (visited set,
+faces, vertex keys) with vertices from FP64 planes.  Large marches are capped with
+``max_regions`` on both sides: both stop after the same breadth-first level."""
+import sys
+
+import numpy as np
+import pytest
+
+from _common import workload
+from analyticmesh_amd import configs
+from analyticmesh_amd.canonical import same_topology, vertex_error
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _bitexact(g, o, what=""):
+    assert len(g.masks) == len(o.masks), f"{what}: {len(g.masks)} vs {len(o.masks)} regions"
+    np.testing.assert_array_equal(g.masks, o.masks, err_msg=what)
+    np.testing.assert_array_equal(g.info[:, :6], o.info[:, :6], err_msg=what)
+    np.testing.assert_array_equal(g.tabs, o.tabs, err_msg=what)
+    np.testing.assert_array_equal(g.vpairs, o.vpairs, err_msg=what)
+    np.testing.assert_array_equal(g.points, o.points, err_msg=what)
+    assert np.array_equal(g.verts.view(np.uint64), o.verts.view(np.uint64)), f"{what}: vertex bits differ"
+
+
+def _topology(g, o, what=""):
+    ok, why = same_topology(g.records(), o.records())
+    assert ok, f"{what}: {why}"
+
+
+@pytest.fixture(scope="module")
+def deep64():
+    import oracle
+    spec, pts, st = workload("deep", n_seeds=64)
+    cap = 100000
+    o = oracle.run_oracle(spec, pts, st, traversal="bfs", max_regions=cap)
+    return spec, pts, st, cap, o
+
+
+def test_deep_fp64_bitexact_vs_oracle(deep64):
+    """4x256 net, 64 seeds, first 100,000 regions (level-capped): FP64 records bit for bit."""
+    from analyticmesh_amd.engine import march
+    spec, pts, st, cap, o = deep64
+    _say(f"[deep fp64 vs oracle] {len(o.masks)} regions")
+    assert o.truncated and len(o.masks) >= cap
+    g = march(spec, pts, st, precision="fp64", max_regions=cap)
+    _bitexact(g, o, "deep fp64")
+
+
+@pytest.mark.parametrize("prec", ["fp64ray", "fp32", "bf16x3"])
+def test_deep_reduced_forms_topology_vs_oracle(prec, deep64):
+    """Each checked form on the 4x256 net gives the oracle's topology; vertices within 1e-12."""
+    from analyticmesh_amd.engine import march
+    spec, pts, st, cap, o = deep64
+    _say(f"[deep {prec} vs oracle]")
+    g = march(spec, pts, st, precision=prec, max_regions=cap)
+    _topology(g, o, f"deep {prec}")
+    err = vertex_error(g.records(), o.records())
+    _say(f"  redo {int(g.stats[9])} re-solved {int(g.stats[33])} vertex err {err:.2e}")
+    assert err is not None and err < 1e-12
+
+
+def _batch_seeds(specs, n):
+    out = []
+    for s in specs:
+        p = configs.seeds_fibonacci(s, n)
+        out.append((p, configs.mask_of(s, p)))
+    return out
+
+
+def test_batch64_every_net_vs_oracle():
+    """All 64 nets of the batch workload (16 seeds each, the first 8,000 regions of each, level-capped),
+    meshed in one job: per net, the FP64 form equals the oracle's records bit for bit and the FP32
+    form gives the oracle's topology."""
+    import oracle
+    from analyticmesh_amd.engine import march_batch
+    specs = configs.batch64(0)
+    seeds = _batch_seeds(specs, 16)
+    cap = 8000
+    _say(f"[batch64 vs oracle] {len(specs)} nets, cap {cap}")
+    f64, _ = march_batch(specs, seeds, precision="fp64", combine=False, max_regions=cap)
+    f32, _ = march_batch(specs, seeds, precision="fp32", combine=False, max_regions=cap)
+    full = 0
+    for k, (s, (p, st)) in enumerate(zip(specs, seeds)):
+        o = oracle.run_oracle(s, p, st, traversal="bfs", max_regions=cap)
+        full += not o.truncated
+        _bitexact(f64[k], o, s.name)
+        _topology(f32[k], o, s.name + " fp32")
+    _say(f"  {full} nets meshed completely, {len(specs) - full} capped")
+
+
+@pytest.mark.parametrize("name", ["skip_mlp2", "skip_custom2"])
+def test_skip_nets_fp64_bitexact_vs_oracle(name):
+    """Identity and transform skips from the input and from hidden layers (the reference's
+    test_mlp_2 / test_custom_2 architectures): the FP64 region kernel's records equal the
+    oracle's bit for bit; the reduced forms take the FP64 path on such nets, same records."""
+    import oracle
+    from analyticmesh_amd.engine import march
+    spec = getattr(configs, name)(0)
+    pts = configs.seeds_fibonacci(spec, 4)
+    st = configs.mask_of(spec, pts)
+    cap = 30000
+    _say(f"[{name} vs oracle] L={spec.L}")
+    o = oracle.run_oracle(spec, pts, st, traversal="bfs", max_regions=cap)
+    g = march(spec, pts, st, precision="fp64", max_regions=cap)
+    _bitexact(g, o, name)
+    for prec in ("fp64ray", "fp32"):
+        r = march(spec, pts, st, precision=prec, max_regions=cap)
+        _topology(r, o, f"{name} {prec}")
+        if prec == "fp32":
+            assert int(r.stats[34]) == 1   # ran on the FP64 region kernel
+    g.combine()
+
+
+def test_wide_layer_vs_oracle():
+    """A first hidden layer of 320 (wider than the tiled chain): FP64 bit for bit, fp64ray topology."""
+    import oracle
+    from analyticmesh_amd.engine import march
+    spec = configs.wide(0)
+    pts = configs.seeds_fibonacci(spec, 4)
+    st = configs.mask_of(spec, pts)
+    cap = 30000
+    _say(f"[wide vs oracle] nodes {spec.nodes}")
+    o = oracle.run_oracle(spec, pts, st, traversal="bfs", max_regions=cap)
+    g = march(spec, pts, st, precision="fp64", max_regions=cap)
+    _bitexact(g, o, "wide fp64")
+    r = march(spec, pts, st, precision="fp64ray", max_regions=cap)
+    _topology(r, o, "wide fp64ray")
+
+
+def _box(h):
+    """Six half-spaces |x|, |y|, |z| <= h, with the reference's 1e-8 jitter on w (backend/main.py:427)."""
+    ew = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1]], dtype=np.float64) + 1e-8
+    eb = np.full(6, -h)
+    return ew, eb
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp64ray", "fp32"])
+def test_extra_planes_vs_oracle(prec):
+    """E = 6 extra planes (a box |x|, |y|, |z| <= 0.42 that cuts the sphere): FP64 records bit for bit, other forms the
+    oracle's topology (backend/inc/process.h:189-192)."""
+    import oracle
+    from analyticmesh_amd.engine import march
+    spec = configs.sphere(0)
+    pts = configs.seeds_fibonacci(spec, 8)
+    st = configs.mask_of(spec, pts)
+    h = 0.42
+    ew, eb = _box(h)
+    _say(f"[extra planes] {prec}")
+    o = oracle.run_oracle(spec, pts, st, traversal="bfs", extra_w=ew, extra_b=eb)
+    g = march(spec, pts, st, precision=prec, extra_w=ew, extra_b=eb)
+    assert len(o.masks) > 1000
+    if prec == "fp64":
+        _bitexact(g, o, "extra planes fp64")
+    else:
+        _topology(g, o, f"extra planes {prec}")
+    # the planes cut: some vertices lie on the box, none outside it
+    V, _ = g.combine()
+    assert np.abs(V).max() <= h + 1e-6
+    assert (np.abs(np.abs(V).max(axis=1) - h) < 1e-6).any()
+
+
+def test_skip_net_with_extra_planes_vs_oracle():
+    import oracle
+    from analyticmesh_amd.engine import march
+    spec = configs.skip_custom2(0)
+    pts = configs.seeds_fibonacci(spec, 4)
+    st = configs.mask_of(spec, pts)
+    ew, eb = _box(0.35)
+    _say("[skip + extra planes]")
+    o = oracle.run_oracle(spec, pts, st, traversal="bfs", extra_w=ew, extra_b=eb)
+    g = march(spec, pts, st, precision="fp64", extra_w=ew, extra_b=eb)
+    _bitexact(g, o, "skip_custom2 + box")
+
+
+def test_small_capacities():
+    """Forced small capacities: a 2^10-slot visited table (grown and rehashed level by level) gives
+    the default run's records; a region limit fails with AM_ERR_CAPACITY (4) and leaves the
+    context usable; vm = 3 truncates polygons exactly as the oracle does (Q3)."""
+    import oracle
+    from analyticmesh_amd.engine import Engine, march
+    spec, pts, st = workload("sphere", n_seeds=8)
+    _say("[capacities]")
+    a = march(spec, pts, st, precision="fp64")
+    b = march(spec, pts, st, precision="fp64", table_log2=10)
+    _bitexact(b, a, "table_log2=10")
+    eng = Engine(spec.nodes, precision="fp64")
+    eng.set_option("region_limit", 1000)
+    W = [w.astype(np.float64) for w in spec.weights]
+    B = [x.astype(np.float64) for x in spec.biases]
+    with pytest.raises(RuntimeError, match="am error 4"):
+        eng.run(W, B, pts, st)
+    eng.set_option("region_limit", -1)
+    r = eng.run(W, B, pts, st)
+    assert len(r.masks) == len(a.masks)
+    g3 = march(spec, pts, st, precision="fp64", vm=3)
+    o3 = oracle.run_oracle(spec, pts, st, traversal="bfs", vm=3)
+    _bitexact(g3, o3, "vm=3")
+    assert (o3.info[:, 2] <= 3).all()
