@@ -23,8 +23,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-HIP_SOURCES = ["am_region_fp64.hip", "am_chain64.hip", "am_hash.hip", "am_fast.hip", "am_adjoint.hip", "am_layout.hip",
-               "am_engine.hip"]
+HIP_SOURCES = ["am_region_fp64.hip", "am_chain64.hip", "am_hash.hip", "am_combine.hip", "am_fast.hip",
+               "am_adjoint.hip", "am_layout.hip", "am_engine.hip"]
 
 
 def _run(cmd, verbose):
@@ -68,9 +68,15 @@ def build(verbose=False):
            "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
            os.path.join(CSRC, "bind.cpp"), "-o", mod + ".tmp",
            "-L" + HERE, "-lam", "-Wl,-rpath,$ORIGIN"]
-    _wait([(cmd, _run(cmd, verbose))])
+    # host-only polygon-mesh tools (no GPU code)
+    pmod = os.path.join(HERE, "_polylib" + ext)
+    pcmd = ["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall",
+            "-I" + pybind11.get_include(), "-I" + sysconfig.get_paths()["include"],
+            os.path.join(CSRC, "polylib.cpp"), "-o", pmod + ".tmp"]
+    _wait([(cmd, _run(cmd, verbose)), (pcmd, _run(pcmd, verbose))])
     os.replace(mod + ".tmp", mod)
-    return LIB, mod
+    os.replace(pmod + ".tmp", pmod)
+    return LIB, mod, pmod
 
 
 if __name__ == "__main__":
