@@ -96,7 +96,11 @@ __device__ __forceinline__ void load_planes64(const NetDev& net, const uint32_t*
   }
 }
 
-__global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
+// The planes of one tile: regions r0 .. r0 + nreg - 1 of lv (ids[r0 + t] or
+// first + r0 + t), written at planes slots r0 + t.  io supplies the region
+// store, the plane stride and the layer operands.
+__device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO& io, const ChainLevel& lv, int r0,
+                                                int nreg) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -107,13 +111,11 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   uint16_t* klist = reinterpret_cast<uint16_t*>(Xs + (size_t)AM_CXR * AM_CXS + 1);     // [D][AM_KLS], 16-B aligned
   uint32_t* mk = reinterpret_cast<uint32_t*>(klist + (size_t)net.D * AM_KLS);          // [AM_CT][Wd]
   __shared__ int s_nk[AM_MAX_LINEAR];
-  const int r0 = blockIdx.x * AM_CT;
-  const int nreg = min(AM_CT, io.count - r0);
   for (int idx = tid; idx < AM_CT * net.Wd; idx += AM_CHAIN_THREADS) {
     const int t = idx / net.Wd, w = idx - t * net.Wd;
     uint32_t v = 0u;
     if (t < nreg) {
-      const int rid = io.ids ? io.ids[r0 + t] : io.first + r0 + t;
+      const int rid = lv.ids ? lv.ids[r0 + t] : lv.first + r0 + t;
       v = io.reg_mask[(size_t)rid * net.Wd + w];
     }
     mk[idx] = v;
@@ -156,7 +158,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   }
   for (int idx = tid; idx < AM_CN * net.E; idx += AM_CHAIN_THREADS) {
     const int col = idx / net.E, e = idx - col * net.E, t = col >> 2, q = col & 3;
-    if (t < nreg) io.planes[((size_t)(r0 + t) * 4 + q) * Cs + net.L + e] = q < 3 ? net.ew[e * 3 + q] : net.eb[e];
+    if (t < nreg) lv.planes[((size_t)(r0 + t) * 4 + q) * Cs + net.L + e] = q < 3 ? net.ew[e * 3 + q] : net.eb[e];
   }
   __syncthreads();
   const int q_l = lane & 3;           // this lane's column is x, y, z or c of region lane >> 2
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       for (int col = ct; col < AM_CN; col += AM_CHAIN_THREADS / 256) {
         const int t = col >> 2, q = col & 3;
         if (t >= nreg) break;
-        double* dst = io.planes + ((size_t)(r0 + t) * 4 + q) * Cs + out0;
+        double* dst = lv.planes + ((size_t)(r0 + t) * 4 + q) * Cs + out0;
         for (int j = jt; j < nout; j += 256) {
           const double y = Xs[j * AM_CXS + col];
           dst[j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
@@ -305,15 +307,29 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
       }
     } else if (tid < AM_CN && (tid >> 2) < nreg) {
       const double y = Xs[tid];
-      io.eq[(size_t)(r0 + (tid >> 2)) * 4 + (tid & 3)] = (tid & 3) == 3 ? y - net.iso : y;
+      lv.eq[(size_t)(r0 + (tid >> 2)) * 4 + (tid & 3)] = (tid & 3) == 3 ? y - net.iso : y;
     }
     __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count,
-                                                  const int* slots, const double* planes, const double* eqb,
-                                                  int Cs_hbm) {
+__global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
+  const int r0 = blockIdx.x * AM_CT;
+  const ChainLevel lv{io.first, io.count, io.ids, io.planes, io.eq};
+  chain_tile_body(net, io, lv, r0, min(AM_CT, io.count - r0));
+}
+
+// Many networks in one launch: block b takes tiles[b] = (network, r0, regions);
+// the tile's regions are lv.first + r0 + t, all of one network.
+__global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64_multi(const NetDev* nets, const ChainIO* cios,
+                                                                           ChainLevel lv, const int4* tiles) {
+  const int4 tl = tiles[blockIdx.x];
+  chain_tile_body(nets[tl.x], cios[tl.x], lv, tl.y, tl.z);
+}
+
+// The candidate scan of one region (rid) on the planes at slot sl.
+__device__ __forceinline__ void walk_fp64_body(const NetDev& net, const RegionIO& io, int rid, int sl,
+                                               const double* planes, const double* eqb, int Cs_hbm) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
   const int Cs = io.Cs;
@@ -328,25 +344,59 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
   uint32_t* smask = used + ((C + 31) >> 5);
 
   const int lane = threadIdx.x;
+  for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
+  load_planes64(net, io.reg_mask + (size_t)rid * net.Wd, planes + (size_t)sl * 4 * Cs_hbm, Cs_hbm, Px, Py, Pz,
+                Pc, lane, 64);
+  double p[3];
+  p[0] = io.reg_point[(size_t)rid * 3 + 0];
+  p[1] = io.reg_point[(size_t)rid * 3 + 1];
+  p[2] = io.reg_point[(size_t)rid * 3 + 2];
+  const double eq[4] = {eqb[(size_t)sl * 4 + 0], eqb[(size_t)sl * 4 + 1], eqb[(size_t)sl * 4 + 2],
+                        eqb[(size_t)sl * 4 + 3]};
+  __syncthreads();
+  const WalkParams wp{0.f, 0.f, 0.f};
+  walk_region<double, false>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
+  __syncthreads();   // this region's LDS is free for the next
+}
+
+__global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count,
+                                                  const int* slots, const double* planes, const double* eqb,
+                                                  int Cs_hbm) {
   const int total = count ? min(*count, io.count) : io.count;
   // block-stride over the regions (one iteration per block when the grid covers them)
   for (int b = blockIdx.x; b < total; b += gridDim.x) {
     const int sl = slots ? slots[b] : b;     // planes slot of this region
     const int rid = io.ids ? io.ids[sl] : rid0 + sl;
-    for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
-    for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
-    load_planes64(net, io.reg_mask + (size_t)rid * net.Wd, planes + (size_t)sl * 4 * Cs_hbm, Cs_hbm, Px, Py, Pz,
-                  Pc, lane, 64);
-    double p[3];
-    p[0] = io.reg_point[(size_t)rid * 3 + 0];
-    p[1] = io.reg_point[(size_t)rid * 3 + 1];
-    p[2] = io.reg_point[(size_t)rid * 3 + 2];
-    const double eq[4] = {eqb[(size_t)sl * 4 + 0], eqb[(size_t)sl * 4 + 1], eqb[(size_t)sl * 4 + 2],
-                          eqb[(size_t)sl * 4 + 3]};
-    __syncthreads();
-    const WalkParams wp{0.f, 0.f, 0.f};
-    walk_region<double, false>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
-    __syncthreads();   // this region's LDS is free for the next
+    walk_fp64_body(net, io, rid, sl, planes, eqb, Cs_hbm);
+  }
+}
+
+// Network of a region in a batch of networks: the last word of its mask.
+__device__ __forceinline__ int region_net(const RegionIO& io, int Wd, int rid) {
+  return (int)io.reg_mask[(size_t)rid * Wd + Wd - 1];
+}
+
+// LDS layout of the candidate scan for a network of C constraints (the host
+// sizes the launch for the widest network of the batch).
+__device__ __forceinline__ RegionIO scan_layout(const RegionIO& io, int C) {
+  RegionIO r = io;
+  int n2 = AM_TOPK;
+  while (n2 < C) n2 <<= 1;
+  r.Cs = C;
+  r.Csort = n2;
+  return r;
+}
+
+__global__ __launch_bounds__(64) void k_walk_fp64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0,
+                                                        const int* count, const int* slots, const double* planes,
+                                                        const double* eqb, int Cs_hbm) {
+  const int total = count ? min(*count, io.count) : io.count;
+  for (int b = blockIdx.x; b < total; b += gridDim.x) {
+    const int sl = slots ? slots[b] : b;
+    const int rid = rid0 + sl;
+    const NetDev& net = nets[region_net(io, Wd, rid)];
+    walk_fp64_body(net, scan_layout(io, net.C), rid, sl, planes, eqb, Cs_hbm);
   }
 }
 
@@ -354,9 +404,9 @@ __global__ __launch_bounds__(64) void k_walk_fp64(NetDev net, RegionIO io, int r
 // planes are the canonical FP64 planes, so the only uncertainty is the
 // rounding of the walk's own evaluations; regions it cannot certify are
 // flagged (io.flags, level-relative) for the candidate scan.
-__global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
-                                                   const double* planes, const double* eqb, int Cs_hbm,
-                                                   int only_undecided) {
+__device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionIO& io, int rid, int b,
+                                                const WalkParams& wp, const double* planes, const double* eqb,
+                                                int Cs_hbm) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // LDS holds the planes past hidden layer 1 (index g - n1); layer 1's are
   // rebuilt from W0 and b0 where read (PlaneView), which leaves LDS for more
@@ -368,12 +418,8 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net,
   double* Pc = Pz + Cl;
   uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cl);
   const int lane = threadIdx.x;
-  if ((int)blockIdx.x >= io.count) return;
-  const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
-  // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
-  if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
   for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
-  const double* src = planes + (size_t)blockIdx.x * 4 * Cs_hbm + n1;
+  const double* src = planes + (size_t)b * 4 * Cs_hbm + n1;
   for (int i = lane; i < Cl; i += 64 * AM_RAY_WAVES) {
     Px[i] = src[i];
     Py[i] = src[Cs_hbm + i];
@@ -384,11 +430,30 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net,
   p[0] = io.reg_point[(size_t)rid * 3 + 0];
   p[1] = io.reg_point[(size_t)rid * 3 + 1];
   p[2] = io.reg_point[(size_t)rid * 3 + 2];
-  const double eq[4] = {eqb[(size_t)blockIdx.x * 4 + 0], eqb[(size_t)blockIdx.x * 4 + 1],
-                        eqb[(size_t)blockIdx.x * 4 + 2], eqb[(size_t)blockIdx.x * 4 + 3]};
+  const double eq[4] = {eqb[(size_t)b * 4 + 0], eqb[(size_t)b * 4 + 1], eqb[(size_t)b * 4 + 2],
+                        eqb[(size_t)b * 4 + 3]};
   __syncthreads();
   const PlaneView<double> P{Px, Py, Pz, Pc, n1, net.W0, net.b[0], smask};
   walk_ray<double, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
+}
+
+__global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
+                                                   const double* planes, const double* eqb, int Cs_hbm,
+                                                   int only_undecided) {
+  if ((int)blockIdx.x >= io.count) return;
+  const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
+  // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
+  if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
+  walk_ray64_body(net, io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
+}
+
+__global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd,
+                                                                         int rid0, WalkParams wp,
+                                                                         const double* planes, const double* eqb,
+                                                                         int Cs_hbm) {
+  if ((int)blockIdx.x >= io.count) return;
+  const int rid = rid0 + blockIdx.x;
+  walk_ray64_body(nets[region_net(io, Wd, rid)], io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
 }
 
 // Vertices of a region whose polygon (the tight-constraint pair of every

@@ -569,6 +569,175 @@ am_status guarded(am_ctx* c, F&& f) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// A batch of networks marched in one job.  Every network's regions share one
+// region store, visited set and candidate buffer: masks carry one extra word
+// holding the network id (so equal masks of different networks stay distinct
+// keys, and the visited-set and mesh kernels need no change), and each level
+// runs the tiled FP64 chain and the walks for all networks together -- a chain
+// tile holds regions of one network, every other kernel finds a region's
+// network in its mask's last word.  The breadth-first level of each network is
+// the one it has alone, so per network the records are those of its own march.
+// ---------------------------------------------------------------------------
+struct BatchNet {
+  std::vector<int> nodes, off;
+  int D = 0, L = 0, Wn = 0;
+  DevBuf<double> W0;
+  std::vector<DevBuf<double>> WT, Wrow, b, WTp, bpad;
+};
+
+}  // namespace
+
+struct am_batch {
+  am_ctx* c = nullptr;
+  int precision = AM_PREC_FP64_RAY;
+  std::vector<BatchNet> nets;
+  int Wmax = 0, Dmax = 0, Cmax = 0, Lmax = 0;
+  std::vector<NetDev> nets_h;
+  std::vector<ChainIO> cios_h;
+  DevBuf<NetDev> nets_d;
+  DevBuf<ChainIO> cios_d;
+  DevBuf<int4> tiles_d;
+  int4* tiles_h = nullptr;       // pinned staging of a level's tile list
+  size_t tiles_cap = 0;
+  DevBuf<int> hist_d, sel_flags, sel_rank, sel_ids;
+  int* hist_h = nullptr;         // pinned: regions per network of the level
+};
+
+namespace {
+
+// LDS of the candidate scan / the FP64 ray walk / the chain for the widest network of the batch.
+size_t batch_scan_lds(const am_batch* b) {
+  size_t m = 0;
+  for (const BatchNet& n : b->nets) {
+    int Csort = AM_TOPK;
+    while (Csort < n.L) Csort <<= 1;
+    m = std::max(m, (size_t)4 * n.L * 8 + (size_t)Csort * 12 + (size_t)((n.L + 31) / 32) * 4 + (size_t)b->c->Wd * 4 + 64);
+  }
+  return m;
+}
+size_t batch_ray_lds(const am_batch* b) {
+  size_t m = 0;
+  for (const BatchNet& n : b->nets) m = std::max(m, (size_t)4 * (n.L - n.nodes[1]) * 8 + (size_t)b->c->Wd * 4 + 64);
+  return m;
+}
+size_t batch_chain_lds(const am_batch* b) {
+  return ((size_t)AM_CXR * AM_CXS + 1) * sizeof(double) + (size_t)b->Dmax * AM_KLS * sizeof(uint16_t) +
+         (size_t)AM_CT * b->c->Wd * 4 + 16;
+}
+
+// Planes and walks of the level's regions (first .. first + count - 1, grouped
+// by network in ascending order; hist_h has the per-network counts).
+void batch_level(am_batch* b, long long first, long long count, Fp64Counts& t) {
+  am_ctx* c = b->c;
+  const hipStream_t st = c->stream;
+  const int Ch = (b->Cmax + 63) & ~63;
+  size_t chunk = ((size_t)1 << 30) / ((size_t)32 * Ch);
+  chunk = std::max<size_t>(1024, std::min<size_t>(chunk, 65536)) & ~(size_t)(AM_CT - 1);
+  const size_t cap = std::min<size_t>(chunk, (size_t)count);
+  ensure(c->tplanes, cap * 4 * Ch, false, st);
+  ensure(c->teq, cap * 4, false, st);
+  ensure(c->tslots, cap, false, st);
+  ensure(c->tcnt, 1, false, st);
+  // tiles: per chunk, per network segment, AM_CT regions (chunk-relative r0)
+  const int nn = (int)b->nets.size();
+  std::vector<long long> seg(nn + 1, 0);
+  for (int i = 0; i < nn; ++i) seg[i + 1] = seg[i] + b->hist_h[i];
+  if (seg[nn] != count) throw std::runtime_error("batch level: network counts do not add up");
+  const size_t max_tiles = (size_t)(count + AM_CT - 1) / AM_CT + (size_t)nn * ((count + chunk - 1) / chunk + 1);
+  if (max_tiles > b->tiles_cap) {
+    if (b->tiles_h) HIPCHK(hipHostFree(b->tiles_h));
+    b->tiles_h = nullptr;
+    b->tiles_cap = std::max(max_tiles, b->tiles_cap * 2);
+    HIPCHK(hipHostMalloc((void**)&b->tiles_h, b->tiles_cap * sizeof(int4), hipHostMallocDefault));
+    ensure(b->tiles_d, b->tiles_cap, false, st);
+  }
+  std::vector<size_t> chunk_tile0;
+  size_t nt = 0;
+  for (long long cs = 0; cs < count; cs += (long long)chunk) {
+    const long long ce = std::min<long long>(cs + (long long)chunk, count);
+    chunk_tile0.push_back(nt);
+    for (int i = 0; i < nn; ++i) {
+      const long long a = std::max(seg[i], cs), e = std::min(seg[i + 1], ce);
+      for (long long r = a; r < e; r += AM_CT) b->tiles_h[nt++] = make_int4(i, (int)(r - cs), (int)std::min<long long>(AM_CT, e - r), 0);
+    }
+  }
+  chunk_tile0.push_back(nt);
+  HIPCHK(hipMemcpyAsync(b->tiles_d.p, b->tiles_h, nt * sizeof(int4), hipMemcpyHostToDevice, st));
+  const size_t clds = batch_chain_lds(b), slds = batch_scan_lds(b), rlds = batch_ray_lds(b);
+  const bool ray = b->precision == AM_PREC_FP64_RAY && rlds <= c->max_lds;
+  StageTimer& tm = c->timer;
+  size_t k = 0;
+  for (long long cs = 0; cs < count; cs += (long long)chunk, ++k) {
+    const int n = (int)std::min<long long>((long long)chunk, count - cs);
+    const int rid0 = (int)(first + cs);
+    const ChainLevel lv{rid0, n, nullptr, c->tplanes.p, c->teq.p};
+    const int ntile = (int)(chunk_tile0[k + 1] - chunk_tile0[k]);
+    const size_t m0 = tm.mark(st);
+    hipLaunchKernelGGL(k_chain_tile_fp64_multi, dim3(ntile), dim3(AM_CHAIN_THREADS), clds, st,
+                       (const NetDev*)b->nets_d.p, (const ChainIO*)b->cios_d.p, lv,
+                       (const int4*)(b->tiles_d.p + chunk_tile0[k]));
+    HIPCHK(hipGetLastError());
+    t.chain_launches += 1;
+    const size_t m1 = tm.mark(st);
+    tm.span(m0, m1, T_CHAIN64);
+    RegionIO io;
+    io.reg_mask = c->reg_mask.p;
+    io.reg_start = c->reg_start.p;
+    io.reg_point = c->reg_point.p;
+    io.reg_info = c->reg_info.p;
+    io.reg_tab = c->reg_tab.p;
+    io.reg_verts = c->reg_verts.p;
+    io.reg_vpair = c->reg_vpair.p;
+    io.cand_mask = c->cand_mask.p;
+    io.cand_start = c->cand_start.p;
+    io.cand_point = c->cand_point.p;
+    io.first = (int)first;
+    io.count = n;
+    io.Cs = 0;
+    io.Csort = 0;
+    io.flags = ray ? c->rflags.p : nullptr;
+    io.ids = nullptr;
+    if (!ray) {
+      hipLaunchKernelGGL(k_walk_fp64_multi, dim3(n), dim3(64), slds, st, (const NetDev*)b->nets_d.p, io, c->Wd,
+                         rid0, (const int*)nullptr, (const int*)nullptr, (const double*)c->tplanes.p,
+                         (const double*)c->teq.p, Ch);
+      HIPCHK(hipGetLastError());
+      tm.span(m1, tm.mark(st), T_SCAN64);
+      t.scanned_host += n;
+      continue;
+    }
+    hipLaunchKernelGGL(k_walk_ray64_multi, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, st, (const NetDev*)b->nets_d.p,
+                       io, c->Wd, rid0, kRay64, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+    HIPCHK(hipGetLastError());
+    const size_t m2 = tm.mark(st);
+    tm.span(m1, m2, T_RAY64);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->tcnt.p, (size_t)1, 0);
+    hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, st, (const unsigned*)c->rflags.p, n,
+                       (const int*)nullptr, rid0, (int)first, c->tcnt.p, c->tslots.p, c->runacc.p);
+    io.flags = nullptr;
+    hipLaunchKernelGGL(k_walk_fp64_multi, dim3(std::min(n, 2048)), dim3(64), slds, st, (const NetDev*)b->nets_d.p,
+                       io, c->Wd, rid0, (const int*)c->tcnt.p, (const int*)c->tslots.p,
+                       (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+    HIPCHK(hipGetLastError());
+    tm.span(m2, tm.mark(st), T_SCAN64);
+  }
+}
+
+// Regions per network of level first .. first + count - 1 into b->hist_h (waits for the stream).
+void batch_hist(am_batch* b, long long first, long long count) {
+  am_ctx* c = b->c;
+  const hipStream_t st = c->stream;
+  const int nn = (int)b->nets.size();
+  hipLaunchKernelGGL(k_fill_i32, dim3(blocks(nn, 256)), dim3(256), 0, st, b->hist_d.p, (size_t)nn, 0);
+  if (count > 0)
+    hipLaunchKernelGGL(k_net_hist, dim3(blocks((size_t)count, 256)), dim3(256), 0, st, c->reg_mask.p, c->Wd,
+                       (int)first, (int)count, b->hist_d.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->hist_h, b->hist_d.p, nn * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -953,6 +1122,15 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
   });
 }
 
+// dst row i = the first copy_words words of src row ids[i] (row strides in words).
+__global__ void k_gather_words(const uint32_t* src, int src_stride, int copy_words, const int* ids, int n,
+                               uint32_t* dst, int dst_stride) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (size_t)n * copy_words) return;
+  const int i = (int)(t / copy_words), w = (int)(t - (size_t)i * copy_words);
+  dst[(size_t)i * dst_stride + w] = src[(size_t)ids[i] * src_stride + w];
+}
+
 // Region scan of the mesh assembly; returns (faces, items).
 static void combine_counts(am_ctx* c, const CombineIn& in, CombineWork& w, long long& F, long long& N) {
   combine_count(in, w, c->stream);
@@ -1129,11 +1307,11 @@ am_status am_topology_fingerprint(am_ctx* c, unsigned long long* out5) {
   return guarded(c, [&]() -> am_status {
     const hipStream_t st = c->stream;
     DevBuf<unsigned long long> acc;
-    ensure(acc, 5, false, st);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t*)acc.p, (size_t)10, 0u);
+    ensure(acc, 6, false, st);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t*)acc.p, (size_t)12, 0u);
     if (c->R > 0)
       hipLaunchKernelGGL(k_fingerprint, dim3(blocks((size_t)c->R, 256)), dim3(256), 0, st, c->reg_mask.p,
-                         c->reg_info.p, c->reg_vpair.p, (int)c->R, c->Wd, c->vm, c->L, acc.p);
+                         c->reg_info.p, c->reg_vpair.p, (int)c->R, c->Wd, c->Wd, c->vm, c->L, -1, acc.p);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out5, acc.p, 5 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
@@ -1367,6 +1545,378 @@ am_status am_destroy(am_ctx* c) {
 const char* am_last_error(const am_ctx* c) {
   if (!c) return g_create_error.c_str();
   return c->err.c_str();
+}
+
+
+// ---------------------------------------------------------------------------
+// batch of networks
+// ---------------------------------------------------------------------------
+am_status am_batch_create(am_batch** out, int n_nets, const int* n_nodes, const int* const* nodes, int precision) {
+  if (!out) return AM_ERR_ARG;
+  *out = nullptr;
+  if (n_nets < 1 || !n_nodes || !nodes) {
+    g_create_error = "a batch needs at least one network";
+    return AM_ERR_ARG;
+  }
+  if (precision != AM_PREC_FP64 && precision != AM_PREC_FP64_RAY) {
+    g_create_error = "a batch runs the FP64 forms (AM_PREC_FP64, AM_PREC_FP64_RAY)";
+    return AM_ERR_ARG;
+  }
+  if (n_nets > (1 << 20)) {
+    g_create_error = "too many networks in one batch";
+    return AM_ERR_ARG;
+  }
+  am_batch* b = new am_batch();
+  b->precision = precision;
+  int Lmax = 0, Dmax = 0, Wmax = 0;
+  for (int i = 0; i < n_nets; ++i) {
+    const int nn = n_nodes[i];
+    const int* nd = nodes[i];
+    if (nn < 3 || !nd || nd[0] != 3 || nd[nn - 1] != 1 || nn - 1 > AM_MAX_LINEAR) {
+      g_create_error = "network " + std::to_string(i) + ": nodes must start with 3, end with 1, 3..17 entries";
+      delete b;
+      return AM_ERR_ARG;
+    }
+    BatchNet bn;
+    bn.nodes.assign(nd, nd + nn);
+    bn.D = nn - 2;
+    bn.off.assign(bn.D + 1, 0);
+    for (int k = 0; k < bn.D; ++k) bn.off[k + 1] = bn.off[k] + nd[k + 1];
+    bn.L = bn.off[bn.D];
+    bn.Wn = (bn.L + 31) / 32;
+    for (int k = 1; k < nn; ++k)
+      if (nd[k] > AM_CHAIN_MAXW) {
+        g_create_error = "network " + std::to_string(i) + ": a batch takes layers up to 256 wide";
+        delete b;
+        return AM_ERR_ARG;
+      }
+    Lmax = std::max(Lmax, bn.L);
+    Dmax = std::max(Dmax, bn.D);
+    Wmax = std::max(Wmax, bn.Wn);
+    b->nets.push_back(std::move(bn));
+  }
+  if (Wmax + 1 > AM_MAX_WORDS) {
+    g_create_error = "networks too wide for a batch";
+    delete b;
+    return AM_ERR_ARG;
+  }
+  // the shared context: masks of Wmax words + the network word
+  std::vector<int> n0 = {3, std::max(1, Lmax), 1};
+  const int arc = 0;
+  am_ctx* c = nullptr;
+  const am_status s = am_create(&c, 3, n0.data(), &arc, 1, 1, 0, AM_PREC_FP64);
+  if (s != AM_OK) {
+    delete b;
+    return s;
+  }
+  c->Wd = Wmax + 1;
+  c->L = Lmax;
+  c->C = Lmax;
+  b->c = c;
+  b->Wmax = Wmax;
+  b->Dmax = Dmax;
+  b->Lmax = Lmax;
+  b->Cmax = Lmax;
+  *out = b;
+  return AM_OK;
+}
+
+am_status am_batch_set_option(am_batch* b, const char* key, long long value) {
+  if (!b || !key) return AM_ERR_ARG;
+  const std::string k(key);
+  if (k == "vm" || k == "table_log2" || k == "region_limit") return am_set_option(b->c, key, value);
+  return fail(b->c, AM_ERR_ARG, "unknown batch option " + k);
+}
+
+am_status am_batch_run(am_batch* b, const void* const* weights, const void* const* biases, int dtype,
+                       const void* const* points, const uint8_t* const* states, const int* n_seeds, double iso,
+                       int flip_insideout, void* stream) {
+  if (!b) return AM_ERR_ARG;
+  am_ctx* c = b->c;
+  return guarded(c, [&]() -> am_status {
+    if (!weights || !biases || !points || !states || !n_seeds) throw ArgErr{"null argument"};
+    if (dtype != AM_DTYPE_F32 && dtype != AM_DTYPE_F64) throw ArgErr{"dtype must be AM_DTYPE_F32 or AM_DTYPE_F64"};
+    const int nn = (int)b->nets.size();
+    long long total_seeds = 0;
+    for (int i = 0; i < nn; ++i) {
+      if (n_seeds[i] < 1 || !points[i] || !states[i]) throw ArgErr{"every network needs at least one seed"};
+      total_seeds += n_seeds[i];
+    }
+    if (total_seeds > INT_MAX / 2) throw CapErr{"too many seeds"};
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    const hipStream_t st = c->stream;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (double& x : c->stats) x = 0;
+    c->has_run = false;
+    c->has_mesh = false;
+    c->stage_used = 0;
+    const size_t esz = dtype == AM_DTYPE_F64 ? 8 : 4;
+    const int Ch = (b->Cmax + 63) & ~63;
+    b->nets_h.assign(nn, NetDev{});
+    b->cios_h.assign(nn, ChainIO{});
+    size_t wi = 0;
+    for (int i = 0; i < nn; ++i) {
+      BatchNet& bn = b->nets[i];
+      const int nl = bn.D + 1;
+      std::vector<const void*> wv(nl), bv(nl);
+      for (int li = 0; li < nl; ++li) {
+        wv[li] = dev_view(c, weights[wi + li], (size_t)bn.nodes[li + 1] * bn.nodes[li] * esz, st);
+        bv[li] = dev_view(c, biases[wi + li], (size_t)bn.nodes[li + 1] * esz, st);
+      }
+      wi += nl;
+      bn.WT.resize(nl);
+      bn.b.resize(nl);
+      bn.WTp.resize(nl);
+      bn.bpad.resize(nl);
+      upload(c, bn.W0, wv[0], (size_t)bn.nodes[1] * 3, dtype, st);
+      for (int li = 0; li < nl; ++li) {
+        upload(c, bn.b[li], bv[li], (size_t)bn.nodes[li + 1], dtype, st);
+        if (li >= 1) {
+          upload_T(c, bn.WT[li], wv[li], bn.nodes[li + 1], bn.nodes[li], dtype, st);
+          upload_pad(c, bn.WTp[li], bn.bpad[li], wv[li], bv[li], bn.nodes[li + 1], bn.nodes[li], dtype, st);
+        }
+      }
+      NetDev& n = b->nets_h[i];
+      std::memset(&n, 0, sizeof(n));
+      n.D = bn.D; n.L = bn.L; n.E = 0; n.C = bn.L; n.Wd = c->Wd; n.vm = c->vm;
+      for (int k = 0; k < (int)bn.nodes.size(); ++k) n.nodes[k] = bn.nodes[k];
+      for (int k = 0; k <= bn.D; ++k) n.off[k] = bn.off[k];
+      n.W0 = bn.W0.p;
+      for (int li = 0; li < nl; ++li) {
+        n.WT[li] = li >= 1 ? bn.WT[li].p : nullptr;
+        n.b[li] = bn.b[li].p;
+      }
+      n.iso = iso;
+      n.flip = flip_insideout ? 1 : 0;
+      ChainIO& cio = b->cios_h[i];
+      std::memset(&cio, 0, sizeof(cio));
+      cio.Cs = Ch;
+      for (int li = 1; li <= bn.D; ++li) {
+        cio.Wp[li] = bn.WTp[li].p;
+        cio.bp[li] = bn.bpad[li].p;
+        cio.noutp[li] = (bn.nodes[li + 1] + AM_CR - 1) / AM_CR * AM_CR;
+      }
+    }
+    ensure(b->nets_d, (size_t)nn, false, st);
+    HIPCHK(hipMemcpyAsync(b->nets_d.p, b->nets_h.data(), nn * sizeof(NetDev), hipMemcpyHostToDevice, st));
+    ensure(b->cios_d, (size_t)nn, false, st);
+    ensure(b->hist_d, (size_t)nn, false, st);
+    if (!b->hist_h) HIPCHK(hipHostMalloc((void**)&b->hist_h, (size_t)1 << 22, hipHostMallocDefault));
+    if ((size_t)nn * sizeof(int) > ((size_t)1 << 22)) throw CapErr{"too many networks"};
+    const size_t clds = batch_chain_lds(b), slds = batch_scan_lds(b), rlds = batch_ray_lds(b);
+    if (clds > c->max_lds || slds > c->max_lds) throw CapErr{"networks too wide for the batch kernels' LDS"};
+    if (clds > 65536)
+      HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)clds));
+    if (slds > 65536)
+      HIPCHK(hipFuncSetAttribute((const void*)k_walk_fp64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)slds));
+    if (rlds > 65536 && rlds <= c->max_lds)
+      HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)rlds));
+    const auto t_setup = std::chrono::steady_clock::now();
+
+    // seeds of every network -> the first level (network word appended)
+    c->R = 0;
+    c->table_cap = 0;
+    c->table.release();
+    grow_cands(c, (size_t)total_seeds);
+    {
+      std::vector<uint32_t> m((size_t)total_seeds * c->Wd, 0u);
+      std::vector<double> pts((size_t)total_seeds * 3);
+      size_t q = 0;
+      for (int i = 0; i < nn; ++i) {
+        const int L = b->nets[i].L, ns = n_seeds[i];
+        std::vector<uint8_t> hs((size_t)ns * L);
+        HIPCHK(hipMemcpy(hs.data(), states[i], hs.size(), hipMemcpyDefault));
+        std::vector<double> p((size_t)ns * 3);
+        if (dtype == AM_DTYPE_F64) {
+          HIPCHK(hipMemcpy(p.data(), points[i], p.size() * 8, hipMemcpyDefault));
+        } else {
+          std::vector<float> f(p.size());
+          HIPCHK(hipMemcpy(f.data(), points[i], f.size() * 4, hipMemcpyDefault));
+          for (size_t k = 0; k < f.size(); ++k) p[k] = f[k];
+        }
+        for (int s2 = 0; s2 < ns; ++s2, ++q) {
+          uint32_t* mw = m.data() + q * c->Wd;
+          for (int g = 0; g < L; ++g)
+            if (hs[(size_t)s2 * L + g]) mw[g >> 5] |= 1u << (g & 31);
+          mw[c->Wd - 1] = (uint32_t)i;
+          for (int k = 0; k < 3; ++k) pts[q * 3 + k] = p[(size_t)s2 * 3 + k];
+        }
+      }
+      std::vector<int> startv((size_t)total_seeds, -1);
+      HIPCHK(hipMemcpy(c->cand_mask.p, m.data(), m.size() * 4, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->cand_point.p, pts.data(), pts.size() * 8, hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->cand_start.p, startv.data(), startv.size() * 4, hipMemcpyHostToDevice));
+    }
+    c->timer.reset();
+    constexpr int NACC = 2 * (1 + AM_WHY_COUNT);
+    ensure(c->runacc, NACC, false, st);
+    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->runacc.p, (size_t)NACC, 0);
+    long long lvl_first = 0;
+    long long lvl_count = insert_level(c, (size_t)total_seeds);
+    batch_hist(b, lvl_first, lvl_count);
+    long long levels = 0, cands = 0, launches = 0, waits = 1;
+    const uint32_t* mask_at_upload = nullptr;
+    Fp64Counts f64;
+    while (lvl_count > 0) {
+      const size_t ncand = (size_t)lvl_count * c->vm;
+      grow_cands(c, ncand);
+      ensure(c->rflags, (size_t)lvl_count, false, st);
+      if (c->reg_mask.p != mask_at_upload) {
+        // the chain reads region masks through each network's ChainIO
+        for (ChainIO& cio : b->cios_h) cio.reg_mask = c->reg_mask.p;
+        HIPCHK(hipMemcpyAsync(b->cios_d.p, b->cios_h.data(), nn * sizeof(ChainIO), hipMemcpyHostToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
+        mask_at_upload = c->reg_mask.p;
+      }
+      batch_level(b, lvl_first, lvl_count, f64);
+      launches += 1;
+      const size_t mi = c->timer.mark(st);
+      const long long before = c->R;
+      const int added = insert_level(c, ncand);
+      c->timer.span(mi, c->timer.mark(st), T_INSERT);
+      cands += (long long)ncand;
+      levels += 1;
+      lvl_first = before;
+      lvl_count = added;
+      batch_hist(b, lvl_first, lvl_count);
+      waits += 2;
+    }
+    int acc[NACC] = {0};
+    HIPCHK(hipMemcpyAsync(acc, c->runacc.p, sizeof(acc), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    c->timer.resolve();
+    const auto t1 = std::chrono::steady_clock::now();
+    const double* T = c->timer.acc;
+    c->stats[AM_STAT_RUN_MS] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    c->stats[AM_STAT_INSERT_MS] = T[T_INSERT];
+    c->stats[AM_STAT_REGION_MS] = T[T_CHAIN64] + T[T_RAY64] + T[T_SCAN64];
+    c->stats[AM_STAT_GEMM_MS] = T[T_CHAIN64];
+    c->stats[AM_STAT_WALK_MS] = T[T_RAY64] + T[T_SCAN64];
+    c->stats[AM_STAT_LEVELS] = (double)levels;
+    c->stats[AM_STAT_REGIONS] = (double)c->R;
+    c->stats[AM_STAT_CANDIDATES] = (double)cands;
+    c->stats[AM_STAT_REGION_LAUNCHES] = (double)launches;
+    c->stats[AM_STAT_CHAIN64_MS] = T[T_CHAIN64];
+    c->stats[AM_STAT_WALK64_MS] = T[T_SCAN64];
+    c->stats[AM_STAT_RAY64_MS] = T[T_RAY64];
+    c->stats[AM_STAT_SCAN64_REGIONS] = (double)(f64.scanned_host + acc[0]);
+    c->stats[AM_STAT_RAY64_UNDECIDED] = (double)acc[0];
+    c->stats[AM_STAT_CHAIN64_LAUNCHES] = (double)f64.chain_launches;
+    c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
+    c->stats[AM_STAT_HOST_WAITS] = (double)waits;
+    c->has_run = true;
+    return AM_OK;
+  });
+}
+
+am_status am_batch_fingerprint(am_batch* b, int net, unsigned long long* out6) {
+  if (!b || !out6) return AM_ERR_ARG;
+  am_ctx* c = b->c;
+  if (!c->has_run) return fail(c, AM_ERR_STATE, "am_batch_fingerprint called before am_batch_run");
+  if (net < 0 || net >= (int)b->nets.size()) return fail(c, AM_ERR_ARG, "no such network");
+  return guarded(c, [&]() -> am_status {
+    const hipStream_t st = c->stream;
+    DevBuf<unsigned long long> acc;
+    ensure(acc, 6, false, st);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, (uint32_t*)acc.p, (size_t)12, 0u);
+    if (c->R > 0)
+      hipLaunchKernelGGL(k_fingerprint, dim3(blocks((size_t)c->R, 256)), dim3(256), 0, st, c->reg_mask.p,
+                         c->reg_info.p, c->reg_vpair.p, (int)c->R, c->Wd, b->nets[net].Wn, c->vm, b->nets[net].L, net,
+                         acc.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out6, acc.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return AM_OK;
+  });
+}
+
+am_status am_batch_get_regions(am_batch* b, int net, long long* n_regions, uint32_t* masks, int* info, int* tabs,
+                               double* verts, int* vpairs, double* points) {
+  if (!b || !n_regions) return AM_ERR_ARG;
+  am_ctx* c = b->c;
+  if (!c->has_run) return fail(c, AM_ERR_STATE, "am_batch_get_regions called before am_batch_run");
+  if (net < 0 || net >= (int)b->nets.size()) return fail(c, AM_ERR_ARG, "no such network");
+  return guarded(c, [&]() -> am_status {
+    const hipStream_t st = c->stream;
+    const int R = (int)c->R;
+    *n_regions = 0;
+    if (R == 0) return AM_OK;
+    ensure(b->sel_flags, (size_t)R, false, st);
+    ensure(b->sel_rank, (size_t)R + 1, false, st);
+    ensure(b->sel_ids, (size_t)R, false, st);
+    hipLaunchKernelGGL(k_net_flags, dim3(blocks(R, 256)), dim3(256), 0, st, c->reg_mask.p, c->Wd, R, net,
+                       b->sel_flags.p);
+    ensure(c->scan_part, scan_blocks((size_t)R) + 1, false, st);
+    scan_launch<int>(ArraySrc<int>{b->sel_flags.p}, (size_t)R, ArraySink<int>{b->sel_rank.p}, c->scan_part.p,
+                     b->sel_rank.p + R, st);
+    hipLaunchKernelGGL(k_net_scatter, dim3(blocks(R, 256)), dim3(256), 0, st, (const int*)b->sel_flags.p,
+                       (const int*)b->sel_rank.p, R, b->sel_ids.p);
+    HIPCHK(hipGetLastError());
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, b->sel_rank.p + R, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    *n_regions = n;
+    if (n == 0 || (!masks && !info && !tabs && !verts && !vpairs && !points)) return AM_OK;
+    const int vm = c->vm, Wn = b->nets[net].Wn;
+    DevBuf<uint32_t> tmp;
+    auto gather = [&](const void* src, int stride_words, int copy_words, void* dst) {
+      if (!dst) return;
+      ensure(tmp, (size_t)n * copy_words, false, st);
+      const size_t tot = (size_t)n * copy_words;
+      hipLaunchKernelGGL(k_gather_words, dim3(blocks(tot, 256)), dim3(256), 0, st, (const uint32_t*)src,
+                         stride_words, copy_words, (const int*)b->sel_ids.p, n, tmp.p, copy_words);
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(dst, tmp.p, tot * 4, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    };
+    gather(c->reg_mask.p, c->Wd, Wn, masks);
+    gather(c->reg_info.p, AM_INFO_STRIDE, AM_INFO_STRIDE, info);
+    gather(c->reg_tab.p, vm + 1, vm + 1, tabs);
+    gather(c->reg_verts.p, vm * 6, vm * 6, verts);
+    gather(c->reg_vpair.p, vm * 2, vm * 2, vpairs);
+    gather(c->reg_point.p, 6, 6, points);
+    return AM_OK;
+  });
+}
+
+am_status am_batch_combine(am_batch* b, double scale, const double* center3) {
+  if (!b) return AM_ERR_ARG;
+  return am_combine(b->c, scale, center3);
+}
+
+am_status am_batch_get_counts(am_batch* b, long long* n_regions, long long* n_faces, long long* n_vertices,
+                              long long* n_face_indices) {
+  if (!b) return AM_ERR_ARG;
+  return am_get_counts(b->c, n_regions, n_faces, n_vertices, n_face_indices);
+}
+
+am_status am_batch_get_stats(am_batch* b, double* out, int n) {
+  if (!b) return AM_ERR_ARG;
+  return am_get_stats(b->c, out, n);
+}
+
+const char* am_batch_last_error(const am_batch* b) { return b ? am_last_error(b->c) : am_last_error(nullptr); }
+
+am_status am_batch_destroy(am_batch* b) {
+  if (!b) return AM_ERR_ARG;
+  if (b->c) (void)hipStreamSynchronize(b->c->stream);
+  if (b->tiles_h) (void)hipHostFree(b->tiles_h);
+  if (b->hist_h) (void)hipHostFree(b->hist_h);
+  am_ctx* c = b->c;
+  b->nets.clear();
+  b->nets_d.release();
+  b->cios_d.release();
+  b->tiles_d.release();
+  b->hist_d.release();
+  b->sel_flags.release();
+  b->sel_rank.release();
+  b->sel_ids.release();
+  delete b;
+  if (c) am_destroy(c);
+  return AM_OK;
 }
 
 }  // extern "C"

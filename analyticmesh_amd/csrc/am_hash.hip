@@ -8,11 +8,7 @@
 // Among equal candidates of one level the lowest candidate index wins, which
 // makes the next frontier's order independent of atomic timing.
 //
-// Vertex deduplication (replaces the quantised-coordinate hash of
-// backend/inc/polymesh.h:144-233): a vertex is keyed by its two tight
-// constraints (lo, hi) and the region mask with those two bits cleared; every
-// region around the vertex produces the same key.  The first occurrence in
-// (region, vertex) order gives the vertex its index and coordinates.
+// Vertex deduplication lives in am_combine.hip.
 #include "am_device.h"
 #include "am_kernels.h"
 #include "am_hash.h"
@@ -108,110 +104,6 @@ __global__ void k_rehash(const uint32_t* reg_mask, int R, int Wd, uint32_t* tabl
   }
 }
 
-// ---------------------------------------------------------------------------
-// mesh assembly
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void vkey_parts(const int* vpair, const int* info, int vm, int L, int q,
-                                           int& r, int& lo, int& hi) {
-  r = q / vm;
-  const int k = q - r * vm;
-  const int a = vpair[(size_t)r * vm * 2 + k * 2 + 0];
-  const int b = vpair[(size_t)r * vm * 2 + k * 2 + 1];
-  lo = a < b ? a : b;
-  hi = a < b ? b : a;
-  (void)info;
-  (void)L;
-}
-
-__device__ __forceinline__ uint32_t masked_word(const uint32_t* m, int w, int lo, int hi, int L) {
-  uint32_t x = m[w];
-  if (lo >= 0 && lo < L && (lo >> 5) == w) x &= ~(1u << (lo & 31));
-  if (hi >= 0 && hi < L && (hi >> 5) == w) x &= ~(1u << (hi & 31));
-  return x;
-}
-
-__global__ void k_vinsert(VertexIO io) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= io.n) return;
-  const int r = q / io.vm, k = q - (q / io.vm) * io.vm;
-  const int nv = io.info[(size_t)r * AM_INFO_STRIDE + 2];
-  if (nv < 3 || k >= nv) { io.status[q] = 0; return; }
-  int rr, lo, hi;
-  vkey_parts(io.vpair, io.info, io.vm, io.L, q, rr, lo, hi);
-  const uint32_t* m = io.reg_mask + (size_t)r * io.Wd;
-  uint64_t h = mix64(AM_HSEED ^ (uint64_t(uint32_t(lo)) << 32 | uint32_t(hi)));
-  for (int w = 0; w < io.Wd; ++w)
-    h = mix64(h ^ (uint64_t(masked_word(m, w, lo, hi, io.L)) + 0x9e3779b97f4a7c15ULL * uint64_t(w + 1)));
-  uint32_t s = (uint32_t)h & io.cap_mask;
-  for (uint32_t probes = 0; probes <= io.cap_mask; ++probes) {
-    const uint32_t prev = atomicCAS(&io.table[s], AM_EMPTY, (uint32_t)q);
-    if (prev == AM_EMPTY) {
-      io.status[q] = 1;
-      atomicMin(&io.winner[q], q);
-      return;
-    }
-    int r2, lo2, hi2;
-    vkey_parts(io.vpair, io.info, io.vm, io.L, (int)prev, r2, lo2, hi2);
-    bool eq = (lo2 == lo && hi2 == hi);
-    if (eq) {
-      const uint32_t* m2 = io.reg_mask + (size_t)r2 * io.Wd;
-      for (int w = 0; w < io.Wd && eq; ++w)
-        eq = masked_word(m, w, lo, hi, io.L) == masked_word(m2, w, lo2, hi2, io.L);
-    }
-    if (eq) {
-      io.status[q] = 2;
-      io.owner[q] = (int)prev;
-      atomicMin(&io.winner[prev], q);
-      return;
-    }
-    s = (s + 1) & io.cap_mask;
-  }
-  io.status[q] = 4;
-}
-
-__global__ void k_vmark_first(const int* status, const int* winner, int* first, int n) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  if (status[q] == 1) first[winner[q]] = 1;
-}
-
-// Vertex index of every (region, vertex) item and the unique vertex coordinates.
-__global__ void k_vassign(VertexIO io, const int* first, const int* rank, int* vid, double* out_xyz,
-                          double scale, double cx, double cy, double cz) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= io.n) return;
-  const int st = io.status[q];
-  if (st != 1 && st != 2) { vid[q] = -1; return; }
-  const int grp = st == 1 ? q : io.owner[q];
-  const int w = io.winner[grp];
-  const int id = rank[w];
-  vid[q] = id;
-  if (first[q]) {
-    out_xyz[(size_t)id * 3 + 0] = io.verts[(size_t)q * 3 + 0] * scale + cx;
-    out_xyz[(size_t)id * 3 + 1] = io.verts[(size_t)q * 3 + 1] * scale + cy;
-    out_xyz[(size_t)id * 3 + 2] = io.verts[(size_t)q * 3 + 2] * scale + cz;
-  }
-}
-
-__global__ void k_face_flags(const int* info, int R, int* fflag, int* fnv) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R) return;
-  const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
-  fflag[r] = nv >= 3 ? 1 : 0;
-  fnv[r] = nv >= 3 ? nv : 0;
-}
-
-__global__ void k_face_write(const int* info, const int* fflag, const int* frank, const int* foff, const int* vid,
-                             int R, int vm, int* face_off, int* face_idx) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= R || !fflag[r]) return;
-  const int f = frank[r];
-  const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
-  const int o = foff[r];
-  face_off[f] = o;
-  for (int k = 0; k < nv; ++k) face_idx[o + k] = vid[(size_t)r * vm + k];
-}
-
 }  // namespace am
 
 namespace am {
@@ -235,16 +127,20 @@ struct RowHash {
   }
 };
 
-// out[0..2]: visited, faces, vertex-key sums; out[3], out[4]: faces, vertex uses.
+// out[0..2]: visited, faces, vertex-key sums; out[3], out[4]: faces, vertex uses;
+// out[5]: regions.  Rows hash the first Wh of a mask's Wd words; with net >= 0
+// only regions whose last mask word equals net count (a batch of networks).
 __global__ __launch_bounds__(256) void k_fingerprint(const uint32_t* reg_mask, const int* info, const int* vpair,
-                                                     int R, int Wd, int vm, int L, unsigned long long* out) {
-  __shared__ unsigned long long red[5][256];
+                                                     int R, int Wd, int Wh, int vm, int L, int net,
+                                                     unsigned long long* out) {
+  __shared__ unsigned long long red[6][256];
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  uint64_t acc[5] = {0, 0, 0, 0, 0};
-  if (r < R) {
+  uint64_t acc[6] = {0, 0, 0, 0, 0, 0};
+  if (r < R && (net < 0 || reg_mask[(size_t)r * Wd + Wd - 1] == (uint32_t)net)) {
     const uint32_t* m = reg_mask + (size_t)r * Wd;
+    acc[5] = 1;
     RowHash hv;
-    for (int w = 0; w < Wd; ++w) hv.col((int64_t)m[w]);
+    for (int w = 0; w < Wh; ++w) hv.col((int64_t)m[w]);
     acc[0] = hv.done();
     const int nv = info[(size_t)r * AM_INFO_STRIDE + 2];
     if (nv >= 3) {
@@ -257,7 +153,7 @@ __global__ __launch_bounds__(256) void k_fingerprint(const uint32_t* reg_mask, c
         if (key < best) { best = key; i0 = k; }
       }
       RowHash hf;
-      for (int w = 0; w < Wd; ++w) hf.col((int64_t)m[w]);
+      for (int w = 0; w < Wh; ++w) hf.col((int64_t)m[w]);
       hf.col(nv);
       int k = i0;
       for (int s = 0; s < vm; ++s) {
@@ -277,7 +173,7 @@ __global__ __launch_bounds__(256) void k_fingerprint(const uint32_t* reg_mask, c
         RowHash hk;
         hk.col(lo);
         hk.col(hi);
-        for (int w = 0; w < Wd; ++w) {
+        for (int w = 0; w < Wh; ++w) {
           uint32_t x = m[w];
           if (lo >= 0 && lo < L && (lo >> 5) == w) x &= ~(1u << (lo & 31));
           if (hi >= 0 && hi < L && (hi >> 5) == w) x &= ~(1u << (hi & 31));
@@ -288,14 +184,32 @@ __global__ __launch_bounds__(256) void k_fingerprint(const uint32_t* reg_mask, c
       acc[4] = (uint64_t)nv;
     }
   }
-  for (int q = 0; q < 5; ++q) red[q][threadIdx.x] = acc[q];
+  for (int q = 0; q < 6; ++q) red[q][threadIdx.x] = acc[q];
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s)
-      for (int q = 0; q < 5; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+      for (int q = 0; q < 6; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 5) atomicAdd(&out[threadIdx.x], red[threadIdx.x][0]);
+  if (threadIdx.x < 6) atomicAdd(&out[threadIdx.x], red[threadIdx.x][0]);
+}
+
+// Regions per network of level regions first .. first + count - 1 (network =
+// last mask word).
+__global__ void k_net_hist(const uint32_t* reg_mask, int Wd, int first, int count, int* hist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) atomicAdd(&hist[reg_mask[(size_t)(first + i) * Wd + Wd - 1]], 1);
+}
+
+// Ids of the regions of network `net` (last mask word), in region order:
+// flags for the scan, then the scatter.
+__global__ void k_net_flags(const uint32_t* reg_mask, int Wd, int R, int net, int* flags) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R) flags[r] = reg_mask[(size_t)r * Wd + Wd - 1] == (uint32_t)net ? 1 : 0;
+}
+__global__ void k_net_scatter(const int* flags, const int* rank, int R, int* ids) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r < R && flags[r]) ids[rank[r]] = r;
 }
 
 }  // namespace am

@@ -69,7 +69,18 @@ struct ChainIO {
   int noutp[AM_MAX_LINEAR];         // n_out rounded up to AM_CR
 };
 
+// The regions and plane buffers of one tiled-chain launch.
+struct ChainLevel {
+  int first, count;           // regions first .. first + count - 1 (or ids[0 .. count))
+  const int* ids;
+  double* planes;             // [count][4][Cs]
+  double* eq;                 // [count][4]
+};
+
 __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
+// A batch of networks (each region's network is the last word of its mask):
+// tiles[b] = (network, first slot, regions, -) of one network.
+__global__ void k_chain_tile_fp64_multi(const NetDev* nets, const ChainIO* cios, ChainLevel lv, const int4* tiles);
 // The walk of k_region_fp64 on planes from k_chain_tile_fp64.  Block b takes
 // planes slot s = slots[b] (or b); its region is io.ids[s] when ids are given,
 // else rid0 + s; candidate slots are numbered from io.first as in k_region_fp64.
@@ -80,6 +91,13 @@ struct WalkParams;
 // only_undecided: blocks whose region is not flagged AM_UNDECIDED in io.flags exit.
 __global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, const double* planes,
                              const double* eq, int Cs_hbm, int only_undecided);
+// Batch-of-networks forms (region rid0 + b; its network is mask word Wd - 1;
+// the scan's LDS layout follows each network's C, the launch is sized for the
+// widest network).
+__global__ void k_walk_fp64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0, const int* count,
+                                  const int* slots, const double* planes, const double* eq, int Cs_hbm);
+__global__ void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0, WalkParams wp,
+                                   const double* planes, const double* eq, int Cs_hbm);
 // Chunk-local slots of the regions k_walk_ray64 left undecided (cnt[0] counts
 // them); run[0] accumulates the count and run[1 + w] the reasons, per run.
 __global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,

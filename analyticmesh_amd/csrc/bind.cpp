@@ -164,6 +164,110 @@ struct Engine {
   }
 };
 
+struct Batch {
+  am_batch* b = nullptr;
+  int vm = 20;
+  std::vector<int> Wn;
+
+  Batch(const std::vector<std::vector<int>>& nets, int precision) {
+    std::vector<int> nn;
+    std::vector<const int*> ptrs;
+    for (auto& n : nets) {
+      nn.push_back((int)n.size());
+      ptrs.push_back(n.data());
+      int L = 0;
+      for (size_t k = 1; k + 1 < n.size(); ++k) L += n[k];
+      Wn.push_back((L + 31) / 32);
+    }
+    const am_status s = am_batch_create(&b, (int)nets.size(), nn.data(), ptrs.data(), precision);
+    if (s != AM_OK) throw std::runtime_error("am error " + std::to_string(s) + ": " + am_last_error(nullptr));
+  }
+  ~Batch() {
+    if (b) am_batch_destroy(b);
+  }
+  void chk(am_status s) {
+    if (s != AM_OK) throw std::runtime_error("am error " + std::to_string(s) + ": " + am_batch_last_error(b));
+  }
+  void set_option(const std::string& key, long long value) {
+    chk(am_batch_set_option(b, key.c_str(), value));
+    if (key == "vm") vm = (int)value;
+  }
+  void run(const std::vector<uintptr_t>& weights, const std::vector<uintptr_t>& biases, int dtype,
+           const std::vector<uintptr_t>& points, const std::vector<uintptr_t>& states, const std::vector<int>& n_seeds,
+           double iso, bool flip, uintptr_t stream) {
+    std::vector<const void*> w(weights.size()), bb(biases.size()), p(points.size());
+    std::vector<const uint8_t*> st(states.size());
+    for (size_t i = 0; i < w.size(); ++i) w[i] = (const void*)weights[i];
+    for (size_t i = 0; i < bb.size(); ++i) bb[i] = (const void*)biases[i];
+    for (size_t i = 0; i < p.size(); ++i) p[i] = (const void*)points[i];
+    for (size_t i = 0; i < st.size(); ++i) st[i] = (const uint8_t*)states[i];
+    if (p.size() != Wn.size() || st.size() != Wn.size() || n_seeds.size() != Wn.size())
+      throw std::runtime_error("one seed set per network");
+    am_status s;
+    {
+      py::gil_scoped_release nogil;
+      s = am_batch_run(b, w.data(), bb.data(), dtype, p.data(), st.data(), n_seeds.data(), iso, flip ? 1 : 0,
+                       (void*)stream);
+    }
+    chk(s);
+  }
+  py::dict fingerprint(int net) {
+    unsigned long long f[6];
+    chk(am_batch_fingerprint(b, net, f));
+    char buf[3][17];
+    for (int i = 0; i < 3; ++i) std::snprintf(buf[i], sizeof(buf[i]), "%016llx", f[i]);
+    py::dict d;
+    d["regions"] = (long long)f[5];
+    d["faces"] = (long long)f[3];
+    d["vertex_uses"] = (long long)f[4];
+    d["visited"] = std::string(buf[0]);
+    d["faces_hash"] = std::string(buf[1]);
+    d["vertex_keys"] = std::string(buf[2]);
+    return d;
+  }
+  py::dict regions(int net) {
+    if (net < 0 || net >= (int)Wn.size()) throw std::runtime_error("no such network");
+    long long r = 0;
+    chk(am_batch_get_regions(b, net, &r, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr));
+    py::array_t<uint32_t> masks({(py::ssize_t)r, (py::ssize_t)Wn[net]});
+    py::array_t<int> info({(py::ssize_t)r, (py::ssize_t)8});
+    py::array_t<int> tabs({(py::ssize_t)r, (py::ssize_t)(vm + 1)});
+    py::array_t<double> verts({(py::ssize_t)r, (py::ssize_t)vm, (py::ssize_t)3});
+    py::array_t<int> vpairs({(py::ssize_t)r, (py::ssize_t)vm, (py::ssize_t)2});
+    py::array_t<double> points({(py::ssize_t)r, (py::ssize_t)3});
+    if (r)
+      chk(am_batch_get_regions(b, net, &r, masks.mutable_data(), info.mutable_data(), tabs.mutable_data(),
+                               verts.mutable_data(), vpairs.mutable_data(), points.mutable_data()));
+    py::dict d;
+    d["masks"] = masks;
+    d["info"] = info;
+    d["tabs"] = tabs;
+    d["verts"] = verts;
+    d["vpairs"] = vpairs;
+    d["points"] = points;
+    return d;
+  }
+  void combine(double scale, const std::vector<double>& center) {
+    if (center.size() != 3) throw std::runtime_error("center must have 3 entries");
+    am_status s;
+    {
+      py::gil_scoped_release nogil;
+      s = am_batch_combine(b, scale, center.data());
+    }
+    chk(s);
+  }
+  py::tuple counts() {
+    long long r, f, v, n;
+    chk(am_batch_get_counts(b, &r, &f, &v, &n));
+    return py::make_tuple(r, f, v, n);
+  }
+  py::array_t<double> stats() {
+    py::array_t<double> s((py::ssize_t)AM_STAT_COUNT);
+    chk(am_batch_get_stats(b, s.mutable_data(), AM_STAT_COUNT));
+    return s;
+  }
+};
+
 }  // namespace
 
 PYBIND11_MODULE(_am, m) {
@@ -190,4 +294,13 @@ PYBIND11_MODULE(_am, m) {
       .def("planes", &Engine::planes)
       .def("fingerprint", &Engine::fingerprint)
       .def("stats", &Engine::stats);
+  py::class_<Batch>(m, "Batch")
+      .def(py::init<const std::vector<std::vector<int>>&, int>(), py::arg("nets"), py::arg("precision"))
+      .def("set_option", &Batch::set_option)
+      .def("run", &Batch::run)
+      .def("fingerprint", &Batch::fingerprint)
+      .def("regions", &Batch::regions)
+      .def("combine", &Batch::combine)
+      .def("counts", &Batch::counts)
+      .def("stats", &Batch::stats);
 }

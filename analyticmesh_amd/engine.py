@@ -264,3 +264,74 @@ def march_batch(specs, seeds, precision="fp32", workers=16, vm=20, walk="ray", c
     with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as ex:
         out = list(ex.map(one, range(len(specs))))
     return out, time.perf_counter() - t0
+
+
+class BatchResult:
+    """Per-network views of one batch run (records are gathered from device memory on demand)."""
+
+    def __init__(self, batch, specs, vm):
+        self._b = batch
+        self.specs = specs
+        self.vm = vm
+        self.stats = batch.stats()
+
+    def __len__(self):
+        return len(self.specs)
+
+    def fingerprint(self, i):
+        return dict(self._b.fingerprint(int(i)))
+
+    def regions(self, i):
+        return dict(self._b.regions(int(i)))
+
+    def records(self, i):
+        d = self.regions(i)
+        return RegionRecords(d["masks"], d["info"], d["tabs"], d["verts"], d["vpairs"], L=self.specs[i].L)
+
+    def combine(self, scale=1.0, center=(0.0, 0.0, 0.0)):
+        """Vertex dedup and face indexing of every network at once; returns (V, F, face indices) counts."""
+        self._b.combine(float(scale), [float(c) for c in center])
+        r, f, v, n = self._b.counts()
+        self.combine_ms = float(self._b.stats()[10])
+        return int(v), int(f), int(n)
+
+    @property
+    def run_ms(self):
+        return float(self.stats[0])
+
+
+class BatchEngine:
+    """Many networks marched in one job by one set of kernel launches per breadth-first level
+    (``am_batch_*``).  precision: "fp64" (bit-exact scan) or "fp64ray"."""
+
+    def __init__(self, specs, precision="fp64ray", vm=20):
+        m = native()
+        self.specs = list(specs)
+        for s in self.specs:
+            if getattr(s, "has_skips", False):
+                raise ValueError(f"{s.name}: a batch takes networks without skips")
+        self.vm = int(vm)
+        self._b = m.Batch([list(map(int, s.nodes)) for s in self.specs], PRECISIONS[precision])
+        self._b.set_option("vm", self.vm)
+        self._keep = []
+
+    def run(self, seeds, iso=0.0, flip=False, dtype=np.float64):
+        """seeds: list of (points (N,3), states (N,L)) per network."""
+        keep = []
+
+        def cont(a, dt):
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return int(a.ctypes.data)
+
+        W, B, P, S, N = [], [], [], [], []
+        for spec, (pts, st) in zip(self.specs, seeds):
+            W += [cont(w, dtype) for w in spec.weights]
+            B += [cont(b, dtype) for b in spec.biases]
+            pts = np.asarray(pts, dtype=np.float64).reshape(-1, 3)
+            P.append(cont(pts, dtype))
+            S.append(cont(np.asarray(st).reshape(len(pts), spec.L), np.uint8))
+            N.append(len(pts))
+        self._b.run(W, B, 1 if dtype == np.float64 else 0, P, S, N, float(iso), bool(flip), 0)
+        self._keep = keep
+        return BatchResult(self._b, self.specs, self.vm)

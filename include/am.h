@@ -154,8 +154,53 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_VERT64_MS 32    /* FP64 vertex re-solve of regions the reduced-precision walk decided (HIP events) */
 #define AM_STAT_VERT64_REGIONS 33 /* regions whose vertices were re-solved from FP64 planes */
 #define AM_STAT_FP64_ONLY 34    /* 1: a reduced form ran on the FP64 region kernel (skips or layers over 256 wide) */
-#define AM_STAT_COUNT 35
+#define AM_STAT_HOST_WAITS 35   /* host waits on the device during the run (batch of networks) */
+#define AM_STAT_COUNT 36
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
+
+/* ---- Many networks in one job (no counterpart in the reference, which marches
+ * one network per Init and re-runs Init when the architecture changes,
+ * backend/main.py:437-448, backend/src/cuam_kernel.cu:53-98).  All networks of a
+ * batch march together: each breadth-first level of every network goes through
+ * one launch of each kernel (the FP64 plane chain over tiles of one network,
+ * the walks over regions of any network), with one region store and visited
+ * set.  Per network the records equal those of its own am_run in the same
+ * precision.  Networks: no skips, no extra planes, layers up to 256 wide. */
+typedef struct am_batch am_batch;
+
+/* n_nets networks; network i has n_nodes[i] widths nodes[i][0..n_nodes[i]).
+ * precision: AM_PREC_FP64 (candidate scan, the FP64 records of am_run) or
+ * AM_PREC_FP64_RAY (checked FP64 ray walk, scan where undecided). */
+am_status am_batch_create(am_batch** out, int n_nets, const int* n_nodes, const int* const* nodes, int precision);
+
+/* "vm", "table_log2", "region_limit" as for am_set_option. */
+am_status am_batch_set_option(am_batch* b, const char* key, long long value);
+
+/* weights / biases: the linear layers of network 0, then of network 1, ...
+ * (n_nodes[i] - 1 arrays each, shaped as for am_run); points[i] (n_seeds[i] x 3)
+ * and states[i] (n_seeds[i] x L_i bytes) are network i's seeds. */
+am_status am_batch_run(am_batch* b, const void* const* weights, const void* const* biases, int dtype,
+                       const void* const* points, const uint8_t* const* states, const int* n_seeds, double iso,
+                       int flip_insideout, void* stream);
+
+/* Topology fingerprint of one network's regions (as am_topology_fingerprint;
+ * out6[5] = its region count). */
+am_status am_batch_fingerprint(am_batch* b, int net, unsigned long long* out6);
+
+/* One network's region records in its visiting order (layouts of
+ * am_get_regions with that network's mask words); n_regions receives the
+ * count; with every array NULL only the count is returned. */
+am_status am_batch_get_regions(am_batch* b, int net, long long* n_regions, uint32_t* masks, int* info, int* tabs,
+                               double* verts, int* vpairs, double* points);
+
+/* Vertex dedup and face indexing of all networks at once (vertices of
+ * different networks never merge); counts and stats as for one context. */
+am_status am_batch_combine(am_batch* b, double scale, const double* center3);
+am_status am_batch_get_counts(am_batch* b, long long* n_regions, long long* n_faces, long long* n_vertices,
+                              long long* n_face_indices);
+am_status am_batch_get_stats(am_batch* b, double* out, int n);
+const char* am_batch_last_error(const am_batch* b);
+am_status am_batch_destroy(am_batch* b);
 
 #ifdef __cplusplus
 }
