@@ -5,6 +5,12 @@
 
 #include "am_limits.h"
 
+// am_combine_noalloc: largest inputs its static scratch takes, and the scratch size
+#define AM_NOALLOC_MAX_REGIONS (1 << 18)
+#define AM_NOALLOC_MAX_ITEMS (1 << 20)
+#define AM_NOALLOC_SCRATCH_BYTES ((size_t)28 << 20)
+#define AM_NOALLOC_MAX_DEVICES 64
+
 namespace am {
 
 struct CombineIn {
@@ -49,5 +55,9 @@ void combine_dedup(const CombineIn& in, CombineWork& w, long long N, hipStream_t
 // face-index array, vertex coordinates (v * scale + center) and face offsets
 void combine_assign(const CombineIn& in, const CombineWork& w, const CombineOut& out, long long N, double scale,
                     const double* c3, hipStream_t st);
+
+// Device address and size of this GPU's static scratch (the address is looked
+// up once per device, with that device current).
+hipError_t noalloc_scratch(void** out, size_t* bytes);
 
 }  // namespace am

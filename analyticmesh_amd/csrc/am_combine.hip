@@ -22,10 +22,12 @@
 //   4. k_cassign, one thread per item: vertex index into the face-index array,
 //      coordinates by the first item, face offsets by each face's first item.
 //
-// Scratch comes from a caller-provided workspace (CombineWork).
+// Scratch comes from a caller-provided workspace (CombineWork), so the same
+// kernels serve the allocating entry point and the one that allocates nothing.
 #include <algorithm>
 #include <climits>
 #include <cstdio>
+#include <mutex>
 
 #include "am_combine.h"
 #include "am_device.h"
@@ -224,6 +226,35 @@ void combine_assign(const CombineIn& in, const CombineWork& w, const CombineOut&
                     const double* c3, hipStream_t st) {
   hipLaunchKernelGGL(k_cassign, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, in, w, out, (int)N, scale,
                      c3[0], c3[1], c3[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of am_combine_noalloc: one static device array per GPU, reached
+// through its device address (hipGetSymbolAddress, once per device).  Calls on
+// one GPU share it: overlapping calls race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[AM_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[AM_NOALLOC_MAX_DEVICES] = {nullptr};
+}  // namespace
+
+hipError_t noalloc_scratch(void** out, size_t* bytes) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= AM_NOALLOC_MAX_DEVICES) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  *bytes = sizeof(scratch);
+  return hipSuccess;
 }
 
 }  // namespace am

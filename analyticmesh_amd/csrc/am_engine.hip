@@ -1212,6 +1212,72 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
   });
 }
 
+am_status am_combine_noalloc(am_ctx* c, double scale, const double* center3, double* vertices,
+                             long long vertex_cap, int* face_offsets, long long face_cap, int* face_indices,
+                             long long index_cap, long long* counts3) {
+  if (!c) return AM_ERR_ARG;
+  if (!c->has_run) return fail(c, AM_ERR_STATE, "am_combine_noalloc called before am_run");
+  return guarded(c, [&]() -> am_status {
+    if (!vertices || !face_offsets || !face_indices || !counts3) throw ArgErr{"null output buffer"};
+    for (const void* p : {(const void*)vertices, (const void*)face_offsets, (const void*)face_indices}) {
+      hipPointerAttribute_t a{};
+      if (hipPointerGetAttributes(&a, p) != hipSuccess ||
+          (a.type != hipMemoryTypeDevice && a.type != hipMemoryTypeManaged)) {
+        (void)hipGetLastError();
+        throw ArgErr{"am_combine_noalloc writes its outputs from the device: pass device memory"};
+      }
+    }
+    const hipStream_t st = c->stream;
+    const double c3[3] = {center3 ? center3[0] : 0.0, center3 ? center3[1] : 0.0, center3 ? center3[2] : 0.0};
+    const long long R = c->R;
+    counts3[0] = counts3[1] = counts3[2] = 0;
+    if (R > AM_NOALLOC_MAX_REGIONS)
+      throw CapErr{"am_combine_noalloc takes at most " + std::to_string(AM_NOALLOC_MAX_REGIONS) + " regions (" +
+                   std::to_string(R) + " given)"};
+    if (face_cap < 1) throw CapErr{"face_offsets needs room for at least one entry"};
+    if (R == 0) {
+      const int zero = 0;
+      HIPCHK(hipMemcpyAsync(face_offsets, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+      HIPCHK(hipStreamSynchronize(st));
+      return AM_OK;
+    }
+    void* base = nullptr;
+    size_t bytes = 0;
+    HIPCHK(noalloc_scratch(&base, &bytes));
+    const size_t rb = combine_region_bytes(R);
+    const CombineIn in = combine_in(c);
+    CombineWork w;
+    combine_carve_regions((char*)base, R, w);
+    long long F = 0, N = 0;
+    combine_counts(c, in, w, F, N);
+    if (N > AM_NOALLOC_MAX_ITEMS || rb + combine_item_bytes(N) > bytes)
+      throw CapErr{"am_combine_noalloc takes at most " + std::to_string(AM_NOALLOC_MAX_ITEMS) +
+                   " vertex uses (" + std::to_string(N) + " in this mesh)"};
+    if (F + 1 > face_cap || N > index_cap)
+      throw CapErr{"output too small: the mesh has " + std::to_string(F) + " faces and " + std::to_string(N) +
+                   " face indices"};
+    long long V = 0;
+    if (N > 0) {
+      combine_carve_items((char*)base + rb, N, w);
+      combine_dedup(in, w, N, st);
+      V = combine_vertices(c, w);
+      if (V > vertex_cap)
+        throw CapErr{"output too small: the mesh has " + std::to_string(V) + " vertices"};
+      CombineOut out{vertices, face_offsets, face_indices, (int)F};
+      combine_assign(in, w, out, N, scale, c3, st);
+      HIPCHK(hipGetLastError());
+    } else {
+      const int zero = 0;
+      HIPCHK(hipMemcpyAsync(face_offsets, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    counts3[0] = V;
+    counts3[1] = F;
+    counts3[2] = N;
+    return AM_OK;
+  });
+}
+
 am_status am_get_counts(am_ctx* c, long long* nr, long long* nf, long long* nv, long long* ni) {
   if (!c) return AM_ERR_ARG;
   if (nr) *nr = c->R;

@@ -157,6 +157,8 @@ struct Engine {
     return d;
   }
 
+  uintptr_t ctx_address() const { return (uintptr_t)c; }
+
   py::array_t<double> stats() {
     py::array_t<double> s((py::ssize_t)AM_STAT_COUNT);
     check(c, am_get_stats(c, s.mutable_data(), AM_STAT_COUNT));
@@ -293,6 +295,7 @@ PYBIND11_MODULE(_am, m) {
       .def("adjoint", &Engine::adjoint)
       .def("planes", &Engine::planes)
       .def("fingerprint", &Engine::fingerprint)
+      .def("ctx_address", &Engine::ctx_address)
       .def("stats", &Engine::stats);
   py::class_<Batch>(m, "Batch")
       .def(py::init<const std::vector<std::vector<int>>&, int>(), py::arg("nets"), py::arg("precision"))

@@ -65,6 +65,21 @@ am_status am_run(am_ctx* ctx, const void* const* weights, const void* const* bia
  * deduplicates vertices and indexes faces; output vertex = v * scale + center. */
 am_status am_combine(am_ctx* ctx, double scale, const double* center3);
 
+/* Variant of am_combine that allocates nothing (no counterpart in the reference;
+ * for callers that cannot allocate device memory: code under graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same mesh as
+ * am_combine + am_get_mesh, written into the caller's DEVICE arrays: vertices
+ * (vertex_cap x 3 double), face_offsets (face_cap >= faces + 1 ints),
+ * face_indices (index_cap ints); counts3 (host) receives vertices, faces, face
+ * indices.  Scratch is a static device array of 28 MiB per GPU (held until the
+ * process ends), which bounds the input to 2^18 regions and 2^20 vertex uses;
+ * larger meshes, or outputs too small, fail with AM_ERR_CAPACITY.  All calls on
+ * one GPU share that scratch: calls that overlap on one GPU race.  The
+ * context's own mesh (am_get_mesh, am_export_ply, am_adjoint) is not changed. */
+am_status am_combine_noalloc(am_ctx* ctx, double scale, const double* center3, double* vertices,
+                             long long vertex_cap, int* face_offsets, long long face_cap, int* face_indices,
+                             long long index_cap, long long* counts3);
+
 /* Replaces ExportMesh(file_path, is_polymesh, is_float32)
  * (backend/src/cuam_kernel.cu:205-246, backend/inc/polymesh.h:350-423):
  * binary little-endian PLY, polygons or fan triangles, float or double xyz. */
