@@ -404,6 +404,7 @@ __global__ __launch_bounds__(64) void k_walk_fp64_multi(const NetDev* nets, Regi
 // planes are the canonical FP64 planes, so the only uncertainty is the
 // rounding of the walk's own evaluations; regions it cannot certify are
 // flagged (io.flags, level-relative) for the candidate scan.
+template <int NW>
 __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionIO& io, int rid, int b,
                                                 const WalkParams& wp, const double* planes, const double* eqb,
                                                 int Cs_hbm) {
@@ -418,9 +419,9 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
   double* Pc = Pz + Cl;
   uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cl);
   const int lane = threadIdx.x;
-  for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  for (int w = lane; w < net.Wd; w += 64 * NW) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)b * 4 * Cs_hbm + n1;
-  for (int i = lane; i < Cl; i += 64 * AM_RAY_WAVES) {
+  for (int i = lane; i < Cl; i += 64 * NW) {
     Px[i] = src[i];
     Py[i] = src[Cs_hbm + i];
     Pz[i] = src[2 * (size_t)Cs_hbm + i];
@@ -434,27 +435,40 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
                         eqb[(size_t)b * 4 + 3]};
   __syncthreads();
   const PlaneView<double> P{Px, Py, Pz, Pc, n1, net.W0, net.b[0], smask};
-  walk_ray<double, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
+  walk_ray<double, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
 }
 
-__global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
-                                                   const double* planes, const double* eqb, int Cs_hbm,
-                                                   int only_undecided) {
+// NW waves per region (the records do not depend on NW: every reduction breaks
+// ties by constraint index and each serial step runs on one wave).
+template <int NW>
+__global__ __launch_bounds__(64 * NW, 20 / NW) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
+                                                        const double* planes, const double* eqb, int Cs_hbm,
+                                                        int only_undecided) {
   if ((int)blockIdx.x >= io.count) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
   // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
   if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
-  walk_ray64_body(net, io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
+  walk_ray64_body<NW>(net, io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
 }
 
-__global__ __launch_bounds__(64 * AM_RAY_WAVES, 5) void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd,
-                                                                         int rid0, WalkParams wp,
-                                                                         const double* planes, const double* eqb,
-                                                                         int Cs_hbm) {
+template <int NW>
+__global__ __launch_bounds__(64 * NW, 20 / NW) void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0,
+                                                              WalkParams wp, const double* planes, const double* eqb,
+                                                              int Cs_hbm) {
   if ((int)blockIdx.x >= io.count) return;
   const int rid = rid0 + blockIdx.x;
-  walk_ray64_body(nets[region_net(io, Wd, rid)], io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
+  walk_ray64_body<NW>(nets[region_net(io, Wd, rid)], io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
 }
+
+template __global__ void k_walk_ray64<2>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int);
+template __global__ void k_walk_ray64<4>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int);
+template __global__ void k_walk_ray64<8>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int);
+template __global__ void k_walk_ray64_multi<2>(const NetDev*, RegionIO, int, int, WalkParams, const double*,
+                                               const double*, int);
+template __global__ void k_walk_ray64_multi<4>(const NetDev*, RegionIO, int, int, WalkParams, const double*,
+                                               const double*, int);
+template __global__ void k_walk_ray64_multi<8>(const NetDev*, RegionIO, int, int, WalkParams, const double*,
+                                               const double*, int);
 
 // Vertices of a region whose polygon (the tight-constraint pair of every
 // vertex, its order and orientation) a reduced-precision walk certified: each

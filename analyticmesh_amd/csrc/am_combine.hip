@@ -13,7 +13,7 @@
 // Passes:
 //   1. region scan: packed (face flag << 32 | face size) -> first item and face
 //      rank of every region; the down-sweep also writes each item's region.
-//   2. k_cinsert, one wave per region: the wave reads the region's mask once
+//   2. k_cinsert, sixteen lanes per region: they read the region's mask once
 //      (coalesced); the key hash is a sum of per-word hashes, so each item's
 //      hash is the region's sum with the one or two changed words swapped out.
 //      Items claim a table slot by CAS; an item whose key is already present
@@ -37,9 +37,6 @@ namespace am {
 
 #define AM_CEMPTY 0xFFFFFFFFu
 
-__device__ __forceinline__ uint64_t word_hash(uint32_t x, int w) {
-  return mix64((uint64_t(x) << 32) ^ (0x9e3779b97f4a7c15ULL * uint64_t(w + 1)));
-}
 __device__ __forceinline__ uint64_t pair_hash(int lo, int hi) {
   return mix64((uint64_t(uint32_t(lo)) << 32) | uint32_t(hi));
 }
@@ -64,69 +61,76 @@ struct FlagSrc {
   __device__ unsigned operator()(size_t i) const { return p[i]; }
 };
 
-// One wave per region, four regions per 256-thread block.
+// Sixteen lanes per region, sixteen regions per 256-thread block: the lanes of
+// a region read its mask once (coalesced) into LDS and sum the per-word
+// hashes; each lane then takes items lane, lane + 16, ... of the region.
+#define AM_CG 16
 __global__ __launch_bounds__(256) void k_cinsert(CombineIn in, CombineWork w) {
-  __shared__ uint32_t s_mask[4][AM_MAX_WORDS];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int r = blockIdx.x * 4 + wid;
+  __shared__ uint32_t s_mask[256 / AM_CG][AM_MAX_WORDS];
+  const int g = threadIdx.x & (AM_CG - 1), grp = threadIdx.x / AM_CG;
+  const int r = blockIdx.x * (256 / AM_CG) + grp;
   const bool live = r < in.R;
   const int nv = live ? in.info[(size_t)r * AM_INFO_STRIDE + 2] : 0;
-  uint64_t g = 0;
-  if (live && lane < in.Wd) {
-    const uint32_t x = in.reg_mask[(size_t)r * in.Wd + lane];
-    s_mask[wid][lane] = x;
-    g = word_hash(x, lane);
-  }
-  // wave sum of the per-word hashes (mod 2^64)
+  uint64_t hs = 0;
+  if (live && nv >= 3)
+    for (int q = g; q < in.Wd; q += AM_CG) {
+      const uint32_t x = in.reg_mask[(size_t)r * in.Wd + q];
+      s_mask[grp][q] = x;
+      hs += word_hash(x, q);
+    }
+  // sum over the region's 16 lanes (mod 2^64)
 #pragma unroll
-  for (int d = 32; d > 0; d >>= 1) {
-    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)g, d, 64);
-    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(g >> 32), d, 64);
-    g += ((uint64_t)hi << 32) | lo;
+  for (int d = AM_CG / 2; d > 0; d >>= 1) {
+    const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)hs, d, 64);
+    const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(hs >> 32), d, 64);
+    hs += ((uint64_t)hi << 32) | lo;
   }
   __syncthreads();
-  if (!live || nv < 3 || lane >= nv) return;
-  const uint32_t* m = s_mask[wid];
-  const int item = (int)(w.roff[r] & 0xffffffffu) + lane;
-  const int a = in.vpair[((size_t)r * in.vm + lane) * 2 + 0], b = in.vpair[((size_t)r * in.vm + lane) * 2 + 1];
-  const int lo = a < b ? a : b, hi = a < b ? b : a;
-  // key hash: the region's word-hash sum with the words holding bits lo and hi cleared
-  uint64_t h = g;
-  const bool lo_in = lo >= 0 && lo < in.L, hi_in = hi >= 0 && hi < in.L;
-  const int wl = lo_in ? lo >> 5 : -1, wh = hi_in ? hi >> 5 : -1;
-  if (wl >= 0) {
-    uint32_t x = m[wl] & ~(1u << (lo & 31));
-    if (wh == wl) x &= ~(1u << (hi & 31));
-    h += word_hash(x, wl) - word_hash(m[wl], wl);
-  }
-  if (wh >= 0 && wh != wl) h += word_hash(m[wh] & ~(1u << (hi & 31)), wh) - word_hash(m[wh], wh);
-  h = mix64(h ^ pair_hash(lo, hi));
-  uint32_t s = (uint32_t)h & w.cap_mask;
-  for (uint32_t probes = 0; probes <= w.cap_mask; ++probes) {
-    const uint32_t prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
-    if (prev == AM_CEMPTY) break;
-    // compare keys with the slot's item (records are immutable here; any item
-    // the slot holds has the key of the item that claimed it)
-    const int r2 = w.item_region[prev];
-    const int k2 = (int)prev - (int)(w.roff[r2] & 0xffffffffu);
-    const int a2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 0], b2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 1];
-    bool eq = (a2 < b2 ? a2 : b2) == lo && (a2 < b2 ? b2 : a2) == hi;
-    if (eq) {
-      const uint32_t* m2 = in.reg_mask + (size_t)r2 * in.Wd;
-      for (int q = 0; q < in.Wd && eq; ++q) {
-        uint32_t x = m[q], y = m2[q];
-        if (q == wl) { x &= ~(1u << (lo & 31)); y &= ~(1u << (lo & 31)); }
-        if (q == wh) { x &= ~(1u << (hi & 31)); y &= ~(1u << (hi & 31)); }
-        eq = x == y;
+  if (!live || nv < 3) return;
+  const uint32_t* m = s_mask[grp];
+  const int item0 = (int)(w.roff[r] & 0xffffffffu);
+  for (int k = g; k < nv; k += AM_CG) {
+    const int item = item0 + k;
+    const int a = in.vpair[((size_t)r * in.vm + k) * 2 + 0], b = in.vpair[((size_t)r * in.vm + k) * 2 + 1];
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    // key hash: the region's word-hash sum with the words holding bits lo and hi cleared
+    uint64_t h = hs;
+    const bool lo_in = lo >= 0 && lo < in.L, hi_in = hi >= 0 && hi < in.L;
+    const int wl = lo_in ? lo >> 5 : -1, wh = hi_in ? hi >> 5 : -1;
+    const uint32_t bl = lo_in ? 1u << (lo & 31) : 0u, bh = hi_in ? 1u << (hi & 31) : 0u;
+    if (wl >= 0) {
+      uint32_t x = m[wl] & ~bl;
+      if (wh == wl) x &= ~bh;
+      h += word_hash(x, wl) - word_hash(m[wl], wl);
+    }
+    if (wh >= 0 && wh != wl) h += word_hash(m[wh] & ~bh, wh) - word_hash(m[wh], wh);
+    h = mix64(h ^ pair_hash(lo, hi));
+    uint32_t s = (uint32_t)h & w.cap_mask;
+    for (uint32_t probes = 0; probes <= w.cap_mask; ++probes) {
+      const uint32_t prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
+      if (prev == AM_CEMPTY) break;
+      // compare keys with the slot's item (records are immutable here; any item
+      // the slot holds has the key of the item that claimed it)
+      const int r2 = w.item_region[prev];
+      const int k2 = (int)prev - (int)(w.roff[r2] & 0xffffffffu);
+      const int a2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 0], b2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 1];
+      if ((a2 < b2 ? a2 : b2) == lo && (a2 < b2 ? b2 : a2) == hi) {
+        // the pairs agree, so bits lo and hi are the same words in both keys
+        const uint32_t* m2 = in.reg_mask + (size_t)r2 * in.Wd;
+        uint32_t diff = 0;
+        for (int q = 0; q < in.Wd; ++q) {
+          const uint32_t clr = (q == wl ? bl : 0u) | (q == wh ? bh : 0u);
+          diff |= (m[q] ^ m2[q]) & ~clr;
+        }
+        if (diff == 0) {
+          atomicMin(&w.table[s], (uint32_t)item);
+          break;
+        }
       }
+      s = (s + 1) & w.cap_mask;
     }
-    if (eq) {
-      atomicMin(&w.table[s], (uint32_t)item);
-      break;
-    }
-    s = (s + 1) & w.cap_mask;
+    w.slot[item] = (int)s;
   }
-  w.slot[item] = (int)s;
 }
 
 __global__ void k_cflag(CombineWork w, int N) {
@@ -216,7 +220,7 @@ void combine_dedup(const CombineIn& in, CombineWork& w, long long N, hipStream_t
   hipLaunchKernelGGL(k_citem_region, dim3((unsigned)((in.R + 255) / 256)), dim3(256), 0, st,
                      (const unsigned long long*)w.roff, in.info, in.R, w.item_region);
   hipLaunchKernelGGL(k_cfill, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, w.table, cap);
-  hipLaunchKernelGGL(k_cinsert, dim3((unsigned)((in.R + 3) / 4)), dim3(256), 0, st, in, w);
+  hipLaunchKernelGGL(k_cinsert, dim3((unsigned)((in.R + 256 / AM_CG - 1) / (256 / AM_CG))), dim3(256), 0, st, in, w);
   hipLaunchKernelGGL(k_cflag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, w, (int)N);
   scan_launch<unsigned>(FlagSrc{w.vflag}, (size_t)N, ArraySink<unsigned>{w.vrank}, w.ipart,
                         (unsigned*)(w.totals + 1), st);

@@ -106,6 +106,18 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
   x ^= x >> 33;
   return x;
 }
+// Hash of word x at position w.  Keys hash to the sum (mod 2^64) of their
+// words' hashes, so a key that differs from a known one in one word is hashed
+// in O(1): the known sum with that word's term swapped.
+__device__ __forceinline__ uint64_t word_hash(uint32_t x, int w) {
+  return mix64((uint64_t(x) << 32) ^ (0x9e3779b97f4a7c15ULL * uint64_t(w + 1)));
+}
+__device__ __forceinline__ uint64_t key_hash_sum(const uint32_t* m, int nw) {
+  uint64_t h = 0;
+  for (int w = 0; w < nw; ++w) h += word_hash(m[w], w);
+  return h;
+}
+
 __device__ __forceinline__ uint64_t hash_words(const uint32_t* m, int nw, uint64_t seed) {
   uint64_t h = seed;
   for (int w = 0; w < nw; ++w) h = mix64(h ^ (uint64_t(m[w]) + 0x9e3779b97f4a7c15ULL * uint64_t(w + 1)));

@@ -114,6 +114,11 @@ struct am_ctx {
   size_t max_lds = 65536;
   long long max_regions = -1;
   long long region_limit = -1;   // AM_ERR_CAPACITY past this many regions (-1: the engine's own limit)
+  int ray_waves = 4;             // waves per region of the FP64 ray walk (2, 4, 8)
+  bool graph_levels = false;     // each level's launches captured into a graph and launched as one
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};   // alternate levels: one may still run while the other is updated
+  long long graph_instantiations = 0, graph_updates = 0;
+  int scan_blocks = 2048;        // grid of the candidate scan over undecided regions
   bool adjoint_tiled = true;   // am_adjoint: tiled chain + GEMM pull-back where the tiled chain runs
   bool adjoint_mfma = true;    // am_adjoint's GEMMs on the FP64 matrix cores (else VALU fmas)
   int table_log2 = 22;
@@ -135,8 +140,9 @@ struct am_ctx {
 
   // level candidates
   DevBuf<uint32_t> cand_mask;
-  DevBuf<int> cand_start, status, winner, slot, owner, first, link, rank;
+  DevBuf<int> cand_start, slot, rank;
   DevBuf<double> cand_point;
+  DevBuf<uint64_t> cand_hash;
   DevBuf<int> scan_part;
   DevBuf<unsigned> rflags;
 
@@ -162,7 +168,7 @@ struct am_ctx {
   std::deque<DevBuf<char>> stage;
   size_t stage_used = 0;
 
-  // pinned, coherent host word the device writes scan totals into (scan_flags_to_host)
+  // pinned, coherent host word the device writes scan totals into (arm_host_total)
   int* host_total = nullptr;
   int* host_total_dev = nullptr;
 
@@ -202,31 +208,24 @@ void grow_cands(am_ctx* c, size_t n) {
   ensure(c->cand_mask, n * c->Wd, false, st);
   ensure(c->cand_start, n, false, st);
   ensure(c->cand_point, n * 3, false, st);
-  ensure(c->status, n, false, st);
-  ensure(c->winner, n, false, st);
+  ensure(c->cand_hash, n, false, st);
   ensure(c->slot, n, false, st);
-  ensure(c->owner, n, false, st);
-  ensure(c->first, n, false, st);
-  ensure(c->link, n, false, st);
   ensure(c->rank, n, false, st);
 }
 
-// Exclusive scan of n flags into rank (hand-written scan, am_scan.h).  The
+// The level's scan (hand-written, am_scan.h) over the new-region flags: the
 // total is written by the device straight into a pinned, coherent host word,
 // which the host then polls (wait_total): a short tight spin, then a yield
 // between polls so that host threads driving other contexts (a batch of
 // networks) keep their cores, and a stream sync after 100 ms, which also
 // reports a failed kernel.  Kernels enqueued after the scan may still run when
 // the word is read; later work on the stream stays ordered behind them.
-void scan_flags_to_host(am_ctx* c, const int* flags, int* rank, size_t n) {
+void arm_host_total(am_ctx* c) {
   if (!c->host_total) {
     HIPCHK(hipHostMalloc((void**)&c->host_total, 256, hipHostMallocMapped | hipHostMallocCoherent));
     HIPCHK(hipHostGetDevicePointer((void**)&c->host_total_dev, c->host_total, 0));
   }
   *reinterpret_cast<volatile int*>(c->host_total) = -1;
-  ensure(c->scan_part, scan_blocks(n) + 1, false, c->stream);
-  scan_launch<int>(ArraySrc<int>{flags}, n, ArraySink<int>{rank}, c->scan_part.p, c->host_total_dev, c->stream);
-  HIPCHK(hipGetLastError());
 }
 
 int wait_total(am_ctx* c) {
@@ -259,44 +258,77 @@ void ensure_table(am_ctx* c, size_t entries) {
   HIPCHK(hipGetLastError());
 }
 
-// Insert n candidates (already in cand_*) and append the new ones as regions.
-int insert_level(am_ctx* c, size_t n) {
-  if (n == 0) return 0;
-  const hipStream_t st = c->stream;
+// Visited-set stage of a level, in three parts so that the launches can be
+// captured into a graph: insert_prepare grows the table and the region store
+// (host work: allocations, the rehash), insert_launch enqueues the kernels
+// (nothing else), insert_finish waits for the count and commits late where the
+// store had to grow after it.
+struct InsertPlan {
+  size_t n = 0;
+  bool early = false;
+  int R0 = 0;
+};
+
+InsertPlan insert_prepare(am_ctx* c, size_t n) {
+  InsertPlan p;
+  p.n = n;
+  p.R0 = (int)c->R;
+  if (n == 0) return p;
   if (n > (size_t)INT_MAX / 2) throw CapErr{"too many candidates in one level"};
   ensure_table(c, (size_t)c->R + n);
   // Where the region store holds the worst case (every candidate new; grown
   // to it for levels up to 4M candidates), the new regions are committed
   // before the host learns how many there are: one wait per level, no host
   // round trip between the count and the commit.
-  const bool early = (size_t)c->R + n <= c->reg_cap || n <= ((size_t)1 << 22);
-  if (early) grow_regions(c, (size_t)c->R + n);
-  hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, c->winner.p, n, INT_MAX);
-  hipLaunchKernelGGL(k_fill_i32, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, n, 0);
+  p.early = (size_t)c->R + n <= c->reg_cap || n <= ((size_t)1 << 22);
+  if (p.early) grow_regions(c, (size_t)c->R + n);
+  ensure(c->scan_part, scan_blocks(n) + 1, false, c->stream);
+  arm_host_total(c);
+  return p;
+}
+
+CommitIO commit_io(const am_ctx* c, int R0) {
+  CommitIO cio;
+  cio.slot = c->slot.p;
+  cio.cand_mask = c->cand_mask.p;
+  cio.cand_start = c->cand_start.p;
+  cio.cand_point = c->cand_point.p;
+  cio.reg_mask = c->reg_mask.p;
+  cio.reg_start = c->reg_start.p;
+  cio.reg_point = c->reg_point.p;
+  cio.table = c->table.p;
+  cio.Wd = c->Wd;
+  cio.R0 = R0;
+  return cio;
+}
+
+void insert_launch(am_ctx* c, const InsertPlan& p) {
+  if (p.n == 0) return;
+  const hipStream_t st = c->stream;
+  const size_t n = p.n;
   InsertIO io;
   io.cand_mask = c->cand_mask.p;
+  io.cand_hash = c->cand_hash.p;
   io.cand_start = c->cand_start.p;
   io.reg_mask = c->reg_mask.p;
   io.table = c->table.p;
-  io.status = c->status.p;
-  io.winner = c->winner.p;
   io.slot = c->slot.p;
-  io.owner = c->owner.p;
   io.n = (int)n;
   io.Wd = c->Wd;
   io.cap_mask = (uint32_t)(c->table_cap - 1);
   hipLaunchKernelGGL(k_insert, dim3(blocks(n, 256)), dim3(256), 0, st, io);
-  hipLaunchKernelGGL(k_mark_first, dim3(blocks(n, 256)), dim3(256), 0, st, c->status.p, c->winner.p, c->first.p,
-                     c->link.p, (int)n);
   HIPCHK(hipGetLastError());
-  const auto commit = [&]() {
-    hipLaunchKernelGGL(k_commit, dim3(blocks(n, 256)), dim3(256), 0, st, c->first.p, c->rank.p, c->link.p,
-                       c->slot.p, c->cand_mask.p, c->cand_start.p, c->cand_point.p, c->reg_mask.p, c->reg_start.p,
-                       c->reg_point.p, c->table.p, (int)n, c->Wd, (int)c->R);
-    HIPCHK(hipGetLastError());
-  };
-  scan_flags_to_host(c, c->first.p, c->rank.p, n);
-  if (early) commit();
+  // ranks of the new-region flags; with the store already large enough the
+  // commit is enqueued before the host learns the count
+  const CommitIO cio = commit_io(c, p.R0);
+  scan_launch<int>(FirstSrc{cio}, n, ArraySink<int>{c->rank.p}, c->scan_part.p, c->host_total_dev, st);
+  if (p.early)
+    hipLaunchKernelGGL(k_commit, dim3(blocks(n, 256)), dim3(256), 0, st, cio, (const int*)c->rank.p, (int)n);
+  HIPCHK(hipGetLastError());
+}
+
+int insert_finish(am_ctx* c, const InsertPlan& p) {
+  if (p.n == 0) return 0;
   const int total = wait_total(c);
   if (total == 0) return 0;
   // am_combine indexes (region, vertex slot) items with 32-bit ints: R * vm must stay below 2^31
@@ -304,12 +336,70 @@ int insert_level(am_ctx* c, size_t n) {
   if (c->region_limit >= 0) limit = std::min(limit, c->region_limit);
   if ((long long)c->R + total > limit)
     throw CapErr{"region count would exceed the limit of " + std::to_string(limit) + " regions"};
-  if (!early) {
+  if (!p.early) {
     grow_regions(c, (size_t)c->R + total);
-    commit();
+    const CommitIO cio = commit_io(c, p.R0);
+    hipLaunchKernelGGL(k_commit, dim3(blocks(p.n, 256)), dim3(256), 0, c->stream, cio, (const int*)c->rank.p,
+                       (int)p.n);
+    HIPCHK(hipGetLastError());
   }
   c->R += total;
   return total;
+}
+
+// Capture the launches `enqueue` makes on the context's stream into a graph
+// and launch it.  Levels alternate between two executable graphs: the one of
+// two levels back is updated in place when the kernels are the same (only
+// arguments and grids differ), else instantiated anew.  (When a level's count
+// reaches the host, that level's last kernels may still run; the graph two
+// levels back has finished.)  The timer is off meanwhile (no event nodes) and
+// nothing in the capture allocates, copies synchronously or sets memory.
+template <typename F>
+void launch_as_graph(am_ctx* c, int parity, F&& enqueue) {
+  hipGraphExec_t& exec = c->gexec[parity & 1];
+  const hipStream_t st = c->stream;
+  hipGraph_t g = nullptr;
+  c->timer.enabled = false;
+  HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  try {
+    enqueue();
+  } catch (...) {
+    (void)hipStreamEndCapture(st, &g);
+    if (g) (void)hipGraphDestroy(g);
+    c->timer.enabled = true;
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(st, &g));
+  c->timer.enabled = true;
+  if (exec) {
+    hipGraphNode_t err_node = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(exec, g, &err_node, &res) == hipSuccess && res == hipGraphExecUpdateSuccess) {
+      c->graph_updates += 1;
+    } else {
+      (void)hipGetLastError();
+      (void)hipGraphExecDestroy(exec);
+      exec = nullptr;
+    }
+  }
+  if (!exec) {
+    const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      throw HipErr{e, "hipGraphInstantiate"};
+    }
+    c->graph_instantiations += 1;
+  }
+  const hipError_t e = hipGraphLaunch(exec, st);
+  (void)hipGraphDestroy(g);
+  HIPCHK(e);
+}
+
+// Insert n candidates (already in cand_*) and append the new ones as regions.
+int insert_level(am_ctx* c, size_t n) {
+  const InsertPlan p = insert_prepare(c, n);
+  insert_launch(c, p);
+  return insert_finish(c, p);
 }
 
 size_t region_lds_bytes(const am_ctx* c, int& Cs, int& Csort) {
@@ -334,6 +424,7 @@ void launch_region_fp64(am_ctx* c, int first, int count, const int* ids = nullpt
   io.cand_mask = c->cand_mask.p;
   io.cand_start = c->cand_start.p;
   io.cand_point = c->cand_point.p;
+  io.cand_hash = c->cand_hash.p;
   io.first = first;
   io.count = count;
   io.Cs = Cs;
@@ -398,6 +489,46 @@ ChainIO chain_io(const am_ctx* c, const int* ids, int first, int n, int Ch) {
   return cio;
 }
 
+// The FP64 ray walk with the context's waves per region.
+void launch_ray64(am_ctx* c, int n, size_t lds, const RegionIO& io, int rid0, int Ch, int only_undecided) {
+  const double* P = c->tplanes.p;
+  const double* E = c->teq.p;
+  switch (c->ray_waves) {
+    case 2:
+      hipLaunchKernelGGL(k_walk_ray64<2>, dim3(n), dim3(128), lds, c->stream, c->net, io, rid0, kRay64, P, E, Ch,
+                         only_undecided);
+      break;
+    case 8:
+      hipLaunchKernelGGL(k_walk_ray64<8>, dim3(n), dim3(512), lds, c->stream, c->net, io, rid0, kRay64, P, E, Ch,
+                         only_undecided);
+      break;
+    default:
+      hipLaunchKernelGGL(k_walk_ray64<4>, dim3(n), dim3(256), lds, c->stream, c->net, io, rid0, kRay64, P, E, Ch,
+                         only_undecided);
+  }
+}
+
+void set_ray64_lds(size_t lds) {
+  HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+}
+
+// Buffers of run_fp64_regions for a level of `count` regions (planes of one
+// chunk, the undecided-slot list); returns the chunk size.  Idempotent, so it
+// can run before a capture and again, as a no-op, inside it.
+size_t fp64_prepare(am_ctx* c, int count) {
+  const int Ch = (c->C + 63) & ~63;
+  size_t chunk = ((size_t)1 << 30) / ((size_t)32 * Ch);
+  chunk = std::max<size_t>(1024, std::min<size_t>(chunk, 65536)) & ~(size_t)(AM_CT - 1);
+  const size_t cap = std::min<size_t>(chunk, (size_t)count);
+  ensure(c->tplanes, cap * 4 * Ch, false, c->stream);
+  ensure(c->teq, cap * 4, false, c->stream);
+  ensure(c->tslots, cap, false, c->stream);
+  ensure(c->tcnt, 1, false, c->stream);
+  return chunk;
+}
+
 void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t,
                       long long undecided = -1) {
   if (!c->tile_ok) {
@@ -417,15 +548,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   const size_t wlds = region_lds_bytes(c, Cs, Csort);
   const size_t rlds = ray64_lds_bytes(c);
   const int Ch = (c->C + 63) & ~63;
-  size_t chunk = ((size_t)1 << 30) / ((size_t)32 * Ch);
-  chunk = std::max<size_t>(1024, std::min<size_t>(chunk, 65536)) & ~(size_t)(AM_CT - 1);
-  const size_t cap = std::min<size_t>(chunk, (size_t)count);
-  ensure(c->tplanes, cap * 4 * Ch, false, c->stream);
-  ensure(c->teq, cap * 4, false, c->stream);
-  if (mode == 1 || after_walk) {
-    ensure(c->tslots, cap, false, c->stream);
-    ensure(c->tcnt, 1, false, c->stream);
-  }
+  const size_t chunk = fp64_prepare(c, count);
   const size_t clds = chain_lds_bytes(c);
   StageTimer& tm = c->timer;
   for (int off = 0; off < count; off += (int)chunk) {
@@ -442,6 +565,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.cand_mask = c->cand_mask.p;
     io.cand_start = c->cand_start.p;
     io.cand_point = c->cand_point.p;
+    io.cand_hash = c->cand_hash.p;
     io.first = lvl_first;
     io.count = n;
     io.Cs = Cs;
@@ -473,8 +597,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
       continue;
     }
     if (mode == 1) {
-      hipLaunchKernelGGL(k_walk_ray64, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, c->stream, c->net, io, first + off,
-                         kRay64, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch, after_walk ? 1 : 0);
+      launch_ray64(c, n, rlds, io, first + off, Ch, after_walk ? 1 : 0);
       HIPCHK(hipGetLastError());
       const size_t m2 = tm.mark(c->stream);
       tm.span(m1v, m2, T_RAY64);
@@ -486,7 +609,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     HIPCHK(hipGetLastError());
     io.flags = nullptr;
     const size_t m3 = tm.mark(c->stream);
-    hipLaunchKernelGGL(k_walk_fp64, dim3(std::min(n, 2048)), dim3(64), wlds, c->stream, c->net, io, first + off,
+    hipLaunchKernelGGL(k_walk_fp64, dim3(std::min(n, c->scan_blocks)), dim3(64), wlds, c->stream, c->net, io, first + off,
                        (const int*)c->tcnt.p, (const int*)c->tslots.p, (const double*)c->tplanes.p,
                        (const double*)c->teq.p, Ch);
     HIPCHK(hipGetLastError());
@@ -628,9 +751,21 @@ size_t batch_chain_lds(const am_batch* b) {
 
 // Planes and walks of the level's regions (first .. first + count - 1, grouped
 // by network in ascending order; hist_h has the per-network counts).
-void batch_level(am_batch* b, long long first, long long count, Fp64Counts& t) {
+struct BatchLevelPlan {
+  long long first = 0, count = 0;
+  size_t chunk = 0, nt = 0;
+  std::vector<size_t> chunk_tile0;
+};
+
+void batch_level_launch(am_batch* b, const BatchLevelPlan& pl, Fp64Counts& t);
+
+// Host part of a level: buffers and the tile list (in pinned memory).
+BatchLevelPlan batch_level_prepare(am_batch* b, long long first, long long count) {
   am_ctx* c = b->c;
   const hipStream_t st = c->stream;
+  BatchLevelPlan pl;
+  pl.first = first;
+  pl.count = count;
   const int Ch = (b->Cmax + 63) & ~63;
   size_t chunk = ((size_t)1 << 30) / ((size_t)32 * Ch);
   chunk = std::max<size_t>(1024, std::min<size_t>(chunk, 65536)) & ~(size_t)(AM_CT - 1);
@@ -663,7 +798,23 @@ void batch_level(am_batch* b, long long first, long long count, Fp64Counts& t) {
     }
   }
   chunk_tile0.push_back(nt);
+  pl.chunk = chunk;
+  pl.nt = nt;
+  pl.chunk_tile0 = std::move(chunk_tile0);
+  // stream-ordered after the previous level's kernels (which read the old
+  // list) and kept out of graph captures (no memcpy nodes)
   HIPCHK(hipMemcpyAsync(b->tiles_d.p, b->tiles_h, nt * sizeof(int4), hipMemcpyHostToDevice, st));
+  return pl;
+}
+
+// The level's launches (kernels only: capturable).
+void batch_level_launch(am_batch* b, const BatchLevelPlan& pl, Fp64Counts& t) {
+  am_ctx* c = b->c;
+  const hipStream_t st = c->stream;
+  const long long first = pl.first, count = pl.count;
+  const size_t chunk = pl.chunk;
+  const std::vector<size_t>& chunk_tile0 = pl.chunk_tile0;
+  const int Ch = (b->Cmax + 63) & ~63;
   const size_t clds = batch_chain_lds(b), slds = batch_scan_lds(b), rlds = batch_ray_lds(b);
   const bool ray = b->precision == AM_PREC_FP64_RAY && rlds <= c->max_lds;
   StageTimer& tm = c->timer;
@@ -692,6 +843,7 @@ void batch_level(am_batch* b, long long first, long long count, Fp64Counts& t) {
     io.cand_mask = c->cand_mask.p;
     io.cand_start = c->cand_start.p;
     io.cand_point = c->cand_point.p;
+    io.cand_hash = c->cand_hash.p;
     io.first = (int)first;
     io.count = n;
     io.Cs = 0;
@@ -707,8 +859,21 @@ void batch_level(am_batch* b, long long first, long long count, Fp64Counts& t) {
       t.scanned_host += n;
       continue;
     }
-    hipLaunchKernelGGL(k_walk_ray64_multi, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, st, (const NetDev*)b->nets_d.p,
-                       io, c->Wd, rid0, kRay64, (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
+    {
+      const NetDev* nd = (const NetDev*)b->nets_d.p;
+      const double* P = c->tplanes.p;
+      const double* E = c->teq.p;
+      switch (c->ray_waves) {
+        case 2:
+          hipLaunchKernelGGL(k_walk_ray64_multi<2>, dim3(n), dim3(128), rlds, st, nd, io, c->Wd, rid0, kRay64, P, E, Ch);
+          break;
+        case 8:
+          hipLaunchKernelGGL(k_walk_ray64_multi<8>, dim3(n), dim3(512), rlds, st, nd, io, c->Wd, rid0, kRay64, P, E, Ch);
+          break;
+        default:
+          hipLaunchKernelGGL(k_walk_ray64_multi<4>, dim3(n), dim3(256), rlds, st, nd, io, c->Wd, rid0, kRay64, P, E, Ch);
+      }
+    }
     HIPCHK(hipGetLastError());
     const size_t m2 = tm.mark(st);
     tm.span(m1, m2, T_RAY64);
@@ -716,7 +881,7 @@ void batch_level(am_batch* b, long long first, long long count, Fp64Counts& t) {
     hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, st, (const unsigned*)c->rflags.p, n,
                        (const int*)nullptr, rid0, (int)first, c->tcnt.p, c->tslots.p, c->runacc.p);
     io.flags = nullptr;
-    hipLaunchKernelGGL(k_walk_fp64_multi, dim3(std::min(n, 2048)), dim3(64), slds, st, (const NetDev*)b->nets_d.p,
+    hipLaunchKernelGGL(k_walk_fp64_multi, dim3(std::min(n, c->scan_blocks)), dim3(64), slds, st, (const NetDev*)b->nets_d.p,
                        io, c->Wd, rid0, (const int*)c->tcnt.p, (const int*)c->tslots.p,
                        (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
     HIPCHK(hipGetLastError());
@@ -850,6 +1015,15 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
     c->max_regions = value;
   } else if (k == "region_limit") {
     c->region_limit = value;
+  } else if (k == "graph_levels") {
+    if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "graph_levels must be 0 or 1");
+    c->graph_levels = value != 0;
+  } else if (k == "ray_waves") {
+    if (value != 2 && value != 4 && value != 8) return fail(c, AM_ERR_ARG, "ray_waves must be 2, 4 or 8");
+    c->ray_waves = (int)value;
+  } else if (k == "scan_blocks") {
+    if (value < 64 || value > (1 << 20)) return fail(c, AM_ERR_ARG, "scan_blocks must be in [64, 2^20]");
+    c->scan_blocks = (int)value;
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
@@ -995,8 +1169,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     const bool fp64_form = !reduced_asked || !c->tile_ok;
     if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, wv.data(), bv.data(), dtype, st);
     if (c->tile_ok && ray64_lds_bytes(c) > 65536 && ray64_lds_bytes(c) <= c->max_lds)
-      HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)ray64_lds_bytes(c)));
+      set_ray64_lds(ray64_lds_bytes(c));
 
     const auto t_setup = std::chrono::steady_clock::now();
     // ---- seeds -> first level (backend/inc/states.h:147-176)
@@ -1024,6 +1197,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       HIPCHK(hipMemcpy(c->cand_point.p, pts.data(), pts.size() * 8, hipMemcpyHostToDevice));
       HIPCHK(hipMemcpy(c->cand_start.p, startv.data(), startv.size() * 4, hipMemcpyHostToDevice));
     }
+    hipLaunchKernelGGL(k_hash_keys, dim3(blocks(n_seeds, 256)), dim3(256), 0, st, (const uint32_t*)c->cand_mask.p,
+                       n_seeds, c->Wd, c->cand_hash.p);
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
     long long levels = 0, cands = 0, launches = 0, processed = 0;
@@ -1040,6 +1215,25 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       const size_t ncand = (size_t)lvl_count * c->vm;
       grow_cands(c, ncand);
       ensure(c->rflags, (size_t)lvl_count, false, st);
+      if (fp64_form && c->graph_levels) {
+        // the level's launches (planes, walks, visited-set insert and commit)
+        // as one graph; the host work (growth) happens before the capture
+        if (c->tile_ok) fp64_prepare(c, (int)lvl_count);
+        const InsertPlan plan = insert_prepare(c, ncand);
+        launch_as_graph(c, (int)levels, [&]() {
+          run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr,
+                           c->precision == AM_PREC_FP64 ? 0 : 1, f64);
+          insert_launch(c, plan);
+        });
+        const long long before = c->R;
+        const int added = insert_finish(c, plan);
+        launches += 1;
+        cands += (long long)ncand;
+        levels += 1;
+        lvl_first = before;
+        lvl_count = added;
+        continue;
+      }
       if (fp64_form) {
         run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr,
                          c->precision == AM_PREC_FP64 ? 0 : 1, f64);
@@ -1052,6 +1246,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.reg_info = c->reg_info.p; fl.reg_tab = c->reg_tab.p; fl.reg_verts = c->reg_verts.p;
         fl.reg_vpair = c->reg_vpair.p; fl.cand_mask = c->cand_mask.p; fl.cand_start = c->cand_start.p;
         fl.cand_point = c->cand_point.p;
+        fl.cand_hash = c->cand_hash.p;
         FastTimes ft{};
         fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT);
         // FP64 planes of the whole level: the regions the walk decided get
@@ -1116,6 +1311,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_VERT64_REGIONS] = (double)vert64;
     c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
     c->stats[AM_STAT_FP64_ONLY] = reduced_asked && fp64_form ? 1.0 : 0.0;
+    c->stats[AM_STAT_HOST_WAITS] = (double)(levels + 1);
+    c->stats[AM_STAT_GRAPH_LAUNCHES] = c->graph_levels && fp64_form ? (double)levels : 0.0;
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;
     return AM_OK;
@@ -1598,6 +1795,9 @@ am_status am_export_ply(am_ctx* c, const char* path, int is_polymesh, int is_flo
 
 am_status am_destroy(am_ctx* c) {
   if (!c) return AM_ERR_ARG;
+  if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
+  for (hipGraphExec_t& e : c->gexec)
+    if (e) (void)hipGraphExecDestroy(e);
   c->timer.release();
   fast_release(c->fbuf);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
@@ -1690,7 +1890,9 @@ am_status am_batch_create(am_batch** out, int n_nets, const int* n_nodes, const 
 am_status am_batch_set_option(am_batch* b, const char* key, long long value) {
   if (!b || !key) return AM_ERR_ARG;
   const std::string k(key);
-  if (k == "vm" || k == "table_log2" || k == "region_limit") return am_set_option(b->c, key, value);
+  if (k == "vm" || k == "table_log2" || k == "region_limit" || k == "ray_waves" || k == "scan_blocks" ||
+      k == "graph_levels")
+    return am_set_option(b->c, key, value);
   return fail(b->c, AM_ERR_ARG, "unknown batch option " + k);
 }
 
@@ -1778,8 +1980,9 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
       HIPCHK(hipFuncSetAttribute((const void*)k_walk_fp64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)slds));
     if (rlds > 65536 && rlds <= c->max_lds)
-      HIPCHK(hipFuncSetAttribute((const void*)k_walk_ray64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)rlds));
+      for (const void* f : {(const void*)k_walk_ray64_multi<2>, (const void*)k_walk_ray64_multi<4>,
+                            (const void*)k_walk_ray64_multi<8>})
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
     const auto t_setup = std::chrono::steady_clock::now();
 
     // seeds of every network -> the first level (network word appended)
@@ -1820,6 +2023,8 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
     constexpr int NACC = 2 * (1 + AM_WHY_COUNT);
     ensure(c->runacc, NACC, false, st);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->runacc.p, (size_t)NACC, 0);
+    hipLaunchKernelGGL(k_hash_keys, dim3(blocks((size_t)total_seeds, 256)), dim3(256), 0, st,
+                       (const uint32_t*)c->cand_mask.p, (int)total_seeds, c->Wd, c->cand_hash.p);
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)total_seeds);
     batch_hist(b, lvl_first, lvl_count);
@@ -1830,6 +2035,10 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
       const size_t ncand = (size_t)lvl_count * c->vm;
       grow_cands(c, ncand);
       ensure(c->rflags, (size_t)lvl_count, false, st);
+      // with graph-captured levels the store grows before the level's kernels
+      // are captured, so the chain must see the grown store's masks
+      InsertPlan ip;
+      if (c->graph_levels) ip = insert_prepare(c, ncand);
       if (c->reg_mask.p != mask_at_upload) {
         // the chain reads region masks through each network's ChainIO
         for (ChainIO& cio : b->cios_h) cio.reg_mask = c->reg_mask.p;
@@ -1837,12 +2046,22 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
         HIPCHK(hipStreamSynchronize(st));
         mask_at_upload = c->reg_mask.p;
       }
-      batch_level(b, lvl_first, lvl_count, f64);
+      const BatchLevelPlan pl = batch_level_prepare(b, lvl_first, lvl_count);
       launches += 1;
-      const size_t mi = c->timer.mark(st);
       const long long before = c->R;
-      const int added = insert_level(c, ncand);
-      c->timer.span(mi, c->timer.mark(st), T_INSERT);
+      int added = 0;
+      if (c->graph_levels) {
+        launch_as_graph(c, (int)levels, [&]() {
+          batch_level_launch(b, pl, f64);
+          insert_launch(c, ip);
+        });
+        added = insert_finish(c, ip);
+      } else {
+        batch_level_launch(b, pl, f64);
+        const size_t mi = c->timer.mark(st);
+        added = insert_level(c, ncand);
+        c->timer.span(mi, c->timer.mark(st), T_INSERT);
+      }
       cands += (long long)ncand;
       levels += 1;
       lvl_first = before;
@@ -1873,6 +2092,7 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
     c->stats[AM_STAT_CHAIN64_LAUNCHES] = (double)f64.chain_launches;
     c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
     c->stats[AM_STAT_HOST_WAITS] = (double)waits;
+    c->stats[AM_STAT_GRAPH_LAUNCHES] = c->graph_levels ? (double)levels : 0.0;
     c->has_run = true;
     return AM_OK;
   });

@@ -59,6 +59,7 @@ struct FastLevel {
   uint32_t* cand_mask;
   int* cand_start;
   double* cand_point;
+  uint64_t* cand_hash;
 };
 
 struct FastTimes {

@@ -623,6 +623,7 @@ void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& 
     io.cand_mask = lvl.cand_mask;
     io.cand_start = lvl.cand_start;
     io.cand_point = lvl.cand_point;
+    io.cand_hash = lvl.cand_hash;
     io.first = lvl.first;          // candidate slots / flags are level-relative
     io.count = n;
     io.Cs = CsL;

@@ -10,11 +10,54 @@ namespace am {
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v);
 __global__ void k_fill_i32(int* p, size_t n, int v);
 __global__ void k_insert(InsertIO io);
-__global__ void k_mark_first(const int* status, const int* winner, int* first, int* link, int n);
-__global__ void k_commit(const int* first, const int* rank, const int* link, const int* slot,
-                         const uint32_t* cand_mask, const int* cand_start, const double* cand_point,
-                         uint32_t* reg_mask, int* reg_start, double* reg_point, uint32_t* table,
-                         int n, int Wd, int R0);
+
+// Commit of a level's new regions: the candidates, their slots, the store.
+struct CommitIO {
+  const int* slot;
+  const uint32_t* cand_mask;
+  const int* cand_start;
+  const double* cand_point;
+  uint32_t* reg_mask;
+  int* reg_start;
+  double* reg_point;
+  uint32_t* table;
+  int Wd, R0;
+};
+#define AM_EMPTY 0xFFFFFFFFu
+#define AM_PEND 0x80000000u
+
+// A candidate is new (becomes a region) when it is the smallest of its pending
+// group: its slot still holds pending | c.  Only that candidate's own commit
+// rewrites the slot, so the flag reads the same before and during the commit.
+__device__ __forceinline__ int first_flag(const CommitIO& io, size_t c) {
+  const int s = io.slot[c];
+  return s >= 0 && io.table[s] == (AM_PEND | (uint32_t)c) ? 1 : 0;
+}
+
+__device__ __forceinline__ void commit_one(const CommitIO& io, size_t c, int rank) {
+  const int rid = io.R0 + rank;
+  for (int w = 0; w < io.Wd; ++w) io.reg_mask[(size_t)rid * io.Wd + w] = io.cand_mask[c * io.Wd + w];
+  io.reg_start[rid] = io.cand_start[c];
+  io.reg_point[(size_t)rid * 3 + 0] = io.cand_point[c * 3 + 0];
+  io.reg_point[(size_t)rid * 3 + 1] = io.cand_point[c * 3 + 1];
+  io.reg_point[(size_t)rid * 3 + 2] = io.cand_point[c * 3 + 2];
+  io.table[io.slot[c]] = (uint32_t)rid;
+}
+
+// scan source: 1 for a candidate that becomes a region
+struct FirstSrc {
+  CommitIO io;
+  __device__ int operator()(size_t c) const { return first_flag(io, c); }
+};
+// scan sink that commits each new region as its rank is known
+struct CommitSink {
+  CommitIO io;
+  __device__ void operator()(size_t c, int prefix, int v) const {
+    if (v) commit_one(io, c, prefix);
+  }
+};
+__global__ void k_commit(CommitIO io, const int* rank, int n);
+__global__ void k_hash_keys(const uint32_t* masks, int n, int Wd, uint64_t* out);
 __global__ void k_rehash(const uint32_t* reg_mask, int R, int Wd, uint32_t* table, uint32_t cap_mask);
 __global__ void k_fingerprint(const uint32_t* reg_mask, const int* info, const int* vpair, int R, int Wd, int Wh,
                               int vm, int L, int net, unsigned long long* out);

@@ -15,14 +15,13 @@
 
 namespace am {
 
-#define AM_EMPTY 0xFFFFFFFFu
-#define AM_PEND 0x80000000u
-#define AM_HSEED 0x6a09e667f3bcc909ULL
 
+// All loads issued before any comparison decides (no early exit), so a
+// thread's key reads overlap instead of waiting on each other.
 __device__ __forceinline__ bool words_equal(const uint32_t* a, const uint32_t* b, int n) {
-  for (int w = 0; w < n; ++w)
-    if (a[w] != b[w]) return false;
-  return true;
+  uint32_t diff = 0;
+  for (int w = 0; w < n; ++w) diff |= a[w] ^ b[w];
+  return diff == 0;
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -35,69 +34,54 @@ __global__ void k_fill_i32(int* p, size_t n, int v) {
   if (i < n) p[i] = v;
 }
 
+// Candidate c claims a slot (CAS) or joins the pending group of an equal key
+// there, lowering the slot to the group's smallest candidate (atomicMin on
+// pending | c); a key already in the store, or an empty candidate, gets slot -1.
 __global__ void k_insert(InsertIO io) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= io.n) return;
-  if (io.cand_start[c] == -2) { io.status[c] = 0; return; }
+  if (io.cand_start[c] == -2) { io.slot[c] = -1; return; }
   const uint32_t* key = io.cand_mask + (size_t)c * io.Wd;
-  uint32_t s = (uint32_t)hash_words(key, io.Wd, AM_HSEED) & io.cap_mask;
+  uint32_t s = (uint32_t)mix64(io.cand_hash[c]) & io.cap_mask;
   for (uint32_t probes = 0; probes <= io.cap_mask; ++probes) {
     const uint32_t prev = atomicCAS(&io.table[s], AM_EMPTY, AM_PEND | (uint32_t)c);
     if (prev == AM_EMPTY) {
-      io.status[c] = 1;
       io.slot[c] = (int)s;
-      atomicMin(&io.winner[c], c);
       return;
     }
     if (prev & AM_PEND) {
       const int o = (int)(prev & ~AM_PEND);
       if (words_equal(key, io.cand_mask + (size_t)o * io.Wd, io.Wd)) {
-        io.status[c] = 2;
-        io.owner[c] = o;
-        atomicMin(&io.winner[o], c);
+        atomicMin(&io.table[s], AM_PEND | (uint32_t)c);
+        io.slot[c] = (int)s;
         return;
       }
     } else if (words_equal(key, io.reg_mask + (size_t)prev * io.Wd, io.Wd)) {
-      io.status[c] = 3;
+      io.slot[c] = -1;
       return;
     }
     s = (s + 1) & io.cap_mask;
   }
-  io.status[c] = 4;  // table full: the host sizes the table so this never happens
+  io.slot[c] = -1;  // table full: the host sizes the table so this never happens
 }
 
-// first[winner] = 1 and link[winner] = owner for every group of equal candidates.
-__global__ void k_mark_first(const int* status, const int* winner, int* first, int* link, int n) {
+// New regions get ids R0 + rank in candidate order (rank from a stored scan).
+__global__ void k_commit(CommitIO io, const int* rank, int n) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  if (status[c] == 1) {
-    const int w = winner[c];
-    first[w] = 1;
-    link[w] = c;
-  }
+  if (c < n && first_flag(io, c)) commit_one(io, c, rank[c]);
 }
 
-// New regions get ids R0 + rank in candidate order; the table slot is finalised.
-__global__ void k_commit(const int* first, const int* rank, const int* link, const int* slot,
-                         const uint32_t* cand_mask, const int* cand_start, const double* cand_point,
-                         uint32_t* reg_mask, int* reg_start, double* reg_point, uint32_t* table,
-                         int n, int Wd, int R0) {
+// Key hash sums of n candidate masks given whole (the seeds).
+__global__ void k_hash_keys(const uint32_t* masks, int n, int Wd, uint64_t* out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n || !first[c]) return;
-  const int rid = R0 + rank[c];
-  for (int w = 0; w < Wd; ++w) reg_mask[(size_t)rid * Wd + w] = cand_mask[(size_t)c * Wd + w];
-  reg_start[rid] = cand_start[c];
-  reg_point[(size_t)rid * 3 + 0] = cand_point[(size_t)c * 3 + 0];
-  reg_point[(size_t)rid * 3 + 1] = cand_point[(size_t)c * 3 + 1];
-  reg_point[(size_t)rid * 3 + 2] = cand_point[(size_t)c * 3 + 2];
-  table[slot[link[c]]] = (uint32_t)rid;
+  if (c < n) out[c] = key_hash_sum(masks + (size_t)c * Wd, Wd);
 }
 
 // Re-insert every region id into an empty (larger) table.
 __global__ void k_rehash(const uint32_t* reg_mask, int R, int Wd, uint32_t* table, uint32_t cap_mask) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
-  uint32_t s = (uint32_t)hash_words(reg_mask + (size_t)r * Wd, Wd, AM_HSEED) & cap_mask;
+  uint32_t s = (uint32_t)mix64(key_hash_sum(reg_mask + (size_t)r * Wd, Wd)) & cap_mask;
   for (uint32_t probes = 0; probes <= cap_mask; ++probes) {
     if (atomicCAS(&table[s], AM_EMPTY, (uint32_t)r) == AM_EMPTY) return;
     s = (s + 1) & cap_mask;

@@ -19,6 +19,7 @@ struct RegionIO {
   uint32_t* cand_mask;        // [count * vm][Wd]
   int* cand_start;            // [count * vm]   -2: no candidate in this slot
   double* cand_point;         // [count * vm][3]
+  uint64_t* cand_hash;        // [count * vm] key hash sum of the candidate's mask (key_hash_sum)
   int first;                  // region id of blockIdx.x == 0
   int count;                  // regions in this launch
   int Cs;                     // stride of one SoA plane array in LDS
@@ -30,13 +31,11 @@ struct RegionIO {
 // Visited-set insertion of one level's candidates.
 struct InsertIO {
   const uint32_t* cand_mask;  // [n][Wd]
+  const uint64_t* cand_hash;  // [n] key_hash_sum of each candidate's mask
   const int* cand_start;      // [n]  -2 = empty slot
   const uint32_t* reg_mask;   // [R][Wd]
   uint32_t* table;            // [cap]  0xFFFFFFFF empty, 0x80000000|cand pending, else region id
-  int* status;                // [n]  0 empty, 1 new (slot owner), 2 duplicate, 3 visited
-  int* winner;                // [n]  min candidate index of the owner's group
-  int* slot;                  // [n]  table slot of an owner
-  int* owner;                 // [n]  owner candidate of a duplicate
+  int* slot;                  // [n]  table slot of a pending candidate, -1: empty or already visited
   int n;
   int Wd;
   uint32_t cap_mask;
@@ -89,6 +88,8 @@ __global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count,
                             const double* planes, const double* eq, int Cs_hbm);
 struct WalkParams;
 // only_undecided: blocks whose region is not flagged AM_UNDECIDED in io.flags exit.
+// NW waves per region: 2, 4 or 8.
+template <int NW>
 __global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, const double* planes,
                              const double* eq, int Cs_hbm, int only_undecided);
 // Batch-of-networks forms (region rid0 + b; its network is mask word Wd - 1;
@@ -96,6 +97,7 @@ __global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, c
 // widest network).
 __global__ void k_walk_fp64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0, const int* count,
                                   const int* slots, const double* planes, const double* eq, int Cs_hbm);
+template <int NW>
 __global__ void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0, WalkParams wp,
                                    const double* planes, const double* eq, int Cs_hbm);
 // Chunk-local slots of the regions k_walk_ray64 left undecided (cnt[0] counts

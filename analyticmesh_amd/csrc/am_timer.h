@@ -21,12 +21,14 @@ struct StageTimer {
   };
   std::vector<Span> spans;
   double acc[T_SLOTS] = {0};
+  bool enabled = true;   // off while launches are captured into a graph (no event nodes)
 
   static void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
   }
   // Record an event on `st`; returns its index for span().
   size_t mark(hipStream_t st) {
+    if (!enabled) return 0;
     if (used == pool.size()) {
       hipEvent_t e;
       check(hipEventCreate(&e), "hipEventCreate");
@@ -35,7 +37,9 @@ struct StageTimer {
     check(hipEventRecord(pool[used], st), "hipEventRecord");
     return used++;
   }
-  void span(size_t a, size_t b, int slot) { spans.push_back({a, b, slot}); }
+  void span(size_t a, size_t b, int slot) {
+    if (enabled) spans.push_back({a, b, slot});
+  }
   // The stream must have drained (or the events completed).
   void resolve() {
     for (const Span& s : spans) {

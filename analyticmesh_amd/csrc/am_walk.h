@@ -271,6 +271,27 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
       io.cand_mask[(slot0 + m) * Wd + w] = word;
     }
   }
+  // each candidate's key hash: the region's sum of word hashes with the
+  // flipped word's term swapped (wave 0; Wd <= 64 and vm <= 64)
+  if (lane < 64) {
+    uint64_t g = 0;
+    if (lane < Wd) g = word_hash(smask[lane], lane);
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+      const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)g, d, 64);
+      const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(g >> 32), d, 64);
+      g += ((uint64_t)hi << 32) | lo;
+    }
+    if (lane < vm) {
+      int t = -1;
+      if (lane + 1 < ntab) t = s_tab[lane];
+      if (t >= 0 && t < L) {
+        const int wt = t >> 5;
+        const uint32_t x = smask[wt];
+        io.cand_hash[slot0 + lane] = g - word_hash(x, wt) + word_hash(x ^ (1u << (t & 31)), wt);
+      }
+    }
+  }
 }
 
 // Distance key of constraint g at p: s|s|/|a|^2 orders like the signed

@@ -143,7 +143,10 @@ class Engine:
     """One engine context for a fixed architecture (mirrors ``Init``)."""
 
     def __init__(self, nodes, arc_table=None, num_extra_constraints=0, precision="fp64", vm=20,
-                 table_log2=None, walk="ray"):
+                 table_log2=None, walk="ray", ray_waves=None, scan_blocks=None, graph_levels=False):
+        """ray_waves / scan_blocks: launch shapes (see ``autotune``); None takes the tuned
+        shape stored for the nearest network shape.  graph_levels: launch each level of the
+        FP64 forms as one captured hipGraph."""
         m = native()
         nodes = [int(n) for n in nodes]
         D = len(nodes) - 2
@@ -165,6 +168,13 @@ class Engine:
         if table_log2 is not None:
             self._eng.set_option("table_log2", int(table_log2))
         self._eng.set_option("walk", {"ray": 0, "scan": 1}[walk])
+        from .autotune import lookup
+        tuned = lookup(nodes)
+        self.ray_waves = int(ray_waves if ray_waves is not None else tuned["ray_waves"])
+        self.scan_blocks = int(scan_blocks if scan_blocks is not None else tuned["scan_blocks"])
+        self._eng.set_option("ray_waves", self.ray_waves)
+        self._eng.set_option("scan_blocks", self.scan_blocks)
+        self._eng.set_option("graph_levels", 1 if graph_levels else 0)
         self._generation = 0
 
     def set_option(self, key, value):
@@ -217,7 +227,7 @@ class Engine:
 
 
 def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip=False, max_regions=None,
-          table_log2=None, walk="ray", extra_w=None, extra_b=None):
+          table_log2=None, walk="ray", extra_w=None, extra_b=None, **engine_options):
     """Mesh one NetSpec (with its skips, if any) from explicit seeds (points (N,3) float64,
     states (N,L) bool).
 
@@ -226,7 +236,7 @@ def march(spec, seed_points, seed_states, precision="fp64", vm=20, iso=0.0, flip
     extra_w (E,3), extra_b (E,): extra half-spaces a.x + c <= 0, used as given."""
     E = 0 if extra_w is None else len(extra_w)
     eng = Engine(spec.nodes, getattr(spec, "arc_table", None), E, precision=precision, vm=vm,
-                 table_log2=table_log2, walk=walk)
+                 table_log2=table_log2, walk=walk, **engine_options)
     if max_regions is not None:
         eng.set_option("max_regions", max_regions)
     pts = np.ascontiguousarray(seed_points, dtype=np.float64).reshape(-1, 3)
@@ -304,7 +314,7 @@ class BatchEngine:
     """Many networks marched in one job by one set of kernel launches per breadth-first level
     (``am_batch_*``).  precision: "fp64" (bit-exact scan) or "fp64ray"."""
 
-    def __init__(self, specs, precision="fp64ray", vm=20):
+    def __init__(self, specs, precision="fp64ray", vm=20, ray_waves=None, scan_blocks=None, graph_levels=False):
         m = native()
         self.specs = list(specs)
         for s in self.specs:
@@ -313,6 +323,12 @@ class BatchEngine:
         self.vm = int(vm)
         self._b = m.Batch([list(map(int, s.nodes)) for s in self.specs], PRECISIONS[precision])
         self._b.set_option("vm", self.vm)
+        from .autotune import lookup
+        widest = max(self.specs, key=lambda s: (max(s.nodes[1:-1]), len(s.nodes)))
+        tuned = lookup(widest.nodes)
+        self._b.set_option("ray_waves", int(ray_waves if ray_waves is not None else tuned["ray_waves"]))
+        self._b.set_option("scan_blocks", int(scan_blocks if scan_blocks is not None else tuned["scan_blocks"]))
+        self._b.set_option("graph_levels", 1 if graph_levels else 0)
         self._keep = []
 
     def run(self, seeds, iso=0.0, flip=False, dtype=np.float64):

@@ -41,10 +41,12 @@ def make_workload(n_seeds):
     return spec, pts, states
 
 
-def run_form(spec, pts, states, precision, steps, warmup):
+def run_form(spec, pts, states, form, steps, warmup):
+    """form: a precision name, with "+graph" for graph-captured levels."""
     import torch
     from analyticmesh_amd.engine import Engine
-    eng = Engine(spec.nodes, None, 0, precision=precision, vm=20)
+    precision, _, opt = form.partition("+")
+    eng = Engine(spec.nodes, None, 0, precision=precision, vm=20, graph_levels=(opt == "graph"))
     dev = torch.device("cuda:0")
     W = [torch.from_numpy(w).to(dev) for w in spec.weights]
     B = [torch.from_numpy(b).to(dev) for b in spec.biases]
@@ -161,11 +163,13 @@ def traffic_from_profile(kernel):
     return None
 
 
-def batch64_extra(n_nets, seeds_per_net, golden):
-    """The 64-net batch workload meshed in one job per form; per-net topology against the oracle's
-    committed fingerprints and between the forms."""
+def batch64_extra(n_nets, seeds_per_net, golden, steps=2):
+    """The 64-net batch workload: the batch engine (every network's levels through one launch of
+    each kernel; FP64 ray walk, without and with graph-captured levels) and, for comparison, one
+    context per network driven by a pool of host threads (FP32 form).  Per-net topology against
+    the oracle's committed fingerprints."""
     from analyticmesh_amd import configs
-    from analyticmesh_amd.engine import march_batch
+    from analyticmesh_amd.engine import BatchEngine, march_batch
     specs = configs.batch64(0)[:n_nets]
     seeds = []
     for s in specs:
@@ -173,21 +177,36 @@ def batch64_extra(n_nets, seeds_per_net, golden):
         seeds.append((p, configs.mask_of(s, p)))
     ref = (golden or {}).get("batch64")
     out = {"workload": f"batch64(seed 0): {len(specs)} nets, depth U{{2..6}}, width U{{32,64,128,256}}, "
-                       f"{seeds_per_net} seeds each"}
-    fps = {}
-    for f in ("fp32", "fp64"):
-        march_batch(specs[:2], seeds[:2], precision=f, workers=2, combine=False)   # warm up
-        res, dt = march_batch(specs, seeds, precision=f, combine=True)
-        fps[f] = [r.fingerprint() for r in res]
-        faces = sum(int(x["faces"]) for x in fps[f])
-        out[f] = {"faces": faces, "seconds": round(dt, 4), "faces_per_s": round(faces / dt, 1),
-                  "levels": int(sum(float(r.stats[6]) for r in res)),
-                  "host_waits": int(sum(float(r.stats[6]) + 1 for r in res))}
-        if ref:
-            out[f]["nets_identical_to_oracle"] = sum(1 for a, b in zip(fps[f], ref) if same_fp(a, b))
-        del res
-    out["nets"] = len(specs)
-    out["nets_fp32_identical_to_fp64"] = sum(1 for a, b in zip(fps["fp32"], fps["fp64"]) if same_fp(a, b))
+                       f"{seeds_per_net} seeds each", "nets": len(specs)}
+
+    def check(fps):
+        return sum(1 for a, b in zip(fps, ref) if same_fp(a, b)) if ref else None
+
+    for name, graph in (("batch_engine_fp64ray", False), ("batch_engine_fp64ray_graph", True)):
+        eng = BatchEngine(specs, precision="fp64ray", graph_levels=graph)
+        eng.run(seeds[:]).combine()                       # warm up
+        ts = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            res = eng.run(seeds)
+            res.combine()
+            ts.append(time.perf_counter() - t0)
+        fps = [res.fingerprint(i) for i in range(len(specs))]
+        faces = sum(int(x["faces"]) for x in fps)
+        dt = min(ts)
+        st = res.stats
+        out[name] = {"faces": faces, "seconds": round(dt, 4), "faces_per_s": round(faces / dt, 1),
+                     "levels": int(st[6]), "host_waits": int(st[35]), "graph_launches": int(st[36]),
+                     "chain_ms": round(float(st[13]), 3), "ray_walk_ms": round(float(st[28]), 3),
+                     "insert_ms": round(float(st[2]), 3), "combine_ms": round(float(res.combine_ms), 3),
+                     "nets_identical_to_oracle": check(fps)}
+        del res, eng
+    march_batch(specs[:2], seeds[:2], precision="fp32", workers=2, combine=False)   # warm up
+    res, dt = march_batch(specs, seeds, precision="fp32", combine=True)
+    fps = [r.fingerprint() for r in res]
+    faces = sum(int(x["faces"]) for x in fps)
+    out["thread_pool_fp32"] = {"faces": faces, "seconds": round(dt, 4), "faces_per_s": round(faces / dt, 1),
+                               "contexts": len(specs), "nets_identical_to_oracle": check(fps)}
     return out
 
 
@@ -236,7 +255,7 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--cpu-regions", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,bf16x3,fp64ray,fp64")
+    ap.add_argument("--forms", default="fp32,bf16x3,fp64ray,fp64ray+graph,fp64")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-MLP batch")
     ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint (one JSON line)")
@@ -306,7 +325,7 @@ def main():
             if f != "fp64" and out.get(f):
                 same = same_fp(out[f]["fingerprint"], out["fp64"]["fingerprint"])
                 topo[f] = "identical" if same else "DIFFERENT"
-                if same:
+                if same and "+" not in f:
                     vert_err[f] = vertex_error_vs(out[f]["res"], ref_res, spec.L)
     golden = golden_fingerprints()
     gold = (golden or {}).get("deep_1024") if args.seeds == 1024 else None
@@ -333,7 +352,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64", "fp64ray": "f64"}[head],
+        "dtype": {"fp32": "f32", "bf16x3": "bf16x3", "fp64": "f64", "fp64ray": "f64"}[head.split("+")[0]],
         "data": "synthetic (seeded geometric-init weights, bisection seeds)",
         "config": {"workload": "deep 3->256x4->1 ReLU MLP, r=0.6, sigma=0.05, seed 0, 1024 seeds",
                    "faces": h["faces"], "regions": h["regions"], "form": head},

@@ -134,7 +134,13 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * (stop after the breadth-first level during which this many regions were
  * processed, -1 = unlimited), "region_limit" (am_run fails with
  * AM_ERR_CAPACITY once more regions than this would be stored; -1 = the
- * engine's own limit, min(2^30, (2^31 - 1) / vm)), "walk" (0 ray walk, 1
+ * engine's own limit, min(2^30, (2^31 - 1) / vm)), "ray_waves" (waves per
+ * region of the FP64 ray walk: 2, 4 (default) or 8; the records do not depend
+ * on it), "scan_blocks" (grid of the candidate scan over the regions the ray
+ * walk leaves undecided, default 2048), "graph_levels" (1: each breadth-first
+ * level's launches, FP64 forms, are captured into a hipGraph and launched as
+ * one; the executable graph is updated in place from level to level),
+ * "walk" (0 ray walk, 1
  * candidate scan, for the reduced forms), "adjoint_tiled" (1 default: the
  * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
  * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU). */
@@ -169,8 +175,9 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_VERT64_MS 32    /* FP64 vertex re-solve of regions the reduced-precision walk decided (HIP events) */
 #define AM_STAT_VERT64_REGIONS 33 /* regions whose vertices were re-solved from FP64 planes */
 #define AM_STAT_FP64_ONLY 34    /* 1: a reduced form ran on the FP64 region kernel (skips or layers over 256 wide) */
-#define AM_STAT_HOST_WAITS 35   /* host waits on the device during the run (batch of networks) */
-#define AM_STAT_COUNT 36
+#define AM_STAT_HOST_WAITS 35   /* host waits on the device during the run */
+#define AM_STAT_GRAPH_LAUNCHES 36 /* levels launched as one captured graph ("graph_levels") */
+#define AM_STAT_COUNT 37
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 
 /* ---- Many networks in one job (no counterpart in the reference, which marches
@@ -188,7 +195,7 @@ typedef struct am_batch am_batch;
  * AM_PREC_FP64_RAY (checked FP64 ray walk, scan where undecided). */
 am_status am_batch_create(am_batch** out, int n_nets, const int* n_nodes, const int* const* nodes, int precision);
 
-/* "vm", "table_log2", "region_limit" as for am_set_option. */
+/* "vm", "table_log2", "region_limit", "ray_waves", "scan_blocks" as for am_set_option. */
 am_status am_batch_set_option(am_batch* b, const char* key, long long value);
 
 /* weights / biases: the linear layers of network 0, then of network 1, ...
