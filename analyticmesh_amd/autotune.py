@@ -114,6 +114,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--seeds", type=int, default=256)
     ap.add_argument("--repeats", type=int, default=2)
+    ap.add_argument("--out", default=TABLE, help="table to update (default: the package's tuning.json)")
     args = ap.parse_args()
     # the benchmark net and one net of each width of the batch workload
     work = [configs.deep(0)]
@@ -126,7 +127,7 @@ def main():
     for spec in work:
         pts = configs.seeds_fibonacci(spec, args.seeds if spec.L > 512 else 16)
         st = configs.mask_of(spec, pts)
-        e = tune(spec, pts, st, repeats=args.repeats)
+        e = tune(spec, pts, st, repeats=args.repeats, path=args.out)
         print(json.dumps({shape_key(spec.nodes): e["best"], "workload": spec.name}), flush=True)
 
 

@@ -40,7 +40,11 @@ __global__ void k_adj_mfma(const double* A, int lda, const double* B, int ldb, d
                            int kchunk, const uint32_t* mask, int Wd, int n0);
 __global__ void k_adj_scatter(const int* dn, const int* dg, const double* dv, int count, int cap, int o, int n,
                               double* G);
-__global__ void k_col_sum(const double* G, int count, int M, int qsel, double* out, int stride_out);
-__global__ void k_col_sum4(const double* G, int count, int M, double* outw, double* outb);
+// Column sums of G over regions, one partial per block of 256 regions
+// (k_col_sum: component qsel, part [blocks][M]; k_col_sum4: all four, part
+// [blocks][4][M]), and the ordered sum of partial slices into an output.
+__global__ void k_col_sum(const double* G, int count, int M, int qsel, double* part);
+__global__ void k_col_sum4(const double* G, int count, int M, double* part);
+__global__ void k_slice_sum(const double* part, int nslice, size_t sstride, int n, double* out, int ostride);
 
 }  // namespace am

@@ -270,8 +270,10 @@ __global__ __launch_bounds__(64) void k_adjoint_direct(NetDev net, AdjointIO io,
 //   MODE 0 (pull-back): rows are (region, component), region m / 4 of the
 //     tile's mask rows; C[m][n] = acc where that region's neuron n0 + n is
 //     active, else +0.  K whole (one block in z).
-//   MODE 1 (weight gradient): atomicAdd of acc into C[m][n]; K split over
-//     blockIdx.z in chunks of kchunk.
+//   MODE 1 (weight gradient): K split over blockIdx.z in chunks of kchunk;
+//     block z stores its partial product in slice z of C (C + z * M * ldc),
+//     which k_slice_sum then adds in ascending z: no atomics, so the sums and
+//     the gradients are the same from run to run.
 template <bool TA, int MODE>
 __global__ __launch_bounds__(256) void k_adj_gemm(const double* __restrict__ A, int lda, const double* __restrict__ B,
                                                   int ldb, double* C, int ldc, int M, int N, int K, int kchunk,
@@ -338,7 +340,7 @@ __global__ __launch_bounds__(256) void k_adj_gemm(const double* __restrict__ A, 
       if (MODE == 0) {
         C[(size_t)m * ldc + n] = mbit(mask + (size_t)(m >> 2) * Wd, n0 + n) ? acc[i][j] : 0.0;
       } else {
-        atomicAdd(&C[(size_t)m * ldc + n], acc[i][j]);
+        C[((size_t)blockIdx.z * M + m) * ldc + n] = acc[i][j];
       }
     }
   }
@@ -421,7 +423,7 @@ __global__ __launch_bounds__(256) void k_adj_mfma(const double* __restrict__ A, 
         if (MODE == 0) {
           C[(size_t)m * ldc + n] = mbit(mask + (size_t)(m >> 2) * Wd, n0 + n) ? acc[i][j][v] : 0.0;
         } else {
-          atomicAdd(&C[(size_t)m * ldc + n], acc[i][j][v]);
+          C[((size_t)blockIdx.z * M + m) * ldc + n] = acc[i][j][v];
         }
       }
     }
@@ -447,28 +449,26 @@ __global__ void k_adj_scatter(const int* dn, const int* dg, const double* dv, in
   for (int c = 0; c < 4; ++c) G[((size_t)b * 4 + c) * n + g] += dv[((size_t)b * cap + j) * 4 + c];
 }
 
-// out[j * stride_out] += sum over b < count of G[b][qsel][j]: 64 columns and
-// 256 regions per block, four partial sums per column added in LDS.
-__global__ __launch_bounds__(256) void k_col_sum(const double* G, int count, int M, int qsel, double* out,
-                                                 int stride_out) {
-  __shared__ double part[4][64];
+// part[y][j] = sum over the 256 regions b of block y of G[b][qsel][j]: 64
+// columns per block, four partial sums per column added in LDS in a fixed
+// order.  k_slice_sum adds the blocks' partials.
+__global__ __launch_bounds__(256) void k_col_sum(const double* G, int count, int M, int qsel, double* part) {
+  __shared__ double ps[4][64];
   const int jx = threadIdx.x & 63, by = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + jx;
   const int b0 = blockIdx.y * 256, b1 = min(count, b0 + 256);
   double acc = 0.0;
   if (j < M)
     for (int b = b0 + by; b < b1; b += 4) acc += G[((size_t)b * 4 + qsel) * M + j];
-  part[by][jx] = acc;
+  ps[by][jx] = acc;
   __syncthreads();
-  if (by == 0 && j < M) atomicAdd(&out[(size_t)j * stride_out], ((part[0][jx] + part[1][jx]) + part[2][jx]) + part[3][jx]);
+  if (by == 0 && j < M) part[(size_t)blockIdx.y * M + j] = ((ps[0][jx] + ps[1][jx]) + ps[2][jx]) + ps[3][jx];
 }
 
-// All four components at once (the first layer): outw[j * 3 + q] += sum over
-// b of G[b][q][j] for q < 3, outb[j] += the same for q = 3.  64 columns and
-// 256 regions per block, four partial sums per column and component added in
-// LDS.
-__global__ __launch_bounds__(256) void k_col_sum4(const double* G, int count, int M, double* outw, double* outb) {
-  __shared__ double part[4][4][64];
+// All four components at once (the first layer): part[y][q][j] = sum over the
+// 256 regions b of block y of G[b][q][j].
+__global__ __launch_bounds__(256) void k_col_sum4(const double* G, int count, int M, double* part) {
+  __shared__ double ps[4][4][64];
   const int jx = threadIdx.x & 63, by = threadIdx.x >> 6;
   const int j = blockIdx.x * 64 + jx;
   const int b0 = blockIdx.y * 256, b1 = min(count, b0 + 256);
@@ -476,13 +476,22 @@ __global__ __launch_bounds__(256) void k_col_sum4(const double* G, int count, in
   if (j < M)
     for (int b = b0 + by; b < b1; b += 4)
       for (int q = 0; q < 4; ++q) acc[q] += G[((size_t)b * 4 + q) * M + j];
-  for (int q = 0; q < 4; ++q) part[by][q][jx] = acc[q];
+  for (int q = 0; q < 4; ++q) ps[by][q][jx] = acc[q];
   __syncthreads();
   if (by == 0 && j < M)
-    for (int q = 0; q < 4; ++q) {
-      const double v = ((part[0][q][jx] + part[1][q][jx]) + part[2][q][jx]) + part[3][q][jx];
-      atomicAdd(q < 3 ? &outw[(size_t)j * 3 + q] : &outb[j], v);
-    }
+    for (int q = 0; q < 4; ++q)
+      part[((size_t)blockIdx.y * 4 + q) * M + j] = ((ps[0][q][jx] + ps[1][q][jx]) + ps[2][q][jx]) + ps[3][q][jx];
+}
+
+// out[i * ostride] += part[0][i] + part[1][i] + ... + part[nslice - 1][i],
+// added in that order (slice z at part + z * sstride), i < n.
+__global__ __launch_bounds__(256) void k_slice_sum(const double* part, int nslice, size_t sstride, int n, double* out,
+                                                   int ostride) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  double s = out[(size_t)i * ostride];
+  for (int z = 0; z < nslice; ++z) s += part[(size_t)z * sstride + i];
+  out[(size_t)i * ostride] = s;
 }
 
 }  // namespace am

@@ -177,6 +177,7 @@ struct am_ctx {
   DevBuf<double> tplanes, teq;
   // am_adjoint's buffers, kept from call to call: dL/dv, dW, db, G and P per layer
   DevBuf<double> adj_gv, adj_dv;
+  DevBuf<double> adj_part;   // per-block partial sums of dW / db, added in a fixed order
   DevBuf<int> adj_dn, adj_dg;
   std::vector<DevBuf<double>> adj_dW, adj_db, adj_G, adj_P;
   DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
@@ -1653,6 +1654,28 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
         HIPCHK(hipFuncSetAttribute((const void*)k_adjoint_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)lds));
     }
+    // dW_li = G_li+1^T P_li with K = 4 x regions split into nz slices (at
+    // most 512 K each, the slices of one layer within 32 MiB), each stored
+    // whole and added in ascending order: the gradients do not depend on the
+    // order in which blocks finish
+    auto k_slices = [&](int K, int M, int N, int& kchunk) {
+      const long long cap = std::max<long long>(8, ((long long)1 << 22) / ((long long)M * N));
+      const int nz = (int)std::min<long long>((K + 511) / 512, cap);
+      kchunk = ((K + nz - 1) / nz + 15) & ~15;
+      return (K + kchunk - 1) / kchunk;
+    };
+    {
+      size_t need = 0;
+      const int ny = (chunk + 255) / 256;
+      for (int li = 1; li < nl; ++li) {
+        int kc;
+        const int M = c->nodes[li + 1], N = c->nodes[li];
+        need = std::max(need, (size_t)k_slices(chunk * 4, M, N, kc) * M * N);
+        need = std::max(need, (size_t)ny * M);
+      }
+      need = std::max(need, (size_t)ny * 4 * c->nodes[1]);
+      ensure(c->adj_part, need, false, st);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     for (int r0 = 0; r0 < R; r0 += CH) {
       const int cnt = std::min(CH, R - r0);
@@ -1699,20 +1722,31 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
         hipLaunchKernelGGL(k_adjoint_fp64, dim3(cnt), dim3(64), lds, st, c->net, io);
         HIPCHK(hipGetLastError());
       }
-      // dW_li = sum over (region, component) of G_li+1^T P_li; db_li the c components
-      const int K = cnt * 4, kchunk = 512;
+      // dW_li = sum over (region, component) of G_li+1^T P_li; db_li the c
+      // components; partials of blocks added in a fixed order (k_slice_sum)
+      const int K = cnt * 4, ny = (cnt + 255) / 256;
+      double* part = c->adj_part.p;
+      auto slice_sum = [&](const double* src, int nslice, size_t sstride, int n, double* out, int ostride) {
+        hipLaunchKernelGGL(k_slice_sum, dim3((n + 255) / 256), dim3(256), 0, st, src, nslice, sstride, n, out,
+                           ostride);
+      };
       for (int li = 1; li < nl; ++li) {
         const int M = c->nodes[li + 1], N = c->nodes[li];
+        int kchunk;
+        const int nz = k_slices(K, M, N, kchunk);
         hipLaunchKernelGGL((c->adjoint_mfma ? k_adj_mfma<true, 1> : k_adj_gemm<true, 1>),
-                           dim3((N + 63) / 64, (M + 63) / 64, (K + kchunk - 1) / kchunk), dim3(256), 0, st,
-                           (const double*)Gst[li].p, M, (const double*)Pst[li].p, N, dW[li].p, N, M, N, K, kchunk,
-                           (const uint32_t*)nullptr, 0, 0);
-        hipLaunchKernelGGL(k_col_sum, dim3((M + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
-                           (const double*)Gst[li].p, cnt, M, 3, db[li].p, 1);
+                           dim3((N + 63) / 64, (M + 63) / 64, nz), dim3(256), 0, st, (const double*)Gst[li].p, M,
+                           (const double*)Pst[li].p, N, part, N, M, N, K, kchunk, (const uint32_t*)nullptr, 0, 0);
+        slice_sum(part, nz, (size_t)M * N, M * N, dW[li].p, 1);
+        hipLaunchKernelGGL(k_col_sum, dim3((M + 63) / 64, ny), dim3(256), 0, st, (const double*)Gst[li].p, cnt, M, 3,
+                           part);
+        slice_sum(part, ny, (size_t)M, M, db[li].p, 1);
       }
       const int n1 = c->nodes[1];
-      hipLaunchKernelGGL(k_col_sum4, dim3((n1 + 63) / 64, (cnt + 255) / 256), dim3(256), 0, st,
-                         (const double*)Gst[0].p, cnt, n1, dW[0].p, db[0].p);
+      hipLaunchKernelGGL(k_col_sum4, dim3((n1 + 63) / 64, ny), dim3(256), 0, st, (const double*)Gst[0].p, cnt, n1,
+                         part);
+      for (int q = 0; q < 4; ++q)
+        slice_sum(part + (size_t)q * n1, ny, (size_t)4 * n1, n1, q < 3 ? dW[0].p + q : db[0].p, q < 3 ? 3 : 1);
       HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(st));

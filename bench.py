@@ -147,20 +147,43 @@ def copy_calibration(mb):
     return {"bytes_read": n * 4, "bytes_written": n * 4, "launches": 3}
 
 
-def traffic_from_profile(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed counter summary, if present."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+def profile_value(name, kernel):
+    """Per-region figure of `kernel` from a committed counter summary under profiles/ (its
+    machine-read lines are exactly "kernel value"), or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
         parts = line.split()
-        # machine-read lines are exactly "kernel bytes"; table rows have more fields
         if len(parts) == 2 and parts[0] == kernel:
             try:
                 return float(parts[1])
             except ValueError:
                 return None
     return None
+
+
+def traffic_per_launch(kernel, regions, launches):
+    """HBM bytes per launch: the counter summary's scaled bytes per region times this run's
+    regions per launch."""
+    b = profile_value("hbm_traffic.txt", kernel)
+    return None if b is None else round(b * regions / max(1.0, launches), 1)
+
+
+VALU_PEAK_G = 256 * 4 * 2.4 / 4   # G wave64 VALU instructions/s: one per SIMD every 4 cycles at 2.4 GHz
+
+
+def walk_roofline(kernel, ms, regions, launches):
+    """VALU-issue roofline of a walk kernel: instructions per region from the committed counter
+    summary, times the regions this run walked, over this run's kernel time."""
+    per = profile_value("walk_counters.txt", kernel)
+    if per is None or ms <= 0 or regions <= 0:
+        return None
+    ach = per * regions / (ms / 1e3) / 1e9
+    return {"bound": "valu issue", "kernel": kernel, "achieved": round(ach, 1), "peak": VALU_PEAK_G,
+            "unit": "G VALU wave-instructions/s", "frac": round(ach / VALU_PEAK_G, 4),
+            "traffic": traffic_per_launch(kernel, regions, launches), "kernel_ms": round(ms, 3),
+            "launches": int(launches), "regions": int(regions), "valu_per_region": per}
 
 
 def batch64_extra(n_nets, seeds_per_net, golden, steps=2):
@@ -309,12 +332,23 @@ def main():
         ach = flops_region * k["regions"] / (k["ms"] / 1e3) / 1e12 if k["ms"] > 0 else 0.0
         lines.append({"bound": "mfma" if k["kernel"].startswith("k_gemm") else "fp64 valu", "kernel": k["kernel"],
                       "achieved": round(ach, 3), "peak": k["peak"], "unit": "TFLOP/s",
-                      "frac": round(ach / k["peak"], 5), "traffic": traffic_from_profile(k["kernel"]),
+                      "frac": round(ach / k["peak"], 5),
+                      "traffic": traffic_per_launch(k["kernel"], k["regions"], k["launches"]),
                       "kernel_ms": round(k["ms"], 3), "launches": int(k["launches"]),
                       "avg_launch_ms": round(k["ms"] / k["launches"], 4),
                       "regions": k["regions"], "flops_per_region": flops_region})
     lines.sort(key=lambda r: -r["kernel_ms"])
     roofline = lines[0]
+    # walk kernels of the head form: the reduced-precision walk over every region, the FP64 ray
+    # walk over the regions it left undecided (every region in the fp64ray form)
+    walks = []
+    nlev = max(1.0, float(st[6]))
+    if head in ("fp32", "bf16x3"):
+        walks.append(walk_roofline("k_walk_ray_f32", float(st[4]), h["regions"], nlev))
+        walks.append(walk_roofline("k_walk_ray64<4>", float(st[28]), int(st[9]), float(st[31])))
+    elif head == "fp64ray":
+        walks.append(walk_roofline("k_walk_ray64<4>", float(st[28]), h["regions"], float(st[31])))
+    walks = [w for w in walks if w]
 
     topo = {}
     vert_err = {}
@@ -339,7 +373,10 @@ def main():
                  "emitted_edges": n_edges, "ms": round(ins_ms, 3),
                  "achieved": round(tbytes / (ins_ms / 1e3) / 1e9, 1) if ins_ms > 0 else None,
                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": round(tbytes / (ins_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if ins_ms > 0 else None}
+                 "frac": round(tbytes / (ins_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if ins_ms > 0 else None,
+                 "traffic": (None if profile_value("hbm_traffic.txt", "k_insert") is None else
+                             round((profile_value("hbm_traffic.txt", "k_insert") +
+                                    profile_value("hbm_traffic.txt", "k_commit")) * h["regions"], 1))}
 
     line = {
         "metric": "mesh faces/s, 4x256 ReLU MLP: plain FP32 vs split-BF16 MFMA, same topology",
@@ -384,7 +421,7 @@ def main():
         "topology_vs_fp64": topo,
         "topology_vs_oracle_golden": topo_oracle,
         "roofline": roofline,
-        "roofline_other": lines[1:],
+        "roofline_other": lines[1:] + walks,
         "traversal": traversal,
     }
     log("[bench] adjoint (backward) on the diff 3->128x4->1 net")

@@ -91,7 +91,8 @@ def test_adjoint_tiled_matches_per_region():
     gradients as GEMMs over all regions, on the FP64 matrix cores or as VALU
     fmas) against the one-kernel-per-region path (chain and pull-back in one
     kernel).  The sums run in different orders; the results agree to 1e-12 of
-    each array's largest entry.  64 seeds: 4 chunks."""
+    each array's largest entry.  64 seeds: 4 chunks.  A second call gives the
+    same gradients bit for bit (partial sums are added in a fixed order)."""
     from analyticmesh_amd.engine import march
     spec, pts, st = workload("diff", n_seeds=64)
     _say("[adjoint] tiled (MFMA, VALU) vs per-region, diff seeds=64")
@@ -108,6 +109,9 @@ def test_adjoint_tiled_matches_per_region():
     finally:
         res._eng.set_option("adjoint_tiled", 1)
         res._eng.set_option("adjoint_mfma", 1)
+    again = res.adjoint(g, shapes)
+    for a, b in zip(again[0] + again[1], runs["mfma"][0] + runs["mfma"][1]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64)), "the adjoint is not deterministic"
     ref_W, ref_b = runs["per-region"]
     for name in ("mfma", "valu"):
         dW, db = runs[name]
