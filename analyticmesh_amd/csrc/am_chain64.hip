@@ -315,8 +315,10 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
 
 __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
   const int r0 = blockIdx.x * AM_CT;
-  const ChainLevel lv{io.first, io.count, io.ids, io.planes, io.eq};
-  chain_tile_body(net, io, lv, r0, min(AM_CT, io.count - r0));
+  const int cnt = launch_count(io);
+  if (r0 >= cnt) return;
+  const ChainLevel lv{io.first, cnt, io.ids, io.planes, io.eq};
+  chain_tile_body(net, io, lv, r0, min(AM_CT, cnt - r0));
 }
 
 // Many networks in one launch: block b takes tiles[b] = (network, r0, regions);
@@ -444,7 +446,7 @@ template <int NW>
 __global__ __launch_bounds__(64 * NW, 20 / NW) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
                                                         const double* planes, const double* eqb, int Cs_hbm,
                                                         int only_undecided) {
-  if ((int)blockIdx.x >= io.count) return;
+  if ((int)blockIdx.x >= launch_count(io)) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
   // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
   if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
@@ -478,7 +480,7 @@ template __global__ void k_walk_ray64_multi<8>(const NetDev*, RegionIO, int, int
 __global__ __launch_bounds__(64) void k_vertex_fp64(NetDev net, RegionIO io, int rid0, const double* planes,
                                                     const double* eqb, int Cs_hbm) {
   const int b = blockIdx.x;
-  if (b >= io.count) return;
+  if (b >= launch_count(io)) return;
   const int rid = io.ids ? io.ids[b] : rid0 + b;
   if (io.flags && (io.flags[rid - io.first] & AM_UNDECIDED)) return;   // left to the FP64 walk
   const int vm = net.vm;
@@ -505,10 +507,10 @@ __global__ __launch_bounds__(64) void k_vertex_fp64(NetDev net, RegionIO io, int
 
 // Chunk-local slots of the regions k_walk_ray64 left undecided: region of
 // slot b is ids[b] or rid0 + b; its flag is level-relative.
-__global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
-                                int* slots, int* run) {
+__global__ void k_collect_slots(const unsigned* flags, int n, const int* dn, int dbase, const int* ids, int rid0,
+                                int lvl_first, int* cnt, int* slots, int* run) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
+  if (b >= (dn ? min(n, *dn - dbase) : n)) return;
   const int rid = ids ? ids[b] : rid0 + b;
   const unsigned f = flags[rid - lvl_first];
   if (f & AM_UNDECIDED) {

@@ -181,6 +181,7 @@ struct am_ctx {
   DevBuf<int> adj_dn, adj_dg;
   std::vector<DevBuf<double>> adj_dW, adj_db, adj_G, adj_P;
   DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
+  DevBuf<int> redo_ids, redo_cnt;         // regions a reduced-precision walk left to FP64, their count
   DevBuf<int> runacc;                     // per run: FP64 ray walk undecided count, then per reason
   StageTimer timer;
   std::vector<DevBuf<double>> WTp, bpad;   // W_li^T and b_li with n_out padded to AM_CR (zeros)
@@ -530,8 +531,22 @@ size_t fp64_prepare(am_ctx* c, int count) {
   return chunk;
 }
 
+// Regions a reduced-precision walk left to FP64 (undecided, or decided with a
+// vertex to re-solve from FP64 planes), compacted in region order.
+struct RedoSrc {
+  const unsigned* flags;
+  __device__ int operator()(size_t i) const { return (flags[i] & (AM_UNDECIDED | AM_VERT64)) ? 1 : 0; }
+};
+struct RedoSink {
+  int* ids;
+  int first;
+  __device__ void operator()(size_t i, int prefix, int v) const {
+    if (v) ids[prefix] = first + (int)i;
+  }
+};
+
 void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t,
-                      long long undecided = -1) {
+                      long long undecided = -1, const int* dcount = nullptr) {
   if (!c->tile_ok) {
     // one region per workgroup: the chain and the candidate scan in one kernel
     if (ids) {
@@ -554,7 +569,9 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   StageTimer& tm = c->timer;
   for (int off = 0; off < count; off += (int)chunk) {
     const int n = std::min<int>((int)chunk, count - off);
-    const ChainIO cio = chain_io(c, ids ? ids + off : nullptr, first + off, n, Ch);
+    ChainIO cio = chain_io(c, ids ? ids + off : nullptr, first + off, n, Ch);
+    cio.dcount = dcount;
+    cio.dbase = off;
     RegionIO io;
     io.reg_mask = c->reg_mask.p;
     io.reg_start = c->reg_start.p;
@@ -573,6 +590,8 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.Csort = Csort;
     io.flags = (mode == 1 || after_walk) ? c->rflags.p : nullptr;
     io.ids = ids ? ids + off : nullptr;
+    io.dcount = dcount;
+    io.dbase = off;
     const size_t m0 = tm.mark(c->stream);
     hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
                        c->net, cio);
@@ -605,8 +624,8 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     }
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->tcnt.p, (size_t)1, 0);
     hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const unsigned*)c->rflags.p,
-                       n, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p, c->tslots.p,
-                       c->runacc.p);
+                       n, dcount, off, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p,
+                       c->tslots.p, c->runacc.p);
     HIPCHK(hipGetLastError());
     io.flags = nullptr;
     const size_t m3 = tm.mark(c->stream);
@@ -880,7 +899,8 @@ void batch_level_launch(am_batch* b, const BatchLevelPlan& pl, Fp64Counts& t) {
     tm.span(m1, m2, T_RAY64);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->tcnt.p, (size_t)1, 0);
     hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, st, (const unsigned*)c->rflags.p, n,
-                       (const int*)nullptr, rid0, (int)first, c->tcnt.p, c->tslots.p, c->runacc.p);
+                       (const int*)nullptr, 0, (const int*)nullptr, rid0, (int)first, c->tcnt.p, c->tslots.p,
+                       c->runacc.p);
     io.flags = nullptr;
     hipLaunchKernelGGL(k_walk_fp64_multi, dim3(std::min(n, c->scan_blocks)), dim3(64), slds, st, (const NetDev*)b->nets_d.p,
                        io, c->Wd, rid0, (const int*)c->tcnt.p, (const int*)c->tslots.p,
@@ -1028,6 +1048,10 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
+  } else if (k == "reduced_vertices") {
+    if (value != 0 && value != 1)
+      return fail(c, AM_ERR_ARG, "reduced_vertices must be 0 (FP64 re-solve) or 1 (estimate)");
+    c->fast.vertex_mode = (int)value;
   } else if (k == "adjoint_mfma") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "adjoint_mfma must be 0 or 1");
     c->adjoint_mfma = value != 0;
@@ -1202,13 +1226,14 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
                        n_seeds, c->Wd, c->cand_hash.p);
     long long lvl_first = 0;
     long long lvl_count = insert_level(c, (size_t)n_seeds);
-    long long levels = 0, cands = 0, launches = 0, processed = 0;
+    long long levels = 0, cands = 0, launches = 0;
     long long why[AM_WHY_COUNT] = {0};
     Fp64Counts f64;
     c->timer.reset();
     // run counters on the device: [0, 1 + W) the FP64 ray walk's undecided
-    // regions and reasons, [1 + W, 2 + 2W) the reduced-precision walk's
-    constexpr int NACC = 2 * (1 + AM_WHY_COUNT);
+    // regions and reasons, [1 + W, 2 + 2W) the reduced-precision walk's, then
+    // the regions it decided with a vertex left to FP64
+    constexpr int NACC = 2 * (1 + AM_WHY_COUNT) + 1;
     ensure(c->runacc, NACC, false, st);
     hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, st, c->runacc.p, (size_t)NACC, 0);
     while (lvl_count > 0) {
@@ -1250,14 +1275,20 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.cand_hash = c->cand_hash.p;
         FastTimes ft{};
         fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT);
-        // FP64 planes of the whole level: the regions the walk decided get
-        // their vertices re-solved from them; the undecided ones are re-decided
-        // in place (records and candidate slots rewritten) by the checked FP64
-        // ray walk, and by the candidate scan where that is undecided too.
-        // (The split is known on the device only: both kernels cover the
-        // level and skip the other kind's regions.)
-        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr, 2, f64);
-        processed += lvl_count;
+        // FP64 planes of the regions the walk left to FP64 only (their list
+        // and count stay on the device; the launches are sized for the whole
+        // level and their blocks past the count exit): the ones it decided
+        // with a vertex it could not vouch for get their vertices re-solved
+        // from them; the undecided ones are re-decided in place (records and
+        // candidate slots rewritten) by the checked FP64 ray walk, and by the
+        // candidate scan where that is undecided too.
+        ensure(c->redo_ids, (size_t)lvl_count, false, st);
+        ensure(c->redo_cnt, 1, false, st);
+        ensure(c->scan_part, scan_blocks((size_t)lvl_count) + 1, false, st);
+        scan_launch<int>(RedoSrc{c->rflags.p}, (size_t)lvl_count, RedoSink{c->redo_ids.p, (int)lvl_first},
+                         c->scan_part.p, c->redo_cnt.p, st);
+        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, c->redo_ids.p, 2, f64, -1,
+                         c->redo_cnt.p);
         launches += ft.launches;
       }
       const size_t mi = c->timer.mark(st);
@@ -1276,7 +1307,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     int acc[NACC] = {0};
     HIPCHK(hipMemcpyAsync(acc, c->runacc.p, sizeof(acc), hipMemcpyDeviceToHost, st));
     const long long redo = fp64_form ? 0 : acc[1 + AM_WHY_COUNT];
-    const long long vert64 = fp64_form ? 0 : processed - redo;
+    const long long vert64 = fp64_form ? 0 : acc[2 + 2 * AM_WHY_COUNT];
     HIPCHK(hipStreamSynchronize(st));
     c->timer.resolve();
     const auto t1 = std::chrono::steady_clock::now();
@@ -1540,7 +1571,7 @@ am_status am_get_regions(am_ctx* c, uint32_t* masks, int* info, int* tabs, doubl
   });
 }
 
-am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, float* eq) {
+am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, float* eq, float* sigma) {
   if (!c || !masks || !planes || !eq || n < 1) return AM_ERR_ARG;
   if (!c->has_run || c->precision == AM_PREC_FP64 || c->precision == AM_PREC_FP64_RAY || !c->tile_ok)
     return fail(c, AM_ERR_STATE, "am_planes needs a reduced-precision context after am_run (on a net the "
@@ -1551,16 +1582,24 @@ am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, floa
     DevBuf<uint32_t> m;
     DevBuf<float> p, e;
     ensure(m, (size_t)n * c->Wd, false, st);
-    ensure(p, (size_t)n * 4 * Cs, false, st);
-    ensure(e, (size_t)n * 4, false, st);
+    ensure(p, (size_t)n * AM_PCOMP * Cs, false, st);
+    ensure(e, (size_t)n * AM_EQS, false, st);
     HIPCHK(hipMemcpy(m.p, masks, (size_t)n * c->Wd * 4, hipMemcpyDefault));
     fast_planes(c->fast, m.p, n, p.p, e.p, st);
-    std::vector<float> hp((size_t)n * 4 * Cs);
+    std::vector<float> hp((size_t)n * AM_PCOMP * Cs), he((size_t)n * AM_EQS);
     HIPCHK(hipMemcpy(hp.data(), p.p, hp.size() * 4, hipMemcpyDeviceToHost));
-    for (int r = 0; r < n; ++r)
-      for (int q = 0; q < 4; ++q)
-        std::memcpy(planes + ((size_t)r * 4 + q) * c->C, hp.data() + ((size_t)r * 4 + q) * Cs, (size_t)c->C * 4);
-    HIPCHK(hipMemcpy(eq, e.p, (size_t)n * 4 * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(he.data(), e.p, he.size() * 4, hipMemcpyDeviceToHost));
+    for (int r = 0; r < n; ++r) {
+      for (int q = 0; q < 4; ++q) {
+        std::memcpy(planes + ((size_t)r * 4 + q) * c->C, hp.data() + ((size_t)r * AM_PCOMP + q) * Cs,
+                    (size_t)c->C * 4);
+        eq[(size_t)r * 4 + q] = he[(size_t)r * AM_EQS + q];
+      }
+      if (sigma) {
+        std::memcpy(sigma + (size_t)r * (c->C + 1), hp.data() + ((size_t)r * AM_PCOMP + 4) * Cs, (size_t)c->C * 4);
+        sigma[(size_t)r * (c->C + 1) + c->C] = he[(size_t)r * AM_EQS + 4];
+      }
+    }
     return AM_OK;
   });
 }

@@ -13,6 +13,8 @@
 namespace am {
 
 #define AM_GT 8                 // regions per GEMM workgroup
+#define AM_PCOMP 5              // plane record components in HBM: x, y, z, c, sigma
+#define AM_EQS 8                // floats per region of the face-plane record (x, y, z, c, sigma, pad)
 #define AM_GN (4 * AM_GT)       // GEMM columns: x, y, z, c of each region
 
 struct FastNetDev {
@@ -28,6 +30,14 @@ struct FastNetDev {
   const float* Wp[AM_MAX_LINEAR];       // fp32 W^T, k-major [n_in + 1][nrb * 16], row n_in = +0
   const uint16_t* Wh[AM_MAX_LINEAR];    // bf16 hi A operands, [nrb][nkb][64 lanes][8]
   const uint16_t* Wl[AM_MAX_LINEAR];    // bf16 lo A operands
+  // Error estimates of the planes: per region and plane j of layer li + 1 the
+  // variance V_j = sum_k W_jk^2 m_k (V_k + ck[li] |P_k|^2) + ck[li] b_j^2 (V_k
+  // of the layer's input planes, 0 for hidden layer 1's exact planes), a
+  // bf16 MFMA product with A = W^2 in the layout of Wh ([nrb][nkq][64][8]).
+  const uint16_t* Wsq[AM_MAX_LINEAR];
+  int nkq[AM_MAX_LINEAR];       // 32-wide k blocks of layer li's W^2
+  float ck[AM_MAX_LINEAR];      // variance per squared term of layer li's sums
+  int sig;                      // 1: compute the estimates (planes component 4, eq[4])
   const float* ew;
   const float* eb;
   float iso;
@@ -37,14 +47,16 @@ struct FastNetDev {
 struct FastNet {
   int precision = 0;
   int walk_mode = 0;            // 0: ray-shooting walk, 1: the reference's candidate scan (checked)
+  int vertex_mode = 0;          // 0: every vertex re-solved from FP64 planes; 1: vertices solved from the
+                                // reduced planes kept where their error estimate is within 1e-5
   bool ready = false;
   FastNetDev dev{};
   WalkParams wp{};
 };
 
 struct FastBuffers {
-  void* mem[40] = {nullptr};
-  size_t bytes[40] = {0};
+  void* mem[64] = {nullptr};
+  size_t bytes[64] = {0};
 };
 
 struct FastLevel {
@@ -78,7 +90,8 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
 void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
                 StageTimer& timer, FastTimes& t, unsigned* flags, int* acc);
 // Planes of n given regions (their masks in `reg_mask`) with the prepared
-// reduced-precision form: planes [n][4][Cs] f32 (signed), eq [n][4] f32.
+// reduced-precision form: planes [n][AM_PCOMP][Cs] f32 (x, y, z, c signed, then
+// each plane's error estimate sigma), eq [n][AM_EQS] f32 (x, y, z, c, sigma).
 void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, float* eq, hipStream_t st);
 void fast_release(FastBuffers& b);
 

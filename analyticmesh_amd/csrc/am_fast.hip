@@ -10,9 +10,16 @@
 //     a k-major copy of W^T, X is staged already compacted;
 //   * split-BF16: w = wh + wl, x = xh + xl (bf16 parts), y = wh.xh + wh.xl + wl.xh
 //     with v_mfma_f32_16x16x32_bf16 and FP32 accumulation.
+// Beside each plane the kernel writes an estimate sigma of its error: the
+// variance of the accumulated rounding (and, split-BF16, of the split) is
+// propagated through the chain, V' = W^2 m (V + ck |P|^2), as one more small
+// product per layer (bf16 MFMA, A = W^2; the estimate needs a few bits only).
+// The checked walk bounds each plane by lam sigma plus a few ulps instead of
+// one relative bound for every plane of the net.
 // The signed planes go to HBM, one SoA record per region, and the checked FP32
 // walk reads them back into LDS.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 #include <vector>
@@ -70,23 +77,33 @@ __device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32
     }
   }
   // signed planes to HBM: thread = constraint (consecutive threads, consecutive addresses)
+  // (hidden layer 1's planes are rows of W0 and b0, exact: sigma 0)
   for (int k = tid; k < n1; k += 256) {
     const float v4[4] = {f.W0[k * 3 + 0], f.W0[k * 3 + 1], f.W0[k * 3 + 2], f.bias[0][k]};
     for (int t = 0; t < nreg; ++t) {
       const bool on = mbit(mk + t * f.Wd, k);
-      float* dst = io.planes + (size_t)(r0 + t) * 4 * f.Cs + k;
+      float* dst = io.planes + (size_t)(r0 + t) * AM_PCOMP * f.Cs + k;
 #pragma unroll
       for (int q = 0; q < 4; ++q) dst[(size_t)q * f.Cs] = on ? -v4[q] : v4[q];
+      dst[(size_t)4 * f.Cs] = 0.f;
     }
   }
-  for (int idx = tid; idx < AM_GN * f.E; idx += 256) {
-    const int col = idx / f.E, e = idx - col * f.E, t = col >> 2, q = col & 3;
-    if (t < nreg) io.planes[((size_t)(r0 + t) * 4 + q) * f.Cs + f.L + e] = q < 3 ? f.ew[e * 3 + q] : f.eb[e];
+  for (int idx = tid; idx < AM_GT * AM_PCOMP * f.E; idx += 256) {
+    const int col = idx / f.E, e = idx - col * f.E, t = col / AM_PCOMP, q = col - t * AM_PCOMP;
+    if (t < nreg)
+      io.planes[((size_t)(r0 + t) * AM_PCOMP + q) * f.Cs + f.L + e] = q < 3 ? f.ew[e * 3 + q] : (q == 3 ? f.eb[e] : 0.f);
   }
 }
 
+// Input column of the error-estimate product for an input plane with
+// variance V and squared norm n2 (active in its region): V + ck n2, capped to
+// stay finite in bf16.
+__device__ __forceinline__ uint16_t sig_input(float V, float n2, float ck) {
+  return bf16_rn(fminf(V + ck * n2, 1e30f));
+}
+
 template <int PREC>
-__global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
+__global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, GemmIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int KS = f.Kpad + 8;   // bf16 column stride (halves)
@@ -102,6 +119,15 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   uint32_t* mk = reinterpret_cast<uint32_t*>(smem_raw + xbytes);
   const int KLF = f.Kpad;                                            // entries of one layer's input list
   uint16_t* klist = reinterpret_cast<uint16_t*>(mk + AM_GT * f.Wd);  // fp32: [D][KLF]
+  // Error-estimate product input, element (k, t) for region t < AM_GT, bf16;
+  // after the product has read it, the same slot holds the output V (k, t),
+  // bf16.  FP32: in the 16-byte padding of row k of the X / Y buffer (columns
+  // AM_GN .., unused by the planes); split-BF16 (X there is in bf16 parts
+  // over the whole buffer): a [16][KS] array of its own.
+  const size_t qoff = (xbytes + (size_t)AM_GT * f.Wd * 4 + (PREC == 0 ? (size_t)f.D * KLF * 2 : 0) + 15) & ~(size_t)15;
+  uint16_t* Xq = reinterpret_cast<uint16_t*>(smem_raw + qoff);
+  uint16_t* Ypad = reinterpret_cast<uint16_t*>(Ys) + 2 * AM_GN;
+  auto xq_at = [&](int k, int t) -> uint16_t& { return PREC == 0 ? Ypad[k * 2 * AM_XS + t] : Xq[t * KS + k]; };
   __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_GT;
   const int nreg = min(AM_GT, io.count - r0);
@@ -115,7 +141,20 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
   } else {
     for (int i = tid; i < 2 * AM_GN * KS; i += 256) Xh[i] = 0;
   }
+  if (f.sig && PREC != 0)
+    for (int i = tid; i < 16 * KS; i += 256) Xq[i] = 0;
   __syncthreads();
+  if (f.sig) {
+    // hidden layer 1's planes are exact (V = 0)
+    const int n1 = f.nodes[1];
+    for (int idx = tid; idx < n1 * AM_GT; idx += 256) {
+      const int k = idx / AM_GT, t = idx - k * AM_GT;
+      if (mbit(mk + t * f.Wd, k)) {
+        const float a = f.W0[k * 3 + 0], b = f.W0[k * 3 + 1], c = f.W0[k * 3 + 2], d = f.bias[0][k];
+        xq_at(k, t) = sig_input(0.f, a * a + b * b + c * c + d * d, f.ck[1]);
+      }
+    }
+  }
   if (PREC == 0) {
     // Inputs of linear layer li that are active in some region of the tile,
     // ascending, padded to a multiple of 16 with n_li (the +0 row of W^T and,
@@ -147,6 +186,53 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
     const int nout = f.nodes[li + 1];
     const int nrb = f.nrb[li], nkb = f.nkb[li];
     const float* bias = f.bias[li];
+    if (f.sig) {
+      // the error-estimate product of this wave's row blocks, V = W^2 . Xq;
+      // lane (column t, rows 4 (lane >> 4) + i of block rb) holds region t's V
+      // of plane rb * 16 + 4 (lane >> 4) + i
+      const int nuq = (nrb - wave + 3) / 4;
+      f32x4 accq[AM_GRBW];
+#pragma unroll
+      for (int u = 0; u < AM_GRBW; ++u) accq[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+      if (nuq > 0) {
+        const int nkq = f.nkq[li];
+        const int tq = lane & 15, kq = 8 * (lane >> 4);
+        const uint16_t* xq0 = Xq + tq * KS + kq;
+        const uint4* Aq = reinterpret_cast<const uint4*>(f.Wsq[li]) + lane;
+        for (int kb = 0; kb < nkq; ++kb) {
+          bf16x8 vb;
+          if (PREC == 0) {
+            // column tq, rows kb * 32 + kq .. + 7 from the row paddings (+0 for columns past AM_GT)
+            uint16_t h[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) h[e] = tq < AM_GT ? Ypad[(kb * 32 + kq + e) * 2 * AM_XS + tq] : (uint16_t)0;
+            vb = __builtin_bit_cast(bf16x8, h);
+          } else {
+            vb = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xq0 + kb * 32));
+          }
+#pragma unroll
+          for (int u = 0; u < AM_GRBW; ++u)
+            if (u < nuq) {
+              const bf16x8 va = __builtin_bit_cast(bf16x8, Aq[((size_t)(wave + 4 * u) * nkq + kb) * 64]);
+              accq[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, accq[u], 0, 0, 0);
+            }
+        }
+      }
+      __syncthreads();   // every wave has read Xq
+      const int t = lane & 15;
+#pragma unroll
+      for (int u = 0; u < AM_GRBW; ++u) {
+        const int rb = wave + 4 * u;
+        if (rb < nrb && t < AM_GT) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int j = rb * 16 + 4 * (lane >> 4) + i;
+            if (j < nout) xq_at(j, t) = bf16_rn(fminf(fmaxf(accq[u][i], 0.f) + f.ck[li] * bias[j] * bias[j], 1e30f));
+          }
+        }
+      }
+      // (read after the barriers that follow the layer's k loop)
+    }
     // Row blocks rb = wave + 4 u of this wave (n_out <= 64 * AM_GRBW).  The k
     // loop is outermost, so each B fragment read from LDS feeds every row block.
     f32x4 accs[AM_GRBW][2];
@@ -248,12 +334,45 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
       for (int j = tid; j < nout; j += 256) {
         for (int t = 0; t < nreg; ++t) {
           const bool on = mbit(mk + t * f.Wd, out0 + j);
-          float* dst = io.planes + (size_t)(r0 + t) * 4 * f.Cs + out0 + j;
+          float* dst = io.planes + (size_t)(r0 + t) * AM_PCOMP * f.Cs + out0 + j;
           const float4 y = *reinterpret_cast<const float4*>(Ys + j * AM_XS + 4 * t);
           dst[0] = on ? -y.x : y.x;
           dst[f.Cs] = on ? -y.y : y.y;
           dst[2 * (size_t)f.Cs] = on ? -y.z : y.z;
           dst[3 * (size_t)f.Cs] = on ? -y.w : y.w;
+        }
+      }
+      if (!f.sig) {
+        for (int idx = tid; idx < nreg * nout; idx += 256) {
+          const int t = idx / nout, j = idx - t * nout;
+          io.planes[((size_t)(r0 + t) * AM_PCOMP + 4) * f.Cs + out0 + j] = 0.f;
+        }
+      } else {
+        // estimates of this layer's planes to HBM (consecutive threads,
+        // consecutive planes) and the next product's input: values first,
+        // then a barrier (V and the next input share their slots), then the writes
+        const float ckn = f.ck[li + 1];
+        const int rows = f.nkq[li + 1] * 32;   // rows the next product reads
+        constexpr int QPT = AM_GT * AM_CHAIN_MAXW / 256;   // values per thread (rows <= 256)
+        uint16_t qv[QPT];
+#pragma unroll
+        for (int c = 0; c < QPT; ++c) {
+          const int idx = tid + 256 * c;
+          const int t = idx / rows, j = idx - t * rows;
+          qv[c] = 0;
+          if (t < AM_GT && j < nout) {
+            const float V = bf16_to_f(xq_at(j, t));
+            if (t < nreg) io.planes[((size_t)(r0 + t) * AM_PCOMP + 4) * f.Cs + out0 + j] = 1.02f * __builtin_sqrtf(V);
+            const float4 y = *reinterpret_cast<const float4*>(Ys + j * AM_XS + 4 * t);
+            if (mbit(mk + t * f.Wd, out0 + j)) qv[c] = sig_input(V, y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w, ckn);
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < QPT; ++c) {
+          const int idx = tid + 256 * c;
+          const int t = idx / rows, j = idx - t * rows;
+          if (t < AM_GT) xq_at(j, t) = qv[c];
         }
       }
       if (PREC == 0) {
@@ -314,13 +433,17 @@ __global__ __launch_bounds__(256) void k_gemm(FastNetDev f, GemmIO io) {
           }
         }
       }
-    } else if (tid < AM_GN) {
-      const int t = tid >> 2, q = tid & 3;
-      if (t < nreg) {
-        float v = Ys[tid];
-        if (q == 3) v = v - f.iso;
-        io.eq[(size_t)(r0 + t) * 4 + q] = v;
+    } else {
+      if (tid < AM_GN) {
+        const int t = tid >> 2, q = tid & 3;
+        if (t < nreg) {
+          float v = Ys[tid];
+          if (q == 3) v = v - f.iso;
+          io.eq[(size_t)(r0 + t) * AM_EQS + q] = v;
+        }
       }
+      // the face plane's estimate
+      if (tid < nreg) io.eq[(size_t)(r0 + tid) * AM_EQS + 4] = f.sig ? 1.02f * __builtin_sqrtf(bf16_to_f(xq_at(0, tid))) : 0.f;
     }
     __syncthreads();
   }
@@ -336,7 +459,8 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
   float* Py = Px + Cs;
   float* Pz = Py + Cs;
   float* Pc = Pz + Cs;
-  float* keys = Pc + Cs;
+  float* Pe = Pc + Cs;
+  float* keys = Pe + Cs;
   int* order = reinterpret_cast<int*>(keys + N2);
   uint32_t* used = reinterpret_cast<uint32_t*>(order + N2);
   uint32_t* smask = used + ((C + 31) >> 5);
@@ -344,20 +468,23 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
   const int b = blockIdx.x;          // index within the chunk
   if (b >= io.count) return;
   const int rid = chunk_first + b;
-  const float* src = planes + (size_t)b * 4 * Cs_hbm;
+  const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
   for (int g = lane; g < C; g += 64) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * Cs_hbm + g];
     Pc[g] = src[3 * Cs_hbm + g];
+    Pe[g] = src[4 * Cs_hbm + g];
   }
   for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
-  const float eq[4] = {eqbuf[b * 4 + 0], eqbuf[b * 4 + 1], eqbuf[b * 4 + 2], eqbuf[b * 4 + 3]};
+  const float* eb = eqbuf + (size_t)b * AM_EQS;
+  const float eq[4] = {eb[0], eb[1], eb[2], eb[3]};
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
-  walk_region<float, true>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p);
+  walk_region<float, true>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p,
+                           wp.lam > 0.f ? Pe : nullptr, eb[4]);
 }
 
 // Ray-shooting FP32 walk: LDS holds only the planes and the mask.
@@ -370,35 +497,41 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
   float* Py = Px + Cs;
   float* Pz = Py + Cs;
   float* Pc = Pz + Cs;
-  uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cs);
+  float* Pe = Pc + Cs;
+  uint32_t* smask = reinterpret_cast<uint32_t*>(Pe + Cs);
   const int lane = threadIdx.x;
   const int b = blockIdx.x;
   if (b >= io.count) return;
   const int rid = chunk_first + b;
-  const float* src = planes + (size_t)b * 4 * Cs_hbm;
+  const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
   for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * Cs_hbm + g];
     Pc[g] = src[3 * Cs_hbm + g];
+    Pe[g] = src[4 * Cs_hbm + g];
   }
   for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
-  const float eq[4] = {eqbuf[b * 4 + 0], eqbuf[b * 4 + 1], eqbuf[b * 4 + 2], eqbuf[b * 4 + 3]};
+  const float* eb = eqbuf + (size_t)b * AM_EQS;
+  const float eq[4] = {eb[0], eb[1], eb[2], eb[3]};
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
-  const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask};
-  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
+  const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask, wp.lam > 0.f ? Pe : nullptr};
+  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p, eb[4]);
 }
 
-// Regions the walk left undecided, added to cnt[0], and their reasons, added
-// to cnt[1 + w] (counted per wave, one atomic per wave and counter).
+// Regions the walk left undecided, added to cnt[0], their reasons, added to
+// cnt[1 + w], and the decided ones with a vertex left to FP64, added to
+// cnt[1 + AM_WHY_COUNT] (counted per wave, one atomic per wave and counter).
 __global__ void k_count_undecided(const unsigned* flags, int n, int* cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned f = i < n ? flags[i] : 0u;
   const bool lead = (threadIdx.x & 63) == 0;
   const int u = __popcll(__ballot((f & AM_UNDECIDED) != 0u));
   if (lead && u) atomicAdd(cnt, u);
+  const int vv = __popcll(__ballot((f & (AM_UNDECIDED | AM_VERT64)) == AM_VERT64));
+  if (lead && vv) atomicAdd(cnt + 1 + AM_WHY_COUNT, vv);
   for (int w = 0; w < AM_WHY_COUNT; ++w) {
     const int k = __popcll(__ballot((f & AM_UNDECIDED) && (f & AM_WHY_BIT(w))));
     if (lead && k) atomicAdd(cnt + 1 + w, k);
@@ -421,6 +554,7 @@ enum Slot {
   S_W0 = 0, S_EW, S_EB, S_PLANES, S_EQ,
   S_BIAS0 = 8,      // 8..23: biases per linear layer
   S_WP0 = 24,       // 24..39: operand buffers per linear layer (fp32, or hi+lo packed)
+  S_WQ0 = 40,       // 40..55: W^2 operands of the error estimates per linear layer
 };
 
 // Buffer s with at least `bytes`; grows geometrically (levels grow run by run).
@@ -466,11 +600,30 @@ float host_bf16_f(uint16_t h) {
 }
 
 size_t gemm_lds(const FastNetDev& d, int prec) {
-  // one buffer for a layer's input and its outputs (k_gemm), the masks, the fp32 input lists
+  // one buffer for a layer's input and its outputs (k_gemm), the masks, the fp32 input lists,
+  // the error-estimate product's input (16 columns of bf16)
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
   const size_t xs = prec == 0 ? ys : std::max(ys, (size_t)2 * AM_GN * (d.Kpad + 8) * 2);
   const size_t kls = prec == 0 ? (size_t)d.D * d.Kpad * 2 : 0;
-  return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
+  if (prec == 0) return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;   // (estimates in the row paddings)
+  const size_t q = (xs + (size_t)AM_GT * d.Wd * 4 + kls + 15) & ~(size_t)15;
+  return q + (size_t)16 * (d.Kpad + 8) * 2 + 16;
+}
+
+// bf16 MFMA A operand [nrb][nkb][64 lanes][8] of an (nout x nin) matrix given
+// element-wise by `at(row, k)` (+0 past the matrix)
+template <typename F>
+std::vector<uint16_t> mfma_a_bf16(int nout, int nin, F at) {
+  const int nrb = (nout + 15) / 16, nkb = (nin + 31) / 32;
+  std::vector<uint16_t> out((size_t)nrb * nkb * 64 * 8, 0);
+  for (int rb = 0; rb < nrb; ++rb)
+    for (int kb = 0; kb < nkb; ++kb)
+      for (int l = 0; l < 64; ++l)
+        for (int e = 0; e < 8; ++e) {
+          const int row = rb * 16 + (l & 15), k = kb * 32 + 8 * (l >> 4) + e;
+          out[(((size_t)rb * nkb + kb) * 64 + l) * 8 + e] = (row < nout && k < nin) ? at(row, k) : 0;
+        }
+  return out;
 }
 
 }  // namespace
@@ -559,6 +712,30 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
       d.Wl[li] = p + n;
     }
   }
+  // error estimates (FP32 form): W^2 operands and the variance per squared
+  // term of each layer's sums, 4 u^2 (K + 2) for K = n_in products and the
+  // bias: u^2 per rounding and u^2 K for the accumulation, times 4 because the
+  // errors measured against FP64 on the matrix cores' accumulation order run
+  // about 3x those of a blocked CPU sum.  The split-BF16 form carries no
+  // estimates: its 16-bit products leave most regions undecidable even with
+  // per-plane bounds, so the product would cost more than it saves.
+  if (prec == 0) {
+    const double u2 = std::ldexp(1.0, -48);
+    for (int li = 1; li <= net.D; ++li) {
+      const int nin = net.nodes[li], nout = net.nodes[li + 1];
+      auto w = to_float(weights[li], (size_t)nout * nin, dtype, st);
+      auto q = mfma_a_bf16(nout, nin, [&](int row, int k) {
+        const float x = w[(size_t)row * nin + k];
+        return host_bf16(x * x);
+      });
+      uint16_t* p = (uint16_t*)slot(b, S_WQ0 + li, q.size() * 2);
+      FCHK(hipMemcpy(p, q.data(), q.size() * 2, hipMemcpyHostToDevice));
+      d.Wsq[li] = p;
+      d.nkq[li] = (nin + 31) / 32;
+      d.ck[li] = (float)(4.0 * u2 * (nin + 2));
+    }
+    d.sig = 1;
+  }
   const size_t lds = gemm_lds(d, prec);
   if (lds > 65536) {
     if (prec == 0)
@@ -566,11 +743,15 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
     else
       FCHK(hipFuncSetAttribute((const void*)k_gemm<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   }
-  // Relative error bounds of the planes (relative to each plane's largest
-  // coefficient): about 3x the largest error measured against FP64 planes on
-  // the 4x256 and 2x64 networks (FP32: 2.6e-5, split-BF16: 5.1e-4).
-  if (prec == 0) f.wp = WalkParams{8e-5f, 1e-3f, 1e3f};
-  else f.wp = WalkParams{2e-3f, 2e-2f, 1e2f};
+  // FP32: plane error bounds lam sigma per plane plus 8 u of its largest
+  // coefficient (the walk's own FP32 evaluations), lam = 3: the largest error
+  // measured against FP64 on 1.5M planes of the 4x256 net was 0.7 sigma.
+  // Vertex estimates (reduced_vertices = 1) use 0.5 sigma.  Split-BF16: one
+  // relative bound for every plane, 2e-3 of its largest coefficient (about 4x
+  // the largest error measured), and every vertex re-solved from FP64 planes.
+  const float lam_v = f.vertex_mode == 1 ? 0.5f : -1.f;
+  if (prec == 0) f.wp = WalkParams{4.8e-7f, 1e-3f, 1e3f, 3.f, lam_v};
+  else f.wp = WalkParams{2e-3f, 2e-2f, 1e2f, 0.f, -1.f};
   f.ready = true;
 }
 
@@ -581,16 +762,16 @@ void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& 
   t.launches = 0;
   const int CH = 1 << 16;
   const int chunk = std::min(lvl.count, CH);
-  float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * 4 * d.Cs * 4);
-  float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * 4 * 4);
+  float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * AM_PCOMP * d.Cs * 4);
+  float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * AM_EQS * 4);
 
   int Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
   while (Csort < net.C) Csort <<= 1;
   const int CsL = net.C;
-  const size_t wlds = (size_t)4 * CsL * 4 + (size_t)Csort * 8 + (size_t)((net.C + 31) / 32) * 4 + (size_t)net.Wd * 4 + 64;
+  const size_t wlds = (size_t)5 * CsL * 4 + (size_t)Csort * 8 + (size_t)((net.C + 31) / 32) * 4 + (size_t)net.Wd * 4 + 64;
   if (wlds > 65536)
     FCHK(hipFuncSetAttribute((const void*)k_walk_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wlds));
-  const size_t rlds = (size_t)4 * CsL * 4 + (size_t)net.Wd * 4 + 64;
+  const size_t rlds = (size_t)5 * CsL * 4 + (size_t)net.Wd * 4 + 64;
   if (rlds > 65536)
     FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
   const size_t glds = gemm_lds(d, prec);
@@ -662,7 +843,7 @@ void fast_planes(FastNet& f, const uint32_t* reg_mask, int n, float* planes, flo
 }
 
 void fast_release(FastBuffers& b) {
-  for (int i = 0; i < 40; ++i) {
+  for (int i = 0; i < 64; ++i) {
     if (b.mem[i]) (void)hipFree(b.mem[i]);
     b.mem[i] = nullptr;
     b.bytes[i] = 0;

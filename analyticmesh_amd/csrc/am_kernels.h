@@ -26,7 +26,15 @@ struct RegionIO {
   int Csort;                  // power of two >= C (sort width)
   unsigned* flags;            // [count] per-region flags (bit 0: undecided in reduced precision)
   const int* ids;             // optional: region ids to process (FP64 re-decision); else first + blockIdx.x
+  const int* dcount = nullptr;  // optional: ids past *dcount - dbase are not valid (a launch sized on
+  int dbase = 0;                // the host for the worst case; blocks past the device count exit)
 };
+
+// Regions of a launch whose count may be known on the device only.
+template <typename IO>
+__device__ __forceinline__ int launch_count(const IO& io) {
+  return io.dcount ? min(io.count, *io.dcount - io.dbase) : io.count;
+}
 
 // Visited-set insertion of one level's candidates.
 struct InsertIO {
@@ -66,6 +74,8 @@ struct ChainIO {
   const double* Wp[AM_MAX_LINEAR];  // li >= 1: W_li^T padded, [n_li][noutp[li]] (zero rows past n_out)
   const double* bp[AM_MAX_LINEAR];  // li >= 1: b_li padded to noutp[li]
   int noutp[AM_MAX_LINEAR];         // n_out rounded up to AM_CR
+  const int* dcount = nullptr;      // as in RegionIO: regions past *dcount - dbase are not valid
+  int dbase = 0;
 };
 
 // The regions and plane buffers of one tiled-chain launch.
@@ -102,8 +112,9 @@ __global__ void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd, int 
                                    const double* planes, const double* eq, int Cs_hbm);
 // Chunk-local slots of the regions k_walk_ray64 left undecided (cnt[0] counts
 // them); run[0] accumulates the count and run[1 + w] the reasons, per run.
-__global__ void k_collect_slots(const unsigned* flags, int n, const int* ids, int rid0, int lvl_first, int* cnt,
-                                int* slots, int* run);
+// With dn, entries past *dn - dbase are not valid.
+__global__ void k_collect_slots(const unsigned* flags, int n, const int* dn, int dbase, const int* ids, int rid0,
+                                int lvl_first, int* cnt, int* slots, int* run);
 // FP64 vertices of regions whose topology a reduced-precision walk decided:
 // block b re-solves the face vertices of region io.ids[b] (or rid0 + b) from
 // the FP64 planes of slot b (its two tight constraints and the face plane);

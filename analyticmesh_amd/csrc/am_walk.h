@@ -18,6 +18,11 @@
 namespace am {
 
 #define AM_UNDECIDED 0x1u
+// A reduced-precision walk decided the region but cannot vouch for one of its
+// vertices to 1e-5 (relative to |v|_inf, at least 1e-3): its vertices are
+// re-solved from FP64 planes.
+#define AM_VERT64 0x40000000u
+#define AM_VERT_TOL 1e-5f
 // Why a reduced-precision walk left its region to FP64 (flags bit 1 + reason index).
 #define AM_WHY_SEED 0       // seed: no plane bounds the seed's shot
 #define AM_WHY_START 1      // start edge interval empty or not finite
@@ -35,9 +40,15 @@ namespace am {
 #define AM_WHY_BIT(w) (1u << (1 + (w)))
 
 struct WalkParams {
-  float eps;        // relative error bound of the planes (CHECKED only)
+  float eps;        // relative error bound of the planes (CHECKED only); with
+                    // per-plane errors, the relative share on top of them
   float eps_turn;   // relative bound for the turn and orientation tests
   float h_max;      // largest Hadamard ratio of a vertex system that is trusted
+  float lam = 0.f;  // per-plane errors: a plane's bound is lam sigma (sigma its
+                    // error estimate) plus eps of its largest coefficient
+  float lam_v = 0.f;  // per-plane errors: the multiple of sigma in the vertex
+                      // accuracy estimate; < 0: every vertex is left to FP64
+                      // (AM_VERT64 on every decided region), 0: none is
 };
 
 template <typename T>
@@ -50,24 +61,29 @@ __device__ __forceinline__ T hadamard_ratio(const T* p1, const T* p2, const T* p
 }
 
 // Error bound of constraint signs at the vertex v of the system A = [p1; p2; p3]
-// when every plane's coefficients carry an error of at most eps times its
-// largest coefficient.  Perturbing the system by E moves the vertex by
-// dv = -A^-1 (E v + e) to first order, so constraint g's sign moves by
-// lambda_g . (E v + e) with lambda_g = A^-T a_g; its rows are the cross
-// products of the system's normals over the determinant.  The bound for g is
-//   eps m_g + kappa eps sum_i |lambda_g,i| m_i,   m_x = max|P_x| (1 + |v|_1),
-// where kappa = 1 / (1 - 2 rho), rho = eps (1 + H) (H the Hadamard ratio),
-// covers the second-order term; systems with rho > 1/4 are not trusted.
-// eps includes the working-precision rounding of the vertex solve.
-// Since |lambda_g,i| <= |n_i|_1 max|a_g| (n_i the rows of A^-T), the bound is
-// at most max|P_g| E with the per-vertex constant E = eps (1 + |v|_1) +
-// sum_i kappa eps m_i |n_i|_1.  FP64 planes (eps ~ 1e-13) use that cheap form;
-// coarse planes keep the per-constraint lambda form, which certifies more.
+// when plane x's coefficients carry an error of at most m_x: eps times its
+// largest coefficient, plus (per-plane mode) an absolute share e_x.
+// Perturbing the system by E moves the vertex by dv = -A^-1 (E v + e) to first
+// order, so constraint g's sign moves by lambda_g . (E v + e) with
+// lambda_g = A^-T a_g; its rows are the cross products of the system's normals
+// over the determinant.  The bound for g is
+//   m_g (1 + |v|_1) + kappa sum_i |lambda_g,i| m_i (1 + |v|_1),
+// where kappa = 1 / (1 - 2 rho), rho = r (1 + H) (H the Hadamard ratio, r the
+// largest m_i relative to its plane's largest coefficient), covers the
+// second-order term; systems with rho > 1/4 are not trusted.  The vertex
+// solve's working-precision rounding is added to every m_i.
+// Since |lambda_g,i| <= |n_i|_1 max|a_g| (n_i the rows of A^-T), with plane
+// errors relative to the largest coefficient the bound is at most max|P_g| E
+// with the per-vertex constant E = eps (1 + |v|_1) + sum_i kappa m_i |n_i|_1.
+// FP64 planes (eps ~ 1e-13) use that cheap form; coarse planes keep the
+// per-constraint lambda form, which certifies more.
 template <typename T>
 struct VertexBound {
   T E;                     // FP64: err_g = E max|P_g|
-  T n1[3], n2[3], n3[3];   // coarse planes: rows of A^-T
-  T w1, w2, w3, e0;        // coarse planes: kappa eps m_i, eps (1 + |v|_1)
+  T n1[3], n2[3], n3[3];   // coarse planes: rows of A^-T (columns of A^-1)
+  T w1, w2, w3, e0, eg;    // coarse planes: kappa m_i, eps (1 + |v|_1), lam (1 + |v|_1)
+  T s1, s2, s3;            // coarse planes, per-plane mode: kappa m_i with sigma for lam sigma
+                           // (the vertex's own error estimate)
   bool ok;
 };
 
@@ -78,8 +94,13 @@ __device__ __forceinline__ T coef_max(const T* p) {
 
 // The bound is linear in (1 + |v|_1): vertex_bound_unit computes it for
 // 1 + |v|_1 = 1 (it does not need the vertex), scale_bound applies the factor.
+// e1..e3: absolute coefficient errors of the three planes on top of eps times
+// their largest coefficients, lam sigma_i for error estimates sigma_i (0 when
+// the planes carry eps only); a constraint's own share is then lam sigma_g
+// (checked_verdict takes sigma_g), and the vertex estimate s_i uses lam_v sigma_i.
 template <typename T>
-__device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T* p2, const T* p3, T eps, T h_max) {
+__device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T* p2, const T* p3, T e1, T e2, T e3,
+                                                            T eps, T h_max, T lam, T lam_v) {
   VertexBound<T> b;
   const T c1[3] = {p2[1] * p3[2] - p2[2] * p3[1], p2[2] * p3[0] - p2[0] * p3[2], p2[0] * p3[1] - p2[1] * p3[0]};
   const T c2[3] = {p3[1] * p1[2] - p3[2] * p1[1], p3[2] * p1[0] - p3[0] * p1[2], p3[0] * p1[1] - p3[1] * p1[0]};
@@ -92,11 +113,16 @@ __device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T
   // Cramer's rule in working precision adds an error equivalent to perturbing
   // the system by about 16 u H of its coefficients (each 3x3 expansion's
   // absolute terms sum to at most 5.2 H |det|, with three roundings each)
-  const T u = sizeof(T) == 8 ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);   // 2^-53 / 2^-24
-  eps = eps + T(32) * u * (T(1) + h);
-  const T rho = eps * (T(1) + h);
+  // (per-plane mode, lam > 0: the vertex is solved in FP64 and rounded to T,
+  // which the constraint's own eps share covers)
+  const T u = (sizeof(T) == 8 || lam > T(0)) ? T(1.1102230246251565e-16) : T(5.9604644775390625e-08);
+  const T es = eps + T(32) * u * (T(1) + h);
+  const T q1 = coef_max(p1), q2 = coef_max(p2), q3 = coef_max(p3);
+  // the planes' total coefficient errors
+  const T m1 = es * q1 + e1, m2 = es * q2 + e2, m3 = es * q3 + e3;
+  const T rel = fmax(fmax(m1 / q1, m2 / q2), m3 / q3);
+  const T rho = rel * (T(1) + h);
   b.ok = h <= h_max && rho <= T(0.25);
-  const T vn = T(1);
   // 1.001: slack for the rounding of these bound computations themselves.
   // kappa >= 1.001 / (1 - x) with x = 2 rho <= 1/2 (systems with larger rho
   // are not trusted): 1 / (1 - x) = 1 + x + x^2 / (1 - x) <= 1 + x + 2 x^2
@@ -106,18 +132,27 @@ __device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T
   const T l1 = absT(c1[0]) + absT(c1[1]) + absT(c1[2]);
   const T l2 = absT(c2[0]) + absT(c2[1]) + absT(c2[2]);
   const T l3 = absT(c3[0]) + absT(c3[1]) + absT(c3[2]);
-  const T m1 = coef_max(p1) * vn, m2 = coef_max(p2) * vn, m3 = coef_max(p3) * vn;
-  const T w = kappa * eps * (m1 * l1 + m2 * l2 + m3 * l3) * iadet;
-  b.E = T(1.001) * (eps * vn + w);
+  const T w = kappa * (m1 * l1 + m2 * l2 + m3 * l3) * iadet;
+  b.E = T(1.001) * (es + w);
   if (sizeof(T) == 4) {
     const T inv = det < T(0) ? -iadet : iadet;   // 1 / det
     for (int i = 0; i < 3; ++i) { b.n1[i] = c1[i] * inv; b.n2[i] = c2[i] * inv; b.n3[i] = c3[i] * inv; }
-    b.w1 = kappa * eps * m1;
-    b.w2 = kappa * eps * m2;
-    b.w3 = kappa * eps * m3;
-    b.e0 = T(1.001) * eps * vn;
+    b.w1 = kappa * m1;
+    b.w2 = kappa * m2;
+    b.w3 = kappa * m3;
+    b.e0 = T(1.001) * es;
+    b.eg = T(1.001) * lam;
+    const T il = lam > T(0) ? lam_v / lam : T(0);
+    b.s1 = kappa * (es * q1 + e1 * il);
+    b.s2 = kappa * (es * q2 + e2 * il);
+    b.s3 = kappa * (es * q3 + e3 * il);
   }
   return b;
+}
+
+template <typename T>
+__device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T* p2, const T* p3, T eps, T h_max) {
+  return vertex_bound_unit<T>(p1, p2, p3, T(0), T(0), T(0), eps, h_max, T(0), T(0));
 }
 
 template <typename T>
@@ -127,20 +162,54 @@ __device__ __forceinline__ void scale_bound(VertexBound<T>& b, T vn) {
   b.w2 *= vn;
   b.w3 *= vn;
   b.e0 *= vn;
+  b.eg *= vn;
+  b.s1 *= vn;
+  b.s2 *= vn;
+  b.s3 *= vn;
 }
 
 template <typename T>
-__device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2, const T* p3, const T* v, T eps,
-                                                       T h_max) {
-  VertexBound<T> b = vertex_bound_unit<T>(p1, p2, p3, eps, h_max);
+__device__ __forceinline__ VertexBound<T> vertex_bound(const T* p1, const T* p2, const T* p3, const T* v, T e1, T e2,
+                                                       T e3, T eps, T h_max, T lam, T lam_v) {
+  VertexBound<T> b = vertex_bound_unit<T>(p1, p2, p3, e1, e2, e3, eps, h_max, lam, lam_v);
   scale_bound<T>(b, T(1) + absT(v[0]) + absT(v[1]) + absT(v[2]));
   return b;
 }
 
+// The vertex of planes p1, p2, p3: in FP64 and rounded to T when `wide` (coarse
+// planes, per-plane mode), else in T.
+template <typename T>
+__device__ __forceinline__ void solve_vertex(const T* p1, const T* p2, const T* p3, T* v, bool wide) {
+  if (sizeof(T) == 4 && wide) {
+    const double a[4] = {p1[0], p1[1], p1[2], p1[3]};
+    const double b[4] = {p2[0], p2[1], p2[2], p2[3]};
+    const double c[4] = {p3[0], p3[1], p3[2], p3[3]};
+    double x[3];
+    solve3(a, b, c, x);
+    v[0] = T(x[0]); v[1] = T(x[1]); v[2] = T(x[2]);
+  } else {
+    solve3(p1, p2, p3, v);
+  }
+}
+
+// Coarse planes, per-plane mode: whether the vertex v (solved from the bound's
+// system) is within AM_VERT_TOL of the exact one relative to |v|_inf (at least
+// 1e-3), by the error estimate |dv_x| <= sum_i |A^-1_x,i| s_i.
+template <typename T>
+__device__ __forceinline__ bool vertex_accurate(const VertexBound<T>& b, const T* v) {
+  const T tol = T(AM_VERT_TOL) * fmax(T(1e-3), fmax(fmax(absT(v[0]), absT(v[1])), absT(v[2])));
+  bool ok = true;
+  for (int x = 0; x < 3; ++x)
+    ok = ok && (absT(b.n1[x]) * b.s1 + absT(b.n2[x]) * b.s2 + absT(b.n3[x]) * b.s3 <= tol);
+  return ok;
+}
+
 // Signed-distance verdict of constraint (ax, ay, az, c) at v under the bound:
 // 0 = satisfied, 1 = certainly violated, 2 = undecided.
+// eg: the constraint's error estimate sigma_g (coarse planes, per-plane mode;
+// its bound share is lam sigma_g), else 0.
 template <typename T>
-__device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, const T* v, const VertexBound<T>& b) {
+__device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, T eg, const T* v, const VertexBound<T>& b) {
   const T s = plane_eval(ax, ay, az, c, v);
   const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
   T err;
@@ -148,7 +217,7 @@ __device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, const T* v
     const T l1 = b.n1[0] * ax + b.n1[1] * ay + b.n1[2] * az;
     const T l2 = b.n2[0] * ax + b.n2[1] * ay + b.n2[2] * az;
     const T l3 = b.n3[0] * ax + b.n3[1] * ay + b.n3[2] * az;
-    err = b.e0 * pm + b.w1 * absT(l1) + b.w2 * absT(l2) + b.w3 * absT(l3);
+    err = b.e0 * pm + b.eg * eg + b.w1 * absT(l1) + b.w2 * absT(l2) + b.w3 * absT(l3);
   } else {
     err = b.E * pm;
   }
@@ -159,9 +228,10 @@ __device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, const T* v
 
 // Verdict of one vertex against all constraints owned by this lane.
 // 0 = satisfied, 1 = certainly violated, 2 = undecided.
+// Pe: per-plane error estimates sigma, or nullptr.
 template <typename T, bool CHECKED>
-__device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz, const T* Pc, int C, int lane,
-                                          int ex1, int ex2, const T* v, const VertexBound<T>& vb) {
+__device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz, const T* Pc, const T* Pe, int C,
+                                          int lane, int ex1, int ex2, const T* v, const VertexBound<T>& vb) {
   int verdict = 0;
 #pragma unroll 2
   for (int g = lane; g < C; g += 64) {
@@ -170,7 +240,7 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
     if (!CHECKED) {
       if (violates(ax, ay, az, c, v)) return 1;
     } else {
-      const int vg = checked_verdict(ax, ay, az, c, v, vb);
+      const int vg = checked_verdict(ax, ay, az, c, Pe ? Pe[g] : T(0), v, vb);
       if (vg == 1) return 1;
       if (vg == 2) verdict = 2;
     }
@@ -184,7 +254,8 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
 template <typename T, bool CHECKED, typename V = T>
 __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                               const uint32_t* smask, const T* eq, int* s_tab, V* s_verts, int* s_vpair, int ntab,
-                              int nv, int start_in, int start, int status, bool undecided, unsigned why = 0u) {
+                              int nv, int start_in, int start, int status, bool undecided, unsigned why = 0u,
+                              bool vert64 = false) {
   const int L = net.L, vm = net.vm;
   const int lane = threadIdx.x;     // any block size: strides are blockDim.x
   const int nth = blockDim.x;
@@ -222,7 +293,7 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
     if (lane == 0) io.flags[slot_region] = AM_UNDECIDED | why;
     return;   // the FP64 redo writes this region's record and candidate slots
   }
-  if (CHECKED && lane == 0) io.flags[slot_region] = 0u;
+  if (CHECKED && lane == 0) io.flags[slot_region] = vert64 ? AM_VERT64 : 0u;
 
   // ---- region record
   if (lane == 0) {
@@ -412,10 +483,17 @@ __device__ void wave_top64(const T* Px, const T* Py, const T* Pz, const T* Pc, i
   __syncthreads();
 }
 
+// Pe, eqe: per-plane error estimates of the constraints and the face plane
+// (coarse planes, per-plane mode with wp.lam > 0), as in walk_ray.
 template <typename T, bool CHECKED>
 __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                             T* Px, T* Py, T* Pz, T* Pc, T* keys, int* order, uint32_t* used,
-                            const uint32_t* smask, const T* eq_in, const T* p) {
+                            const uint32_t* smask, const T* eq_in, const T* p, const T* Pe = nullptr,
+                            T eqe = T(0)) {
+  const bool per_plane = CHECKED && sizeof(T) == 4 && wp.lam > 0.f && Pe != nullptr;
+  const T lam = per_plane ? T(wp.lam) : T(0);
+  const T* Pg = per_plane ? Pe : nullptr;
+  bool vinacc = false;
   __shared__ int s_tab[AM_VM_MAX + 2];
   __shared__ T s_verts[AM_VM_MAX * 3];
   __shared__ int s_vpair[AM_VM_MAX * 2];
@@ -458,9 +536,10 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
         rj = order[j];
         const T pi4[4] = {Px[ri], Py[ri], Pz[ri], Pc[ri]};
         const T pj4[4] = {Px[rj], Py[rj], Pz[rj], Pc[rj]};
-        solve3(pi4, pj4, eq, v);
+        solve_vertex(pi4, pj4, eq, v, per_plane);
         if (CHECKED) {
-          vb = vertex_bound<T>(pi4, pj4, eq, v, T(wp.eps), T(wp.h_max));
+          vb = vertex_bound<T>(pi4, pj4, eq, v, lam * (Pg ? Pg[ri] : T(0)), lam * (Pg ? Pg[rj] : T(0)), lam * eqe,
+                               T(wp.eps), T(wp.h_max), lam, T(wp.lam_v));
           if (!vb.ok) verdict = 2;
         }
       }
@@ -470,7 +549,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
           if (!CHECKED) {
             if (violates(Px[g], Py[g], Pz[g], Pc[g], v)) verdict = 1;
           } else {
-            const int vg = checked_verdict(Px[g], Py[g], Pz[g], Pc[g], v, vb);
+            const int vg = checked_verdict(Px[g], Py[g], Pz[g], Pc[g], Pg ? Pg[g] : T(0), v, vb);
             if (vg == 1) verdict = 1;
             else if (vg == 2) verdict = 2;
           }
@@ -515,7 +594,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
         if (!cand) continue;
         const T pr4[4] = {Px[r], Py[r], Pz[r], Pc[r]};
         T vt[3];
-        solve3(pr4, pc4, eq, vt);
+        solve_vertex(pr4, pc4, eq, vt, per_plane);
         if (!CHECKED) {
           // infeasible as soon as one constraint is violated: test them in
           // distance order, a wave-wide chunk at a time, so that a rejected
@@ -539,10 +618,14 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
           v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
           break;
         }
-        const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, vt, T(wp.eps), T(wp.h_max));
-        const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, C, lane, cur, r, vt, vb);
+        const VertexBound<T> vb = vertex_bound<T>(pr4, pc4, eq, vt, lam * (Pg ? Pg[r] : T(0)),
+                                                  lam * (Pg ? Pg[cur] : T(0)), lam * eqe, T(wp.eps), T(wp.h_max), lam,
+                                                  T(wp.lam_v));
+        const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, Pg, C, lane, cur, r, vt, vb);
         if (__any(verdict == 1)) continue;          // certainly infeasible
         if (CHECKED && (__any(verdict == 2) || !vb.ok)) { undecided = true; break; }
+        if (CHECKED && sizeof(T) == 4 && (wp.lam_v < 0.f || (per_plane && wp.lam_v > 0.f && !vertex_accurate(vb, vt))))
+          vinacc = true;
         win = r;
         v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
         break;
@@ -618,7 +701,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
     }
   }
   finish_region<T, CHECKED>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv, start_in,
-                            start, status, undecided, undecided ? AM_WHY_BIT(AM_WHY_SCAN) : 0u);
+                            start, status, undecided, undecided ? AM_WHY_BIT(AM_WHY_SCAN) : 0u, vinacc);
 }
 
 // Minimum of (t, idx) over the wave, ties to the smaller index.
@@ -710,6 +793,8 @@ struct PlaneView {
   const double* W0;
   const double* b0;
   const uint32_t* mask;
+  const T* Pe = nullptr;   // per-plane error estimates sigma (indexed g - g0), or none
+  __device__ __forceinline__ T err(int g) const { return (Pe && g >= g0) ? Pe[g - g0] : T(0); }
   __device__ __forceinline__ void get(int g, T* p) const {
     if (g < g0) {
       const bool on = mbit(mask, g);
@@ -797,7 +882,7 @@ __device__ __forceinline__ int check_and_shoot(const PlaneView<T>& P, int C, con
     P.get(g, pg);
     const T ax = pg[0], ay = pg[1], az = pg[2], c = pg[3];
     if (g != c1 && g != c2) {
-      const int vg = checked_verdict(ax, ay, az, c, v, vb);
+      const int vg = checked_verdict(ax, ay, az, c, P.err(g), v, vb);
       bits |= vg == 1 ? 1 : (vg == 2 ? 2 : 0);
     }
     if (g != s1 && g != s2) {
@@ -857,9 +942,15 @@ __device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T
 // edge direction, and the turn test (with, for coarse planes, the FP64 vertex)
 // run side by side and meet in LDS at one barrier; the passes over the
 // constraints use every wave.
+// eqe: the face plane's error estimate (per-plane mode: coarse planes with
+// P.Pe and wp.lam > 0; the vertices are then solved in FP64 from the coarse
+// planes, and a region whose vertices the estimate cannot put within
+// AM_VERT_TOL gets the AM_VERT64 flag).
 template <typename T, int NW>
 __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
-                         const PlaneView<T>& P, const uint32_t* smask, const T* eq, const T* p) {
+                         const PlaneView<T>& P, const uint32_t* smask, const T* eq, const T* p, T eqe = T(0)) {
+  const bool per_plane = sizeof(T) == 4 && wp.lam > 0.f && P.Pe != nullptr;
+  bool vinacc = false;
   __shared__ int s_tab[AM_VM_MAX + 2];
   __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
   __shared__ int s_vpair[AM_VM_MAX * 2];
@@ -923,14 +1014,26 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         P.get(r, pr4);
         P.get(cur, pc4);
         T v[3];
-        solve3_rcp(pr4, pc4, eq, v);
+        if (per_plane) {
+          const double a4[4] = {pr4[0], pr4[1], pr4[2], pr4[3]};
+          const double b4[4] = {pc4[0], pc4[1], pc4[2], pc4[3]};
+          const double e4[4] = {eq[0], eq[1], eq[2], eq[3]};
+          double vd[3];
+          solve3(a4, b4, e4, vd);
+          v[0] = T(vd[0]); v[1] = T(vd[1]); v[2] = T(vd[2]);
+          if (lane == 0) { s_vd[0] = vd[0]; s_vd[1] = vd[1]; s_vd[2] = vd[2]; }
+        } else {
+          solve3_rcp(pr4, pc4, eq, v);
+        }
         if (lane == 0) { s_v[0] = v[0]; s_v[1] = v[1]; s_v[2] = v[2]; }
       }
       if (wave == 1 % NW) {   // its error bound
         T pr4[4], pc4[4];
         P.get(r, pr4);
         P.get(cur, pc4);
-        const VertexBound<T> vb = vertex_bound_unit<T>(pr4, pc4, eq, T(wp.eps), T(wp.h_max));
+        const T lam = per_plane ? T(wp.lam) : T(0);
+        const VertexBound<T> vb = vertex_bound_unit<T>(pr4, pc4, eq, lam * P.err(r), lam * P.err(cur), lam * eqe,
+                                                       T(wp.eps), T(wp.h_max), lam, T(wp.lam_v));
         if (lane == 0) s_vb = vb;
       }
       if (wave == 2 % NW) {   // the next edge runs along plane r, away from the plane we came along
@@ -956,7 +1059,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
                         (absT(A12 * C2 * C3) + absT(A23 * C1 * C2) + absT(A13 * C2 * C2) + absT(A22 * C1 * C3));
           if (!(val > T(wp.eps_turn) * mag)) w = AM_WHY_BIT(AM_WHY_TURN);
         }
-        if (sizeof(T) == 4) {
+        if (sizeof(T) == 4 && !per_plane) {
           T ar[4], ac[4];
           P.get(r, ar);
           P.get(cur, ac);
@@ -996,6 +1099,8 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         else why |= AM_WHY_BIT(AM_WHY_HMAX);
         break;
       }
+      if (sizeof(T) == 4 && (wp.lam_v < 0.f || (per_plane && wp.lam_v > 0.f && !vertex_accurate(vb, v))))
+        vinacc = true;
       // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
       // planes the certified check above already puts the new vertex farther
       // than that from the last one; with FP64 planes it may not, so vertices
@@ -1028,7 +1133,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   }
   __syncthreads();   // the last records are visible to every thread
   finish_region<T, true, double>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv,
-                                 start_in, start, AM_ST_OK, undecided, why);
+                                 start_in, start, AM_ST_OK, undecided, why, vinacc);
 }
 
 }  // namespace am

@@ -136,8 +136,9 @@ struct Engine {
     const int n = (int)masks.shape(0);
     py::array_t<float> P({(py::ssize_t)n, (py::ssize_t)4, (py::ssize_t)C});
     py::array_t<float> E({(py::ssize_t)n, (py::ssize_t)4});
-    check(c, am_planes(c, masks.data(), n, P.mutable_data(), E.mutable_data()));
-    return py::make_tuple(P, E);
+    py::array_t<float> S({(py::ssize_t)n, (py::ssize_t)(C + 1)});
+    check(c, am_planes(c, masks.data(), n, P.mutable_data(), E.mutable_data(), S.mutable_data()));
+    return py::make_tuple(P, E, S);
   }
 
   py::dict fingerprint() {

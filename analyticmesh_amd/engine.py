@@ -143,10 +143,13 @@ class Engine:
     """One engine context for a fixed architecture (mirrors ``Init``)."""
 
     def __init__(self, nodes, arc_table=None, num_extra_constraints=0, precision="fp64", vm=20,
-                 table_log2=None, walk="ray", ray_waves=None, scan_blocks=None, graph_levels=False):
+                 table_log2=None, walk="ray", ray_waves=None, scan_blocks=None, graph_levels=False,
+                 reduced_vertices="fp64"):
         """ray_waves / scan_blocks: launch shapes (see ``autotune``); None takes the tuned
         shape stored for the nearest network shape.  graph_levels: launch each level of the
-        FP64 forms as one captured hipGraph."""
+        FP64 forms as one captured hipGraph.  reduced_vertices (fp32 / bf16x3 forms): "fp64"
+        re-solves every vertex from FP64 planes; "estimate" keeps the vertex solved from the
+        reduced planes where the face kernel's error estimate puts it within 1e-5 of |v|."""
         m = native()
         nodes = [int(n) for n in nodes]
         D = len(nodes) - 2
@@ -168,6 +171,7 @@ class Engine:
         if table_log2 is not None:
             self._eng.set_option("table_log2", int(table_log2))
         self._eng.set_option("walk", {"ray": 0, "scan": 1}[walk])
+        self._eng.set_option("reduced_vertices", {"fp64": 0, "estimate": 1}[reduced_vertices])
         from .autotune import lookup
         tuned = lookup(nodes)
         self.ray_waves = int(ray_waves if ray_waves is not None else tuned["ray_waves"])
@@ -180,9 +184,12 @@ class Engine:
     def set_option(self, key, value):
         self._eng.set_option(key, int(value))
 
-    def planes(self, masks):
-        """Reduced-precision planes of the given packed masks: ((n, 4, C) f32 signed, (n, 4) eq)."""
-        return self._eng.planes(np.ascontiguousarray(masks, dtype=np.uint32), self.L + self.E)
+    def planes(self, masks, with_sigma=False):
+        """Reduced-precision planes of the given packed masks: ((n, 4, C) f32 signed, (n, 4) eq), and
+        with ``with_sigma`` the face kernel's error estimate of every plane, (n, C + 1) (the face
+        plane's last)."""
+        P, E, S = self._eng.planes(np.ascontiguousarray(masks, dtype=np.uint32), self.L + self.E)
+        return (P, E, S) if with_sigma else (P, E)
 
     def run(self, weights, biases, points, states, tms=(), extra_w=None, extra_b=None, iso=0.0, flip=False,
             stream=0):

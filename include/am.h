@@ -118,9 +118,11 @@ am_status am_adjoint(am_ctx* ctx, const void* grad_vertices, int dtype, void* co
 /* Constraint planes of n regions (packed masks [n][W]) computed by the
  * context's reduced-precision face kernel, for accuracy checks: planes
  * [n][4][C] f32 (x, y, z, c arrays, signed so that a.x + c <= 0 inside),
- * eq [n][4] f32 (f = iso plane).  Needs a prior am_run on this context.
+ * eq [n][4] f32 (f = iso plane), and, when sigma is not NULL, the kernel's
+ * error estimate of each plane, sigma [n][C + 1] f32 (the constraints', then
+ * the face plane's).  Needs a prior am_run on this context.
  * No counterpart in the reference (its planes stay internal). */
-am_status am_planes(am_ctx* ctx, const uint32_t* masks, int n, float* planes, float* eq);
+am_status am_planes(am_ctx* ctx, const uint32_t* masks, int n, float* planes, float* eq, float* sigma);
 
 /* Order-independent topology fingerprint of the last am_run (no counterpart in
  * the reference; a parity view for full-size runs): out5[0..2] are sums mod
@@ -141,7 +143,11 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * level's launches, FP64 forms, are captured into a hipGraph and launched as
  * one; the executable graph is updated in place from level to level),
  * "walk" (0 ray walk, 1
- * candidate scan, for the reduced forms), "adjoint_tiled" (1 default: the
+ * candidate scan, for the reduced forms), "reduced_vertices" (reduced forms:
+ * 0 default, every vertex of a region their walk decides is re-solved from
+ * FP64 planes; 1 the vertex solved in FP64 from the reduced planes is kept
+ * where the face kernel's error estimate puts it within 1e-5 of |v|, the
+ * other regions' are re-solved), "adjoint_tiled" (1 default: the
  * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
  * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU). */
 am_status am_set_option(am_ctx* ctx, const char* key, long long value);

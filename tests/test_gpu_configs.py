@@ -58,7 +58,8 @@ def test_deep_fp64_bitexact_vs_oracle(deep64):
 
 @pytest.mark.parametrize("prec", ["fp64ray", "fp32", "bf16x3"])
 def test_deep_reduced_forms_topology_vs_oracle(prec, deep64):
-    """Each checked form on the 4x256 net gives the oracle's topology; vertices within 1e-12."""
+    """Each checked form on the 4x256 net gives the oracle's topology; vertices within 1e-12 (FP64
+    ray form) or 1e-5 (reduced forms, relative to |v|)."""
     from analyticmesh_amd.engine import march
     spec, pts, st, cap, o = deep64
     _say(f"[deep {prec} vs oracle]")
@@ -66,7 +67,7 @@ def test_deep_reduced_forms_topology_vs_oracle(prec, deep64):
     _topology(g, o, f"deep {prec}")
     err = vertex_error(g.records(), o.records())
     _say(f"  redo {int(g.stats[9])} re-solved {int(g.stats[33])} vertex err {err:.2e}")
-    assert err is not None and err < 1e-12
+    assert err is not None and err < (1e-12 if prec == "fp64ray" else 1e-5)
 
 
 def _batch_seeds(specs, n):

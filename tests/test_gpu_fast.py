@@ -2,12 +2,11 @@
 ray walk) against FP64.
 
 Every form must give the FP64 topology exactly (same visited regions, faces
-and vertex keys).  Vertices (VERT_TOL): every form solves each vertex from the
-FP64 planes (the FP32 and split-BF16 forms re-solve the vertices of the
-regions their own walk decided), so only the order of the 3x3 solve's rows
-can differ from the scan's.  Solving from the reduced-precision planes
-instead gives errors up to 3.2e-5 on the 4x256 net (FP32 plane error 2.6e-5
-of the largest coefficient), over the 1e-5 the forms are held to."""
+and vertex keys).  Vertices (VERT_TOL, relative to |v|): the FP64 ray form
+solves them from the FP64 planes, so only the order of the 3x3 solve's rows
+can differ from the scan's.  The reduced forms solve each vertex in FP64 from
+their own planes and keep it where the face kernel's error estimate puts it
+within 1e-5; the other regions' vertices are re-solved from FP64 planes."""
 import sys
 
 import numpy as np
@@ -17,7 +16,7 @@ from _common import assert_same_topology, max_vertex_error, workload
 
 pytestmark = pytest.mark.gpu
 
-VERT_TOL = {"fp64ray": 1e-12, "fp32": 1e-12, "bf16x3": 1e-12}
+VERT_TOL = {"fp64ray": 1e-12, "fp32": 1e-5, "bf16x3": 1e-5}
 
 
 def _say(*a):
@@ -55,5 +54,5 @@ def test_fast_deep_vs_fp64(prec):
     err = max_vertex_error(a, b)
     _say(f"  regions {len(a)} redo {int(g.stats[9])} re-solved {int(g.stats[33])} max vertex rel err {err:.2e}")
     if prec != "fp64ray":
-        assert int(g.stats[9]) + int(g.stats[33]) == len(a)
+        assert int(g.stats[9]) + int(g.stats[33]) <= len(a)
     assert err < VERT_TOL[prec]

@@ -138,4 +138,4 @@ def test_iso_flip_vm_vs_oracle(prec, oracle_mod):
     else:
         a, b = g.records(), o.records()
         assert_same_topology(a, b)
-        assert max_vertex_error(a, b) < 1e-10
+        assert max_vertex_error(a, b) < (1e-10 if prec == "fp64ray" else 1e-5)
