@@ -23,7 +23,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",
 ]
-HIP_SOURCES = ["am_region_fp64.hip", "am_chain64.hip", "am_hash.hip", "am_combine.hip", "am_fast.hip",
+HIP_SOURCES = ["am_region_fp64.hip", "am_chain64.hip", "am_hash.hip", "am_combine.hip", "am_fast.hip", "am_newton.hip",
                "am_adjoint.hip", "am_layout.hip", "am_engine.hip"]
 
 

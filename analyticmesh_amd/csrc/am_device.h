@@ -32,6 +32,14 @@ __device__ __forceinline__ float absT(float a) { return __builtin_fabsf(a); }
 
 __device__ __forceinline__ bool mbit(const uint32_t* m, int g) { return (m[g >> 5] >> (g & 31)) & 1u; }
 
+// bfloat16 bits of x (round to nearest even) and back.
+__device__ __forceinline__ uint16_t bf16_bits(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+
 // s = a.v + c, evaluated as fma(ax, vx, fma(ay, vy, fma(az, vz, c))).
 template <typename T>
 __device__ __forceinline__ T plane_eval(T ax, T ay, T az, T c, const T* v) {

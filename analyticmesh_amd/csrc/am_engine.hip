@@ -25,6 +25,7 @@
 #include "am_kernels.h"
 #include "am_fast.h"
 #include "am_adjoint.h"
+#include "am_newton.h"
 #include "am_combine.h"
 #include "am_scan.h"
 
@@ -182,6 +183,8 @@ struct am_ctx {
   std::vector<DevBuf<double>> adj_dW, adj_db, adj_G, adj_P;
   DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
   DevBuf<int> redo_ids, redo_cnt;         // regions a reduced-precision walk left to FP64, their count
+  DevBuf<uint16_t> reg_jinv;              // [R][vm][9] bf16: the reduced walk's inverse vertex systems
+  bool jinv_on = false;                   // this run keeps them (reduced form, Newton vertices)
   DevBuf<int> runacc;                     // per run: FP64 ray walk undecided count, then per reason
   StageTimer timer;
   std::vector<DevBuf<double>> WTp, bpad;   // W_li^T and b_li with n_out padded to AM_CR (zeros)
@@ -202,6 +205,7 @@ void grow_regions(am_ctx* c, size_t need) {
   ensure(c->reg_tab, cap * (c->vm + 1), true, st);
   ensure(c->reg_verts, cap * c->vm * 3, true, st);
   ensure(c->reg_vpair, cap * c->vm * 2, true, st);
+  if (c->jinv_on) ensure(c->reg_jinv, cap * c->vm * 9, true, st);
   c->reg_cap = cap;
 }
 
@@ -1024,7 +1028,7 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
       // the region store's tab, vertex and pair arrays are strided by vm: drop
       // them (a failed am_run may have sized them already)
       c->reg_mask.release(); c->reg_start.release(); c->reg_point.release(); c->reg_info.release();
-      c->reg_tab.release(); c->reg_verts.release(); c->reg_vpair.release();
+      c->reg_tab.release(); c->reg_verts.release(); c->reg_vpair.release(); c->reg_jinv.release();
       c->reg_cap = 0;
       c->R = 0;
     }
@@ -1049,8 +1053,8 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
   } else if (k == "reduced_vertices") {
-    if (value != 0 && value != 1)
-      return fail(c, AM_ERR_ARG, "reduced_vertices must be 0 (FP64 re-solve) or 1 (estimate)");
+    if (value < 0 || value > 2)
+      return fail(c, AM_ERR_ARG, "reduced_vertices must be 0 (FP64 re-solve), 1 (estimate) or 2 (Newton)");
     c->fast.vertex_mode = (int)value;
   } else if (k == "adjoint_mfma") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "adjoint_mfma must be 0 or 1");
@@ -1063,6 +1067,11 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   }
   return AM_OK;
 }
+
+// FP64 Newton step for the vertices the reduced walk solved from its planes
+// (defined with the mesh assembly, whose vertex dedup it uses); returns the
+// number of vertices refined.
+long long newton_refine(am_ctx* c);
 
 am_status am_run(am_ctx* c, const void* const* weights, const void* const* biases, int n_tm,
                  const int* tm_shapes, const void* const* tms, int dtype, const void* points,
@@ -1192,7 +1201,20 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     // those forms on the FP64 region kernel, with the same topology.
     const bool reduced_asked = c->precision == AM_PREC_FP32 || c->precision == AM_PREC_SPLIT_BF16X3;
     const bool fp64_form = !reduced_asked || !c->tile_ok;
+    // FP32 form, Newton vertices: the walk keeps each vertex's inverse system
+    // (the Newton kernel takes layers up to AM_CHAIN_MAXW wide, which the
+    // reduced forms require anyway, and the ray walk)
+    c->jinv_on = !fp64_form && c->precision == AM_PREC_FP32 && c->fast.vertex_mode == 2 &&
+                 c->fast.walk_mode == 0 && newton_lds_bytes(c->Wd) <= c->max_lds;
+    const int vmode = c->fast.vertex_mode;
+    if (!fp64_form && c->precision == AM_PREC_FP32 && vmode == 2 && !c->jinv_on) c->fast.vertex_mode = 0;
     if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, wv.data(), bv.data(), dtype, st);
+    c->fast.vertex_mode = vmode;
+    if (c->jinv_on) {
+      ensure(c->reg_jinv, c->reg_cap * c->vm * 9, true, st);
+      HIPCHK(hipFuncSetAttribute((const void*)k_newton_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)newton_lds_bytes(c->Wd)));
+    }
     if (c->tile_ok && ray64_lds_bytes(c) > 65536 && ray64_lds_bytes(c) <= c->max_lds)
       set_ray64_lds(ray64_lds_bytes(c));
 
@@ -1273,6 +1295,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.reg_vpair = c->reg_vpair.p; fl.cand_mask = c->cand_mask.p; fl.cand_start = c->cand_start.p;
         fl.cand_point = c->cand_point.p;
         fl.cand_hash = c->cand_hash.p;
+        fl.reg_jinv = c->jinv_on ? c->reg_jinv.p : nullptr;
         FastTimes ft{};
         fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT);
         // FP64 planes of the regions the walk left to FP64 only (their list
@@ -1303,6 +1326,12 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     if (lvl_count > 0 && c->max_regions >= 0) {
       // regions found but not processed: drop them from the store
       c->R = lvl_first;
+    }
+    long long newton_n = 0;
+    if (c->jinv_on && c->R > 0) {
+      const size_t mn = c->timer.mark(st);
+      newton_n = newton_refine(c);
+      c->timer.span(mn, c->timer.mark(st), T_NEWTON);
     }
     int acc[NACC] = {0};
     HIPCHK(hipMemcpyAsync(acc, c->runacc.p, sizeof(acc), hipMemcpyDeviceToHost, st));
@@ -1345,6 +1374,8 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_FP64_ONLY] = reduced_asked && fp64_form ? 1.0 : 0.0;
     c->stats[AM_STAT_HOST_WAITS] = (double)(levels + 1);
     c->stats[AM_STAT_GRAPH_LAUNCHES] = c->graph_levels && fp64_form ? (double)levels : 0.0;
+    c->stats[AM_STAT_NEWTON_MS] = T[T_NEWTON];
+    c->stats[AM_STAT_NEWTON_VERTICES] = (double)newton_n;
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];
     c->has_run = true;
     return AM_OK;
@@ -1388,6 +1419,67 @@ static CombineIn combine_in(const am_ctx* c) {
   in.vm = c->vm;
   in.L = c->L;
   return in;
+}
+
+// The first use of each vertex whose record carries the reduced walk's
+// inverse system (info[7] = 1), compacted in use order.
+struct NewtonSrc {
+  const unsigned* vflag;
+  const int* item_region;
+  const int* info;
+  __device__ int operator()(size_t i) const {
+    return (vflag[i] && info[(size_t)item_region[i] * AM_INFO_STRIDE + 7] == 1) ? 1 : 0;
+  }
+};
+struct NewtonSink {
+  int* items;
+  __device__ void operator()(size_t i, int prefix, int v) const {
+    if (v) items[prefix] = (int)i;
+  }
+};
+
+long long newton_refine(am_ctx* c) {
+  const hipStream_t st = c->stream;
+  const size_t R = (size_t)c->R;
+  if (R * c->vm > (size_t)INT_MAX / 2) throw CapErr{"too many (region, vertex) slots for the vertex refinement"};
+  const CombineIn in = combine_in(c);
+  CombineWork& w = c->cwork;
+  w = CombineWork{};
+  ensure(c->cw_reg, combine_region_bytes((long long)R), false, st);
+  combine_carve_regions(c->cw_reg.p, (long long)R, w);
+  long long F = 0, N = 0;
+  combine_counts(c, in, w, F, N);
+  if (N == 0) return 0;
+  ensure(c->cw_item, combine_item_bytes(N), false, st);
+  combine_carve_items(c->cw_item.p, N, w);
+  combine_dedup(in, w, N, st);
+  ensure(c->redo_ids, (size_t)N, false, st);
+  ensure(c->redo_cnt, 1, false, st);
+  ensure(c->scan_part, scan_blocks((size_t)N) + 1, false, st);
+  scan_launch<int>(NewtonSrc{w.vflag, w.item_region, c->reg_info.p}, (size_t)N, NewtonSink{c->redo_ids.p},
+                   c->scan_part.p, c->redo_cnt.p, st);
+  int n = 0;
+  HIPCHK(hipMemcpyAsync(&n, c->redo_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  NewtonIO io;
+  io.reg_mask = c->reg_mask.p;
+  io.vpair = c->reg_vpair.p;
+  io.verts = c->reg_verts.p;
+  io.jinv = c->reg_jinv.p;
+  io.items = c->redo_ids.p;
+  io.count = c->redo_cnt.p;
+  io.item_region = w.item_region;
+  io.roff = w.roff;
+  if (n > 0) {
+    const int Ch = (c->C + 63) & ~63;
+    hipLaunchKernelGGL(k_newton_fp64, dim3((unsigned)((n + AM_NT - 1) / AM_NT)), dim3(AM_NEWTON_THREADS),
+                       newton_lds_bytes(c->Wd), st, c->net, chain_io(c, nullptr, 0, 0, Ch), io);
+    HIPCHK(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_newton_scatter, dim3(blocks((size_t)N, 256)), dim3(256), 0, st, io, (const uint32_t*)w.table,
+                     (const int*)w.slot, (const unsigned*)w.vflag, (const int*)c->reg_info.p, (int)N, c->vm);
+  HIPCHK(hipGetLastError());
+  return n;
 }
 
 am_status am_combine(am_ctx* c, double scale, const double* center3) {

@@ -47,8 +47,10 @@ struct FastNetDev {
 struct FastNet {
   int precision = 0;
   int walk_mode = 0;            // 0: ray-shooting walk, 1: the reference's candidate scan (checked)
-  int vertex_mode = 0;          // 0: every vertex re-solved from FP64 planes; 1: vertices solved from the
-                                // reduced planes kept where their error estimate is within 1e-5
+  int vertex_mode = 2;          // FP32 form: 0 every vertex re-solved from FP64 planes; 1 vertices solved
+                                // from the reduced planes kept where their error estimate is within 1e-5;
+                                // 2 (default) vertices solved from the reduced planes, then one FP64
+                                // Newton step after the march (the walk keeps their inverse systems)
   bool ready = false;
   FastNetDev dev{};
   WalkParams wp{};
@@ -72,6 +74,7 @@ struct FastLevel {
   int* cand_start;
   double* cand_point;
   uint64_t* cand_hash;
+  uint16_t* reg_jinv = nullptr;   // optional: the walk's inverse vertex systems (Newton vertices)
 };
 
 struct FastTimes {

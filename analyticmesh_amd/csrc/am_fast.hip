@@ -749,7 +749,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   // Vertex estimates (reduced_vertices = 1) use 0.5 sigma.  Split-BF16: one
   // relative bound for every plane, 2e-3 of its largest coefficient (about 4x
   // the largest error measured), and every vertex re-solved from FP64 planes.
-  const float lam_v = f.vertex_mode == 1 ? 0.5f : -1.f;
+  const float lam_v = f.vertex_mode == 1 ? 0.5f : (f.vertex_mode == 2 ? 0.f : -1.f);
   if (prec == 0) f.wp = WalkParams{4.8e-7f, 1e-3f, 1e3f, 3.f, lam_v};
   else f.wp = WalkParams{2e-3f, 2e-2f, 1e2f, 0.f, -1.f};
   f.ready = true;
@@ -805,6 +805,7 @@ void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& 
     io.cand_start = lvl.cand_start;
     io.cand_point = lvl.cand_point;
     io.cand_hash = lvl.cand_hash;
+    io.reg_jinv = lvl.reg_jinv;
     io.first = lvl.first;          // candidate slots / flags are level-relative
     io.count = n;
     io.Cs = CsL;

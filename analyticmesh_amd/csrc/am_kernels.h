@@ -28,6 +28,9 @@ struct RegionIO {
   const int* ids;             // optional: region ids to process (FP64 re-decision); else first + blockIdx.x
   const int* dcount = nullptr;  // optional: ids past *dcount - dbase are not valid (a launch sized on
   int dbase = 0;                // the host for the worst case; blocks past the device count exit)
+  uint16_t* reg_jinv = nullptr; // optional [R][vm][9] bf16: per vertex the inverse of its 3x3 system
+                                // from the reduced planes (columns for the new, walked and face plane),
+                                // for the FP64 Newton step (a record with it has info[7] = 1)
 };
 
 // Regions of a launch whose count may be known on the device only.

@@ -10,7 +10,7 @@
 
 namespace am {
 
-enum TimerSlot { T_CHAIN64 = 0, T_RAY64, T_SCAN64, T_REGION64, T_GEMM, T_WALK, T_INSERT, T_VERT64, T_SLOTS };
+enum TimerSlot { T_CHAIN64 = 0, T_RAY64, T_SCAN64, T_REGION64, T_GEMM, T_WALK, T_INSERT, T_VERT64, T_NEWTON, T_SLOTS };
 
 struct StageTimer {
   std::vector<hipEvent_t> pool;

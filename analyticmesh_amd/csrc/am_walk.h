@@ -255,7 +255,7 @@ template <typename T, bool CHECKED, typename V = T>
 __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                               const uint32_t* smask, const T* eq, int* s_tab, V* s_verts, int* s_vpair, int ntab,
                               int nv, int start_in, int start, int status, bool undecided, unsigned why = 0u,
-                              bool vert64 = false) {
+                              bool vert64 = false, float* s_jinv = nullptr) {
   const int L = net.L, vm = net.vm;
   const int lane = threadIdx.x;     // any block size: strides are blockDim.x
   const int nth = blockDim.x;
@@ -285,6 +285,12 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
           s_vpair[a * 2 + c] = s_vpair[b * 2 + c];
           s_vpair[b * 2 + c] = t;
         }
+        if (s_jinv)
+          for (int c = 0; c < 9; ++c) {
+            const float t = s_jinv[a * 9 + c];
+            s_jinv[a * 9 + c] = s_jinv[b * 9 + c];
+            s_jinv[b * 9 + c] = t;
+          }
       }
       __syncthreads();
   }
@@ -299,8 +305,11 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
   if (lane == 0) {
     int* inf = io.reg_info + (size_t)rid * AM_INFO_STRIDE;
     inf[0] = start_in; inf[1] = start; inf[2] = nv; inf[3] = ntab;
-    inf[4] = reversed; inf[5] = status; inf[6] = CHECKED ? 2 : (io.ids ? 1 : 0); inf[7] = 0;
+    inf[4] = reversed; inf[5] = status; inf[6] = CHECKED ? 2 : (io.ids ? 1 : 0);
+    inf[7] = (s_jinv && io.reg_jinv) ? 1 : 0;
   }
+  if (s_jinv && io.reg_jinv)
+    for (int i = lane; i < nv * 9; i += nth) io.reg_jinv[(size_t)rid * vm * 9 + i] = bf16_bits(s_jinv[i]);
   for (int i = lane; i < vm + 1; i += nth) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
   for (int i = lane; i < vm * 3; i += nth)
     io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[i] : 0.0;
@@ -954,6 +963,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   __shared__ int s_tab[AM_VM_MAX + 2];
   __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
   __shared__ int s_vpair[AM_VM_MAX * 2];
+  __shared__ float s_jinv[AM_VM_MAX * 9];      // per-plane mode: inverse systems for the Newton step
   __shared__ T s_v[6];                 // the vertex and the next edge direction
   __shared__ VertexBound<T> s_vb;      // its bound per unit of 1 + |v|_1
   __shared__ unsigned s_turn;          // turn test: 0 passed, else its reason bit
@@ -1118,6 +1128,12 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         s_vpair[nv * 2 + 0] = r;
         s_vpair[nv * 2 + 1] = cur;
         s_tab[ntab] = r;
+        if (per_plane)
+          for (int x = 0; x < 3; ++x) {
+            s_jinv[nv * 9 + x] = float(vb.n1[x]);
+            s_jinv[nv * 9 + 3 + x] = float(vb.n2[x]);
+            s_jinv[nv * 9 + 6 + x] = float(vb.n3[x]);
+          }
       }
       nv += 1;
       ntab += 1;
@@ -1133,7 +1149,7 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   }
   __syncthreads();   // the last records are visible to every thread
   finish_region<T, true, double>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv,
-                                 start_in, start, AM_ST_OK, undecided, why, vinacc);
+                                 start_in, start, AM_ST_OK, undecided, why, vinacc, per_plane ? s_jinv : nullptr);
 }
 
 }  // namespace am

@@ -415,6 +415,8 @@ def main():
             "fp64_ray_undecided": int(out[f]["stats"][30]),
             "fp64_scanned_regions": int(out[f]["stats"][29]),
             "insert_ms": round(float(out[f]["stats"][2]), 3),
+            "newton_ms": round(float(out[f]["stats"][37]), 3),
+            "newton_vertices": int(out[f]["stats"][38]),
             "combine_ms": round(float(getattr(out[f]["res"], "combine_ms", 0.0)), 3),
             "levels": int(out[f]["stats"][6]),
         }) for f in forms},

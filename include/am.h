@@ -101,7 +101,9 @@ am_status am_get_mesh(am_ctx* ctx, double* vertices, int* face_offsets, int* fac
 
 /* Per-region records in visiting order (parity view): masks [R][W] u32,
  * info [R][8] (start queued, start used, vertices, tab entries, reversed,
- * status, flags, 0), tabs [R][vm+1], verts [R][vm][3], vpairs [R][vm][2],
+ * status, source: 0 or 1 an FP64 walk, 2 a reduced-precision walk, then 1
+ * where the FP32 form's Newton step refined the vertices, else 0), tabs
+ * [R][vm+1], verts [R][vm][3], vpairs [R][vm][2],
  * points [R][3] (seed point each region was queued with).  Any pointer may be NULL. */
 am_status am_get_regions(am_ctx* ctx, uint32_t* masks, int* info, int* tabs, double* verts, int* vpairs,
                          double* points);
@@ -143,11 +145,14 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * level's launches, FP64 forms, are captured into a hipGraph and launched as
  * one; the executable graph is updated in place from level to level),
  * "walk" (0 ray walk, 1
- * candidate scan, for the reduced forms), "reduced_vertices" (reduced forms:
- * 0 default, every vertex of a region their walk decides is re-solved from
- * FP64 planes; 1 the vertex solved in FP64 from the reduced planes is kept
- * where the face kernel's error estimate puts it within 1e-5 of |v|, the
- * other regions' are re-solved), "adjoint_tiled" (1 default: the
+ * candidate scan, for the reduced forms), "reduced_vertices" (FP32 form: 2
+ * default, each vertex solved in FP64 from the reduced planes takes one FP64
+ * Newton step after the march, its residuals from an FP64 forward pass of the
+ * network under its region's mask; 0 every vertex of a region the reduced walk
+ * decides is re-solved from FP64 planes; 1 the reduced-plane vertex is kept
+ * where the face kernel's error estimate puts it within 1e-5 of |v|, the other
+ * regions' are re-solved; the split-BF16 form always re-solves),
+ * "adjoint_tiled" (1 default: the
  * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
  * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU). */
 am_status am_set_option(am_ctx* ctx, const char* key, long long value);
@@ -183,7 +188,9 @@ am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 #define AM_STAT_FP64_ONLY 34    /* 1: a reduced form ran on the FP64 region kernel (skips or layers over 256 wide) */
 #define AM_STAT_HOST_WAITS 35   /* host waits on the device during the run */
 #define AM_STAT_GRAPH_LAUNCHES 36 /* levels launched as one captured graph ("graph_levels") */
-#define AM_STAT_COUNT 37
+#define AM_STAT_NEWTON_MS 37    /* FP32 form: FP64 Newton step of the vertices (HIP events, with its vertex dedup) */
+#define AM_STAT_NEWTON_VERTICES 38 /* vertices it refined */
+#define AM_STAT_COUNT 39
 am_status am_get_stats(am_ctx* ctx, double* out, int n);
 
 /* ---- Many networks in one job (no counterpart in the reference, which marches

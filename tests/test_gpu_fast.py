@@ -52,7 +52,8 @@ def test_fast_deep_vs_fp64(prec):
     # both stop after the same number of BFS levels; compare what both finished
     assert_same_topology(a, b)
     err = max_vertex_error(a, b)
-    _say(f"  regions {len(a)} redo {int(g.stats[9])} re-solved {int(g.stats[33])} max vertex rel err {err:.2e}")
+    _say(f"  regions {len(a)} redo {int(g.stats[9])} re-solved {int(g.stats[33])} Newton {int(g.stats[38])} "
+         f"max vertex rel err {err:.2e}")
     if prec != "fp64ray":
         assert int(g.stats[9]) + int(g.stats[33]) <= len(a)
     assert err < VERT_TOL[prec]
