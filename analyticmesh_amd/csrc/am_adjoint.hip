@@ -483,15 +483,23 @@ __global__ __launch_bounds__(256) void k_col_sum4(const double* G, int count, in
       part[((size_t)blockIdx.y * 4 + q) * M + j] = ((ps[0][q][jx] + ps[1][q][jx]) + ps[2][q][jx]) + ps[3][q][jx];
 }
 
-// out[i * ostride] += part[0][i] + part[1][i] + ... + part[nslice - 1][i],
-// added in that order (slice z at part + z * sstride), i < n.
+// out[i * ostride] += the sum over slices z < nslice of part[z][i] (slice z at
+// part + z * sstride), i < n: eight running sums over z = u mod 8, u = 0..7,
+// then added pairwise in a fixed order (the loads of eight slices are in
+// flight at once; the result does not depend on timing).
 __global__ __launch_bounds__(256) void k_slice_sum(const double* part, int nslice, size_t sstride, int n, double* out,
                                                    int ostride) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  double s = out[(size_t)i * ostride];
-  for (int z = 0; z < nslice; ++z) s += part[(size_t)z * sstride + i];
-  out[(size_t)i * ostride] = s;
+  double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int z = 0;
+  for (; z + 8 <= nslice; z += 8) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) a[u] += part[(size_t)(z + u) * sstride + i];
+  }
+  for (int u = 0; z + u < nslice; ++u) a[u] += part[(size_t)(z + u) * sstride + i];
+  const double s = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
+  out[(size_t)i * ostride] = out[(size_t)i * ostride] + s;
 }
 
 }  // namespace am

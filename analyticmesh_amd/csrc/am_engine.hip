@@ -1687,9 +1687,11 @@ am_status am_planes(am_ctx* c, const uint32_t* masks, int n, float* planes, floa
                     (size_t)c->C * 4);
         eq[(size_t)r * 4 + q] = he[(size_t)r * AM_EQS + q];
       }
-      if (sigma) {
+      if (sigma && c->fast.dev.sig) {
         std::memcpy(sigma + (size_t)r * (c->C + 1), hp.data() + ((size_t)r * AM_PCOMP + 4) * Cs, (size_t)c->C * 4);
         sigma[(size_t)r * (c->C + 1) + c->C] = he[(size_t)r * AM_EQS + 4];
+      } else if (sigma) {
+        std::fill(sigma + (size_t)r * (c->C + 1), sigma + (size_t)(r + 1) * (c->C + 1), 0.f);   // no estimates
       }
     }
     return AM_OK;
@@ -1786,11 +1788,11 @@ am_status am_adjoint(am_ctx* c, const void* grad_vertices, int dtype, void* cons
                                    (int)lds));
     }
     // dW_li = G_li+1^T P_li with K = 4 x regions split into nz slices (at
-    // most 512 K each, the slices of one layer within 32 MiB), each stored
-    // whole and added in ascending order: the gradients do not depend on the
-    // order in which blocks finish
+    // least 512 K each, at most 128 slices within 32 MiB per layer), each
+    // stored whole and added in a fixed order: the gradients do not depend on
+    // the order in which blocks finish
     auto k_slices = [&](int K, int M, int N, int& kchunk) {
-      const long long cap = std::max<long long>(8, ((long long)1 << 22) / ((long long)M * N));
+      const long long cap = std::min<long long>(128, std::max<long long>(8, ((long long)1 << 22) / ((long long)M * N)));
       const int nz = (int)std::min<long long>((K + 511) / 512, cap);
       kchunk = ((K + nz - 1) / nz + 15) & ~15;
       return (K + kchunk - 1) / kchunk;

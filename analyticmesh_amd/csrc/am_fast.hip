@@ -85,12 +85,12 @@ __device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32
       float* dst = io.planes + (size_t)(r0 + t) * AM_PCOMP * f.Cs + k;
 #pragma unroll
       for (int q = 0; q < 4; ++q) dst[(size_t)q * f.Cs] = on ? -v4[q] : v4[q];
-      dst[(size_t)4 * f.Cs] = 0.f;
+      if (f.sig) dst[(size_t)4 * f.Cs] = 0.f;
     }
   }
   for (int idx = tid; idx < AM_GT * AM_PCOMP * f.E; idx += 256) {
     const int col = idx / f.E, e = idx - col * f.E, t = col / AM_PCOMP, q = col - t * AM_PCOMP;
-    if (t < nreg)
+    if (t < nreg && (q < 4 || f.sig))
       io.planes[((size_t)(r0 + t) * AM_PCOMP + q) * f.Cs + f.L + e] = q < 3 ? f.ew[e * 3 + q] : (q == 3 ? f.eb[e] : 0.f);
   }
 }
@@ -103,7 +103,7 @@ __device__ __forceinline__ uint16_t sig_input(float V, float n2, float ck) {
 }
 
 template <int PREC>
-__global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, GemmIO io) {
+__global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, GemmIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int KS = f.Kpad + 8;   // bf16 column stride (halves)
@@ -119,15 +119,13 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
   uint32_t* mk = reinterpret_cast<uint32_t*>(smem_raw + xbytes);
   const int KLF = f.Kpad;                                            // entries of one layer's input list
   uint16_t* klist = reinterpret_cast<uint16_t*>(mk + AM_GT * f.Wd);  // fp32: [D][KLF]
-  // Error-estimate product input, element (k, t) for region t < AM_GT, bf16;
-  // after the product has read it, the same slot holds the output V (k, t),
-  // bf16.  FP32: in the 16-byte padding of row k of the X / Y buffer (columns
-  // AM_GN .., unused by the planes); split-BF16 (X there is in bf16 parts
-  // over the whole buffer): a [16][KS] array of its own.
-  const size_t qoff = (xbytes + (size_t)AM_GT * f.Wd * 4 + (PREC == 0 ? (size_t)f.D * KLF * 2 : 0) + 15) & ~(size_t)15;
-  uint16_t* Xq = reinterpret_cast<uint16_t*>(smem_raw + qoff);
+  // Error estimates (FP32 form only): the product's input, element (k, t)
+  // for region t < AM_GT, bf16, in the 16-byte padding of row k of the X / Y
+  // buffer (columns AM_GN .., unused by the planes); after the product has
+  // read it, the same slot holds the output V (k, t), bf16.
+  constexpr bool SIG = PREC == 0;
   uint16_t* Ypad = reinterpret_cast<uint16_t*>(Ys) + 2 * AM_GN;
-  auto xq_at = [&](int k, int t) -> uint16_t& { return PREC == 0 ? Ypad[k * 2 * AM_XS + t] : Xq[t * KS + k]; };
+  auto xq_at = [&](int k, int t) -> uint16_t& { return Ypad[k * 2 * AM_XS + t]; };
   __shared__ int s_nk[AM_MAX_LINEAR];
   const int r0 = blockIdx.x * AM_GT;
   const int nreg = min(AM_GT, io.count - r0);
@@ -141,10 +139,8 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
   } else {
     for (int i = tid; i < 2 * AM_GN * KS; i += 256) Xh[i] = 0;
   }
-  if (f.sig && PREC != 0)
-    for (int i = tid; i < 16 * KS; i += 256) Xq[i] = 0;
   __syncthreads();
-  if (f.sig) {
+  if (SIG && f.sig) {
     // hidden layer 1's planes are exact (V = 0)
     const int n1 = f.nodes[1];
     for (int idx = tid; idx < n1 * AM_GT; idx += 256) {
@@ -186,8 +182,8 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
     const int nout = f.nodes[li + 1];
     const int nrb = f.nrb[li], nkb = f.nkb[li];
     const float* bias = f.bias[li];
-    if (f.sig) {
-      // the error-estimate product of this wave's row blocks, V = W^2 . Xq;
+    if (SIG && f.sig) {
+      // the error-estimate product of this wave's row blocks, V = W^2 . (input);
       // lane (column t, rows 4 (lane >> 4) + i of block rb) holds region t's V
       // of plane rb * 16 + 4 (lane >> 4) + i
       const int nuq = (nrb - wave + 3) / 4;
@@ -197,19 +193,13 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
       if (nuq > 0) {
         const int nkq = f.nkq[li];
         const int tq = lane & 15, kq = 8 * (lane >> 4);
-        const uint16_t* xq0 = Xq + tq * KS + kq;
         const uint4* Aq = reinterpret_cast<const uint4*>(f.Wsq[li]) + lane;
         for (int kb = 0; kb < nkq; ++kb) {
-          bf16x8 vb;
-          if (PREC == 0) {
-            // column tq, rows kb * 32 + kq .. + 7 from the row paddings (+0 for columns past AM_GT)
-            uint16_t h[8];
+          // column tq, rows kb * 32 + kq .. + 7 from the row paddings (+0 for columns past AM_GT)
+          uint16_t h[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) h[e] = tq < AM_GT ? Ypad[(kb * 32 + kq + e) * 2 * AM_XS + tq] : (uint16_t)0;
-            vb = __builtin_bit_cast(bf16x8, h);
-          } else {
-            vb = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(xq0 + kb * 32));
-          }
+          for (int e = 0; e < 8; ++e) h[e] = tq < AM_GT ? Ypad[(kb * 32 + kq + e) * 2 * AM_XS + tq] : (uint16_t)0;
+          const bf16x8 vb = __builtin_bit_cast(bf16x8, h);
 #pragma unroll
           for (int u = 0; u < AM_GRBW; ++u)
             if (u < nuq) {
@@ -218,7 +208,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
             }
         }
       }
-      __syncthreads();   // every wave has read Xq
+      __syncthreads();   // every wave has read the product's input
       const int t = lane & 15;
 #pragma unroll
       for (int u = 0; u < AM_GRBW; ++u) {
@@ -342,12 +332,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
           dst[3 * (size_t)f.Cs] = on ? -y.w : y.w;
         }
       }
-      if (!f.sig) {
-        for (int idx = tid; idx < nreg * nout; idx += 256) {
-          const int t = idx / nout, j = idx - t * nout;
-          io.planes[((size_t)(r0 + t) * AM_PCOMP + 4) * f.Cs + out0 + j] = 0.f;
-        }
-      } else {
+      if (SIG && f.sig) {
         // estimates of this layer's planes to HBM (consecutive threads,
         // consecutive planes) and the next product's input: values first,
         // then a barrier (V and the next input share their slots), then the writes
@@ -443,7 +428,8 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 2) void k_gemm(FastNetDev f, G
         }
       }
       // the face plane's estimate
-      if (tid < nreg) io.eq[(size_t)(r0 + tid) * AM_EQS + 4] = f.sig ? 1.02f * __builtin_sqrtf(bf16_to_f(xq_at(0, tid))) : 0.f;
+      if (SIG && f.sig && tid < nreg)
+        io.eq[(size_t)(r0 + tid) * AM_EQS + 4] = 1.02f * __builtin_sqrtf(bf16_to_f(xq_at(0, tid)));
     }
     __syncthreads();
   }
@@ -469,12 +455,13 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
   if (b >= io.count) return;
   const int rid = chunk_first + b;
   const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
+  const bool with_sigma = wp.lam > 0.f;
   for (int g = lane; g < C; g += 64) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * Cs_hbm + g];
     Pc[g] = src[3 * Cs_hbm + g];
-    Pe[g] = src[4 * Cs_hbm + g];
+    if (with_sigma) Pe[g] = src[4 * Cs_hbm + g];
   }
   for (int w = lane; w < net.Wd; w += 64) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   for (int w = lane; w < ((C + 31) >> 5); w += 64) used[w] = 0u;
@@ -484,7 +471,7 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
   walk_region<float, true>(net, io, wp, rid, rid - io.first, Px, Py, Pz, Pc, keys, order, used, smask, eq, p,
-                           wp.lam > 0.f ? Pe : nullptr, eb[4]);
+                           with_sigma ? Pe : nullptr, with_sigma ? eb[4] : 0.f);
 }
 
 // Ray-shooting FP32 walk: LDS holds only the planes and the mask.
@@ -504,12 +491,13 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
   if (b >= io.count) return;
   const int rid = chunk_first + b;
   const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
+  const bool with_sigma = wp.lam > 0.f;
   for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * Cs_hbm + g];
     Pc[g] = src[3 * Cs_hbm + g];
-    Pe[g] = src[4 * Cs_hbm + g];
+    if (with_sigma) Pe[g] = src[4 * Cs_hbm + g];
   }
   for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const float* eb = eqbuf + (size_t)b * AM_EQS;
@@ -517,8 +505,8 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
-  const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask, wp.lam > 0.f ? Pe : nullptr};
-  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p, eb[4]);
+  const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask, with_sigma ? Pe : nullptr};
+  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p, with_sigma ? eb[4] : 0.f);
 }
 
 // Regions the walk left undecided, added to cnt[0], their reasons, added to
@@ -600,14 +588,12 @@ float host_bf16_f(uint16_t h) {
 }
 
 size_t gemm_lds(const FastNetDev& d, int prec) {
-  // one buffer for a layer's input and its outputs (k_gemm), the masks, the fp32 input lists,
-  // the error-estimate product's input (16 columns of bf16)
+  // one buffer for a layer's input and its outputs (k_gemm; FP32: the error
+  // estimates in its row paddings), the masks, the fp32 input lists
   const size_t ys = (size_t)d.Kpad * AM_XS * 4;
   const size_t xs = prec == 0 ? ys : std::max(ys, (size_t)2 * AM_GN * (d.Kpad + 8) * 2);
   const size_t kls = prec == 0 ? (size_t)d.D * d.Kpad * 2 : 0;
-  if (prec == 0) return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;   // (estimates in the row paddings)
-  const size_t q = (xs + (size_t)AM_GT * d.Wd * 4 + kls + 15) & ~(size_t)15;
-  return q + (size_t)16 * (d.Kpad + 8) * 2 + 16;
+  return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
 }
 
 // bf16 MFMA A operand [nrb][nkb][64 lanes][8] of an (nout x nin) matrix given

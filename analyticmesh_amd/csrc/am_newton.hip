@@ -91,7 +91,22 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
 #pragma unroll
       for (int jj = 0; jj < AM_NROW; ++jj) acc[jj] = bp[jj];
       const double* wr = cio.Wp[li] + row0;
-      for (int k = 0; k < nin; ++k) {
+      // four k steps per iteration, so that the weight rows of the next
+      // steps load while this step's fmas run
+      const int nk4 = nin & ~3;
+      int k = 0;
+      for (; k < nk4; k += 4) {
+        double x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = X[(size_t)(k + u) * AM_NT + lane];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const double* w = wr + (size_t)(k + u) * ld;
+#pragma unroll
+          for (int jj = 0; jj < AM_NROW; ++jj) acc[jj] = __builtin_fma(w[jj], x[u], acc[jj]);
+        }
+      }
+      for (; k < nin; ++k) {
         const double x = X[(size_t)k * AM_NT + lane];
         const double* w = wr + (size_t)k * ld;
 #pragma unroll

@@ -322,7 +322,9 @@ def main():
         kernels.append(dict(kernel=GEMM_KERNEL[head], ms=float(st[3]), regions=h["regions"], peak=PEAK_TFLOPS[head],
                             launches=max(1.0, float(st[11]))))
         if float(st[13]) > 0:
-            kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=h["regions"],
+            # the FP64 chain runs on the regions the reduced walk left to FP64 (undecided, or
+            # decided with vertices re-solved from FP64 planes)
+            kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=int(st[9]) + int(st[33]),
                                 peak=PEAK_TFLOPS["fp64"], launches=max(1.0, float(st[31]))))
     else:
         kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=h["regions"],
