@@ -15,15 +15,17 @@
 // FP64-solve accuracy without FP64 planes (backend/inc/kernel.h:555-573 solves
 // the same system from FP64 planes).
 //
-// One workgroup of 16 waves takes 64 vertices: lane l is vertex l, wave w owns
-// output rows 16 w .. 16 w + 15 of every layer.  A layer's input (one column
-// per vertex, masked) lives in LDS; each weight row segment is a wave-uniform
-// scalar load, so every fma takes one LDS read and one register operand.
+// One workgroup of 16 waves takes 64 vertices (one column each), wave w owns
+// output rows 16 w .. 16 w + 15 of every layer.  A layer's input (masked) lives
+// in LDS; the products run on the FP64 matrix cores, the A fragments (weights)
+// streamed from memory a few steps ahead.
 #include "am_device.h"
 #include "am_kernels.h"
 #include "am_newton.h"
 
 namespace am {
+
+typedef double am_f64x4 __attribute__((ext_vector_type(4)));
 
 __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, ChainIO cio, NewtonIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -80,59 +82,76 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
     }
   }
   __syncthreads();
-  double f = 0.0;
+  // Layers 2 .. D + 1 on the FP64 matrix cores: wave w computes rows
+  // row0 .. row0 + 15 for all 64 vertices as four 16 x 16 blocks of
+  // v_mfma_f64_16x16x4f64 per 4 inputs (A: lane l holds W[row0 + l % 16][k0 + l / 16];
+  // B: X[k0 + l / 16][16 cb + l % 16]; result element v of lane l is row l / 16 + 4 v,
+  // column l % 16 of the block).  The A fragments of the next steps are in flight
+  // while a step's products run.
+  __shared__ double s_f[AM_NT];
+  const int l16 = lane & 15, l4 = lane >> 4;
   for (int li = 1; li <= net.D; ++li) {
     const int nin = net.nodes[li], nout = net.nodes[li + 1], ld = cio.noutp[li];
     const int row0 = wave * AM_NROW;
     const bool out_layer = li == net.D;
-    double acc[AM_NROW];
+    am_f64x4 acc[AM_NT / 16];
+#pragma unroll
+    for (int cb = 0; cb < AM_NT / 16; ++cb) acc[cb] = am_f64x4{0.0, 0.0, 0.0, 0.0};
     if (row0 < nout) {
-      const double* bp = cio.bp[li] + row0;
+      // W^T is zero past row nin (and up to AM_CHAIN_MAXW), so k may run to a multiple of 4
+      const double* wa = cio.Wp[li] + (size_t)l4 * ld + row0 + l16;
+      const int nk = (nin + 3) & ~3;
+      constexpr int PF = 4;   // A fragments in flight
+      double a[PF];
 #pragma unroll
-      for (int jj = 0; jj < AM_NROW; ++jj) acc[jj] = bp[jj];
-      const double* wr = cio.Wp[li] + row0;
-      // four k steps per iteration, so that the weight rows of the next
-      // steps load while this step's fmas run
-      const int nk4 = nin & ~3;
-      int k = 0;
-      for (; k < nk4; k += 4) {
-        double x[4];
+      for (int u = 0; u < PF; ++u) a[u] = 4 * u < nk ? wa[(size_t)(4 * u) * ld] : 0.0;
+      for (int k0 = 0; k0 < nk; k0 += 4 * PF) {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = X[(size_t)(k + u) * AM_NT + lane];
+        for (int u = 0; u < PF; ++u) {
+          const int kk = k0 + 4 * u;
+          if (kk < nk) {
+            const int kr = kk + l4;
+            double b[AM_NT / 16];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const double* w = wr + (size_t)(k + u) * ld;
+            for (int cb = 0; cb < AM_NT / 16; ++cb) b[cb] = kr < nin ? X[(size_t)kr * AM_NT + 16 * cb + l16] : 0.0;
+            const double av = a[u];
+            const int kn = kk + 4 * PF;
+            a[u] = kn < nk ? wa[(size_t)kn * ld] : 0.0;
 #pragma unroll
-          for (int jj = 0; jj < AM_NROW; ++jj) acc[jj] = __builtin_fma(w[jj], x[u], acc[jj]);
+            for (int cb = 0; cb < AM_NT / 16; ++cb)
+              acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[cb], acc[cb], 0, 0, 0);
+          }
         }
-      }
-      for (; k < nin; ++k) {
-        const double x = X[(size_t)k * AM_NT + lane];
-        const double* w = wr + (size_t)k * ld;
-#pragma unroll
-        for (int jj = 0; jj < AM_NROW; ++jj) acc[jj] = __builtin_fma(w[jj], x, acc[jj]);
       }
     }
     __syncthreads();   // every wave has read this layer's input
     if (row0 < nout) {
-      if (out_layer) {
-        if (wave == 0) f = acc[0];
-      } else {
-        const int out0 = net.off[li];
+      const int out0 = out_layer ? 0 : net.off[li];
 #pragma unroll
-        for (int jj = 0; jj < AM_NROW; ++jj) {
-          const int j = row0 + jj;
+      for (int cb = 0; cb < AM_NT / 16; ++cb) {
+        const int c = 16 * cb + l16;
+        const int ac = s_ab[c][0], bc = s_ab[c][1];
+        const uint32_t* mc = smask + (size_t)c * Wd;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int j = row0 + l4 + 4 * v;
           if (j < nout) {
-            const int g = out0 + j;
-            if (g == ca) s_z[lane][0] = acc[jj];
-            if (g == cb) s_z[lane][1] = acc[jj];
-            X[(size_t)j * AM_NT + lane] = mbit(mk, g) ? acc[jj] : 0.0;
+            const double z = acc[cb][v] + cio.bp[li][j];
+            if (out_layer) {
+              s_f[c] = z;   // nout == 1: row 0
+            } else {
+              const int g = out0 + j;
+              if (g == ac) s_z[c][0] = z;
+              if (g == bc) s_z[c][1] = z;
+              X[(size_t)j * AM_NT + c] = mbit(mc, g) ? z : 0.0;
+            }
           }
         }
       }
     }
     __syncthreads();
   }
+  const double f = s_f[lane];
   if (wave != 0 || lane >= n) return;
   // residuals of the signed planes of the vertex's system and the step
   const int r = s_rk[lane][0], k = s_rk[lane][1];
