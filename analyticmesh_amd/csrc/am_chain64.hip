@@ -99,6 +99,16 @@ __device__ __forceinline__ void load_planes64(const NetDev& net, const uint32_t*
 // The planes of one tile: regions r0 .. r0 + nreg - 1 of lv (ids[r0 + t] or
 // first + r0 + t), written at planes slots r0 + t.  io supplies the region
 // store, the plane stride and the layer operands.
+// MF: the layer products on the FP64 matrix cores (four 16 x 16 blocks of
+// v_mfma_f64_16x16x4f64 per wave and 4 list entries; A: lane l holds
+// W[row0 + l % 16][list entry i0 + l / 16], B: X[i0 + l / 16][16 cb + l % 16];
+// result element v of lane l is row row0 + l / 16 + 4 v, column 16 cb + l % 16),
+// the bias added at the end: the planes equal the sequential-fma ones up to the
+// rounding of a different summation order, so the FP64 form (records bit for
+// bit) keeps MF = false.
+typedef double am_f64x4 __attribute__((ext_vector_type(4)));
+
+template <bool MF>
 __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO& io, const ChainLevel& lv, int r0,
                                                 int nreg) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -125,7 +135,7 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
   // is skipped when it is inactive in all AM_CT regions (its X row is +0 in
   // every column, and fma(w, +0, acc) = acc).  Wave li-1 lists layer li's in
   // ascending order and pads the list with the +0 row AM_CHAIN_MAXW up to a
-  // multiple of AM_CPF, plus AM_CPF more entries for the operand pipeline.
+  // multiple of AM_CPF, plus 2 AM_CPF more entries for the operand pipelines.
   if (wave < net.D) {
     const int li = wave + 1;
     const int nin = net.nodes[li], in0 = li == 1 ? 0 : net.off[li - 1];
@@ -141,7 +151,7 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
       base += __popcll(bal);
     }
     const int nkp = (base + AM_CPF - 1) / AM_CPF * AM_CPF;
-    for (int i = base + lane; i < nkp + AM_CPF; i += 64) kl[i] = (uint16_t)AM_CHAIN_MAXW;
+    for (int i = base + lane; i < nkp + 2 * AM_CPF; i += 64) kl[i] = (uint16_t)AM_CHAIN_MAXW;
     if (lane == 0) s_nk[li] = nkp;
   }
   __syncthreads();
@@ -171,6 +181,7 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
     const bool narrow = nout < AM_CR;
     const bool active = !narrow && row0 < nout;   // wave-uniform
     double acc[AM_CR];
+    am_f64x4 acc4[AM_CN / 16];
     double acc_n = 0.0;
     if (narrow && wave < nout) {
       const uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
@@ -194,7 +205,34 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
       }
       acc_n = a;
     }
-    if (active) {
+    if (MF && active) {
+      const uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
+      const int nkp = s_nk[li];   // a multiple of AM_CPF; AM_CPF more entries name the +0 row
+      const int l16 = lane & 15, l4 = lane >> 4;
+      const double* wcol = io.Wp[li] + row0 + l16;
+#pragma unroll
+      for (int cb = 0; cb < AM_CN / 16; ++cb) acc4[cb] = am_f64x4{0.0, 0.0, 0.0, 0.0};
+      // A fragments in flight: one list group ahead (two measured slower)
+      constexpr int PF = AM_CPF / 4;
+      double a[PF];
+#pragma unroll
+      for (int u = 0; u < PF; ++u) a[u] = wcol[(size_t)kl[4 * u + l4] * noutp];
+      for (int i0 = 0; i0 < nkp; i0 += AM_CPF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int ii = i0 + 4 * u;
+          double b[AM_CN / 16];
+#pragma unroll
+          for (int cb = 0; cb < AM_CN / 16; ++cb) b[cb] = Xs[(size_t)(ii + l4) * AM_CXS + 16 * cb + l16];
+          const double av = a[u];
+          a[u] = wcol[(size_t)kl[ii + AM_CPF + l4] * noutp];
+#pragma unroll
+          for (int cb = 0; cb < AM_CN / 16; ++cb)
+            acc4[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[cb], acc4[cb], 0, 0, 0);
+        }
+      }
+    }
+    if (!MF && active) {
       const double* bias = io.bp[li] + row0;
 #pragma unroll
       for (int j = 0; j < AM_CR; ++j) acc[j] = q_l == 3 ? bias[j] + 0.0 : 0.0;
@@ -248,7 +286,20 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
     }
     __syncthreads();                  // every wave is done reading this layer's input
     // park the layer's outputs in LDS (unsigned, unmasked), row = output index
-    if (active) {
+    if (MF && active) {
+      const int l16 = lane & 15, l4 = lane >> 4;
+      const bool cbias = (l16 & 3) == 3;   // columns 16 cb + l16: component c
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + l4 + 4 * v;
+        if (row < nout) {
+          const double b = cbias ? io.bp[li][row] : 0.0;
+#pragma unroll
+          for (int cb = 0; cb < AM_CN / 16; ++cb) Xs[row * AM_CXS + 16 * cb + l16] = acc4[cb][v] + b;
+        }
+      }
+    }
+    if (!MF && active) {
 #pragma unroll
       for (int j = 0; j < AM_CR; ++j) {
         const int row = row0 + j;
@@ -318,16 +369,27 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   const int cnt = launch_count(io);
   if (r0 >= cnt) return;
   const ChainLevel lv{io.first, cnt, io.ids, io.planes, io.eq};
-  chain_tile_body(net, io, lv, r0, min(AM_CT, cnt - r0));
+  chain_tile_body<false>(net, io, lv, r0, min(AM_CT, cnt - r0));
+}
+
+__global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_mfma(NetDev net, ChainIO io) {
+  const int r0 = blockIdx.x * AM_CT;
+  const int cnt = launch_count(io);
+  if (r0 >= cnt) return;
+  const ChainLevel lv{io.first, cnt, io.ids, io.planes, io.eq};
+  chain_tile_body<true>(net, io, lv, r0, min(AM_CT, cnt - r0));
 }
 
 // Many networks in one launch: block b takes tiles[b] = (network, r0, regions);
 // the tile's regions are lv.first + r0 + t, all of one network.
+template <bool MF>
 __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64_multi(const NetDev* nets, const ChainIO* cios,
                                                                            ChainLevel lv, const int4* tiles) {
   const int4 tl = tiles[blockIdx.x];
-  chain_tile_body(nets[tl.x], cios[tl.x], lv, tl.y, tl.z);
+  chain_tile_body<MF>(nets[tl.x], cios[tl.x], lv, tl.y, tl.z);
 }
+template __global__ void k_chain_tile_fp64_multi<false>(const NetDev*, const ChainIO*, ChainLevel, const int4*);
+template __global__ void k_chain_tile_fp64_multi<true>(const NetDev*, const ChainIO*, ChainLevel, const int4*);
 
 // The candidate scan of one region (rid) on the planes at slot sl.
 __device__ __forceinline__ void walk_fp64_body(const NetDev& net, const RegionIO& io, int rid, int sl,

@@ -175,6 +175,7 @@ struct am_ctx {
 
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
+  bool chain_mfma = true;     // the certified forms' FP64 chain on the matrix cores ("chain_mfma")
   DevBuf<double> tplanes, teq;
   // am_adjoint's buffers, kept from call to call: dL/dv, dW, db, G and P per layer
   DevBuf<double> adj_gv, adj_dv;
@@ -457,6 +458,10 @@ struct Fp64Counts {
 // Checked FP64 ray walk: planes are exact FP64, so eps only covers the walk's
 // own rounding (the vertex solve's share is added per system).
 const WalkParams kRay64{1e-13f, 1e-12f, 1e6f};
+// The same walk on planes from the matrix-core chain (another summation
+// order: up to a few 1e-14 of a plane's largest coefficient from the
+// sequential-fma planes on the 4x256 net); the bound keeps a margin of 10x.
+const WalkParams kRay64Mf{1e-12f, 1e-11f, 1e6f};
 
 // planes past hidden layer 1 (the walk rebuilds layer 1's) and the mask
 size_t ray64_lds_bytes(const am_ctx* c) {
@@ -496,7 +501,8 @@ ChainIO chain_io(const am_ctx* c, const int* ids, int first, int n, int Ch) {
 }
 
 // The FP64 ray walk with the context's waves per region.
-void launch_ray64(am_ctx* c, int n, size_t lds, const RegionIO& io, int rid0, int Ch, int only_undecided) {
+void launch_ray64(am_ctx* c, int n, size_t lds, const RegionIO& io, int rid0, int Ch, int only_undecided,
+                  const WalkParams& kRay64) {
   const double* P = c->tplanes.p;
   const double* E = c->teq.p;
   switch (c->ray_waves) {
@@ -597,8 +603,11 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     io.dcount = dcount;
     io.dbase = off;
     const size_t m0 = tm.mark(c->stream);
-    hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
-                       c->net, cio);
+    // the records of the candidate scan on every region (mode 0) are the
+    // oracle's bit for bit: its planes come from the sequential-fma chain
+    const bool mf = c->chain_mfma && (mode != 0 || after_walk);
+    hipLaunchKernelGGL((mf ? k_chain_tile_mfma : k_chain_tile_fp64), dim3((n + AM_CT - 1) / AM_CT),
+                       dim3(AM_CHAIN_THREADS), clds, c->stream, c->net, cio);
     HIPCHK(hipGetLastError());
     t.chain_launches += 1;
     const size_t m1 = tm.mark(c->stream);
@@ -621,7 +630,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
       continue;
     }
     if (mode == 1) {
-      launch_ray64(c, n, rlds, io, first + off, Ch, after_walk ? 1 : 0);
+      launch_ray64(c, n, rlds, io, first + off, Ch, after_walk ? 1 : 0, mf ? kRay64Mf : kRay64);
       HIPCHK(hipGetLastError());
       const size_t m2 = tm.mark(c->stream);
       tm.span(m1v, m2, T_RAY64);
@@ -841,6 +850,9 @@ void batch_level_launch(am_batch* b, const BatchLevelPlan& pl, Fp64Counts& t) {
   const int Ch = (b->Cmax + 63) & ~63;
   const size_t clds = batch_chain_lds(b), slds = batch_scan_lds(b), rlds = batch_ray_lds(b);
   const bool ray = b->precision == AM_PREC_FP64_RAY && rlds <= c->max_lds;
+  // the ray walk certifies its decisions: its planes may come from the matrix cores
+  const bool mf = ray && c->chain_mfma;
+  const WalkParams& kr = mf ? kRay64Mf : kRay64;
   StageTimer& tm = c->timer;
   size_t k = 0;
   for (long long cs = 0; cs < count; cs += (long long)chunk, ++k) {
@@ -849,8 +861,8 @@ void batch_level_launch(am_batch* b, const BatchLevelPlan& pl, Fp64Counts& t) {
     const ChainLevel lv{rid0, n, nullptr, c->tplanes.p, c->teq.p};
     const int ntile = (int)(chunk_tile0[k + 1] - chunk_tile0[k]);
     const size_t m0 = tm.mark(st);
-    hipLaunchKernelGGL(k_chain_tile_fp64_multi, dim3(ntile), dim3(AM_CHAIN_THREADS), clds, st,
-                       (const NetDev*)b->nets_d.p, (const ChainIO*)b->cios_d.p, lv,
+    hipLaunchKernelGGL((mf ? k_chain_tile_fp64_multi<true> : k_chain_tile_fp64_multi<false>), dim3(ntile),
+                       dim3(AM_CHAIN_THREADS), clds, st, (const NetDev*)b->nets_d.p, (const ChainIO*)b->cios_d.p, lv,
                        (const int4*)(b->tiles_d.p + chunk_tile0[k]));
     HIPCHK(hipGetLastError());
     t.chain_launches += 1;
@@ -889,13 +901,13 @@ void batch_level_launch(am_batch* b, const BatchLevelPlan& pl, Fp64Counts& t) {
       const double* E = c->teq.p;
       switch (c->ray_waves) {
         case 2:
-          hipLaunchKernelGGL(k_walk_ray64_multi<2>, dim3(n), dim3(128), rlds, st, nd, io, c->Wd, rid0, kRay64, P, E, Ch);
+          hipLaunchKernelGGL(k_walk_ray64_multi<2>, dim3(n), dim3(128), rlds, st, nd, io, c->Wd, rid0, kr, P, E, Ch);
           break;
         case 8:
-          hipLaunchKernelGGL(k_walk_ray64_multi<8>, dim3(n), dim3(512), rlds, st, nd, io, c->Wd, rid0, kRay64, P, E, Ch);
+          hipLaunchKernelGGL(k_walk_ray64_multi<8>, dim3(n), dim3(512), rlds, st, nd, io, c->Wd, rid0, kr, P, E, Ch);
           break;
         default:
-          hipLaunchKernelGGL(k_walk_ray64_multi<4>, dim3(n), dim3(256), rlds, st, nd, io, c->Wd, rid0, kRay64, P, E, Ch);
+          hipLaunchKernelGGL(k_walk_ray64_multi<4>, dim3(n), dim3(256), rlds, st, nd, io, c->Wd, rid0, kr, P, E, Ch);
       }
     }
     HIPCHK(hipGetLastError());
@@ -1052,6 +1064,9 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
+  } else if (k == "chain_mfma") {
+    if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "chain_mfma must be 0 or 1");
+    c->chain_mfma = value != 0;
   } else if (k == "reduced_vertices") {
     if (value < 0 || value > 2)
       return fail(c, AM_ERR_ARG, "reduced_vertices must be 0 (FP64 re-solve), 1 (estimate) or 2 (Newton)");
@@ -1192,9 +1207,12 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     if (c->tile_ok) {
       const size_t clds = chain_lds_bytes(c);
       if (clds > c->max_lds) c->tile_ok = false;
-      else if (clds > 65536)
+      else if (clds > 65536) {
+        HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)clds));
         HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)clds));
+      }
     }
     // The reduced forms need the tiled FP64 chain for their vertices and
     // re-decisions; nets it does not take (skips, layers over 256 wide) run
@@ -2140,9 +2158,12 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
     if ((size_t)nn * sizeof(int) > ((size_t)1 << 22)) throw CapErr{"too many networks"};
     const size_t clds = batch_chain_lds(b), slds = batch_scan_lds(b), rlds = batch_ray_lds(b);
     if (clds > c->max_lds || slds > c->max_lds) throw CapErr{"networks too wide for the batch kernels' LDS"};
-    if (clds > 65536)
-      HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)clds));
+    if (clds > 65536) {
+      HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64_multi<false>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
+      HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64_multi<true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)clds));
+    }
     if (slds > 65536)
       HIPCHK(hipFuncSetAttribute((const void*)k_walk_fp64_multi, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)slds));

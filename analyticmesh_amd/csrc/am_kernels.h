@@ -90,8 +90,13 @@ struct ChainLevel {
 };
 
 __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
+// The same planes with the products on the FP64 matrix cores: equal up to the
+// rounding of another summation order (the forms that certify their decisions).
+__global__ void k_chain_tile_mfma(NetDev net, ChainIO io);
 // A batch of networks (each region's network is the last word of its mask):
 // tiles[b] = (network, first slot, regions, -) of one network.
+// MF: the products on the matrix cores (k_chain_tile_mfma).
+template <bool MF>
 __global__ void k_chain_tile_fp64_multi(const NetDev* nets, const ChainIO* cios, ChainLevel lv, const int4* tiles);
 // The walk of k_region_fp64 on planes from k_chain_tile_fp64.  Block b takes
 // planes slot s = slots[b] (or b); its region is io.ids[s] when ids are given,

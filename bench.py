@@ -26,7 +26,10 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2516.6 / 3.0, "fp64": 78.6, "fp64ray": 78.6}
-GEMM_KERNEL = {"fp32": "k_gemm<0>", "bf16x3": "k_gemm<1>", "fp64ray": "k_chain_tile_fp64", "fp64": "k_chain_tile_fp64"}
+GEMM_KERNEL = {"fp32": "k_gemm<0>", "bf16x3": "k_gemm<1>", "fp64ray": "k_chain_tile_mfma", "fp64": "k_chain_tile_fp64"}
+# the FP64 chain of the certified forms runs on the matrix cores; the bit-exact FP64 form's on VALU
+CHAIN_KERNEL = {"fp32": "k_chain_tile_mfma", "bf16x3": "k_chain_tile_mfma", "fp64ray": "k_chain_tile_mfma",
+                "fp64": "k_chain_tile_fp64"}
 
 
 def log(*a):
@@ -324,15 +327,15 @@ def main():
         if float(st[13]) > 0:
             # the FP64 chain runs on the regions the reduced walk left to FP64 (undecided, or
             # decided with vertices re-solved from FP64 planes)
-            kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=int(st[9]) + int(st[33]),
+            kernels.append(dict(kernel=CHAIN_KERNEL[head], ms=float(st[13]), regions=int(st[9]) + int(st[33]),
                                 peak=PEAK_TFLOPS["fp64"], launches=max(1.0, float(st[31]))))
     else:
-        kernels.append(dict(kernel="k_chain_tile_fp64", ms=float(st[13]), regions=h["regions"],
+        kernels.append(dict(kernel=CHAIN_KERNEL[head.split("+")[0]], ms=float(st[13]), regions=h["regions"],
                             peak=PEAK_TFLOPS["fp64"], launches=max(1.0, float(st[31]))))
     lines = []
     for k in kernels:
         ach = flops_region * k["regions"] / (k["ms"] / 1e3) / 1e12 if k["ms"] > 0 else 0.0
-        lines.append({"bound": "mfma" if k["kernel"].startswith("k_gemm") else "fp64 valu", "kernel": k["kernel"],
+        lines.append({"bound": "fp64 valu" if k["kernel"] == "k_chain_tile_fp64" else "mfma", "kernel": k["kernel"],
                       "achieved": round(ach, 3), "peak": k["peak"], "unit": "TFLOP/s",
                       "frac": round(ach / k["peak"], 5),
                       "traffic": traffic_per_launch(k["kernel"], k["regions"], k["launches"]),
