@@ -364,6 +364,170 @@ __device__ __forceinline__ void chain_tile_body(const NetDev& net, const ChainIO
   }
 }
 
+// The matrix-core chain on half tiles: 8 regions (32 columns) per workgroup of
+// 8 waves, each wave two 16-row blocks (rows 32 w .. 32 w + 31) by two 16-column
+// blocks, so one A fragment feeds two products and one B fragment two.  Its
+// LDS (about 75 KB) lets two workgroups share a CU: one runs its products while
+// the other parks, stores and compacts a layer.  Planes as chain_tile_body.
+__device__ __forceinline__ void chain_tile_half(const NetDev& net, const ChainIO& io, const ChainLevel& lv, int r0,
+                                                int nreg) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  double* Xs = reinterpret_cast<double*>(smem_raw);                                   // [AM_CXR][AM_HXS]
+  uint16_t* klist = reinterpret_cast<uint16_t*>(Xs + (size_t)AM_CXR * AM_HXS + 1);     // [D][AM_KLS]
+  uint32_t* mk = reinterpret_cast<uint32_t*>(klist + (size_t)net.D * AM_KLS);          // [AM_HT][Wd]
+  __shared__ int s_nk[AM_MAX_LINEAR];
+  for (int idx = tid; idx < AM_HT * net.Wd; idx += AM_HALF_THREADS) {
+    const int t = idx / net.Wd, w = idx - t * net.Wd;
+    uint32_t v = 0u;
+    if (t < nreg) {
+      const int rid = lv.ids ? lv.ids[r0 + t] : lv.first + r0 + t;
+      v = io.reg_mask[(size_t)rid * net.Wd + w];
+    }
+    mk[idx] = v;
+  }
+  __syncthreads();
+  // active-input lists as in chain_tile_body (waves take layers in turn)
+  for (int li = wave + 1; li <= net.D; li += AM_HALF_THREADS / 64) {
+    const int nin = net.nodes[li], in0 = li == 1 ? 0 : net.off[li - 1];
+    uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
+    int base = 0;
+    for (int kb = 0; kb < nin; kb += 64) {
+      const int k = kb + lane;
+      bool act = false;
+      if (k < nin)
+        for (int t = 0; t < AM_HT; ++t) act = act || mbit(mk + t * net.Wd, in0 + k);
+      const unsigned long long bal = __ballot(act);
+      if (act) kl[base + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = (uint16_t)k;
+      base += __popcll(bal);
+    }
+    const int nkp = (base + AM_CPF - 1) / AM_CPF * AM_CPF;
+    for (int i = base + lane; i < nkp + AM_CPF; i += 64) kl[i] = (uint16_t)AM_CHAIN_MAXW;
+    if (lane == 0) s_nk[li] = nkp;
+  }
+  __syncthreads();
+  const size_t Cs = (size_t)io.Cs;
+  const int n1 = net.nodes[1];
+  for (int idx = tid; idx < s_nk[1] * AM_HN; idx += AM_HALF_THREADS) {
+    const int i = idx / AM_HN, col = idx - i * AM_HN, t = col >> 2, q = col & 3;
+    const int k = klist[i];
+    double x = 0.0;
+    if (k < n1 && mbit(mk + t * net.Wd, k)) x = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
+    Xs[i * AM_HXS + col] = x;
+  }
+  for (int idx = tid; idx < AM_HN * net.E; idx += AM_HALF_THREADS) {
+    const int col = idx / net.E, e = idx - col * net.E, t = col >> 2, q = col & 3;
+    if (t < nreg) lv.planes[((size_t)(r0 + t) * 4 + q) * Cs + net.L + e] = q < 3 ? net.ew[e * 3 + q] : net.eb[e];
+  }
+  __syncthreads();
+  const int l16 = lane & 15, l4 = lane >> 4;
+  for (int li = 1; li <= net.D; ++li) {
+    const int nout = net.nodes[li + 1], noutp = io.noutp[li];
+    const int row0 = wave * 32;
+    // row blocks of this wave with rows (W^T holds noutp >= nout columns, zero past nout)
+    const int nrb = row0 >= nout ? 0 : (row0 + 16 >= noutp ? 1 : 2);
+    am_f64x4 acc[2][2];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = am_f64x4{0.0, 0.0, 0.0, 0.0};
+    if (nrb > 0) {
+      const uint16_t* kl = klist + (size_t)(li - 1) * AM_KLS;
+      const int nkp = s_nk[li];
+      const double* wcol = io.Wp[li] + row0 + l16;
+      const int rb1 = nrb > 1 ? 16 : 0;   // the second block's column offset (re-reads the first when absent)
+      constexpr int PF = AM_CPF / 4;
+      double a0[PF], a1[PF];
+#pragma unroll
+      for (int u = 0; u < PF; ++u) {
+        const double* w = wcol + (size_t)kl[4 * u + l4] * noutp;
+        a0[u] = w[0];
+        a1[u] = w[rb1];
+      }
+      for (int i0 = 0; i0 < nkp; i0 += AM_CPF) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+          const int ii = i0 + 4 * u;
+          const double b0 = Xs[(size_t)(ii + l4) * AM_HXS + l16];
+          const double b1 = Xs[(size_t)(ii + l4) * AM_HXS + 16 + l16];
+          const double x0 = a0[u], x1 = a1[u];
+          const double* w = wcol + (size_t)kl[ii + AM_CPF + l4] * noutp;
+          a0[u] = w[0];
+          a1[u] = w[rb1];
+          acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, b0, acc[0][0], 0, 0, 0);
+          acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, b1, acc[0][1], 0, 0, 0);
+          if (nrb > 1) {
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, b1, acc[1][1], 0, 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();                  // every wave is done reading this layer's input
+    {
+      const bool cbias = (l16 & 3) == 3;
+#pragma unroll
+      for (int rb = 0; rb < 2; ++rb) {
+        if (rb >= nrb) continue;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int row = row0 + 16 * rb + l4 + 4 * v;
+          if (row < nout) {
+            const double b = cbias ? io.bp[li][row] : 0.0;
+            Xs[row * AM_HXS + l16] = acc[rb][0][v] + b;
+            Xs[row * AM_HXS + 16 + l16] = acc[rb][1][v] + b;
+          }
+        }
+      }
+    }
+    __syncthreads();
+    if (li < net.D) {
+      const int out0 = net.off[li];
+      const int jt = tid & 255, ct = tid >> 8;
+      for (int col = ct; col < AM_HN; col += AM_HALF_THREADS / 256) {
+        const int t = col >> 2, q = col & 3;
+        if (t >= nreg) break;
+        double* dst = lv.planes + ((size_t)(r0 + t) * 4 + q) * Cs + out0;
+        for (int j = jt; j < nout; j += 256) {
+          const double y = Xs[j * AM_HXS + col];
+          dst[j] = mbit(mk + t * net.Wd, out0 + j) ? -y : y;
+        }
+      }
+      const uint16_t* kl2 = klist + (size_t)li * AM_KLS;
+      const int nkp2 = s_nk[li + 1];
+      constexpr int PPT = AM_HALF_THREADS / AM_HN;                  // positions per pass of the block
+      constexpr int XH = AM_CHAIN_MAXW / PPT / 2;                   // passes per half
+      int tq = tid;
+      asm volatile("" : "+v"(tq));
+      const int col = tq & (AM_HN - 1), t = col >> 2, p0 = tq / AM_HN;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        double xv[XH];
+#pragma unroll
+        for (int c = 0; c < XH; ++c) {
+          const int i = (h * XH + c) * PPT + p0;
+          xv[c] = 0.0;
+          if (i < nkp2) {
+            const int k = kl2[i];
+            if (k < nout && mbit(mk + t * net.Wd, out0 + k)) xv[c] = Xs[k * AM_HXS + col];
+          }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < XH; ++c) {
+          const int i = (h * XH + c) * PPT + p0;
+          if (i < nkp2) Xs[i * AM_HXS + col] = xv[c];
+        }
+      }
+    } else if (tid < AM_HN && (tid >> 2) < nreg) {
+      const double y = Xs[tid];
+      lv.eq[(size_t)(r0 + (tid >> 2)) * 4 + (tid & 3)] = (tid & 3) == 3 ? y - net.iso : y;
+    }
+    __syncthreads();
+  }
+}
+
 __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
   const int r0 = blockIdx.x * AM_CT;
   const int cnt = launch_count(io);
@@ -372,12 +536,12 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
   chain_tile_body<false>(net, io, lv, r0, min(AM_CT, cnt - r0));
 }
 
-__global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_mfma(NetDev net, ChainIO io) {
-  const int r0 = blockIdx.x * AM_CT;
+__global__ __launch_bounds__(AM_HALF_THREADS) void k_chain_tile_mfma(NetDev net, ChainIO io) {
+  const int r0 = blockIdx.x * AM_HT;
   const int cnt = launch_count(io);
   if (r0 >= cnt) return;
   const ChainLevel lv{io.first, cnt, io.ids, io.planes, io.eq};
-  chain_tile_body<true>(net, io, lv, r0, min(AM_CT, cnt - r0));
+  chain_tile_half(net, io, lv, r0, min(AM_HT, cnt - r0));
 }
 
 // Many networks in one launch: block b takes tiles[b] = (network, r0, regions);

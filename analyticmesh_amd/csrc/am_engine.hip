@@ -449,6 +449,12 @@ size_t chain_lds_bytes(const am_ctx* c) {
          (size_t)AM_CT * c->Wd * 4 + 16;
 }
 
+// LDS of k_chain_tile_mfma (half tiles: AM_HT regions)
+size_t chain_half_lds_bytes(const am_ctx* c) {
+  return ((size_t)AM_CXR * AM_HXS + 1) * sizeof(double) + (size_t)c->D * AM_KLS * sizeof(uint16_t) +
+         (size_t)AM_HT * c->Wd * 4 + 16;
+}
+
 // Per-run counters of the FP64 stages (kernel times go to the context's timer;
 // the ray walk's undecided count and reasons accumulate on the device).
 struct Fp64Counts {
@@ -575,7 +581,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   const size_t rlds = ray64_lds_bytes(c);
   const int Ch = (c->C + 63) & ~63;
   const size_t chunk = fp64_prepare(c, count);
-  const size_t clds = chain_lds_bytes(c);
+  const size_t clds = chain_lds_bytes(c), hlds = chain_half_lds_bytes(c);
   StageTimer& tm = c->timer;
   for (int off = 0; off < count; off += (int)chunk) {
     const int n = std::min<int>((int)chunk, count - off);
@@ -606,8 +612,12 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     // the records of the candidate scan on every region (mode 0) are the
     // oracle's bit for bit: its planes come from the sequential-fma chain
     const bool mf = c->chain_mfma && (mode != 0 || after_walk);
-    hipLaunchKernelGGL((mf ? k_chain_tile_mfma : k_chain_tile_fp64), dim3((n + AM_CT - 1) / AM_CT),
-                       dim3(AM_CHAIN_THREADS), clds, c->stream, c->net, cio);
+    if (mf)
+      hipLaunchKernelGGL(k_chain_tile_mfma, dim3((n + AM_HT - 1) / AM_HT), dim3(AM_HALF_THREADS), hlds, c->stream,
+                         c->net, cio);
+    else
+      hipLaunchKernelGGL(k_chain_tile_fp64, dim3((n + AM_CT - 1) / AM_CT), dim3(AM_CHAIN_THREADS), clds, c->stream,
+                         c->net, cio);
     HIPCHK(hipGetLastError());
     t.chain_launches += 1;
     const size_t m1 = tm.mark(c->stream);
@@ -1208,10 +1218,12 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
       const size_t clds = chain_lds_bytes(c);
       if (clds > c->max_lds) c->tile_ok = false;
       else if (clds > 65536) {
-        HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   (int)clds));
         HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_fp64, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)clds));
+        const size_t hlds = chain_half_lds_bytes(c);
+        if (hlds > 65536)
+          HIPCHK(hipFuncSetAttribute((const void*)k_chain_tile_mfma, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)hlds));
       }
     }
     // The reduced forms need the tiled FP64 chain for their vertices and

@@ -65,6 +65,11 @@ __global__ void k_region_fp64(NetDev net, RegionIO io);
 #define AM_KLS (AM_CHAIN_MAXW + 2 * AM_CPF)   // entries of one layer's active-input list
 #define AM_CXR (AM_CHAIN_MAXW + AM_CPF)       // rows of the chain's layer buffer (list positions + prefetch)
 #define AM_CHAIN_THREADS (64 * AM_CHAIN_MAXW / AM_CR)
+// Half tiles of the matrix-core chain (k_chain_tile_mfma)
+#define AM_HT 8                 // regions per workgroup
+#define AM_HN (4 * AM_HT)       // columns
+#define AM_HXS (AM_HN + 1)      // LDS row stride (doubles)
+#define AM_HALF_THREADS 512     // 8 waves, 32 output rows each
 
 struct ChainIO {
   const uint32_t* reg_mask;   // [R][Wd]
@@ -92,6 +97,7 @@ struct ChainLevel {
 __global__ void k_chain_tile_fp64(NetDev net, ChainIO io);
 // The same planes with the products on the FP64 matrix cores: equal up to the
 // rounding of another summation order (the forms that certify their decisions).
+// Tiles of AM_HT regions, AM_HALF_THREADS threads, LDS chain_half_lds_bytes.
 __global__ void k_chain_tile_mfma(NetDev net, ChainIO io);
 // A batch of networks (each region's network is the last word of its mask):
 // tiles[b] = (network, first slot, regions, -) of one network.
