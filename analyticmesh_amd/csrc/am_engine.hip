@@ -145,6 +145,7 @@ struct am_ctx {
   DevBuf<double> cand_point;
   DevBuf<uint64_t> cand_hash;
   DevBuf<int> scan_part;
+  DevBuf<unsigned long long> scan_state;   // tile states of the visited-set stage's one-pass scan
   DevBuf<unsigned> rflags;
 
   // visited table
@@ -289,7 +290,7 @@ InsertPlan insert_prepare(am_ctx* c, size_t n) {
   // round trip between the count and the commit.
   p.early = (size_t)c->R + n <= c->reg_cap || n <= ((size_t)1 << 22);
   if (p.early) grow_regions(c, (size_t)c->R + n);
-  ensure(c->scan_part, scan_blocks(n) + 1, false, c->stream);
+  ensure(c->scan_state, scan_blocks(n) + 1, false, c->stream);
   arm_host_total(c);
   return p;
 }
@@ -323,14 +324,21 @@ void insert_launch(am_ctx* c, const InsertPlan& p) {
   io.n = (int)n;
   io.Wd = c->Wd;
   io.cap_mask = (uint32_t)(c->table_cap - 1);
+  const size_t tiles = scan_blocks(n);
+  io.scan_state = c->scan_state.p;
+  io.scan_words = (int)tiles + 1;
   hipLaunchKernelGGL(k_insert, dim3(blocks(n, 256)), dim3(256), 0, st, io);
   HIPCHK(hipGetLastError());
-  // ranks of the new-region flags; with the store already large enough the
-  // commit is enqueued before the host learns the count
-  const CommitIO cio = commit_io(c, p.R0);
-  scan_launch<int>(FirstSrc{cio}, n, ArraySink<int>{c->rank.p}, c->scan_part.p, c->host_total_dev, st);
-  if (p.early)
-    hipLaunchKernelGGL(k_commit, dim3(blocks(n, 256)), dim3(256), 0, st, cio, (const int*)c->rank.p, (int)n);
+  // the new-region flags' scan in one pass; with the store already large
+  // enough it commits the new regions too, before the host learns the count
+  ScanCommitIO sio;
+  sio.cio = commit_io(c, p.R0);
+  sio.state = c->scan_state.p;
+  sio.n = (int)n;
+  sio.tiles = (int)tiles;
+  sio.rank = p.early ? nullptr : c->rank.p;
+  sio.total = c->host_total_dev;
+  hipLaunchKernelGGL(k_scan_commit, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, sio);
   HIPCHK(hipGetLastError());
 }
 

@@ -57,6 +57,21 @@ struct CommitSink {
   }
 };
 __global__ void k_commit(CommitIO io, const int* rank, int n);
+
+// One-pass scan of the new-region flags of a level and the commit of the new
+// regions (k_scan_commit).  state: [tiles] tile states (flag << 32 | count; flag
+// 1: the tile's own count, 2: its inclusive prefix) and [tiles] the tile
+// counter, all zero at launch (k_insert clears them).  With rank set the
+// kernel only writes the ranks of the new regions (the store grows before
+// k_commit); total (host-mapped) receives the level's count.
+struct ScanCommitIO {
+  CommitIO cio;
+  unsigned long long* state;
+  int n, tiles;
+  int* rank;
+  int* total;
+};
+__global__ void k_scan_commit(ScanCommitIO io);
 __global__ void k_hash_keys(const uint32_t* masks, int n, int Wd, uint64_t* out);
 __global__ void k_rehash(const uint32_t* reg_mask, int R, int Wd, uint32_t* table, uint32_t cap_mask);
 __global__ void k_fingerprint(const uint32_t* reg_mask, const int* info, const int* vpair, int R, int Wd, int Wh,

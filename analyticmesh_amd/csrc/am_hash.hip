@@ -12,16 +12,45 @@
 #include "am_device.h"
 #include "am_kernels.h"
 #include "am_hash.h"
+#include "am_scan.h"
 
 namespace am {
 
 
-// All loads issued before any comparison decides (no early exit), so a
-// thread's key reads overlap instead of waiting on each other.
-__device__ __forceinline__ bool words_equal(const uint32_t* a, const uint32_t* b, int n) {
+// Keys are handled by groups of AM_KG lanes: lane j of a group takes words
+// j, j + AM_KG, ... of a key (16-byte quads when the key is a whole number of
+// quads), so one load instruction of a wave reads AM_KG whole keys' segments,
+// coalesced.
+#define AM_KG 8
+
+// Whether keys a and b (Wd words) are equal, in every lane of the group; a
+// group that does not compare (cmp false, the same in all its lanes) takes
+// part with no loads.  Called by the whole wave.
+__device__ __forceinline__ bool group_equal(const uint32_t* a, const uint32_t* b, int Wd, int j, int g, bool cmp) {
   uint32_t diff = 0;
-  for (int w = 0; w < n; ++w) diff |= a[w] ^ b[w];
-  return diff == 0;
+  if (cmp) {
+    if ((Wd & 3) == 0) {
+      const uint4* a4 = reinterpret_cast<const uint4*>(a);
+      const uint4* b4 = reinterpret_cast<const uint4*>(b);
+      for (int q = j; q < (Wd >> 2); q += AM_KG) {
+        const uint4 x = a4[q], y = b4[q];
+        diff |= (x.x ^ y.x) | (x.y ^ y.y) | (x.z ^ y.z) | (x.w ^ y.w);
+      }
+    } else {
+      for (int w = j; w < Wd; w += AM_KG) diff |= a[w] ^ b[w];
+    }
+  }
+  const unsigned long long bal = __ballot(diff != 0);
+  return ((bal >> (g * AM_KG)) & ((1ull << AM_KG) - 1)) == 0;
+}
+
+// Copy of a key (Wd words) by a group, as group_equal reads it.
+__device__ __forceinline__ void group_copy(uint32_t* dst, const uint32_t* src, int Wd, int j) {
+  if ((Wd & 3) == 0) {
+    for (int q = j; q < (Wd >> 2); q += AM_KG) reinterpret_cast<uint4*>(dst)[q] = reinterpret_cast<const uint4*>(src)[q];
+  } else {
+    for (int w = j; w < Wd; w += AM_KG) dst[w] = src[w];
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -37,32 +66,154 @@ __global__ void k_fill_i32(int* p, size_t n, int v) {
 // Candidate c claims a slot (CAS) or joins the pending group of an equal key
 // there, lowering the slot to the group's smallest candidate (atomicMin on
 // pending | c); a key already in the store, or an empty candidate, gets slot -1.
-__global__ void k_insert(InsertIO io) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= io.n) return;
-  if (io.cand_start[c] == -2) { io.slot[c] = -1; return; }
-  const uint32_t* key = io.cand_mask + (size_t)c * io.Wd;
-  uint32_t s = (uint32_t)mix64(io.cand_hash[c]) & io.cap_mask;
-  for (uint32_t probes = 0; probes <= io.cap_mask; ++probes) {
-    const uint32_t prev = atomicCAS(&io.table[s], AM_EMPTY, AM_PEND | (uint32_t)c);
-    if (prev == AM_EMPTY) {
-      io.slot[c] = (int)s;
-      return;
-    }
-    if (prev & AM_PEND) {
-      const int o = (int)(prev & ~AM_PEND);
-      if (words_equal(key, io.cand_mask + (size_t)o * io.Wd, io.Wd)) {
-        atomicMin(&io.table[s], AM_PEND | (uint32_t)c);
-        io.slot[c] = (int)s;
-        return;
-      }
-    } else if (words_equal(key, io.reg_mask + (size_t)prev * io.Wd, io.Wd)) {
-      io.slot[c] = -1;
-      return;
-    }
-    s = (s + 1) & io.cap_mask;
+// A wave takes 64 consecutive candidate slots: empty ones (no face edge) are
+// settled by their own lane, the others are compacted and probed 64 / AM_KG at
+// a time, one group of lanes per candidate (the group's first lane does the
+// atomics, the group compares keys).  Also clears the tile states of the scan
+// that follows on the stream.
+__global__ __launch_bounds__(256) void k_insert(InsertIO io) {
+  __shared__ int s_list[4][64];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int gtid = blockIdx.x * 256 + tid;
+  if (gtid < io.scan_words) io.scan_state[gtid] = 0ull;
+  const int c = gtid;
+  bool live = false;
+  if (c < io.n) {
+    live = io.cand_start[c] != -2;
+    if (!live) io.slot[c] = -1;
   }
-  io.slot[c] = -1;  // table full: the host sizes the table so this never happens
+  const unsigned long long bal = __ballot(live);
+  const int nlive = __popcll(bal);
+  if (live) s_list[wv][__builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u))] = c;
+  __syncthreads();
+  const int j = lane & (AM_KG - 1), g = lane / AM_KG, lead = lane & ~(AM_KG - 1);
+  for (int p = 0; p < nlive; p += 64 / AM_KG) {
+    const bool act = p + g < nlive;
+    const int cc = act ? s_list[wv][p + g] : 0;
+    const uint32_t* key = io.cand_mask + (size_t)cc * io.Wd;
+    uint32_t s = act ? (uint32_t)mix64(io.cand_hash[cc]) & io.cap_mask : 0u;
+    bool done = !act;
+    int result = -1;
+    for (uint32_t probes = 0; __ballot(!done) != 0ull; ++probes) {
+      uint32_t prev = 0u;
+      if (!done && j == 0) prev = atomicCAS(&io.table[s], AM_EMPTY, AM_PEND | (uint32_t)cc);
+      prev = (uint32_t)__shfl((int)prev, lead, 64);
+      bool cmp = false;
+      const uint32_t* other = key;
+      if (!done) {
+        if (prev == AM_EMPTY) {
+          result = (int)s;
+          done = true;
+        } else {
+          cmp = true;
+          other = (prev & AM_PEND) ? io.cand_mask + (size_t)(prev & ~AM_PEND) * io.Wd
+                                   : io.reg_mask + (size_t)prev * io.Wd;
+        }
+      }
+      const bool eq = group_equal(key, other, io.Wd, j, g, cmp);
+      if (cmp) {
+        if (eq) {
+          if (prev & AM_PEND) {
+            if (j == 0) atomicMin(&io.table[s], AM_PEND | (uint32_t)cc);
+            result = (int)s;
+          }
+          done = true;
+        } else if (probes >= io.cap_mask) {
+          done = true;   // table full: the host sizes the table so this never happens
+        } else {
+          s = (s + 1) & io.cap_mask;
+        }
+      }
+    }
+    if (act && j == 0) io.slot[cc] = result;
+  }
+}
+
+// Decoupled look-back scan (Merrill and Garland, "Single-pass parallel prefix
+// scan with decoupled look-back", 2016) over the new-region flags, with the
+// commit: a block takes the next tile of kScanTile candidates in start order
+// (a counter), publishes the tile's count, then its first wave adds up the
+// states of the tiles before it, 64 at a time, back to the nearest inclusive
+// prefix, and publishes its own.  A tile publishes its count before it waits
+// and waits only on tiles that started before it, so every wait ends.  The new
+// regions of the tile are then written by groups of AM_KG lanes (mask, start,
+// point, and the table slot now holding the region id).
+__global__ __launch_bounds__(kScanThreads) void k_scan_commit(ScanCommitIO io) {
+  __shared__ int s_tile, s_excl;
+  __shared__ int s_wave[kScanThreads / 64];
+  __shared__ int s_list[kScanTile];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid == 0) s_tile = (int)atomicAdd(&io.state[io.tiles], 1ull);
+  __syncthreads();
+  const int t = s_tile;
+  const size_t base = (size_t)t * kScanTile + (size_t)tid * kScanItems;
+  int v[kScanItems];
+  int sum = 0;
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    v[it] = base + it < (size_t)io.n ? first_flag(io.cio, base + it) : 0;
+    sum += v[it];
+  }
+  int tot;
+  const int off = block_excl_scan(sum, s_wave, &tot);
+  if (tid < 64) {
+    constexpr unsigned long long F1 = 1ull << 32, F2 = 2ull << 32;
+    if (lane == 0)
+      __hip_atomic_store(&io.state[t], (t == 0 ? F2 : F1) | (unsigned)tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int excl = 0;
+    if (t > 0) {
+      for (int look = t - 1;; look -= 64) {
+        const int idx = look - lane;   // lane 0: the nearest tile
+        unsigned long long st = idx >= 0 ? __hip_atomic_load(&io.state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                         : F2;
+        while (__ballot((st >> 32) == 0) != 0ull) {
+          if ((st >> 32) == 0) {
+            __builtin_amdgcn_s_sleep(1);
+            st = __hip_atomic_load(&io.state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+        }
+        const unsigned long long b2 = __ballot((st >> 32) == 2);
+        const int stop = b2 ? __ffsll((long long)b2) - 1 : 63;
+        int x = lane <= stop ? (int)(unsigned)st : 0;
+#pragma unroll
+        for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+        excl += x;
+        if (b2) break;
+      }
+      if (lane == 0)
+        __hip_atomic_store(&io.state[t], F2 | (unsigned)(excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (lane == 0) {
+      s_excl = excl;
+      if (t == io.tiles - 1 && io.total) *io.total = excl + tot;
+    }
+  }
+  __syncthreads();
+  const int excl = s_excl;
+  int run = off;
+#pragma unroll
+  for (int it = 0; it < kScanItems; ++it) {
+    if (v[it]) {
+      if (io.rank) io.rank[base + it] = excl + run;
+      else s_list[run] = (int)(base + it);
+    }
+    run += v[it];
+  }
+  if (io.rank) return;
+  __syncthreads();
+  const CommitIO& ci = io.cio;
+  const int j = tid & (AM_KG - 1);
+  for (int k = tid / AM_KG; k < tot; k += kScanThreads / AM_KG) {
+    const int cc = s_list[k];
+    const int rid = ci.R0 + excl + k;
+    group_copy(ci.reg_mask + (size_t)rid * ci.Wd, ci.cand_mask + (size_t)cc * ci.Wd, ci.Wd, j);
+    if (j == 0) {
+      ci.reg_start[rid] = ci.cand_start[cc];
+      ci.table[ci.slot[cc]] = (uint32_t)rid;
+    } else if (j <= 3) {
+      ci.reg_point[(size_t)rid * 3 + j - 1] = ci.cand_point[(size_t)cc * 3 + j - 1];
+    }
+  }
 }
 
 // New regions get ids R0 + rank in candidate order (rank from a stored scan).

@@ -50,6 +50,8 @@ struct InsertIO {
   int n;
   int Wd;
   uint32_t cap_mask;
+  unsigned long long* scan_state;  // [scan_words] tile states of the level's one-pass scan, cleared here
+  int scan_words;
 };
 
 __global__ void k_region_fp64(NetDev net, RegionIO io);
