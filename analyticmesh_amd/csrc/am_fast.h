@@ -32,10 +32,8 @@ struct FastNetDev {
   const uint16_t* Wl[AM_MAX_LINEAR];    // bf16 lo A operands
   // Error estimates of the planes: per region and plane j of layer li + 1 the
   // variance V_j = sum_k W_jk^2 m_k (V_k + ck[li] |P_k|^2) + ck[li] b_j^2 (V_k
-  // of the layer's input planes, 0 for hidden layer 1's exact planes), a
-  // bf16 MFMA product with A = W^2 in the layout of Wh ([nrb][nkq][64][8]).
-  const uint16_t* Wsq[AM_MAX_LINEAR];
-  int nkq[AM_MAX_LINEAR];       // 32-wide k blocks of layer li's W^2
+  // of the layer's input planes, 0 for hidden layer 1's exact planes), a bf16
+  // MFMA product inside the FP32 k loop (A: the squared weight fragments).
   float ck[AM_MAX_LINEAR];      // variance per squared term of layer li's sums
   int sig;                      // 1: compute the estimates (planes component 4, eq[4])
   const float* ew;

@@ -10,10 +10,11 @@
 //     a k-major copy of W^T, X is staged already compacted;
 //   * split-BF16: w = wh + wl, x = xh + xl (bf16 parts), y = wh.xh + wh.xl + wl.xh
 //     with v_mfma_f32_16x16x32_bf16 and FP32 accumulation.
-// Beside each plane the kernel writes an estimate sigma of its error: the
-// variance of the accumulated rounding (and, split-BF16, of the split) is
-// propagated through the chain, V' = W^2 m (V + ck |P|^2), as one more small
-// product per layer (bf16 MFMA, A = W^2; the estimate needs a few bits only).
+// Beside each plane the FP32 form writes an estimate sigma of its error: the
+// variance of the accumulated rounding is propagated through the chain,
+// V' = W^2 m (V + ck |P|^2), as one more small product inside the k loop (bf16
+// MFMA; its A operand is the square of the weight fragments the loop already
+// holds; the estimate needs a few bits only).
 // The checked walk bounds each plane by lam sigma plus a few ulps instead of
 // one relative bound for every plane of the net.
 // The signed planes go to HBM, one SoA record per region, and the checked FP32
@@ -30,6 +31,7 @@ namespace am {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
 
 #define AM_XS (AM_GN + 4)   // fp32 LDS row stride (floats)
 #define AM_GRBW 4           // 16-row output blocks per wave (layers up to 256 outputs)
@@ -102,6 +104,16 @@ __device__ __forceinline__ uint16_t sig_input(float V, float n2, float ck) {
   return bf16_rn(fminf(V + ck * n2, 1e30f));
 }
 
+// Where the error-estimate input of list position p and region t sits in the
+// row paddings (bf16 units from the first padding): the B operand layout of
+// v_mfma_f32_16x16x16_bf16 for the k loop's step kb = p >> 2 group.  Lane
+// (q, t) of the step group starting at list position 16 g reads positions
+// 16 g + 4 i + q, i = 0..3, as one 8-byte word: row 16 g + 4 q + t / 2, half t % 2.
+// A group's slots lie in its own 16 rows.
+__device__ __forceinline__ int sig_slot(int p, int t) {
+  return ((p & ~15) + 4 * (p & 3) + (t >> 1)) * 2 * AM_XS + (t & 1) * 4 + ((p & 15) >> 2);
+}
+
 template <int PREC>
 __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, GemmIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -140,17 +152,6 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
     for (int i = tid; i < 2 * AM_GN * KS; i += 256) Xh[i] = 0;
   }
   __syncthreads();
-  if (SIG && f.sig) {
-    // hidden layer 1's planes are exact (V = 0)
-    const int n1 = f.nodes[1];
-    for (int idx = tid; idx < n1 * AM_GT; idx += 256) {
-      const int k = idx / AM_GT, t = idx - k * AM_GT;
-      if (mbit(mk + t * f.Wd, k)) {
-        const float a = f.W0[k * 3 + 0], b = f.W0[k * 3 + 1], c = f.W0[k * 3 + 2], d = f.bias[0][k];
-        xq_at(k, t) = sig_input(0.f, a * a + b * b + c * c + d * d, f.ck[1]);
-      }
-    }
-  }
   if (PREC == 0) {
     // Inputs of linear layer li that are active in some region of the tile,
     // ascending, padded to a multiple of 16 with n_li (the +0 row of W^T and,
@@ -176,60 +177,36 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
     __syncthreads();
   }
   stage_layer1<PREC>(f, io, mk, r0, nreg, Xs, Xh, Xl, KS, klist, PREC == 0 ? s_nk[1] : 0);
+  if (SIG) {
+    // the first product's input at the list positions (hidden layer 1's
+    // planes are exact: V = 0)
+    const int n1 = f.nodes[1], nkp1 = s_nk[1];
+    for (int idx = tid; idx < nkp1 * AM_GT; idx += 256) {
+      const int i = idx / AM_GT, t = idx - i * AM_GT;
+      const int k = klist[i];
+      uint16_t x = 0;
+      if (k < n1 && mbit(mk + t * f.Wd, k)) {
+        const float a = f.W0[k * 3 + 0], b = f.W0[k * 3 + 1], c = f.W0[k * 3 + 2], d = f.bias[0][k];
+        x = sig_input(0.f, a * a + b * b + c * c + d * d, f.ck[1]);
+      }
+      Ypad[sig_slot(i, t)] = x;
+    }
+  }
   __syncthreads();
 
   for (int li = 1; li <= f.D; ++li) {
     const int nout = f.nodes[li + 1];
     const int nrb = f.nrb[li], nkb = f.nkb[li];
     const float* bias = f.bias[li];
-    if (SIG && f.sig) {
-      // the error-estimate product of this wave's row blocks, V = W^2 . (input);
-      // lane (column t, rows 4 (lane >> 4) + i of block rb) holds region t's V
-      // of plane rb * 16 + 4 (lane >> 4) + i
-      const int nuq = (nrb - wave + 3) / 4;
-      f32x4 accq[AM_GRBW];
-#pragma unroll
-      for (int u = 0; u < AM_GRBW; ++u) accq[u] = f32x4{0.f, 0.f, 0.f, 0.f};
-      if (nuq > 0) {
-        const int nkq = f.nkq[li];
-        const int tq = lane & 15, kq = 8 * (lane >> 4);
-        const uint4* Aq = reinterpret_cast<const uint4*>(f.Wsq[li]) + lane;
-        for (int kb = 0; kb < nkq; ++kb) {
-          // column tq, rows kb * 32 + kq .. + 7 from the row paddings (+0 for columns past AM_GT)
-          uint16_t h[8];
-#pragma unroll
-          for (int e = 0; e < 8; ++e) h[e] = tq < AM_GT ? Ypad[(kb * 32 + kq + e) * 2 * AM_XS + tq] : (uint16_t)0;
-          const bf16x8 vb = __builtin_bit_cast(bf16x8, h);
-#pragma unroll
-          for (int u = 0; u < AM_GRBW; ++u)
-            if (u < nuq) {
-              const bf16x8 va = __builtin_bit_cast(bf16x8, Aq[((size_t)(wave + 4 * u) * nkq + kb) * 64]);
-              accq[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, accq[u], 0, 0, 0);
-            }
-        }
-      }
-      __syncthreads();   // every wave has read the product's input
-      const int t = lane & 15;
-#pragma unroll
-      for (int u = 0; u < AM_GRBW; ++u) {
-        const int rb = wave + 4 * u;
-        if (rb < nrb && t < AM_GT) {
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int j = rb * 16 + 4 * (lane >> 4) + i;
-            if (j < nout) xq_at(j, t) = bf16_rn(fminf(fmaxf(accq[u][i], 0.f) + f.ck[li] * bias[j] * bias[j], 1e30f));
-          }
-        }
-      }
-      // (read after the barriers that follow the layer's k loop)
-    }
     // Row blocks rb = wave + 4 u of this wave (n_out <= 64 * AM_GRBW).  The k
     // loop is outermost, so each B fragment read from LDS feeds every row block.
     f32x4 accs[AM_GRBW][2];
+    f32x4 accq[AM_GRBW];   // FP32: the error-estimate product V = W^2 . (input) of the row blocks
 #pragma unroll
     for (int u = 0; u < AM_GRBW; ++u) {
       accs[u][0] = f32x4{0.f, 0.f, 0.f, 0.f};
       accs[u][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+      accq[u] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
     const int nu = (nrb - wave + 3) / 4;   // row blocks of this wave (uniform; <= 0: none)
     if (nu <= 0) {
@@ -244,6 +221,10 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
           (void*)f.Wp[li], (short)0, (f.nodes[li] + 1) * ld4, 0x00020000);
       const int cw = (wave * 16 + (lane & 15)) * 4;
       const float* xb = Xs + (lane >> 4) * AM_XS + (lane & 15);
+      // error-estimate operand of lane (q, t): 4 positions of its group as one
+      // 8-byte word (sig_slot; columns 8..15 read region t - 8's, their
+      // results are not used)
+      const uint2* sb = reinterpret_cast<const uint2*>(Ypad + sig_slot(lane >> 4, lane & 7));
       for (int kb = 0; kb < nkl; kb += 4) {
         int ko[4];
         float bx[4][2];
@@ -253,6 +234,8 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
           bx[e][0] = xb[(kb + e) * 4 * AM_XS];
           bx[e][1] = xb[(kb + e) * 4 * AM_XS + 16];
         }
+        uint2 bq = {0u, 0u};
+        if (SIG) bq = sb[2 * kb * AM_XS];   // the group's 16 rows start 4 kb rows (AM_XS / 2 words each) in
 #pragma unroll
         for (int u = 0; u < AM_GRBW; ++u) {
           if (u < nu) {
@@ -264,6 +247,12 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
             for (int e = 0; e < 4; ++e) {
               accs[u][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bx[e][0], accs[u][0], 0, 0, 0);
               accs[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bx[e][1], accs[u][1], 0, 0, 0);
+            }
+            if (SIG) {
+              const uint32_t a01 = (uint32_t)bf16_rn(av[0] * av[0]) | ((uint32_t)bf16_rn(av[1] * av[1]) << 16);
+              const uint32_t a23 = (uint32_t)bf16_rn(av[2] * av[2]) | ((uint32_t)bf16_rn(av[3] * av[3]) << 16);
+              accq[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, uint2{a01, a23}),
+                                                                  __builtin_bit_cast(s16x4, bq), accq[u], 0, 0, 0);
             }
           }
         }
@@ -314,6 +303,8 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
           const float b = bias[row];
           Ys[row * AM_XS + c0] = acc0[i] + ((c0 & 3) == 3 ? b : 0.f);
           Ys[row * AM_XS + 16 + c0] = acc1[i] + (((16 + c0) & 3) == 3 ? b : 0.f);
+          // V of plane row, region c0, in row row's padding
+          if (SIG && c0 < AM_GT) xq_at(row, c0) = bf16_rn(fminf(fmaxf(accq[u][i], 0.f) + f.ck[li] * b * b, 1e30f));
         }
       }
     }
@@ -330,34 +321,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
           dst[f.Cs] = on ? -y.y : y.y;
           dst[2 * (size_t)f.Cs] = on ? -y.z : y.z;
           dst[3 * (size_t)f.Cs] = on ? -y.w : y.w;
-        }
-      }
-      if (SIG && f.sig) {
-        // estimates of this layer's planes to HBM (consecutive threads,
-        // consecutive planes) and the next product's input: values first,
-        // then a barrier (V and the next input share their slots), then the writes
-        const float ckn = f.ck[li + 1];
-        const int rows = f.nkq[li + 1] * 32;   // rows the next product reads
-        constexpr int QPT = AM_GT * AM_CHAIN_MAXW / 256;   // values per thread (rows <= 256)
-        uint16_t qv[QPT];
-#pragma unroll
-        for (int c = 0; c < QPT; ++c) {
-          const int idx = tid + 256 * c;
-          const int t = idx / rows, j = idx - t * rows;
-          qv[c] = 0;
-          if (t < AM_GT && j < nout) {
-            const float V = bf16_to_f(xq_at(j, t));
-            if (t < nreg) io.planes[((size_t)(r0 + t) * AM_PCOMP + 4) * f.Cs + out0 + j] = 1.02f * __builtin_sqrtf(V);
-            const float4 y = *reinterpret_cast<const float4*>(Ys + j * AM_XS + 4 * t);
-            if (mbit(mk + t * f.Wd, out0 + j)) qv[c] = sig_input(V, y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w, ckn);
-          }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int c = 0; c < QPT; ++c) {
-          const int idx = tid + 256 * c;
-          const int t = idx / rows, j = idx - t * rows;
-          if (t < AM_GT) xq_at(j, t) = qv[c];
+          if (SIG && f.sig) dst[4 * (size_t)f.Cs] = 1.02f * __builtin_sqrtf(bf16_to_f(xq_at(j, t)));
         }
       }
       if (PREC == 0) {
@@ -373,22 +337,39 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
         int tq = tid;
         asm volatile("" : "+v"(tq));       // keeps these offsets from being hoisted out of the layer loop
         const int col = tq & (AM_GN - 1), t = col >> 2, p0 = tq / AM_GN;
+        // the next product's input rides along: the x-column thread of each
+        // region also forms m (V + ck |P|^2) for its positions (sig_slot
+        // places them in the rows of the position's own 16-row group, so the
+        // chunk rule holds for them too)
+        const bool sq = SIG && (col & 3) == 0;
+        const float ckn = f.ck[li + 1];
         for (int c0 = 0; c0 < nkp2; c0 += 128) {
           float xv[CH];
+          uint16_t qv[CH];
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
             const int i = c0 + c * PPT + p0;
             xv[c] = 0.f;
+            qv[c] = 0;
             if (i < nkp2) {
               const int k = kl2[i];
-              if (k < nout && mbit(mk + t * f.Wd, out0 + k)) xv[c] = Ys[k * AM_XS + col];
+              if (k < nout && mbit(mk + t * f.Wd, out0 + k)) {
+                xv[c] = Ys[k * AM_XS + col];
+                if (sq) {
+                  const float4 y = *reinterpret_cast<const float4*>(Ys + k * AM_XS + col);
+                  qv[c] = sig_input(bf16_to_f(xq_at(k, t)), y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w, ckn);
+                }
+              }
             }
           }
           __syncthreads();
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
             const int i = c0 + c * PPT + p0;
-            if (i < nkp2) Xs[i * AM_XS + col] = xv[c];
+            if (i < nkp2) {
+              Xs[i * AM_XS + col] = xv[c];
+              if (sq) Ypad[sig_slot(i, t)] = qv[c];
+            }
           }
         }
       }
@@ -542,7 +523,6 @@ enum Slot {
   S_W0 = 0, S_EW, S_EB, S_PLANES, S_EQ,
   S_BIAS0 = 8,      // 8..23: biases per linear layer
   S_WP0 = 24,       // 24..39: operand buffers per linear layer (fp32, or hi+lo packed)
-  S_WQ0 = 40,       // 40..55: W^2 operands of the error estimates per linear layer
 };
 
 // Buffer s with at least `bytes`; grows geometrically (levels grow run by run).
@@ -594,22 +574,6 @@ size_t gemm_lds(const FastNetDev& d, int prec) {
   const size_t xs = prec == 0 ? ys : std::max(ys, (size_t)2 * AM_GN * (d.Kpad + 8) * 2);
   const size_t kls = prec == 0 ? (size_t)d.D * d.Kpad * 2 : 0;
   return xs + (size_t)AM_GT * d.Wd * 4 + kls + 16;
-}
-
-// bf16 MFMA A operand [nrb][nkb][64 lanes][8] of an (nout x nin) matrix given
-// element-wise by `at(row, k)` (+0 past the matrix)
-template <typename F>
-std::vector<uint16_t> mfma_a_bf16(int nout, int nin, F at) {
-  const int nrb = (nout + 15) / 16, nkb = (nin + 31) / 32;
-  std::vector<uint16_t> out((size_t)nrb * nkb * 64 * 8, 0);
-  for (int rb = 0; rb < nrb; ++rb)
-    for (int kb = 0; kb < nkb; ++kb)
-      for (int l = 0; l < 64; ++l)
-        for (int e = 0; e < 8; ++e) {
-          const int row = rb * 16 + (l & 15), k = kb * 32 + 8 * (l >> 4) + e;
-          out[(((size_t)rb * nkb + kb) * 64 + l) * 8 + e] = (row < nout && k < nin) ? at(row, k) : 0;
-        }
-  return out;
 }
 
 }  // namespace
@@ -698,7 +662,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
       d.Wl[li] = p + n;
     }
   }
-  // error estimates (FP32 form): W^2 operands and the variance per squared
+  // error estimates (FP32 form): the variance per squared
   // term of each layer's sums, 4 u^2 (K + 2) for K = n_in products and the
   // bias: u^2 per rounding and u^2 K for the accumulation, times 4 because the
   // errors measured against FP64 on the matrix cores' accumulation order run
@@ -707,19 +671,7 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   // per-plane bounds, so the product would cost more than it saves.
   if (prec == 0) {
     const double u2 = std::ldexp(1.0, -48);
-    for (int li = 1; li <= net.D; ++li) {
-      const int nin = net.nodes[li], nout = net.nodes[li + 1];
-      auto w = to_float(weights[li], (size_t)nout * nin, dtype, st);
-      auto q = mfma_a_bf16(nout, nin, [&](int row, int k) {
-        const float x = w[(size_t)row * nin + k];
-        return host_bf16(x * x);
-      });
-      uint16_t* p = (uint16_t*)slot(b, S_WQ0 + li, q.size() * 2);
-      FCHK(hipMemcpy(p, q.data(), q.size() * 2, hipMemcpyHostToDevice));
-      d.Wsq[li] = p;
-      d.nkq[li] = (nin + 31) / 32;
-      d.ck[li] = (float)(4.0 * u2 * (nin + 2));
-    }
+    for (int li = 1; li <= net.D; ++li) d.ck[li] = (float)(4.0 * u2 * (net.nodes[li] + 2));
     d.sig = 1;
   }
   const size_t lds = gemm_lds(d, prec);
