@@ -31,6 +31,7 @@ namespace am {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef short s16x4 __attribute__((ext_vector_type(4)));
 
 #define AM_XS (AM_GN + 4)   // fp32 LDS row stride (floats)
@@ -249,9 +250,10 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
               accs[u][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[e], bx[e][1], accs[u][1], 0, 0, 0);
             }
             if (SIG) {
-              const uint32_t a01 = (uint32_t)bf16_rn(av[0] * av[0]) | ((uint32_t)bf16_rn(av[1] * av[1]) << 16);
-              const uint32_t a23 = (uint32_t)bf16_rn(av[2] * av[2]) | ((uint32_t)bf16_rn(av[3] * av[3]) << 16);
-              accq[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, uint2{a01, a23}),
+              // squares rounded to bf16 (v_cvt_pk_bf16_f32, to nearest even)
+              const bf16x4 a = {(__bf16)(av[0] * av[0]), (__bf16)(av[1] * av[1]), (__bf16)(av[2] * av[2]),
+                                (__bf16)(av[3] * av[3])};
+              accq[u] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s16x4, a),
                                                                   __builtin_bit_cast(s16x4, bq), accq[u], 0, 0, 0);
             }
           }

@@ -7,9 +7,8 @@
 
 namespace am {
 
-#define AM_NT 64                   // vertices per Newton workgroup (one per lane)
-#define AM_NROW 16                 // output rows per wave
-#define AM_NEWTON_THREADS 1024     // 16 waves: rows 0 .. 255 of a layer
+#define AM_NT 32                   // vertices per Newton workgroup (one column each)
+#define AM_NEWTON_THREADS 512      // 8 waves, 32 rows each: rows 0 .. 255 of a layer
 
 struct NewtonIO {
   const uint32_t* reg_mask;        // [R][Wd]

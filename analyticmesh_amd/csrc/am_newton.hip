@@ -15,10 +15,13 @@
 // FP64-solve accuracy without FP64 planes (backend/inc/kernel.h:555-573 solves
 // the same system from FP64 planes).
 //
-// One workgroup of 16 waves takes 64 vertices (one column each), wave w owns
-// output rows 16 w .. 16 w + 15 of every layer.  A layer's input (masked) lives
-// in LDS; the products run on the FP64 matrix cores, the A fragments (weights)
-// streamed from memory a few steps ahead.
+// One workgroup of 8 waves takes 32 vertices (one column each); wave w owns
+// output rows 32 w .. 32 w + 31 of every layer, as two 16-row blocks by two
+// 16-column blocks.  A layer's input (masked) lives in LDS; the products run
+// on the FP64 matrix cores, the A fragments (weights) streamed from memory a
+// few steps ahead.  The LDS (about 70 KB) lets two workgroups share a CU, so
+// one's products overlap the other's layer epilogue.  The output layer (one
+// row) is a VALU dot product split over the waves.
 #include "am_device.h"
 #include "am_kernels.h"
 #include "am_newton.h"
@@ -31,10 +34,12 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* X = reinterpret_cast<double*>(smem_raw);                             // [AM_CHAIN_MAXW][AM_NT]
   uint32_t* smask = reinterpret_cast<uint32_t*>(X + (size_t)AM_CHAIN_MAXW * AM_NT);   // [AM_NT][Wd]
+  constexpr int NPART = AM_NEWTON_THREADS / AM_NT;   // row groups of the VALU layers
   __shared__ double s_v[AM_NT][3];
   __shared__ double s_z[AM_NT][2];
   __shared__ int s_ab[AM_NT][2];
   __shared__ int s_rk[AM_NT][2];
+  __shared__ double s_part[NPART][AM_NT];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int base = blockIdx.x * AM_NT;
@@ -65,102 +70,119 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
     smask[idx] = c < n ? io.reg_mask[(size_t)s_rk[c][0] * Wd + w] : 0u;
   }
   __syncthreads();
-  const uint32_t* mk = smask + (size_t)lane * Wd;
-  const int ca = s_ab[lane][0], cb = s_ab[lane][1];
-  // hidden layer 1: z_j = W0[j] . v + b0[j]
+  // hidden layer 1: z_j = W0[j] . v + b0[j]; thread: column c, rows part, part + NPART, ...
   {
+    const int c = tid % AM_NT, part = tid / AM_NT;
+    const uint32_t* mk = smask + (size_t)c * Wd;
+    const int ca = s_ab[c][0], cb = s_ab[c][1];
     const int n1 = net.nodes[1];
-    const double x0 = s_v[lane][0], x1 = s_v[lane][1], x2 = s_v[lane][2];
-    for (int jj = 0; jj < AM_NROW; ++jj) {
-      const int j = wave * AM_NROW + jj;
-      if (j >= n1) break;
+    const double x0 = s_v[c][0], x1 = s_v[c][1], x2 = s_v[c][2];
+    for (int j = part; j < n1; j += NPART) {
       const double z = __builtin_fma(net.W0[j * 3 + 2], x2, __builtin_fma(net.W0[j * 3 + 1], x1,
                                      __builtin_fma(net.W0[j * 3 + 0], x0, net.b[0][j])));
-      if (j == ca) s_z[lane][0] = z;
-      if (j == cb) s_z[lane][1] = z;
-      X[(size_t)j * AM_NT + lane] = mbit(mk, j) ? z : 0.0;
+      if (j == ca) s_z[c][0] = z;
+      if (j == cb) s_z[c][1] = z;
+      X[(size_t)j * AM_NT + c] = mbit(mk, j) ? z : 0.0;
     }
   }
   __syncthreads();
-  // Layers 2 .. D + 1 on the FP64 matrix cores: wave w computes rows
-  // row0 .. row0 + 15 for all 64 vertices as four 16 x 16 blocks of
-  // v_mfma_f64_16x16x4f64 per 4 inputs (A: lane l holds W[row0 + l % 16][k0 + l / 16];
-  // B: X[k0 + l / 16][16 cb + l % 16]; result element v of lane l is row l / 16 + 4 v,
-  // column l % 16 of the block).  The A fragments of the next steps are in flight
-  // while a step's products run.
-  __shared__ double s_f[AM_NT];
+  // Hidden layers 2 .. D on the FP64 matrix cores: wave w computes rows row0 ..
+  // row0 + 31 for the 32 vertices with v_mfma_f64_16x16x4f64 per 4 inputs (A:
+  // lane l holds W[row + l % 16][k0 + l / 16]; B: X[k0 + l / 16][16 cb + l % 16];
+  // result element v of lane l is row l / 16 + 4 v, column l % 16 of the block).
   const int l16 = lane & 15, l4 = lane >> 4;
-  for (int li = 1; li <= net.D; ++li) {
+  for (int li = 1; li < net.D; ++li) {
     const int nin = net.nodes[li], nout = net.nodes[li + 1], ld = cio.noutp[li];
-    const int row0 = wave * AM_NROW;
-    const bool out_layer = li == net.D;
-    am_f64x4 acc[AM_NT / 16];
+    const int row0 = wave * 32;
+    // row blocks with rows (W^T holds ld >= nout columns, a multiple of 16, zero past nout)
+    const int nrb = row0 >= nout ? 0 : (row0 + 16 >= ld ? 1 : 2);
+    am_f64x4 acc[2][2];
 #pragma unroll
-    for (int cb = 0; cb < AM_NT / 16; ++cb) acc[cb] = am_f64x4{0.0, 0.0, 0.0, 0.0};
-    if (row0 < nout) {
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) acc[rb][cb] = am_f64x4{0.0, 0.0, 0.0, 0.0};
+    if (nrb > 0) {
       // W^T is zero past row nin (and up to AM_CHAIN_MAXW), so k may run to a multiple of 4
       const double* wa = cio.Wp[li] + (size_t)l4 * ld + row0 + l16;
+      const int rb1 = nrb > 1 ? 16 : 0;   // the second block's offset (re-reads the first when absent)
       const int nk = (nin + 3) & ~3;
-      constexpr int PF = 4;   // A fragments in flight
-      double a[PF];
+      constexpr int PF = 4;   // k steps of A fragments in flight
+      double a0[PF], a1[PF];
 #pragma unroll
-      for (int u = 0; u < PF; ++u) a[u] = 4 * u < nk ? wa[(size_t)(4 * u) * ld] : 0.0;
+      for (int u = 0; u < PF; ++u) {
+        const bool in = 4 * u < nk;
+        a0[u] = in ? wa[(size_t)(4 * u) * ld] : 0.0;
+        a1[u] = in ? wa[(size_t)(4 * u) * ld + rb1] : 0.0;
+      }
       for (int k0 = 0; k0 < nk; k0 += 4 * PF) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
           const int kk = k0 + 4 * u;
           if (kk < nk) {
             const int kr = kk + l4;
-            double b[AM_NT / 16];
-#pragma unroll
-            for (int cb = 0; cb < AM_NT / 16; ++cb) b[cb] = kr < nin ? X[(size_t)kr * AM_NT + 16 * cb + l16] : 0.0;
-            const double av = a[u];
+            const double b0 = kr < nin ? X[(size_t)kr * AM_NT + l16] : 0.0;
+            const double b1 = kr < nin ? X[(size_t)kr * AM_NT + 16 + l16] : 0.0;
+            const double x0 = a0[u], x1 = a1[u];
             const int kn = kk + 4 * PF;
-            a[u] = kn < nk ? wa[(size_t)kn * ld] : 0.0;
-#pragma unroll
-            for (int cb = 0; cb < AM_NT / 16; ++cb)
-              acc[cb] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[cb], acc[cb], 0, 0, 0);
+            a0[u] = kn < nk ? wa[(size_t)kn * ld] : 0.0;
+            a1[u] = kn < nk ? wa[(size_t)kn * ld + rb1] : 0.0;
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x0, b1, acc[0][1], 0, 0, 0);
+            if (nrb > 1) {
+              acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, b0, acc[1][0], 0, 0, 0);
+              acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(x1, b1, acc[1][1], 0, 0, 0);
+            }
           }
         }
       }
     }
     __syncthreads();   // every wave has read this layer's input
-    if (row0 < nout) {
-      const int out0 = out_layer ? 0 : net.off[li];
+    const int out0 = net.off[li];
 #pragma unroll
-      for (int cb = 0; cb < AM_NT / 16; ++cb) {
+    for (int rb = 0; rb < 2; ++rb) {
+      if (rb >= nrb) continue;
+#pragma unroll
+      for (int cb = 0; cb < 2; ++cb) {
         const int c = 16 * cb + l16;
         const int ac = s_ab[c][0], bc = s_ab[c][1];
         const uint32_t* mc = smask + (size_t)c * Wd;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-          const int j = row0 + l4 + 4 * v;
+          const int j = row0 + 16 * rb + l4 + 4 * v;
           if (j < nout) {
-            const double z = acc[cb][v] + cio.bp[li][j];
-            if (out_layer) {
-              s_f[c] = z;   // nout == 1: row 0
-            } else {
-              const int g = out0 + j;
-              if (g == ac) s_z[c][0] = z;
-              if (g == bc) s_z[c][1] = z;
-              X[(size_t)j * AM_NT + c] = mbit(mc, g) ? z : 0.0;
-            }
+            const double z = acc[rb][cb][v] + cio.bp[li][j];
+            const int g = out0 + j;
+            if (g == ac) s_z[c][0] = z;
+            if (g == bc) s_z[c][1] = z;
+            X[(size_t)j * AM_NT + c] = mbit(mc, g) ? z : 0.0;
           }
         }
       }
     }
     __syncthreads();
   }
-  const double f = s_f[lane];
-  if (wave != 0 || lane >= n) return;
+  // output layer (one row): f = w . X + b, the k range split over NPART row groups
+  {
+    const int li = net.D, nin = net.nodes[li], ld = cio.noutp[li];
+    const int c = tid % AM_NT, part = tid / AM_NT;
+    const double* w = cio.Wp[li];
+    double acc = 0.0;
+    for (int k = part; k < nin; k += NPART) acc = __builtin_fma(w[(size_t)k * ld], X[(size_t)k * AM_NT + c], acc);
+    s_part[part][c] = acc;
+  }
+  __syncthreads();
+  if (tid >= n) return;
+  double f = cio.bp[net.D][0];
+  for (int q = 0; q < NPART; ++q) f += s_part[q][tid];
   // residuals of the signed planes of the vertex's system and the step
-  const int r = s_rk[lane][0], k = s_rk[lane][1];
-  const double v[3] = {s_v[lane][0], s_v[lane][1], s_v[lane][2]};
+  const uint32_t* mk = smask + (size_t)tid * Wd;
+  const int r = s_rk[tid][0], k = s_rk[tid][1];
+  const double v[3] = {s_v[tid][0], s_v[tid][1], s_v[tid][2]};
   double res[3];
   for (int i = 0; i < 2; ++i) {
-    const int g = s_ab[lane][i];
+    const int g = s_ab[tid][i];
     if (g < net.L) {
-      const double z = s_z[lane][i];
+      const double z = s_z[tid][i];
       res[i] = mbit(mk, g) ? -z : z;
     } else {
       const int e = g - net.L;
