@@ -290,7 +290,7 @@ InsertPlan insert_prepare(am_ctx* c, size_t n) {
   // round trip between the count and the commit.
   p.early = (size_t)c->R + n <= c->reg_cap || n <= ((size_t)1 << 22);
   if (p.early) grow_regions(c, (size_t)c->R + n);
-  ensure(c->scan_state, scan_blocks(n) + 1, false, c->stream);
+  ensure(c->scan_state, scan_commit_tiles(n) + 1, false, c->stream);
   arm_host_total(c);
   return p;
 }
@@ -324,7 +324,7 @@ void insert_launch(am_ctx* c, const InsertPlan& p) {
   io.n = (int)n;
   io.Wd = c->Wd;
   io.cap_mask = (uint32_t)(c->table_cap - 1);
-  const size_t tiles = scan_blocks(n);
+  const size_t tiles = scan_commit_tiles(n);
   io.scan_state = c->scan_state.p;
   io.scan_words = (int)tiles + 1;
   hipLaunchKernelGGL(k_insert, dim3(blocks(n, 256)), dim3(256), 0, st, io);
@@ -338,7 +338,7 @@ void insert_launch(am_ctx* c, const InsertPlan& p) {
   sio.tiles = (int)tiles;
   sio.rank = p.early ? nullptr : c->rank.p;
   sio.total = c->host_total_dev;
-  hipLaunchKernelGGL(k_scan_commit, dim3((unsigned)tiles), dim3(kScanThreads), 0, st, sio);
+  hipLaunchKernelGGL(k_scan_commit, dim3((unsigned)tiles), dim3(kScanCommitThreads), 0, st, sio);
   HIPCHK(hipGetLastError());
 }
 

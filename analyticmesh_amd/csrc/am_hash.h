@@ -71,6 +71,10 @@ struct ScanCommitIO {
   int* rank;
   int* total;
 };
+constexpr int kScanCommitThreads = 512;
+constexpr int kScanCommitItems = 16;
+constexpr int kScanCommitTile = kScanCommitThreads * kScanCommitItems;   // candidates per block
+inline size_t scan_commit_tiles(size_t n) { return (n + kScanCommitTile - 1) / kScanCommitTile; }
 __global__ void k_scan_commit(ScanCommitIO io);
 __global__ void k_hash_keys(const uint32_t* masks, int n, int Wd, uint64_t* out);
 __global__ void k_rehash(const uint32_t* reg_mask, int R, int Wd, uint32_t* table, uint32_t cap_mask);

@@ -21,7 +21,7 @@ namespace am {
 // j, j + AM_KG, ... of a key (16-byte quads when the key is a whole number of
 // quads), so one load instruction of a wave reads AM_KG whole keys' segments,
 // coalesced.
-#define AM_KG 8
+#define AM_KG 4
 
 // Whether keys a and b (Wd words) are equal, in every lane of the group; a
 // group that does not compare (cmp false, the same in all its lanes) takes
@@ -131,31 +131,31 @@ __global__ __launch_bounds__(256) void k_insert(InsertIO io) {
 
 // Decoupled look-back scan (Merrill and Garland, "Single-pass parallel prefix
 // scan with decoupled look-back", 2016) over the new-region flags, with the
-// commit: a block takes the next tile of kScanTile candidates in start order
+// commit: a block takes the next tile of kScanCommitTile candidates in start order
 // (a counter), publishes the tile's count, then its first wave adds up the
 // states of the tiles before it, 64 at a time, back to the nearest inclusive
 // prefix, and publishes its own.  A tile publishes its count before it waits
 // and waits only on tiles that started before it, so every wait ends.  The new
 // regions of the tile are then written by groups of AM_KG lanes (mask, start,
 // point, and the table slot now holding the region id).
-__global__ __launch_bounds__(kScanThreads) void k_scan_commit(ScanCommitIO io) {
+__global__ __launch_bounds__(kScanCommitThreads) void k_scan_commit(ScanCommitIO io) {
   __shared__ int s_tile, s_excl;
-  __shared__ int s_wave[kScanThreads / 64];
-  __shared__ int s_list[kScanTile];
+  __shared__ int s_wave[kScanCommitThreads / 64];
+  __shared__ int s_list[kScanCommitTile];
   const int tid = threadIdx.x, lane = tid & 63;
   if (tid == 0) s_tile = (int)atomicAdd(&io.state[io.tiles], 1ull);
   __syncthreads();
   const int t = s_tile;
-  const size_t base = (size_t)t * kScanTile + (size_t)tid * kScanItems;
-  int v[kScanItems];
+  const size_t base = (size_t)t * kScanCommitTile + (size_t)tid * kScanCommitItems;
+  int v[kScanCommitItems];
   int sum = 0;
 #pragma unroll
-  for (int it = 0; it < kScanItems; ++it) {
+  for (int it = 0; it < kScanCommitItems; ++it) {
     v[it] = base + it < (size_t)io.n ? first_flag(io.cio, base + it) : 0;
     sum += v[it];
   }
   int tot;
-  const int off = block_excl_scan(sum, s_wave, &tot);
+  const int off = block_excl_scan<int, kScanCommitThreads>(sum, s_wave, &tot);
   if (tid < 64) {
     constexpr unsigned long long F1 = 1ull << 32, F2 = 2ull << 32;
     if (lane == 0)
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_commit(ScanCommitIO io) {
   const int excl = s_excl;
   int run = off;
 #pragma unroll
-  for (int it = 0; it < kScanItems; ++it) {
+  for (int it = 0; it < kScanCommitItems; ++it) {
     if (v[it]) {
       if (io.rank) io.rank[base + it] = excl + run;
       else s_list[run] = (int)(base + it);
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_commit(ScanCommitIO io) {
   __syncthreads();
   const CommitIO& ci = io.cio;
   const int j = tid & (AM_KG - 1);
-  for (int k = tid / AM_KG; k < tot; k += kScanThreads / AM_KG) {
+  for (int k = tid / AM_KG; k < tot; k += kScanCommitThreads / AM_KG) {
     const int cc = s_list[k];
     const int rid = ci.R0 + excl + k;
     group_copy(ci.reg_mask + (size_t)rid * ci.Wd, ci.cand_mask + (size_t)cc * ci.Wd, ci.Wd, j);

@@ -29,8 +29,9 @@ __device__ __forceinline__ T shfl_up_any(T v, int d) {
   }
 }
 
-// Exclusive scan of v over the block; *total receives the block's sum.
-template <typename T>
+// Exclusive scan of v over the block of NT threads (s_wave: NT / 64 slots);
+// *total receives the block's sum.
+template <typename T, int NT = kScanThreads>
 __device__ __forceinline__ T block_excl_scan(T v, T* s_wave, T* total) {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   T inc = v;
@@ -43,7 +44,7 @@ __device__ __forceinline__ T block_excl_scan(T v, T* s_wave, T* total) {
   __syncthreads();
   T off = 0, tot = 0;
 #pragma unroll
-  for (int w = 0; w < kScanThreads / 64; ++w) {
+  for (int w = 0; w < NT / 64; ++w) {
     const T x = s_wave[w];
     if (w < wid) off += x;
     tot += x;
