@@ -173,6 +173,9 @@ struct am_ctx {
   // pinned, coherent host word the device writes scan totals into (arm_host_total)
   int* host_total = nullptr;
   int* host_total_dev = nullptr;
+  // pinned staging for am_get_mesh into pageable host arrays (grown, kept)
+  unsigned char* mesh_pin = nullptr;
+  size_t mesh_pin_bytes = 0;
 
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
@@ -963,6 +966,37 @@ void batch_hist(am_batch* b, long long first, long long count) {
 // ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
+namespace {
+
+// Whether p is ordinary (pageable) host memory: not device, pinned or managed.
+bool pageable_host(const void* p) {
+  hipPointerAttribute_t a;
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return true;
+  }
+  return a.type == hipMemoryTypeUnregistered;
+}
+
+// n bytes between host buffers, split over up to 8 threads (page faults of a
+// fresh destination and the copy itself then run side by side).
+void par_memcpy(void* dst, const void* src, size_t n) {
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n >> 22));
+  if (nt <= 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const size_t chunk = ((n + nt - 1) / nt + 4095) & ~(size_t)4095;
+  std::vector<std::thread> th;
+  for (size_t a = 0; a < n; a += chunk) {
+    const size_t len = std::min(chunk, n - a);
+    th.emplace_back([=]() { std::memcpy((char*)dst + a, (const char*)src + a, len); });
+  }
+  for (auto& t : th) t.join();
+}
+
+}  // namespace
+
 extern "C" {
 
 am_status am_create(am_ctx** out, int n_nodes, const int* nodes, const int* arc_table, int arc_rows,
@@ -1670,16 +1704,53 @@ am_status am_get_mesh(am_ctx* c, double* vertices, int* face_offsets, int* face_
       if (face_indices) std::memcpy(face_indices, c->mesh_idx.data(), c->mesh_idx.size() * 4);
       return AM_OK;
     }
-    // straight from device memory into the caller's arrays (host or device)
-    if (vertices && c->mesh_V)
-      HIPCHK(hipMemcpy(vertices, c->m_xyz.p, (size_t)c->mesh_V * 3 * 8, hipMemcpyDefault));
+    // Device, pinned or managed caller arrays get the arrays straight from
+    // device memory.  Pageable ones: each array goes by one DMA into a pinned
+    // staging buffer the context keeps, then by a host copy over several
+    // threads (a plain copy into pageable memory stages through small chunks
+    // and one host thread: about 10 GB/s).
+    struct Part {
+      void* dst;
+      const void* src;
+      size_t bytes;
+    };
+    Part parts[3] = {{vertices, c->m_xyz.p, vertices ? (size_t)c->mesh_V * 3 * 8 : 0},
+                     {face_offsets, c->m_face_off.p, face_offsets ? (size_t)c->mesh_F * 4 : 0},
+                     {face_indices, c->m_face_idx.p, face_indices ? (size_t)c->mesh_NI * 4 : 0}};
+    size_t staged = 0;
+    bool stage[3] = {false, false, false};
+    for (int i = 0; i < 3; ++i) {
+      if (!parts[i].bytes) continue;
+      stage[i] = pageable_host(parts[i].dst);
+      if (stage[i]) staged += (parts[i].bytes + 255) & ~(size_t)255;
+      else HIPCHK(hipMemcpy(parts[i].dst, parts[i].src, parts[i].bytes, hipMemcpyDefault));
+    }
+    if (staged) {
+      if (staged > c->mesh_pin_bytes) {
+        if (c->mesh_pin) HIPCHK(hipHostFree(c->mesh_pin));
+        c->mesh_pin = nullptr;
+        c->mesh_pin_bytes = 0;
+        HIPCHK(hipHostMalloc((void**)&c->mesh_pin, staged, hipHostMallocDefault));
+        c->mesh_pin_bytes = staged;
+      }
+      size_t o = 0;
+      for (int i = 0; i < 3; ++i)
+        if (stage[i]) {
+          HIPCHK(hipMemcpyAsync(c->mesh_pin + o, parts[i].src, parts[i].bytes, hipMemcpyDeviceToHost, c->stream));
+          o += (parts[i].bytes + 255) & ~(size_t)255;
+        }
+      HIPCHK(hipStreamSynchronize(c->stream));
+      o = 0;
+      for (int i = 0; i < 3; ++i)
+        if (stage[i]) {
+          par_memcpy(parts[i].dst, c->mesh_pin + o, parts[i].bytes);
+          o += (parts[i].bytes + 255) & ~(size_t)255;
+        }
+    }
     if (face_offsets) {
-      if (c->mesh_F) HIPCHK(hipMemcpy(face_offsets, c->m_face_off.p, (size_t)c->mesh_F * 4, hipMemcpyDefault));
       const int ni = (int)c->mesh_NI;
       HIPCHK(hipMemcpy(face_offsets + c->mesh_F, &ni, 4, hipMemcpyDefault));
     }
-    if (face_indices && c->mesh_NI)
-      HIPCHK(hipMemcpy(face_indices, c->m_face_idx.p, (size_t)c->mesh_NI * 4, hipMemcpyDefault));
     return AM_OK;
   });
 }
@@ -2007,6 +2078,7 @@ am_status am_destroy(am_ctx* c) {
   fast_release(c->fbuf);
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   if (c->host_total) (void)hipHostFree(c->host_total);   // every level waited for its total
+  if (c->mesh_pin) (void)hipHostFree(c->mesh_pin);
   hipStream_t own = c->own_stream;
   delete c;
   if (own) (void)hipStreamDestroy(own);

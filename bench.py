@@ -379,9 +379,10 @@ def main():
                  "achieved": round(tbytes / (ins_ms / 1e3) / 1e9, 1) if ins_ms > 0 else None,
                  "peak": HBM_PEAK_GBS, "unit": "GB/s",
                  "frac": round(tbytes / (ins_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4) if ins_ms > 0 else None,
-                 "traffic": (None if profile_value("hbm_traffic.txt", "k_insert") is None else
+                 "traffic": (None if profile_value("hbm_traffic.txt", "k_insert") is None or
+                             profile_value("hbm_traffic.txt", "k_scan_commit") is None else
                              round((profile_value("hbm_traffic.txt", "k_insert") +
-                                    profile_value("hbm_traffic.txt", "k_commit")) * h["regions"], 1))}
+                                    profile_value("hbm_traffic.txt", "k_scan_commit")) * h["regions"], 1))}
 
     line = {
         "metric": "mesh faces/s, 4x256 ReLU MLP: plain FP32 vs split-BF16 MFMA, same topology",
