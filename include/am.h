@@ -154,7 +154,13 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * regions' are re-solved; the split-BF16 form always re-solves),
  * "adjoint_tiled" (1 default: the
  * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
- * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU). */
+ * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU),
+ * "chain_mfma" (1 default: the FP64 plane chain of the forms that certify
+ * their decisions -- the FP64 ray walk, the reduced forms' FP64 redo, the
+ * batch engine's ray walk -- runs on the FP64 matrix cores, planes equal to
+ * the sequential-fma chain's up to the rounding of another summation order,
+ * same topology; 0: the sequential-fma chain everywhere.  The bit-exact FP64
+ * form always uses the sequential-fma chain). */
 am_status am_set_option(am_ctx* ctx, const char* key, long long value);
 
 /* Timings and counters of the last am_run/am_combine; see AM_STAT_* indices. */
