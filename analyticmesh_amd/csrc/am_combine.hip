@@ -16,8 +16,9 @@
 //   2. k_cinsert, sixteen lanes per region: they read the region's mask once
 //      (coalesced); the key hash is a sum of per-word hashes, so each item's
 //      hash is the region's sum with the one or two changed words swapped out.
-//      Items claim a table slot by CAS; an item whose key is already present
-//      lowers the slot to the smallest item of that key (atomicMin).
+//      Items (four lanes each) claim a table slot by CAS; an item whose key is
+//      already present lowers the slot to the smallest item of that key
+//      (atomicMin).
 //   3. first-occurrence flags, scanned -> vertex ranks and the vertex count.
 //   4. k_cassign, one thread per item: vertex index into the face-index array,
 //      coordinates by the first item, face offsets by each face's first item.
@@ -63,16 +64,23 @@ struct FlagSrc {
 
 // Sixteen lanes per region, sixteen regions per 256-thread block: the lanes of
 // a region read its mask once (coalesced) into LDS and sum the per-word
-// hashes; each lane then takes items lane, lane + 16, ... of the region.
+// hashes.  The region's items then go four at a time, four lanes per item: the
+// item's first lane does the atomics, and a key compare reads the other
+// region's mask as the four lanes' 16-byte quads (or words), one ballot
+// deciding equality.  Every lane of a wave stays in the probe loops, whose
+// trip counts are wave-uniform, so the ballots and shuffles see whole groups.
 #define AM_CG 16
+#define AM_CK 4
 __global__ __launch_bounds__(256) void k_cinsert(CombineIn in, CombineWork w) {
   __shared__ uint32_t s_mask[256 / AM_CG][AM_MAX_WORDS];
+  const int lane = threadIdx.x & 63;
   const int g = threadIdx.x & (AM_CG - 1), grp = threadIdx.x / AM_CG;
   const int r = blockIdx.x * (256 / AM_CG) + grp;
   const bool live = r < in.R;
   const int nv = live ? in.info[(size_t)r * AM_INFO_STRIDE + 2] : 0;
+  const bool has = live && nv >= 3;
   uint64_t hs = 0;
-  if (live && nv >= 3)
+  if (has)
     for (int q = g; q < in.Wd; q += AM_CG) {
       const uint32_t x = in.reg_mask[(size_t)r * in.Wd + q];
       s_mask[grp][q] = x;
@@ -86,50 +94,89 @@ __global__ __launch_bounds__(256) void k_cinsert(CombineIn in, CombineWork w) {
     hs += ((uint64_t)hi << 32) | lo;
   }
   __syncthreads();
-  if (!live || nv < 3) return;
   const uint32_t* m = s_mask[grp];
-  const int item0 = (int)(w.roff[r] & 0xffffffffu);
-  for (int k = g; k < nv; k += AM_CG) {
+  const int item0 = has ? (int)(w.roff[r] & 0xffffffffu) : 0;
+  const int j4 = g & (AM_CK - 1), sg = g / AM_CK, lead = lane & ~(AM_CK - 1);
+  const bool quads = (in.Wd & 3) == 0;
+  for (int k0 = 0; __ballot(has && k0 < nv) != 0ull; k0 += AM_CG / AM_CK) {
+    const int k = k0 + sg;
+    const bool act = has && k < nv;
     const int item = item0 + k;
-    const int a = in.vpair[((size_t)r * in.vm + k) * 2 + 0], b = in.vpair[((size_t)r * in.vm + k) * 2 + 1];
-    const int lo = a < b ? a : b, hi = a < b ? b : a;
-    // key hash: the region's word-hash sum with the words holding bits lo and hi cleared
-    uint64_t h = hs;
-    const bool lo_in = lo >= 0 && lo < in.L, hi_in = hi >= 0 && hi < in.L;
-    const int wl = lo_in ? lo >> 5 : -1, wh = hi_in ? hi >> 5 : -1;
-    const uint32_t bl = lo_in ? 1u << (lo & 31) : 0u, bh = hi_in ? 1u << (hi & 31) : 0u;
-    if (wl >= 0) {
-      uint32_t x = m[wl] & ~bl;
-      if (wh == wl) x &= ~bh;
-      h += word_hash(x, wl) - word_hash(m[wl], wl);
+    int lo = 0, hi = 0, wl = -1, wh = -1;
+    uint32_t bl = 0u, bh = 0u, s = 0u;
+    if (act) {
+      const int a = in.vpair[((size_t)r * in.vm + k) * 2 + 0], b = in.vpair[((size_t)r * in.vm + k) * 2 + 1];
+      lo = a < b ? a : b;
+      hi = a < b ? b : a;
+      // key hash: the region's word-hash sum with the words holding bits lo and hi cleared
+      uint64_t h = hs;
+      const bool lo_in = lo >= 0 && lo < in.L, hi_in = hi >= 0 && hi < in.L;
+      wl = lo_in ? lo >> 5 : -1;
+      wh = hi_in ? hi >> 5 : -1;
+      bl = lo_in ? 1u << (lo & 31) : 0u;
+      bh = hi_in ? 1u << (hi & 31) : 0u;
+      if (wl >= 0) {
+        uint32_t x = m[wl] & ~bl;
+        if (wh == wl) x &= ~bh;
+        h += word_hash(x, wl) - word_hash(m[wl], wl);
+      }
+      if (wh >= 0 && wh != wl) h += word_hash(m[wh] & ~bh, wh) - word_hash(m[wh], wh);
+      h = mix64(h ^ pair_hash(lo, hi));
+      s = (uint32_t)h & w.cap_mask;
     }
-    if (wh >= 0 && wh != wl) h += word_hash(m[wh] & ~bh, wh) - word_hash(m[wh], wh);
-    h = mix64(h ^ pair_hash(lo, hi));
-    uint32_t s = (uint32_t)h & w.cap_mask;
-    for (uint32_t probes = 0; probes <= w.cap_mask; ++probes) {
-      const uint32_t prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
-      if (prev == AM_CEMPTY) break;
-      // compare keys with the slot's item (records are immutable here; any item
-      // the slot holds has the key of the item that claimed it)
-      const int r2 = w.item_region[prev];
-      const int k2 = (int)prev - (int)(w.roff[r2] & 0xffffffffu);
-      const int a2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 0], b2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 1];
-      if ((a2 < b2 ? a2 : b2) == lo && (a2 < b2 ? b2 : a2) == hi) {
-        // the pairs agree, so bits lo and hi are the same words in both keys
-        const uint32_t* m2 = in.reg_mask + (size_t)r2 * in.Wd;
-        uint32_t diff = 0;
-        for (int q = 0; q < in.Wd; ++q) {
-          const uint32_t clr = (q == wl ? bl : 0u) | (q == wh ? bh : 0u);
-          diff |= (m[q] ^ m2[q]) & ~clr;
-        }
-        if (diff == 0) {
-          atomicMin(&w.table[s], (uint32_t)item);
-          break;
+    bool done = !act;
+    for (uint32_t probes = 0; __ballot(!done) != 0ull; ++probes) {
+      uint32_t prev = 0u;
+      if (!done && j4 == 0) prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
+      prev = (uint32_t)__shfl((int)prev, lead, 64);
+      bool cmp = false;
+      const uint32_t* m2 = in.reg_mask;
+      if (!done) {
+        if (prev == AM_CEMPTY) {
+          done = true;
+        } else {
+          // compare keys with the slot's item (records are immutable here; any
+          // item the slot holds has the key of the item that claimed it)
+          const int r2 = w.item_region[prev];
+          const int k2 = (int)prev - (int)(w.roff[r2] & 0xffffffffu);
+          const int a2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 0], b2 = in.vpair[((size_t)r2 * in.vm + k2) * 2 + 1];
+          // when the pairs agree, bits lo and hi are the same words in both keys
+          cmp = (a2 < b2 ? a2 : b2) == lo && (a2 < b2 ? b2 : a2) == hi;
+          m2 = in.reg_mask + (size_t)r2 * in.Wd;
         }
       }
-      s = (s + 1) & w.cap_mask;
+      uint32_t diff = 0u;
+      if (cmp) {
+        if (quads) {
+          const uint4* m24 = reinterpret_cast<const uint4*>(m2);
+          for (int q4 = j4; q4 < (in.Wd >> 2); q4 += AM_CK) {
+            const uint4 y = m24[q4];
+            const uint32_t yv[4] = {y.x, y.y, y.z, y.w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              const int q = 4 * q4 + e;
+              const uint32_t clr = (q == wl ? bl : 0u) | (q == wh ? bh : 0u);
+              diff |= (m[q] ^ yv[e]) & ~clr;
+            }
+          }
+        } else {
+          for (int q = j4; q < in.Wd; q += AM_CK) {
+            const uint32_t clr = (q == wl ? bl : 0u) | (q == wh ? bh : 0u);
+            diff |= (m[q] ^ m2[q]) & ~clr;
+          }
+        }
+      }
+      const unsigned long long bal = __ballot(diff != 0u);
+      const bool eq = ((bal >> lead) & ((1ull << AM_CK) - 1)) == 0ull;
+      if (cmp && eq) {
+        if (j4 == 0) atomicMin(&w.table[s], (uint32_t)item);
+        done = true;
+      } else if (!done) {
+        if (probes >= w.cap_mask) done = true;   // table full: sized so that it never is
+        else s = (s + 1) & w.cap_mask;
+      }
     }
-    w.slot[item] = (int)s;
+    if (act && j4 == 0) w.slot[item] = (int)s;
   }
 }
 
