@@ -72,7 +72,7 @@ struct ScanCommitIO {
   int* total;
 };
 constexpr int kScanCommitThreads = 512;
-constexpr int kScanCommitItems = 16;
+constexpr int kScanCommitItems = 8;
 constexpr int kScanCommitTile = kScanCommitThreads * kScanCommitItems;   // candidates per block
 inline size_t scan_commit_tiles(size_t n) { return (n + kScanCommitTile - 1) / kScanCommitTile; }
 __global__ void k_scan_commit(ScanCommitIO io);

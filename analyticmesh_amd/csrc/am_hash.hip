@@ -202,16 +202,70 @@ __global__ __launch_bounds__(kScanCommitThreads) void k_scan_commit(ScanCommitIO
   if (io.rank) return;
   __syncthreads();
   const CommitIO& ci = io.cio;
-  const int j = tid & (AM_KG - 1);
-  for (int k = tid / AM_KG; k < tot; k += kScanCommitThreads / AM_KG) {
-    const int cc = s_list[k];
-    const int rid = ci.R0 + excl + k;
-    group_copy(ci.reg_mask + (size_t)rid * ci.Wd, ci.cand_mask + (size_t)cc * ci.Wd, ci.Wd, j);
-    if (j == 0) {
-      ci.reg_start[rid] = ci.cand_start[cc];
-      ci.table[ci.slot[cc]] = (uint32_t)rid;
-    } else if (j <= 3) {
-      ci.reg_point[(size_t)rid * 3 + j - 1] = ci.cand_point[(size_t)cc * 3 + j - 1];
+  const int j = tid & (AM_KG - 1), gid = tid / AM_KG;
+  constexpr int NG = kScanCommitThreads / AM_KG;   // groups of the block
+  if ((ci.Wd & 3) == 0) {
+    // U regions per group at a time, all their loads before any store
+    constexpr int U = 4, NQ = AM_MAX_WORDS / 4 / AM_KG;
+    const int nq = ci.Wd >> 2;
+    const uint4* src4 = reinterpret_cast<const uint4*>(ci.cand_mask);
+    uint4* dst4 = reinterpret_cast<uint4*>(ci.reg_mask);
+    for (int k0 = gid; k0 < tot; k0 += NG * U) {
+      uint4 mv[U][NQ];
+      int cc[U], sv[U], sl[U];
+      double pv[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int k = k0 + u * NG;
+        cc[u] = k < tot ? s_list[k] : -1;
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) mv[u][t] = uint4{0u, 0u, 0u, 0u};
+        sv[u] = 0;
+        sl[u] = 0;
+        pv[u] = 0.0;
+        if (cc[u] >= 0) {
+#pragma unroll
+          for (int t = 0; t < NQ; ++t) {
+            const int q = j + t * AM_KG;
+            mv[u][t] = q < nq ? src4[(size_t)cc[u] * nq + q] : uint4{0u, 0u, 0u, 0u};
+          }
+          if (j == 0) {
+            sv[u] = ci.cand_start[cc[u]];
+            sl[u] = ci.slot[cc[u]];
+          } else if (j <= 3) {
+            pv[u] = ci.cand_point[(size_t)cc[u] * 3 + j - 1];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (cc[u] >= 0) {
+          const int rid = ci.R0 + excl + k0 + u * NG;
+#pragma unroll
+          for (int t = 0; t < NQ; ++t) {
+            const int q = j + t * AM_KG;
+            if (q < nq) dst4[(size_t)rid * nq + q] = mv[u][t];
+          }
+          if (j == 0) {
+            ci.reg_start[rid] = sv[u];
+            ci.table[sl[u]] = (uint32_t)rid;
+          } else if (j <= 3) {
+            ci.reg_point[(size_t)rid * 3 + j - 1] = pv[u];
+          }
+        }
+      }
+    }
+  } else {
+    for (int k = gid; k < tot; k += NG) {
+      const int cc = s_list[k];
+      const int rid = ci.R0 + excl + k;
+      group_copy(ci.reg_mask + (size_t)rid * ci.Wd, ci.cand_mask + (size_t)cc * ci.Wd, ci.Wd, j);
+      if (j == 0) {
+        ci.reg_start[rid] = ci.cand_start[cc];
+        ci.table[ci.slot[cc]] = (uint32_t)rid;
+      } else if (j <= 3) {
+        ci.reg_point[(size_t)rid * 3 + j - 1] = ci.cand_point[(size_t)cc * 3 + j - 1];
+      }
     }
   }
 }
