@@ -30,6 +30,12 @@ namespace am {
 
 typedef double am_f64x4 __attribute__((ext_vector_type(4)));
 
+// Element (k, c) of a layer's input in LDS: rows of AM_NT doubles, the odd
+// rows' halves swapped.  A 64-bit LDS read serves 32 lanes per cycle; the
+// product's B reads put rows k and k + 1 (k even) in one such half-wave, which
+// the swap sends to different banks; every other access walks along a row.
+__device__ __forceinline__ int xat(int k, int c) { return k * AM_NT + (c ^ ((k & 1) << 4)); }
+
 __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, ChainIO cio, NewtonIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* X = reinterpret_cast<double*>(smem_raw);                             // [AM_CHAIN_MAXW][AM_NT]
@@ -82,7 +88,7 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
                                      __builtin_fma(net.W0[j * 3 + 0], x0, net.b[0][j])));
       if (j == ca) s_z[c][0] = z;
       if (j == cb) s_z[c][1] = z;
-      X[(size_t)j * AM_NT + c] = mbit(mk, j) ? z : 0.0;
+      X[xat(j, c)] = mbit(mk, j) ? z : 0.0;
     }
   }
   __syncthreads();
@@ -120,8 +126,8 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
           const int kk = k0 + 4 * u;
           if (kk < nk) {
             const int kr = kk + l4;
-            const double b0 = kr < nin ? X[(size_t)kr * AM_NT + l16] : 0.0;
-            const double b1 = kr < nin ? X[(size_t)kr * AM_NT + 16 + l16] : 0.0;
+            const double b0 = kr < nin ? X[xat(kr, l16)] : 0.0;
+            const double b1 = kr < nin ? X[xat(kr, 16 + l16)] : 0.0;
             const double x0 = a0[u], x1 = a1[u];
             const int kn = kk + 4 * PF;
             a0[u] = kn < nk ? wa[(size_t)kn * ld] : 0.0;
@@ -154,7 +160,7 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
             const int g = out0 + j;
             if (g == ac) s_z[c][0] = z;
             if (g == bc) s_z[c][1] = z;
-            X[(size_t)j * AM_NT + c] = mbit(mc, g) ? z : 0.0;
+            X[xat(j, c)] = mbit(mc, g) ? z : 0.0;
           }
         }
       }
@@ -167,7 +173,7 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
     const int c = tid % AM_NT, part = tid / AM_NT;
     const double* w = cio.Wp[li];
     double acc = 0.0;
-    for (int k = part; k < nin; k += NPART) acc = __builtin_fma(w[(size_t)k * ld], X[(size_t)k * AM_NT + c], acc);
+    for (int k = part; k < nin; k += NPART) acc = __builtin_fma(w[(size_t)k * ld], X[xat(k, c)], acc);
     s_part[part][c] = acc;
   }
   __syncthreads();
