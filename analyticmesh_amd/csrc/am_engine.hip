@@ -1116,6 +1116,9 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
   } else if (k == "walk") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "walk must be 0 (ray) or 1 (scan)");
     c->fast.walk_mode = (int)value;
+  } else if (k == "ray_waves_reduced") {
+    if (value != 2 && value != 4 && value != 8) return fail(c, AM_ERR_ARG, "ray_waves_reduced must be 2, 4 or 8");
+    c->fast.ray_waves = (int)value;
   } else if (k == "chain_mfma") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "chain_mfma must be 0 or 1");
     c->chain_mfma = value != 0;

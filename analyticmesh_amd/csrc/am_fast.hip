@@ -457,9 +457,11 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
                            with_sigma ? Pe : nullptr, with_sigma ? eb[4] : 0.f);
 }
 
-// Ray-shooting FP32 walk: LDS holds only the planes and the mask.
-__global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
-                                                     const float* eqbuf, int Cs_hbm, int chunk_first) {
+// Ray-shooting FP32 walk: LDS holds only the planes and the mask; NW waves
+// per region.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
+                                                          const float* eqbuf, int Cs_hbm, int chunk_first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
   const int Cs = io.Cs;
@@ -475,21 +477,21 @@ __global__ __launch_bounds__(64 * AM_RAY_WAVES) void k_walk_ray_f32(NetDev net, 
   const int rid = chunk_first + b;
   const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
   const bool with_sigma = wp.lam > 0.f;
-  for (int g = lane; g < C; g += 64 * AM_RAY_WAVES) {
+  for (int g = lane; g < C; g += 64 * NW) {
     Px[g] = src[g];
     Py[g] = src[Cs_hbm + g];
     Pz[g] = src[2 * Cs_hbm + g];
     Pc[g] = src[3 * Cs_hbm + g];
     if (with_sigma) Pe[g] = src[4 * Cs_hbm + g];
   }
-  for (int w = lane; w < net.Wd; w += 64 * AM_RAY_WAVES) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
+  for (int w = lane; w < net.Wd; w += 64 * NW) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const float* eb = eqbuf + (size_t)b * AM_EQS;
   const float eq[4] = {eb[0], eb[1], eb[2], eb[3]};
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
   const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask, with_sigma ? Pe : nullptr};
-  walk_ray<float, AM_RAY_WAVES>(net, io, wp, rid, rid - io.first, P, smask, eq, p, with_sigma ? eb[4] : 0.f);
+  walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, with_sigma ? eb[4] : 0.f);
 }
 
 // Regions the walk left undecided, added to cnt[0], their reasons, added to
@@ -712,8 +714,11 @@ void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& 
   if (wlds > 65536)
     FCHK(hipFuncSetAttribute((const void*)k_walk_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wlds));
   const size_t rlds = (size_t)5 * CsL * 4 + (size_t)net.Wd * 4 + 64;
-  if (rlds > 65536)
-    FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
+  if (rlds > 65536) {
+    FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
+    FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
+    FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
+  }
   const size_t glds = gemm_lds(d, prec);
 
   for (int c0 = 0; c0 < lvl.count; c0 += CH) {
@@ -752,8 +757,14 @@ void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& 
     io.Csort = Csort;
     io.flags = flags;
     io.ids = nullptr;
-    if (f.walk_mode == 0)
-      hipLaunchKernelGGL(k_walk_ray_f32, dim3(n), dim3(64 * AM_RAY_WAVES), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
+    if (f.walk_mode == 0 && f.ray_waves == 2)
+      hipLaunchKernelGGL(k_walk_ray_f32<2>, dim3(n), dim3(128), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
+                         lvl.first + c0);
+    else if (f.walk_mode == 0 && f.ray_waves == 8)
+      hipLaunchKernelGGL(k_walk_ray_f32<8>, dim3(n), dim3(512), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
+                         lvl.first + c0);
+    else if (f.walk_mode == 0)
+      hipLaunchKernelGGL(k_walk_ray_f32<4>, dim3(n), dim3(256), rlds, st, net, io, f.wp, planes, eqb, d.Cs,
                          lvl.first + c0);
     else
       hipLaunchKernelGGL(k_walk_f32, dim3(n), dim3(64), wlds, st, net, io, f.wp, planes, eqb, d.Cs,

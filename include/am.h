@@ -145,7 +145,9 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * level's launches, FP64 forms, are captured into a hipGraph and launched as
  * one; the executable graph is updated in place from level to level),
  * "walk" (0 ray walk, 1
- * candidate scan, for the reduced forms), "reduced_vertices" (FP32 form: 2
+ * candidate scan, for the reduced forms), "ray_waves_reduced" (waves per
+ * region of the reduced forms' ray walk: 2, 4 (default) or 8; 4 measured
+ * fastest on the 4x256 net, 2 and 8 give the same records), "reduced_vertices" (FP32 form: 2
  * default, each vertex solved in FP64 from the reduced planes takes one FP64
  * Newton step after the march, its residuals from an FP64 forward pass of the
  * network under its region's mask; 0 every vertex of a region the reduced walk
