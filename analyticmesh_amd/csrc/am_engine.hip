@@ -158,6 +158,10 @@ struct am_ctx {
   // offsets and first-occurrence flags) and the mesh in device memory
   DevBuf<char> cw_reg, cw_item;
   CombineWork cwork;
+  // cwork holds the vertex dedup of the current region store (the FP32 form's
+  // Newton step makes it; am_combine reuses it): faces and vertex uses
+  bool dedup_ready = false;
+  long long dedup_F = 0, dedup_N = 0;
   DevBuf<int> m_face_off, m_face_idx;
   DevBuf<double> m_xyz;
   long long mesh_V = 0, mesh_F = 0, mesh_NI = 0;
@@ -1095,6 +1099,7 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
       c->reg_tab.release(); c->reg_verts.release(); c->reg_vpair.release(); c->reg_jinv.release();
       c->reg_cap = 0;
       c->R = 0;
+      c->dedup_ready = false;
     }
     c->vm = (int)value;
   } else if (k == "table_log2") {
@@ -1165,6 +1170,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     for (int i = 0; i < AM_STAT_COUNT; ++i) c->stats[i] = 0;
     c->has_run = false;
     c->has_mesh = false;
+    c->dedup_ready = false;
 
     // ---- network upload (fp64, weights transposed for the chain)
     const int nl = c->D + 1;
@@ -1528,6 +1534,9 @@ long long newton_refine(am_ctx* c) {
   ensure(c->cw_item, combine_item_bytes(N), false, st);
   combine_carve_items(c->cw_item.p, N, w);
   combine_dedup(in, w, N, st);
+  c->dedup_ready = true;
+  c->dedup_F = F;
+  c->dedup_N = N;
   ensure(c->redo_ids, (size_t)N, false, st);
   ensure(c->redo_cnt, 1, false, st);
   ensure(c->scan_part, scan_blocks((size_t)N) + 1, false, st);
@@ -1579,16 +1588,28 @@ am_status am_combine(am_ctx* c, double scale, const double* center3) {
     }
     const CombineIn in = combine_in(c);
     CombineWork& w = c->cwork;
-    w = CombineWork{};
-    ensure(c->cw_reg, combine_region_bytes((long long)R), false, st);
-    combine_carve_regions(c->cw_reg.p, (long long)R, w);
     long long F = 0, N = 0;
-    combine_counts(c, in, w, F, N);
+    if (c->dedup_ready) {
+      // the dedup of this region store is already in cwork (vertex
+      // coordinates, which the Newton step refined, are read by the assign)
+      F = c->dedup_F;
+      N = c->dedup_N;
+    } else {
+      w = CombineWork{};
+      ensure(c->cw_reg, combine_region_bytes((long long)R), false, st);
+      combine_carve_regions(c->cw_reg.p, (long long)R, w);
+      combine_counts(c, in, w, F, N);
+    }
     long long V = 0;
     if (N > 0) {
-      ensure(c->cw_item, combine_item_bytes(N), false, st);
-      combine_carve_items(c->cw_item.p, N, w);
-      combine_dedup(in, w, N, st);
+      if (!c->dedup_ready) {
+        ensure(c->cw_item, combine_item_bytes(N), false, st);
+        combine_carve_items(c->cw_item.p, N, w);
+        combine_dedup(in, w, N, st);
+        c->dedup_ready = true;
+        c->dedup_F = F;
+        c->dedup_N = N;
+      }
       V = combine_vertices(c, w);
       ensure(c->m_xyz, (size_t)std::max<long long>(V * 3, 1), false, st);
       ensure(c->m_face_off, (size_t)F + 1, false, st);
@@ -2197,6 +2218,7 @@ am_status am_batch_run(am_batch* b, const void* const* weights, const void* cons
     for (double& x : c->stats) x = 0;
     c->has_run = false;
     c->has_mesh = false;
+    c->dedup_ready = false;
     c->stage_used = 0;
     const size_t esz = dtype == AM_DTYPE_F64 ? 8 : 4;
     const int Ch = (b->Cmax + 63) & ~63;
