@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <atomic>
 #include <thread>
 #include <climits>
 #include <cstdio>
@@ -180,6 +181,7 @@ struct am_ctx {
   // pinned staging for am_get_mesh into pageable host arrays (grown, kept)
   unsigned char* mesh_pin = nullptr;
   size_t mesh_pin_bytes = 0;
+  std::vector<hipEvent_t> mesh_ev;   // one per 2 MB chunk of the staged copy (created once, kept)
 
   // tiled FP64 chain (nets without skips, layers up to AM_CHAIN_MAXW wide)
   bool tile_ok = false;
@@ -982,23 +984,6 @@ bool pageable_host(const void* p) {
   return a.type == hipMemoryTypeUnregistered;
 }
 
-// n bytes between host buffers, split over up to 8 threads (page faults of a
-// fresh destination and the copy itself then run side by side).
-void par_memcpy(void* dst, const void* src, size_t n) {
-  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n >> 22));
-  if (nt <= 1) {
-    std::memcpy(dst, src, n);
-    return;
-  }
-  const size_t chunk = ((n + nt - 1) / nt + 4095) & ~(size_t)4095;
-  std::vector<std::thread> th;
-  for (size_t a = 0; a < n; a += chunk) {
-    const size_t len = std::min(chunk, n - a);
-    th.emplace_back([=]() { std::memcpy((char*)dst + a, (const char*)src + a, len); });
-  }
-  for (auto& t : th) t.join();
-}
-
 }  // namespace
 
 extern "C" {
@@ -1757,19 +1742,49 @@ am_status am_get_mesh(am_ctx* c, double* vertices, int* face_offsets, int* face_
         HIPCHK(hipHostMalloc((void**)&c->mesh_pin, staged, hipHostMallocDefault));
         c->mesh_pin_bytes = staged;
       }
+      // chunks of 2 MB, each DMA followed by an event; host threads copy a
+      // chunk out as soon as its event has passed, so the copies overlap the
+      // later DMAs
+      struct Chunk {
+        void* dst;
+        const unsigned char* src;
+        size_t len;
+      };
+      std::vector<Chunk> chunks;
+      const size_t kChunk = (size_t)2 << 20;
       size_t o = 0;
       for (int i = 0; i < 3; ++i)
         if (stage[i]) {
-          HIPCHK(hipMemcpyAsync(c->mesh_pin + o, parts[i].src, parts[i].bytes, hipMemcpyDeviceToHost, c->stream));
+          for (size_t a = 0; a < parts[i].bytes; a += kChunk) {
+            const size_t len = std::min(kChunk, parts[i].bytes - a);
+            if (chunks.size() == c->mesh_ev.size()) {
+              hipEvent_t e;
+              HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+              c->mesh_ev.push_back(e);
+            }
+            HIPCHK(hipMemcpyAsync(c->mesh_pin + o + a, (const unsigned char*)parts[i].src + a, len,
+                                  hipMemcpyDeviceToHost, c->stream));
+            HIPCHK(hipEventRecord(c->mesh_ev[chunks.size()], c->stream));
+            chunks.push_back(Chunk{(unsigned char*)parts[i].dst + a, c->mesh_pin + o + a, len});
+          }
           o += (parts[i].bytes + 255) & ~(size_t)255;
         }
-      HIPCHK(hipStreamSynchronize(c->stream));
-      o = 0;
-      for (int i = 0; i < 3; ++i)
-        if (stage[i]) {
-          par_memcpy(parts[i].dst, c->mesh_pin + o, parts[i].bytes);
-          o += (parts[i].bytes + 255) & ~(size_t)255;
+      const int nt = (int)std::min<size_t>(8, chunks.size());
+      std::atomic<int> bad{0};
+      auto work = [&](int id) {
+        for (size_t k = id; k < chunks.size(); k += nt) {
+          if (hipEventSynchronize(c->mesh_ev[k]) != hipSuccess) {
+            bad = 1;
+            return;
+          }
+          std::memcpy(chunks[k].dst, chunks[k].src, chunks[k].len);
         }
+      };
+      std::vector<std::thread> th;
+      for (int id = 1; id < nt; ++id) th.emplace_back(work, id);
+      work(0);
+      for (auto& t : th) t.join();
+      if (bad) HIPCHK(hipStreamSynchronize(c->stream));   // reports the failed copy
     }
     if (face_offsets) {
       const int ni = (int)c->mesh_NI;
@@ -2103,6 +2118,7 @@ am_status am_destroy(am_ctx* c) {
   if (c->own_stream) (void)hipStreamSynchronize(c->own_stream);
   if (c->host_total) (void)hipHostFree(c->host_total);   // every level waited for its total
   if (c->mesh_pin) (void)hipHostFree(c->mesh_pin);
+  for (hipEvent_t e : c->mesh_ev) (void)hipEventDestroy(e);
   hipStream_t own = c->own_stream;
   delete c;
   if (own) (void)hipStreamDestroy(own);
