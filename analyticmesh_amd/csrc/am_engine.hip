@@ -1348,11 +1348,9 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         lvl_count = added;
         continue;
       }
-      if (fp64_form) {
-        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr,
-                         c->precision == AM_PREC_FP64 ? 0 : 1, f64);
-        launches += 1;
-      } else {
+      // the reduced forms' level: face kernel and checked walk, then the FP64
+      // redo of the regions the walk left to FP64
+      auto fast_level_of = [&]() {
         FastLevel fl;
         fl.first = (int)lvl_first;
         fl.count = (int)lvl_count;
@@ -1362,6 +1360,42 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fl.cand_point = c->cand_point.p;
         fl.cand_hash = c->cand_hash.p;
         fl.reg_jinv = c->jinv_on ? c->reg_jinv.p : nullptr;
+        return fl;
+      };
+      if (!fp64_form && c->graph_levels) {
+        // as the FP64 forms' graph above: the host work first (buffers, the
+        // region store's growth), then every launch of the level in one graph
+        if (c->tile_ok) fp64_prepare(c, (int)lvl_count);
+        fast_level_prepare(c->fast, c->fbuf, n, (int)lvl_count);
+        ensure(c->redo_ids, (size_t)lvl_count, false, st);
+        ensure(c->redo_cnt, 1, false, st);
+        ensure(c->scan_part, scan_blocks((size_t)lvl_count) + 1, false, st);
+        const InsertPlan plan = insert_prepare(c, ncand);
+        const FastLevel fl = fast_level_of();
+        FastTimes ft{};
+        launch_as_graph(c, (int)levels, [&]() {
+          fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT, true);
+          scan_launch<int>(RedoSrc{c->rflags.p}, (size_t)lvl_count, RedoSink{c->redo_ids.p, (int)lvl_first},
+                           c->scan_part.p, c->redo_cnt.p, st);
+          run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, c->redo_ids.p, 2, f64, -1,
+                           c->redo_cnt.p);
+          insert_launch(c, plan);
+        });
+        const long long before = c->R;
+        const int added = insert_finish(c, plan);
+        launches += ft.launches;
+        cands += (long long)ncand;
+        levels += 1;
+        lvl_first = before;
+        lvl_count = added;
+        continue;
+      }
+      if (fp64_form) {
+        run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, nullptr,
+                         c->precision == AM_PREC_FP64 ? 0 : 1, f64);
+        launches += 1;
+      } else {
+        const FastLevel fl = fast_level_of();
         FastTimes ft{};
         fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT);
         // FP64 planes of the regions the walk left to FP64 only (their list
@@ -1439,7 +1473,7 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     c->stats[AM_STAT_SETUP_MS] = std::chrono::duration<double, std::milli>(t_setup - t0).count();
     c->stats[AM_STAT_FP64_ONLY] = reduced_asked && fp64_form ? 1.0 : 0.0;
     c->stats[AM_STAT_HOST_WAITS] = (double)(levels + 1);
-    c->stats[AM_STAT_GRAPH_LAUNCHES] = c->graph_levels && fp64_form ? (double)levels : 0.0;
+    c->stats[AM_STAT_GRAPH_LAUNCHES] = c->graph_levels ? (double)levels : 0.0;
     c->stats[AM_STAT_NEWTON_MS] = T[T_NEWTON];
     c->stats[AM_STAT_NEWTON_VERTICES] = (double)newton_n;
     for (int w = 0; w < AM_WHY_COUNT && AM_STAT_WHY0 + w < AM_STAT_COUNT; ++w) c->stats[AM_STAT_WHY0 + w] = (double)why[w];

@@ -89,8 +89,12 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
 // region's records).  The undecided regions are added to acc[0] and their
 // reasons to acc[1 + w] on the device; nothing here waits.
 // Kernel times go to `timer` (T_GEMM, T_WALK).
+// With `prepared`, fast_level_prepare(f, b, net, lvl.count) has run before
+// (then fast_level only enqueues launches: it can be captured into a graph).
 void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc);
+                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc, bool prepared = false);
+// Buffers and launch attributes of fast_level for a level of `count` regions.
+void fast_level_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int count);
 // Planes of n given regions (their masks in `reg_mask`) with the prepared
 // reduced-precision form: planes [n][AM_PCOMP][Cs] f32 (x, y, z, c signed, then
 // each plane's error estimate sigma), eq [n][AM_EQS] f32 (x, y, z, c, sigma).

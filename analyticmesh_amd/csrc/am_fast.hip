@@ -697,28 +697,50 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   f.ready = true;
 }
 
-void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc) {
-  const FastNetDev& d = f.dev;
-  const int prec = f.precision == 1 ? 1 : 0;
-  t.launches = 0;
-  const int CH = 1 << 16;
-  const int chunk = std::min(lvl.count, CH);
-  float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * AM_PCOMP * d.Cs * 4);
-  float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * AM_EQS * 4);
-
+namespace {
+constexpr int kLevelChunk = 1 << 16;   // regions per GEMM + walk launch pair
+int walk_sort_size(const NetDev& net) {
   int Csort = AM_TOPK;   // the walk's top-64 order lives in keys/order[0, 64)
   while (Csort < net.C) Csort <<= 1;
-  const int CsL = net.C;
-  const size_t wlds = (size_t)5 * CsL * 4 + (size_t)Csort * 8 + (size_t)((net.C + 31) / 32) * 4 + (size_t)net.Wd * 4 + 64;
+  return Csort;
+}
+size_t walk_scan_lds(const NetDev& net) {
+  return (size_t)5 * net.C * 4 + (size_t)walk_sort_size(net) * 8 + (size_t)((net.C + 31) / 32) * 4 +
+         (size_t)net.Wd * 4 + 64;
+}
+size_t walk_ray_lds(const NetDev& net) { return (size_t)5 * net.C * 4 + (size_t)net.Wd * 4 + 64; }
+}  // namespace
+
+void fast_level_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int count) {
+  const FastNetDev& d = f.dev;
+  const int chunk = std::min(count, kLevelChunk);
+  (void)slot(b, S_PLANES, (size_t)chunk * AM_PCOMP * d.Cs * 4);
+  (void)slot(b, S_EQ, (size_t)chunk * AM_EQS * 4);
+  const size_t wlds = walk_scan_lds(net);
   if (wlds > 65536)
     FCHK(hipFuncSetAttribute((const void*)k_walk_f32, hipFuncAttributeMaxDynamicSharedMemorySize, (int)wlds));
-  const size_t rlds = (size_t)5 * CsL * 4 + (size_t)net.Wd * 4 + 64;
+  const size_t rlds = walk_ray_lds(net);
   if (rlds > 65536) {
     FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
     FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
     FCHK(hipFuncSetAttribute((const void*)k_walk_ray_f32<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)rlds));
   }
+}
+
+void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
+                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc, bool prepared) {
+  const FastNetDev& d = f.dev;
+  const int prec = f.precision == 1 ? 1 : 0;
+  t.launches = 0;
+  if (!prepared) fast_level_prepare(f, b, net, lvl.count);
+  const int CH = kLevelChunk;
+  const int chunk = std::min(lvl.count, CH);
+  float* planes = (float*)slot(b, S_PLANES, (size_t)chunk * AM_PCOMP * d.Cs * 4);
+  float* eqb = (float*)slot(b, S_EQ, (size_t)chunk * AM_EQS * 4);
+  const int Csort = walk_sort_size(net);
+  const int CsL = net.C;
+  const size_t wlds = walk_scan_lds(net);
+  const size_t rlds = walk_ray_lds(net);
   const size_t glds = gemm_lds(d, prec);
 
   for (int c0 = 0; c0 < lvl.count; c0 += CH) {

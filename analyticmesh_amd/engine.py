@@ -146,8 +146,8 @@ class Engine:
                  table_log2=None, walk="ray", ray_waves=None, scan_blocks=None, graph_levels=False,
                  reduced_vertices="newton"):
         """ray_waves / scan_blocks: launch shapes (see ``autotune``); None takes the tuned
-        shape stored for the nearest network shape.  graph_levels: launch each level of the
-        FP64 forms as one captured hipGraph.  reduced_vertices (fp32 form): "newton" (default)
+        shape stored for the nearest network shape.  graph_levels: launch each level as one
+        captured hipGraph (every form; the stage times of the stats are then not taken).  reduced_vertices (fp32 form): "newton" (default)
         gives each vertex solved from the reduced planes one FP64 Newton step after the march;
         "fp64" re-solves every vertex from FP64 planes; "estimate" keeps the vertex solved from
         the reduced planes where the face kernel's error estimate puts it within 1e-5 of |v|."""

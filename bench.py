@@ -281,7 +281,7 @@ def main():
     ap.add_argument("--seeds", type=int, default=1024)
     ap.add_argument("--cpu-regions", type=int, default=-1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--forms", default="fp32,bf16x3,fp64ray,fp64ray+graph,fp64")
+    ap.add_argument("--forms", default="fp32,fp32+graph,bf16x3,fp64ray,fp64ray+graph,fp64")
     ap.add_argument("--calib-copy-mb", type=int, default=0, help="plain copy of this size for counter scaling")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-MLP batch")
     ap.add_argument("--adjoint-only", action="store_true", help="measure only the adjoint (one JSON line)")

@@ -142,8 +142,9 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * region of the FP64 ray walk: 2, 4 (default) or 8; the records do not depend
  * on it), "scan_blocks" (grid of the candidate scan over the regions the ray
  * walk leaves undecided, default 2048), "graph_levels" (1: each breadth-first
- * level's launches, FP64 forms, are captured into a hipGraph and launched as
- * one; the executable graph is updated in place from level to level),
+ * level's launches, every form, are captured into a hipGraph and launched as
+ * one; the executable graph is updated in place from level to level; the
+ * per-stage kernel times of the stats are not taken then),
  * "walk" (0 ray walk, 1
  * candidate scan, for the reduced forms), "ray_waves_reduced" (waves per
  * region of the reduced forms' ray walk: 2, 4 (default) or 8; 4 measured

@@ -49,7 +49,7 @@ def test_autotuner_picks_and_stores(tmp_path):
     assert autotune.lookup(spec.nodes, path=path) == e["best"]
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp64ray"])
+@pytest.mark.parametrize("prec", ["fp64", "fp64ray", "fp32", "bf16x3"])
 def test_graph_levels_give_the_same_records(prec):
     """Each level captured as a hipGraph (updated in place from level to level) and launched as one:
     the same records as the launch-by-launch run, bit for bit."""
