@@ -208,9 +208,9 @@ __device__ __forceinline__ bool vertex_accurate(const VertexBound<T>& b, const T
 // 0 = satisfied, 1 = certainly violated, 2 = undecided.
 // eg: the constraint's error estimate sigma_g (coarse planes, per-plane mode;
 // its bound share is lam sigma_g), else 0.
+// s: the constraint's value at v, plane_eval(ax, ay, az, c, v).
 template <typename T>
-__device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, T eg, const T* v, const VertexBound<T>& b) {
-  const T s = plane_eval(ax, ay, az, c, v);
+__device__ __forceinline__ int checked_verdict_at(T s, T ax, T ay, T az, T c, T eg, const VertexBound<T>& b) {
   const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
   T err;
   if (sizeof(T) == 4) {
@@ -224,6 +224,11 @@ __device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, T eg, cons
   if (s > err) return 1;
   if (s >= -err) return 2;
   return 0;
+}
+
+template <typename T>
+__device__ __forceinline__ int checked_verdict(T ax, T ay, T az, T c, T eg, const T* v, const VertexBound<T>& b) {
+  return checked_verdict_at(plane_eval(ax, ay, az, c, v), ax, ay, az, c, eg, b);
 }
 
 // Verdict of one vertex against all constraints owned by this lane.
@@ -792,6 +797,16 @@ __device__ __forceinline__ void block_argmin(T& t, int& g) {
 // g >= g0 from LDS (arrays indexed g - g0); constraints g < g0, which are
 // hidden layer 1's, rebuilt from its weights (row g of (W0 | b0), negated where
 // the region's neuron g is active), so their LDS is not needed.
+// x with its sign bit xor-ed with sm's top bit (sm: 0 or 0x80000000).
+__device__ __forceinline__ double flip_sign(double x, uint32_t sm) {
+  uint2 u = __builtin_bit_cast(uint2, x);
+  u.y ^= sm;
+  return __builtin_bit_cast(double, u);
+}
+__device__ __forceinline__ float flip_sign(float x, uint32_t sm) {
+  return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ sm);
+}
+
 template <typename T>
 struct PlaneView {
   const T* Px;
@@ -806,9 +821,10 @@ struct PlaneView {
   __device__ __forceinline__ T err(int g) const { return (Pe && g >= g0) ? Pe[g - g0] : T(0); }
   __device__ __forceinline__ void get(int g, T* p) const {
     if (g < g0) {
-      const bool on = mbit(mask, g);
+      // an active neuron's plane is negated: flip the sign bits (one xor each)
+      const uint32_t sm = mbit(mask, g) ? 0x80000000u : 0u;
       const T v[4] = {T(W0[g * 3 + 0]), T(W0[g * 3 + 1]), T(W0[g * 3 + 2]), T(b0[g])};
-      for (int q = 0; q < 4; ++q) p[q] = on ? -v[q] : v[q];
+      for (int q = 0; q < 4; ++q) p[q] = flip_sign(v[q], sm);
     } else {
       const int i = g - g0;
       p[0] = Px[i];
@@ -890,14 +906,16 @@ __device__ __forceinline__ int check_and_shoot(const PlaneView<T>& P, int C, con
     T pg[4];
     P.get(g, pg);
     const T ax = pg[0], ay = pg[1], az = pg[2], c = pg[3];
+    // the constraint at the vertex, once for both uses
+    const T sv = plane_eval(ax, ay, az, c, v);
     if (g != c1 && g != c2) {
-      const int vg = checked_verdict(ax, ay, az, c, P.err(g), v, vb);
+      const int vg = checked_verdict_at(sv, ax, ay, az, c, P.err(g), vb);
       bits |= vg == 1 ? 1 : (vg == 2 ? 2 : 0);
     }
     if (g != s1 && g != s2) {
       const T al = fmaT(ax, d[0], fmaT(ay, d[1], az * d[2]));
       if (al > T(0)) {
-        const T num = -plane_eval(ax, ay, az, c, v);
+        const T num = -sv;
         if (db == T(0) || num * db < nb * al) { nb = num; db = al; gb = g; }
       }
     }
