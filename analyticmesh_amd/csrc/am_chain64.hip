@@ -641,19 +641,14 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
   // rebuilt from W0 and b0 where read (PlaneView), which leaves LDS for more
   // regions per CU
   const int n1 = net.nodes[1], Cl = net.C - n1;
-  double* Px = reinterpret_cast<double*>(smem_raw);
-  double* Py = Px + Cl;
-  double* Pz = Py + Cl;
-  double* Pc = Pz + Cl;
-  uint32_t* smask = reinterpret_cast<uint32_t*>(Pc + Cl);
+  double2* Q = reinterpret_cast<double2*>(smem_raw);   // (x, y) pairs, then (z, c) pairs
+  uint32_t* smask = reinterpret_cast<uint32_t*>(Q + 2 * Cl);
   const int lane = threadIdx.x;
   for (int w = lane; w < net.Wd; w += 64 * NW) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)b * 4 * Cs_hbm + n1;
   for (int i = lane; i < Cl; i += 64 * NW) {
-    Px[i] = src[i];
-    Py[i] = src[Cs_hbm + i];
-    Pz[i] = src[2 * (size_t)Cs_hbm + i];
-    Pc[i] = src[3 * (size_t)Cs_hbm + i];
+    Q[i] = double2{src[i], src[Cs_hbm + i]};
+    Q[Cl + i] = double2{src[2 * (size_t)Cs_hbm + i], src[3 * (size_t)Cs_hbm + i]};
   }
   double p[3];
   p[0] = io.reg_point[(size_t)rid * 3 + 0];
@@ -662,7 +657,7 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
   const double eq[4] = {eqb[(size_t)b * 4 + 0], eqb[(size_t)b * 4 + 1], eqb[(size_t)b * 4 + 2],
                         eqb[(size_t)b * 4 + 3]};
   __syncthreads();
-  const PlaneView<double> P{Px, Py, Pz, Pc, n1, net.W0, net.b[0], smask};
+  const PlaneView<double> P{reinterpret_cast<const double*>(Q), Cl, n1, net.W0, net.b[0], smask};
   walk_ray<double, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
 }
 

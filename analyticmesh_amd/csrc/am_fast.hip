@@ -465,11 +465,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO i
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
   const int Cs = io.Cs;
-  float* Px = reinterpret_cast<float*>(smem_raw);
-  float* Py = Px + Cs;
-  float* Pz = Py + Cs;
-  float* Pc = Pz + Cs;
-  float* Pe = Pc + Cs;
+  float4* Q = reinterpret_cast<float4*>(smem_raw);   // (x, y, z, c) per constraint
+  float* Pe = reinterpret_cast<float*>(Q + Cs);
   uint32_t* smask = reinterpret_cast<uint32_t*>(Pe + Cs);
   const int lane = threadIdx.x;
   const int b = blockIdx.x;
@@ -478,10 +475,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO i
   const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
   const bool with_sigma = wp.lam > 0.f;
   for (int g = lane; g < C; g += 64 * NW) {
-    Px[g] = src[g];
-    Py[g] = src[Cs_hbm + g];
-    Pz[g] = src[2 * Cs_hbm + g];
-    Pc[g] = src[3 * Cs_hbm + g];
+    Q[g] = float4{src[g], src[Cs_hbm + g], src[2 * Cs_hbm + g], src[3 * Cs_hbm + g]};
     if (with_sigma) Pe[g] = src[4 * Cs_hbm + g];
   }
   for (int w = lane; w < net.Wd; w += 64 * NW) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
@@ -490,7 +484,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO i
   const float p[3] = {(float)io.reg_point[(size_t)rid * 3 + 0], (float)io.reg_point[(size_t)rid * 3 + 1],
                       (float)io.reg_point[(size_t)rid * 3 + 2]};
   __syncthreads();
-  const PlaneView<float> P{Px, Py, Pz, Pc, 0, nullptr, nullptr, smask, with_sigma ? Pe : nullptr};
+  const PlaneView<float> P{reinterpret_cast<const float*>(Q), Cs, 0, nullptr, nullptr, smask,
+                           with_sigma ? Pe : nullptr};
   walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, with_sigma ? eb[4] : 0.f);
 }
 

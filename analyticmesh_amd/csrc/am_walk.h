@@ -809,10 +809,11 @@ __device__ __forceinline__ float flip_sign(float x, uint32_t sm) {
 
 template <typename T>
 struct PlaneView {
-  const T* Px;
-  const T* Py;
-  const T* Pz;
-  const T* Pc;
+  // LDS planes of constraints g >= g0 at index i = g - g0, in 16-byte groups
+  // so that one ds_read_b128 per group serves a lane: FP64, Q[2 i] = (x, y)
+  // and Q[2 (Cl + i)] = (z, c) (two arrays of pairs); FP32, Q[4 i] = (x, y, z, c).
+  const T* Q;
+  int Cl;
   int g0;
   const double* W0;
   const double* b0;
@@ -827,10 +828,20 @@ struct PlaneView {
       for (int q = 0; q < 4; ++q) p[q] = flip_sign(v[q], sm);
     } else {
       const int i = g - g0;
-      p[0] = Px[i];
-      p[1] = Py[i];
-      p[2] = Pz[i];
-      p[3] = Pc[i];
+      if constexpr (sizeof(T) == 8) {
+        const double2 a = reinterpret_cast<const double2*>(Q)[i];
+        const double2 b = reinterpret_cast<const double2*>(Q)[Cl + i];
+        p[0] = a.x;
+        p[1] = a.y;
+        p[2] = b.x;
+        p[3] = b.y;
+      } else {
+        const float4 a = reinterpret_cast<const float4*>(Q)[i];
+        p[0] = a.x;
+        p[1] = a.y;
+        p[2] = a.z;
+        p[3] = a.w;
+      }
     }
   }
 };
