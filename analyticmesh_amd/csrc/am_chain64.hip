@@ -643,6 +643,8 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
   const int n1 = net.nodes[1], Cl = net.C - n1;
   double2* Q = reinterpret_cast<double2*>(smem_raw);   // (x, y) pairs, then (z, c) pairs
   uint32_t* smask = reinterpret_cast<uint32_t*>(Q + 2 * Cl);
+  const WalkRecs rec = carve_walk_recs(reinterpret_cast<unsigned char*>(Q + 2 * Cl) + ((net.Wd * 4 + 15) & ~15),
+                                       net.vm, false);
   const int lane = threadIdx.x;
   for (int w = lane; w < net.Wd; w += 64 * NW) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)b * 4 * Cs_hbm + n1;
@@ -658,13 +660,13 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
                         eqb[(size_t)b * 4 + 3]};
   __syncthreads();
   const PlaneView<double> P{reinterpret_cast<const double*>(Q), Cl, n1, net.W0, net.b[0], smask};
-  walk_ray<double, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p);
+  walk_ray<double, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec);
 }
 
 // NW waves per region (the records do not depend on NW: every reduction breaks
 // ties by constraint index and each serial step runs on one wave).
 template <int NW>
-__global__ __launch_bounds__(64 * NW, 20 / NW) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
+__global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
                                                         const double* planes, const double* eqb, int Cs_hbm,
                                                         int only_undecided) {
   if ((int)blockIdx.x >= launch_count(io)) return;
@@ -675,7 +677,7 @@ __global__ __launch_bounds__(64 * NW, 20 / NW) void k_walk_ray64(NetDev net, Reg
 }
 
 template <int NW>
-__global__ __launch_bounds__(64 * NW, 20 / NW) void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0,
+__global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray64_multi(const NetDev* nets, RegionIO io, int Wd, int rid0,
                                                               WalkParams wp, const double* planes, const double* eqb,
                                                               int Cs_hbm) {
   if ((int)blockIdx.x >= io.count) return;

@@ -488,7 +488,8 @@ const WalkParams kRay64Mf{1e-12f, 1e-11f, 1e6f};
 
 // planes past hidden layer 1 (the walk rebuilds layer 1's) and the mask
 size_t ray64_lds_bytes(const am_ctx* c) {
-  return (size_t)4 * (c->C - c->nodes[1]) * sizeof(double) + (size_t)c->Wd * 4 + 64;
+  return (size_t)4 * (c->C - c->nodes[1]) * sizeof(double) + (((size_t)c->Wd * 4 + 15) & ~(size_t)15) +
+         walk_rec_bytes(c->vm, false);
 }
 
 // FP64 planes + walk of `count` regions (region ids first.., or ids[0..count)),
@@ -801,7 +802,9 @@ size_t batch_scan_lds(const am_batch* b) {
 }
 size_t batch_ray_lds(const am_batch* b) {
   size_t m = 0;
-  for (const BatchNet& n : b->nets) m = std::max(m, (size_t)4 * (n.L - n.nodes[1]) * 8 + (size_t)b->c->Wd * 4 + 64);
+  for (const BatchNet& n : b->nets)
+    m = std::max(m, (size_t)4 * (n.L - n.nodes[1]) * 8 + (((size_t)b->c->Wd * 4 + 15) & ~(size_t)15) +
+                        walk_rec_bytes(b->c->vm, false));
   return m;
 }
 size_t batch_chain_lds(const am_batch* b) {

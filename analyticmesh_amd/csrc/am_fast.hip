@@ -460,7 +460,7 @@ __global__ __launch_bounds__(64) void k_walk_f32(NetDev net, RegionIO io, WalkPa
 // Ray-shooting FP32 walk: LDS holds only the planes and the mask; NW waves
 // per region.
 template <int NW>
-__global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
+__global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray_f32(NetDev net, RegionIO io, WalkParams wp, const float* planes,
                                                           const float* eqbuf, int Cs_hbm, int chunk_first) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int C = net.C;
@@ -468,6 +468,8 @@ __global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO i
   float4* Q = reinterpret_cast<float4*>(smem_raw);   // (x, y, z, c) per constraint
   float* Pe = reinterpret_cast<float*>(Q + Cs);
   uint32_t* smask = reinterpret_cast<uint32_t*>(Pe + Cs);
+  const WalkRecs rec = carve_walk_recs(
+      reinterpret_cast<unsigned char*>(Q) + (((size_t)Cs * 20 + (size_t)net.Wd * 4 + 15) & ~(size_t)15), net.vm, true);
   const int lane = threadIdx.x;
   const int b = blockIdx.x;
   if (b >= io.count) return;
@@ -486,7 +488,7 @@ __global__ __launch_bounds__(64 * NW) void k_walk_ray_f32(NetDev net, RegionIO i
   __syncthreads();
   const PlaneView<float> P{reinterpret_cast<const float*>(Q), Cs, 0, nullptr, nullptr, smask,
                            with_sigma ? Pe : nullptr};
-  walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, with_sigma ? eb[4] : 0.f);
+  walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec, with_sigma ? eb[4] : 0.f);
 }
 
 // Regions the walk left undecided, added to cnt[0], their reasons, added to
@@ -703,7 +705,9 @@ size_t walk_scan_lds(const NetDev& net) {
   return (size_t)5 * net.C * 4 + (size_t)walk_sort_size(net) * 8 + (size_t)((net.C + 31) / 32) * 4 +
          (size_t)net.Wd * 4 + 64;
 }
-size_t walk_ray_lds(const NetDev& net) { return (size_t)5 * net.C * 4 + (size_t)net.Wd * 4 + 64; }
+size_t walk_ray_lds(const NetDev& net) {
+  return (((size_t)net.C * 20 + (size_t)net.Wd * 4 + 15) & ~(size_t)15) + walk_rec_bytes(net.vm, true);
+}
 }  // namespace
 
 void fast_level_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int count) {

@@ -969,6 +969,34 @@ __device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T
   tp = t1; gp = bp; tm = t2; gm = bm;
 }
 
+// The ray walk's per-region records in dynamic LDS, sized by the polygon
+// vertex cap vm (not by AM_VM_MAX): the tab, the vertices, their constraint
+// pairs and (reduced planes, per-plane mode) their inverse systems.  Kernels
+// carve them after their planes; walk_rec_bytes is the space they take.
+struct WalkRecs {
+  int* tab;        // [vm + 2]
+  double* verts;   // [vm][3]
+  int* vpair;      // [vm][2]
+  float* jinv;     // [vm][9], or nullptr
+};
+__host__ __device__ inline size_t walk_rec_bytes(int vm, bool jinv) {
+  auto a16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+  return a16((size_t)vm * 3 * 8) + a16((size_t)vm * 2 * 4) + a16((size_t)(vm + 2) * 4) +
+         (jinv ? a16((size_t)vm * 9 * 4) : 0);
+}
+__device__ inline WalkRecs carve_walk_recs(unsigned char* p, int vm, bool jinv) {
+  auto a16 = [](size_t x) { return (x + 15) & ~(size_t)15; };
+  WalkRecs r;
+  r.verts = reinterpret_cast<double*>(p);
+  p += a16((size_t)vm * 3 * 8);
+  r.vpair = reinterpret_cast<int*>(p);
+  p += a16((size_t)vm * 2 * 4);
+  r.tab = reinterpret_cast<int*>(p);
+  p += a16((size_t)(vm + 2) * 4);
+  r.jinv = jinv ? reinterpret_cast<float*>(p) : nullptr;
+  return r;
+}
+
 // Reduced-precision walk by ray shooting.  From the start plane's edge each
 // step proposes the next edge plane as the nearest plane crossed along the
 // current edge line (one pass over the constraints), then certifies the new
@@ -986,13 +1014,14 @@ __device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T
 // AM_VERT_TOL gets the AM_VERT64 flag).
 template <typename T, int NW>
 __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
-                         const PlaneView<T>& P, const uint32_t* smask, const T* eq, const T* p, T eqe = T(0)) {
+                         const PlaneView<T>& P, const uint32_t* smask, const T* eq, const T* p, const WalkRecs& rec,
+                         T eqe = T(0)) {
   const bool per_plane = sizeof(T) == 4 && wp.lam > 0.f && P.Pe != nullptr;
   bool vinacc = false;
-  __shared__ int s_tab[AM_VM_MAX + 2];
-  __shared__ double s_verts[AM_VM_MAX * 3];    // vertices re-solved in FP64 from the planes
-  __shared__ int s_vpair[AM_VM_MAX * 2];
-  __shared__ float s_jinv[AM_VM_MAX * 9];      // per-plane mode: inverse systems for the Newton step
+  int* const s_tab = rec.tab;
+  double* const s_verts = rec.verts;     // vertices (re-solved in FP64 from the planes)
+  int* const s_vpair = rec.vpair;
+  float* const s_jinv = rec.jinv;        // per-plane mode: inverse systems for the Newton step
   __shared__ T s_v[6];                 // the vertex and the next edge direction
   __shared__ VertexBound<T> s_vb;      // its bound per unit of 1 + |v|_1
   __shared__ unsigned s_turn;          // turn test: 0 passed, else its reason bit
@@ -1178,7 +1207,8 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   }
   __syncthreads();   // the last records are visible to every thread
   finish_region<T, true, double>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv,
-                                 start_in, start, AM_ST_OK, undecided, why, vinacc, per_plane ? s_jinv : nullptr);
+                                 start_in, start, AM_ST_OK, undecided, why, vinacc,
+                                 per_plane ? s_jinv : nullptr);
 }
 
 }  // namespace am
