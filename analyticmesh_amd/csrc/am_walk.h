@@ -793,10 +793,6 @@ __device__ __forceinline__ void block_argmin(T& t, int& g) {
     if (red_t[w] < t || (red_t[w] == t && red_g[w] < g)) { t = red_t[w]; g = red_g[w]; }
 }
 
-// Constraint planes of one region as the ray walk reads them: constraints
-// g >= g0 from LDS (arrays indexed g - g0); constraints g < g0, which are
-// hidden layer 1's, rebuilt from its weights (row g of (W0 | b0), negated where
-// the region's neuron g is active), so their LDS is not needed.
 // x with its sign bit xor-ed with sm's top bit (sm: 0 or 0x80000000).
 __device__ __forceinline__ double flip_sign(double x, uint32_t sm) {
   uint2 u = __builtin_bit_cast(uint2, x);
@@ -807,6 +803,10 @@ __device__ __forceinline__ float flip_sign(float x, uint32_t sm) {
   return __builtin_bit_cast(float, __builtin_bit_cast(uint32_t, x) ^ sm);
 }
 
+// Constraint planes of one region as the ray walk reads them: constraints
+// g >= g0 from LDS (16-byte groups indexed g - g0); constraints g < g0, which are
+// hidden layer 1's, rebuilt from its weights (row g of (W0 | b0), negated where
+// the region's neuron g is active), so their LDS is not needed.
 template <typename T>
 struct PlaneView {
   // LDS planes of constraints g >= g0 at index i = g - g0, in 16-byte groups
