@@ -96,7 +96,13 @@ __global__ __launch_bounds__(256) void k_insert(InsertIO io) {
     int result = -1;
     for (uint32_t probes = 0; __ballot(!done) != 0ull; ++probes) {
       uint32_t prev = 0u;
-      if (!done && j == 0) prev = atomicCAS(&io.table[s], AM_EMPTY, AM_PEND | (uint32_t)cc);
+      if (!done && j == 0) {
+        // read the slot first: most candidates meet a key already there, and
+        // a slot only ever goes empty -> pending c -> pending c' < c within
+        // the kernel, so a non-empty value read here is the key's slot state
+        prev = __hip_atomic_load(&io.table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == AM_EMPTY) prev = atomicCAS(&io.table[s], AM_EMPTY, AM_PEND | (uint32_t)cc);
+      }
       prev = (uint32_t)__shfl((int)prev, lead, 64);
       bool cmp = false;
       const uint32_t* other = key;
@@ -114,7 +120,8 @@ __global__ __launch_bounds__(256) void k_insert(InsertIO io) {
       if (cmp) {
         if (eq) {
           if (prev & AM_PEND) {
-            if (j == 0) atomicMin(&io.table[s], AM_PEND | (uint32_t)cc);
+            // lower the group's smallest candidate (only a smaller index can)
+            if (j == 0 && (uint32_t)cc < (prev & ~AM_PEND)) atomicMin(&io.table[s], AM_PEND | (uint32_t)cc);
             result = (int)s;
           }
           done = true;
