@@ -127,7 +127,11 @@ __global__ __launch_bounds__(256) void k_cinsert(CombineIn in, CombineWork w) {
     bool done = !act;
     for (uint32_t probes = 0; __ballot(!done) != 0ull; ++probes) {
       uint32_t prev = 0u;
-      if (!done && j4 == 0) prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
+      if (!done && j4 == 0) {
+        // read first (a slot only goes empty -> item -> smaller item here), claim when empty
+        prev = __hip_atomic_load(&w.table[s], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (prev == AM_CEMPTY) prev = atomicCAS(&w.table[s], AM_CEMPTY, (uint32_t)item);
+      }
       prev = (uint32_t)__shfl((int)prev, lead, 64);
       bool cmp = false;
       const uint32_t* m2 = in.reg_mask;
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(256) void k_cinsert(CombineIn in, CombineWork w) {
       const unsigned long long bal = __ballot(diff != 0u);
       const bool eq = ((bal >> lead) & ((1ull << AM_CK) - 1)) == 0ull;
       if (cmp && eq) {
-        if (j4 == 0) atomicMin(&w.table[s], (uint32_t)item);
+        if (j4 == 0 && (uint32_t)item < prev) atomicMin(&w.table[s], (uint32_t)item);
         done = true;
       } else if (!done) {
         if (probes >= w.cap_mask) done = true;   // table full: sized so that it never is
