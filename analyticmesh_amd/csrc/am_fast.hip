@@ -52,21 +52,47 @@ __device__ __forceinline__ uint16_t bf16_rn(float x) {
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
+// Input column of the error-estimate product for an input plane with
+// variance V and squared norm n2 (active in its region): V + ck n2, capped to
+// stay finite in bf16.
+__device__ __forceinline__ uint16_t sig_input(float V, float n2, float ck) {
+  return bf16_rn(fminf(V + ck * n2, 1e30f));
+}
+
+// Where the error-estimate input of list position p and region t sits in the
+// row paddings (bf16 units from the first padding): the B operand layout of
+// v_mfma_f32_16x16x16_bf16 for the k loop's step kb = p >> 2 group.  Lane
+// (q, t) of the step group starting at list position 16 g reads positions
+// 16 g + 4 i + q, i = 0..3, as one 8-byte word: row 16 g + 4 q + t / 2, half t % 2.
+// A group's slots lie in its own 16 rows.
+__device__ __forceinline__ int sig_slot(int p, int t) {
+  return ((p & ~15) + 4 * (p & 3) + (t >> 1)) * 2 * AM_XS + (t & 1) * 4 + ((p & 15) >> 2);
+}
+
 // Stage layer-1 planes: X = masked rows of W0 (and b0); signed planes to HBM.
 // FP32: X holds only the tile's active inputs (kl1, nkp1 entries).
 template <int PREC>
 __device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32_t* mk, int r0, int nreg,
-                             float* Xs, uint16_t* Xh, uint16_t* Xl, int KS, const uint16_t* kl1, int nkp1) {
+                             float* Xs, uint16_t* Xh, uint16_t* Xl, int KS, const uint16_t* kl1, int nkp1,
+                             uint16_t* Ypad) {
   const int tid = threadIdx.x;
   const int n1 = f.nodes[1];
   if (PREC == 0) {
-    // X rows = the tile's active inputs of layer 1, in list order (+0 past the list)
-    for (int idx = tid; idx < nkp1 * AM_GN; idx += 256) {
-      const int j = idx / AM_GN, col = idx - j * AM_GN, t = col >> 2, q = col & 3;
-      const int k = kl1[j];
-      float x = 0.f;
-      if (k < n1 && mbit(mk + t * f.Wd, k)) x = q < 3 ? f.W0[k * 3 + q] : f.bias[0][k];
-      Xs[j * AM_XS + col] = x;
+    // X rows = the tile's active inputs of layer 1, in list order (+0 past the
+    // list): a thread takes list position i, reads its row of (W0 | b0) once
+    // and writes the row's 8 regions as float4s; with Ypad, also the first
+    // error-estimate product's input (hidden layer 1's planes are exact: V = 0)
+    for (int i = tid; i < nkp1; i += 256) {
+      const int k = kl1[i];
+      float4 w = {0.f, 0.f, 0.f, 0.f};
+      if (k < n1) w = float4{f.W0[k * 3 + 0], f.W0[k * 3 + 1], f.W0[k * 3 + 2], f.bias[0][k]};
+      const uint16_t xs = Ypad ? sig_input(0.f, w.x * w.x + w.y * w.y + w.z * w.z + w.w * w.w, f.ck[1]) : 0;
+#pragma unroll
+      for (int t = 0; t < AM_GT; ++t) {
+        const bool on = k < n1 && mbit(mk + t * f.Wd, k);
+        *reinterpret_cast<float4*>(Xs + i * AM_XS + 4 * t) = on ? w : float4{0.f, 0.f, 0.f, 0.f};
+        if (Ypad) Ypad[sig_slot(i, t)] = on ? xs : (uint16_t)0;
+      }
     }
   }
   for (int idx = tid; PREC != 0 && idx < n1 * AM_GN; idx += 256) {
@@ -98,22 +124,6 @@ __device__ void stage_layer1(const FastNetDev& f, const GemmIO& io, const uint32
   }
 }
 
-// Input column of the error-estimate product for an input plane with
-// variance V and squared norm n2 (active in its region): V + ck n2, capped to
-// stay finite in bf16.
-__device__ __forceinline__ uint16_t sig_input(float V, float n2, float ck) {
-  return bf16_rn(fminf(V + ck * n2, 1e30f));
-}
-
-// Where the error-estimate input of list position p and region t sits in the
-// row paddings (bf16 units from the first padding): the B operand layout of
-// v_mfma_f32_16x16x16_bf16 for the k loop's step kb = p >> 2 group.  Lane
-// (q, t) of the step group starting at list position 16 g reads positions
-// 16 g + 4 i + q, i = 0..3, as one 8-byte word: row 16 g + 4 q + t / 2, half t % 2.
-// A group's slots lie in its own 16 rows.
-__device__ __forceinline__ int sig_slot(int p, int t) {
-  return ((p & ~15) + 4 * (p & 3) + (t >> 1)) * 2 * AM_XS + (t & 1) * 4 + ((p & 15) >> 2);
-}
 
 template <int PREC>
 __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, GemmIO io) {
@@ -147,11 +157,10 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
     const int t = idx / f.Wd, w = idx - t * f.Wd;
     mk[idx] = t < nreg ? io.reg_mask[(size_t)(io.first + r0 + t) * f.Wd + w] : 0u;
   }
-  if (PREC == 0) {
-    for (int i = tid; i < f.Kpad * AM_XS; i += 256) Xs[i] = 0.f;
-  } else {
+  // (FP32: every LDS element a step reads is written first -- list positions
+  // by the staging and the compaction, parked rows below n_out -- so no clear)
+  if (PREC != 0)
     for (int i = tid; i < 2 * AM_GN * KS; i += 256) Xh[i] = 0;
-  }
   __syncthreads();
   if (PREC == 0) {
     // Inputs of linear layer li that are active in some region of the tile,
@@ -177,22 +186,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
     }
     __syncthreads();
   }
-  stage_layer1<PREC>(f, io, mk, r0, nreg, Xs, Xh, Xl, KS, klist, PREC == 0 ? s_nk[1] : 0);
-  if (SIG) {
-    // the first product's input at the list positions (hidden layer 1's
-    // planes are exact: V = 0)
-    const int n1 = f.nodes[1], nkp1 = s_nk[1];
-    for (int idx = tid; idx < nkp1 * AM_GT; idx += 256) {
-      const int i = idx / AM_GT, t = idx - i * AM_GT;
-      const int k = klist[i];
-      uint16_t x = 0;
-      if (k < n1 && mbit(mk + t * f.Wd, k)) {
-        const float a = f.W0[k * 3 + 0], b = f.W0[k * 3 + 1], c = f.W0[k * 3 + 2], d = f.bias[0][k];
-        x = sig_input(0.f, a * a + b * b + c * c + d * d, f.ck[1]);
-      }
-      Ypad[sig_slot(i, t)] = x;
-    }
-  }
+  stage_layer1<PREC>(f, io, mk, r0, nreg, Xs, Xh, Xl, KS, klist, PREC == 0 ? s_nk[1] : 0, SIG ? Ypad : nullptr);
   __syncthreads();
 
   for (int li = 1; li <= f.D; ++li) {
