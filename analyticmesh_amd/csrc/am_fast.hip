@@ -51,12 +51,14 @@ __device__ __forceinline__ uint16_t bf16_rn(float x) {
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf16_to_f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+// The same rounding (to nearest even, finite x) in one instruction (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint16_t bf16_hw(float x) { return __builtin_bit_cast(uint16_t, (__bf16)x); }
 
 // Input column of the error-estimate product for an input plane with
 // variance V and squared norm n2 (active in its region): V + ck n2, capped to
 // stay finite in bf16.
 __device__ __forceinline__ uint16_t sig_input(float V, float n2, float ck) {
-  return bf16_rn(fminf(V + ck * n2, 1e30f));
+  return bf16_hw(fminf(V + ck * n2, 1e30f));
 }
 
 // Where the error-estimate input of list position p and region t sits in the
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
           Ys[row * AM_XS + c0] = acc0[i] + ((c0 & 3) == 3 ? b : 0.f);
           Ys[row * AM_XS + 16 + c0] = acc1[i] + (((16 + c0) & 3) == 3 ? b : 0.f);
           // V of plane row, region c0, in row row's padding
-          if (SIG && c0 < AM_GT) xq_at(row, c0) = bf16_rn(fminf(fmaxf(accq[u][i], 0.f) + f.ck[li] * b * b, 1e30f));
+          if (SIG && c0 < AM_GT) xq_at(row, c0) = bf16_hw(fminf(fmaxf(accq[u][i], 0.f) + f.ck[li] * b * b, 1e30f));
         }
       }
     }
@@ -328,43 +330,43 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
         // no chunk overwrites a row a later chunk reads.
         const uint16_t* kl2 = klist + (size_t)li * KLF;
         const int nkp2 = s_nk[li + 1];
-        constexpr int PPT = 256 / AM_GN;   // positions per pass of the block
-        constexpr int CH = 128 / PPT;      // passes per chunk
+        // thread (region t, position p0): region t's four columns of positions
+        // p0, p0 + 32, p0 + 64, p0 + 96 of each chunk, as one float4 each (one
+        // list, mask and row read per four values); the next product's input
+        // rides along, m (V + ck |P|^2) from the same float4 (sig_slot places
+        // it in the rows of the position's own 16-row group, so the chunk rule
+        // holds for it too)
+        constexpr int PP = 256 / AM_GT;   // positions per pass of the block
+        constexpr int CH = 128 / PP;      // passes per chunk
         int tq = tid;
         asm volatile("" : "+v"(tq));       // keeps these offsets from being hoisted out of the layer loop
-        const int col = tq & (AM_GN - 1), t = col >> 2, p0 = tq / AM_GN;
-        // the next product's input rides along: the x-column thread of each
-        // region also forms m (V + ck |P|^2) for its positions (sig_slot
-        // places them in the rows of the position's own 16-row group, so the
-        // chunk rule holds for them too)
-        const bool sq = SIG && (col & 3) == 0;
+        const int t = tq & (AM_GT - 1), p0 = tq / AM_GT;
+        const uint32_t* mt = mk + t * f.Wd;
         const float ckn = f.ck[li + 1];
         for (int c0 = 0; c0 < nkp2; c0 += 128) {
-          float xv[CH];
+          float4 xv[CH];
           uint16_t qv[CH];
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
-            const int i = c0 + c * PPT + p0;
-            xv[c] = 0.f;
+            const int i = c0 + c * PP + p0;
+            xv[c] = float4{0.f, 0.f, 0.f, 0.f};
             qv[c] = 0;
             if (i < nkp2) {
               const int k = kl2[i];
-              if (k < nout && mbit(mk + t * f.Wd, out0 + k)) {
-                xv[c] = Ys[k * AM_XS + col];
-                if (sq) {
-                  const float4 y = *reinterpret_cast<const float4*>(Ys + k * AM_XS + col);
-                  qv[c] = sig_input(bf16_to_f(xq_at(k, t)), y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w, ckn);
-                }
+              if (k < nout && mbit(mt, out0 + k)) {
+                const float4 y = *reinterpret_cast<const float4*>(Ys + k * AM_XS + 4 * t);
+                xv[c] = y;
+                if (SIG) qv[c] = sig_input(bf16_to_f(xq_at(k, t)), y.x * y.x + y.y * y.y + y.z * y.z + y.w * y.w, ckn);
               }
             }
           }
           __syncthreads();
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
-            const int i = c0 + c * PPT + p0;
+            const int i = c0 + c * PP + p0;
             if (i < nkp2) {
-              Xs[i * AM_XS + col] = xv[c];
-              if (sq) Ypad[sig_slot(i, t)] = qv[c];
+              *reinterpret_cast<float4*>(Xs + i * AM_XS + 4 * t) = xv[c];
+              if (SIG) Ypad[sig_slot(i, t)] = qv[c];
             }
           }
         }
