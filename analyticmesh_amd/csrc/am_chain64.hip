@@ -409,12 +409,28 @@ __device__ __forceinline__ void chain_tile_half(const NetDev& net, const ChainIO
   __syncthreads();
   const size_t Cs = (size_t)io.Cs;
   const int n1 = net.nodes[1];
-  for (int idx = tid; idx < s_nk[1] * AM_HN; idx += AM_HALF_THREADS) {
-    const int i = idx / AM_HN, col = idx - i * AM_HN, t = col >> 2, q = col & 3;
-    const int k = klist[i];
-    double x = 0.0;
-    if (k < n1 && mbit(mk + t * net.Wd, k)) x = q < 3 ? net.W0[k * 3 + q] : net.b[0][k];
-    Xs[i * AM_HXS + col] = x;
+  // layer-1 input rows: a thread takes list position i and half h of the tile's
+  // regions, reads its row of (W0 | b0) once and writes 4 regions x 4 values
+  {
+    static_assert(AM_HALF_THREADS == 512 && AM_CHAIN_MAXW <= 256, "two halves of 256 positions");
+    const int h = tid >> 8;
+    for (int i = tid & 255; i < s_nk[1]; i += 256) {
+      const int k = klist[i];
+      double w[4] = {0.0, 0.0, 0.0, 0.0};
+      if (k < n1) {
+        w[0] = net.W0[k * 3 + 0];
+        w[1] = net.W0[k * 3 + 1];
+        w[2] = net.W0[k * 3 + 2];
+        w[3] = net.b[0][k];
+      }
+#pragma unroll
+      for (int tt = 0; tt < AM_HT / 2; ++tt) {
+        const int t = h * (AM_HT / 2) + tt;
+        const bool on = k < n1 && mbit(mk + t * net.Wd, k);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Xs[i * AM_HXS + 4 * t + q] = on ? w[q] : 0.0;
+      }
+    }
   }
   for (int idx = tid; idx < AM_HN * net.E; idx += AM_HALF_THREADS) {
     const int col = idx / net.E, e = idx - col * net.E, t = col >> 2, q = col & 3;
