@@ -1116,8 +1116,10 @@ am_status am_set_option(am_ctx* c, const char* key, long long value) {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "chain_mfma must be 0 or 1");
     c->chain_mfma = value != 0;
   } else if (k == "reduced_vertices") {
-    if (value < 0 || value > 2)
-      return fail(c, AM_ERR_ARG, "reduced_vertices must be 0 (FP64 re-solve), 1 (estimate) or 2 (Newton)");
+    if (value < 0 || value > 3)
+      return fail(c, AM_ERR_ARG,
+                  "reduced_vertices must be 0 (FP64 re-solve), 1 (estimate), 2 (Newton) or 3 (Newton where the "
+                  "estimate is not within 1e-5)");
     c->fast.vertex_mode = (int)value;
   } else if (k == "adjoint_mfma") {
     if (value != 0 && value != 1) return fail(c, AM_ERR_ARG, "adjoint_mfma must be 0 or 1");
@@ -1273,10 +1275,10 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
     // FP32 form, Newton vertices: the walk keeps each vertex's inverse system
     // (the Newton kernel takes layers up to AM_CHAIN_MAXW wide, which the
     // reduced forms require anyway, and the ray walk)
-    c->jinv_on = !fp64_form && c->precision == AM_PREC_FP32 && c->fast.vertex_mode == 2 &&
+    c->jinv_on = !fp64_form && c->precision == AM_PREC_FP32 && c->fast.vertex_mode >= 2 &&
                  c->fast.walk_mode == 0 && newton_lds_bytes(c->Wd) <= c->max_lds;
     const int vmode = c->fast.vertex_mode;
-    if (!fp64_form && c->precision == AM_PREC_FP32 && vmode == 2 && !c->jinv_on) c->fast.vertex_mode = 0;
+    if (!fp64_form && c->precision == AM_PREC_FP32 && vmode >= 2 && !c->jinv_on) c->fast.vertex_mode = 0;
     if (!fp64_form) fast_prepare(c->fast, c->fbuf, n, c->precision, wv.data(), bv.data(), dtype, st);
     c->fast.vertex_mode = vmode;
     if (c->jinv_on) {
@@ -1526,12 +1528,23 @@ static CombineIn combine_in(const am_ctx* c) {
 
 // The first use of each vertex whose record carries the reduced walk's
 // inverse system (info[7] = 1), compacted in use order.
+// Vertices of the Newton step: the first use of each vertex, in a region whose
+// walk kept inverse systems, unless the walk marked it as needing no step (a
+// NaN first entry of its inverse system).
 struct NewtonSrc {
   const unsigned* vflag;
   const int* item_region;
   const int* info;
+  const unsigned long long* roff;
+  const uint16_t* jinv;
+  int vm;
   __device__ int operator()(size_t i) const {
-    return (vflag[i] && info[(size_t)item_region[i] * AM_INFO_STRIDE + 7] == 1) ? 1 : 0;
+    if (!vflag[i]) return 0;
+    const int r = item_region[i];
+    if (info[(size_t)r * AM_INFO_STRIDE + 7] != 1) return 0;
+    const int k = (int)i - (int)(roff[r] & 0xffffffffu);
+    const uint16_t h = jinv[((size_t)r * vm + k) * 9];
+    return (h & 0x7fffu) > 0x7f80u ? 0 : 1;
   }
 };
 struct NewtonSink {
@@ -1562,7 +1575,8 @@ long long newton_refine(am_ctx* c) {
   ensure(c->redo_ids, (size_t)N, false, st);
   ensure(c->redo_cnt, 1, false, st);
   ensure(c->scan_part, scan_blocks((size_t)N) + 1, false, st);
-  scan_launch<int>(NewtonSrc{w.vflag, w.item_region, c->reg_info.p}, (size_t)N, NewtonSink{c->redo_ids.p},
+  scan_launch<int>(NewtonSrc{w.vflag, w.item_region, c->reg_info.p, w.roff, c->reg_jinv.p, c->vm}, (size_t)N,
+                   NewtonSink{c->redo_ids.p},
                    c->scan_part.p, c->redo_cnt.p, st);
   int n = 0;
   HIPCHK(hipMemcpyAsync(&n, c->redo_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));

@@ -46,10 +46,11 @@ struct FastNet {
   int precision = 0;
   int walk_mode = 0;            // 0: ray-shooting walk, 1: the reference's candidate scan (checked)
   int ray_waves = 4;            // waves per region of the ray-shooting walk (2, 4, 8)
-  int vertex_mode = 2;          // FP32 form: 0 every vertex re-solved from FP64 planes; 1 vertices solved
+  int vertex_mode = 3;          // FP32 form: 0 every vertex re-solved from FP64 planes; 1 vertices solved
                                 // from the reduced planes kept where their error estimate is within 1e-5;
-                                // 2 (default) vertices solved from the reduced planes, then one FP64
-                                // Newton step after the march (the walk keeps their inverse systems)
+                                // 2 vertices solved from the reduced planes, then one FP64 Newton step
+                                // after the march (the walk keeps their inverse systems); 3 (default)
+                                // as 2, the step only for vertices whose estimate is not within 1e-5
   bool ready = false;
   FastNetDev dev{};
   WalkParams wp{};

@@ -681,12 +681,15 @@ void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, 
   // FP32: plane error bounds lam sigma per plane plus 8 u of its largest
   // coefficient (the walk's own FP32 evaluations), lam = 3: the largest error
   // measured against FP64 on 1.5M planes of the 4x256 net was 0.7 sigma.
-  // Vertex estimates (reduced_vertices = 1) use 0.5 sigma.  Split-BF16: one
-  // relative bound for every plane, 2e-3 of its largest coefficient (about 4x
-  // the largest error measured), and every vertex re-solved from FP64 planes.
-  const float lam_v = f.vertex_mode == 1 ? 0.5f : (f.vertex_mode == 2 ? 0.f : -1.f);
-  if (prec == 0) f.wp = WalkParams{4.8e-7f, 1e-3f, 1e3f, 3.f, lam_v};
-  else f.wp = WalkParams{2e-3f, 2e-2f, 1e2f, 0.f, -1.f};
+  // Vertex estimates (to keep a region's reduced vertices, reduced_vertices = 1,
+  // or to skip a vertex's Newton step, reduced_vertices = 3) use 1 sigma.
+  // Split-BF16: one relative bound for every plane, 2e-3 of its largest
+  // coefficient (about 4x the largest error measured), and every vertex
+  // re-solved from FP64 planes.
+  const int vm = f.vertex_mode;
+  const float lam_v = (vm == 1 || vm == 3) ? 1.f : (vm == 2 ? 0.f : -1.f);
+  if (prec == 0) f.wp = WalkParams{4.8e-7f, 1e-3f, 1e3f, 3.f, lam_v, vm == 3 ? 1 : 0};
+  else f.wp = WalkParams{2e-3f, 2e-2f, 1e2f, 0.f, -1.f, 0};
   f.ready = true;
 }
 

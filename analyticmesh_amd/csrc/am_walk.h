@@ -49,6 +49,10 @@ struct WalkParams {
   float lam_v = 0.f;  // per-plane errors: the multiple of sigma in the vertex
                       // accuracy estimate; < 0: every vertex is left to FP64
                       // (AM_VERT64 on every decided region), 0: none is
+  int vsel = 0;       // with lam_v > 0 and inverse systems kept: a vertex within
+                      // the estimate is marked as needing no Newton step (NaN
+                      // first entry of its inverse system) instead of sending
+                      // its region to FP64
 };
 
 template <typename T>
@@ -142,10 +146,14 @@ __device__ __forceinline__ VertexBound<T> vertex_bound_unit(const T* p1, const T
     b.w3 = kappa * m3;
     b.e0 = T(1.001) * es;
     b.eg = T(1.001) * lam;
+    // the vertex estimate (per-plane mode, where the vertex is solved in FP64):
+    // the planes' FP32 rounding (u of the largest coefficient) and the FP64
+    // solve's, plus lam_v sigma for the error of the sums
     const T il = lam > T(0) ? lam_v / lam : T(0);
-    b.s1 = kappa * (es * q1 + e1 * il);
-    b.s2 = kappa * (es * q2 + e2 * il);
-    b.s3 = kappa * (es * q3 + e3 * il);
+    const T ev = T(5.9604644775390625e-08) + T(32) * T(1.1102230246251565e-16) * (T(1) + h);
+    b.s1 = kappa * (ev * q1 + e1 * il);
+    b.s2 = kappa * (ev * q2 + e2 * il);
+    b.s3 = kappa * (ev * q3 + e3 * il);
   }
   return b;
 }
@@ -193,11 +201,11 @@ __device__ __forceinline__ void solve_vertex(const T* p1, const T* p2, const T* 
 }
 
 // Coarse planes, per-plane mode: whether the vertex v (solved from the bound's
-// system) is within AM_VERT_TOL of the exact one relative to |v|_inf (at least
-// 1e-3), by the error estimate |dv_x| <= sum_i |A^-1_x,i| s_i.
+// system) is within AM_VERT_TOL of the exact one relative to |v|_inf, by the
+// error estimate |dv_x| <= sum_i |A^-1_x,i| s_i (a vertex at the origin never is).
 template <typename T>
 __device__ __forceinline__ bool vertex_accurate(const VertexBound<T>& b, const T* v) {
-  const T tol = T(AM_VERT_TOL) * fmax(T(1e-3), fmax(fmax(absT(v[0]), absT(v[1])), absT(v[2])));
+  const T tol = T(AM_VERT_TOL) * fmax(fmax(absT(v[0]), absT(v[1])), absT(v[2]));
   bool ok = true;
   for (int x = 0; x < 3; ++x)
     ok = ok && (absT(b.n1[x]) * b.s1 + absT(b.n2[x]) * b.s2 + absT(b.n3[x]) * b.s3 <= tol);
@@ -638,7 +646,8 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
         const int verdict = lane_check<T, CHECKED>(Px, Py, Pz, Pc, Pg, C, lane, cur, r, vt, vb);
         if (__any(verdict == 1)) continue;          // certainly infeasible
         if (CHECKED && (__any(verdict == 2) || !vb.ok)) { undecided = true; break; }
-        if (CHECKED && sizeof(T) == 4 && (wp.lam_v < 0.f || (per_plane && wp.lam_v > 0.f && !vertex_accurate(vb, vt))))
+        if (CHECKED && sizeof(T) == 4 &&
+            (wp.lam_v < 0.f || (per_plane && wp.lam_v > 0.f && !wp.vsel && !vertex_accurate(vb, vt))))
           vinacc = true;
         win = r;
         v[0] = vt[0]; v[1] = vt[1]; v[2] = vt[2];
@@ -1167,8 +1176,8 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         else why |= AM_WHY_BIT(AM_WHY_HMAX);
         break;
       }
-      if (sizeof(T) == 4 && (wp.lam_v < 0.f || (per_plane && wp.lam_v > 0.f && !vertex_accurate(vb, v))))
-        vinacc = true;
+      const bool vacc = sizeof(T) == 4 && per_plane && wp.lam_v > 0.f && vertex_accurate(vb, v);
+      if (sizeof(T) == 4 && (wp.lam_v < 0.f || (per_plane && wp.lam_v > 0.f && !wp.vsel && !vacc))) vinacc = true;
       // The reference's "same vertex" test (L1 < 3e-14, FP64): with coarse
       // planes the certified check above already puts the new vertex farther
       // than that from the last one; with FP64 planes it may not, so vertices
@@ -1186,12 +1195,14 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
         s_vpair[nv * 2 + 0] = r;
         s_vpair[nv * 2 + 1] = cur;
         s_tab[ntab] = r;
-        if (per_plane)
+        if (per_plane) {
           for (int x = 0; x < 3; ++x) {
             s_jinv[nv * 9 + x] = float(vb.n1[x]);
             s_jinv[nv * 9 + 3 + x] = float(vb.n2[x]);
             s_jinv[nv * 9 + 6 + x] = float(vb.n3[x]);
           }
+          if (wp.vsel && vacc) s_jinv[nv * 9] = __builtin_nanf("");   // no Newton step needed
+        }
       }
       nv += 1;
       ntab += 1;

@@ -144,13 +144,15 @@ class Engine:
 
     def __init__(self, nodes, arc_table=None, num_extra_constraints=0, precision="fp64", vm=20,
                  table_log2=None, walk="ray", ray_waves=None, scan_blocks=None, graph_levels=False,
-                 reduced_vertices="newton"):
+                 reduced_vertices="newton_where_needed"):
         """ray_waves / scan_blocks: launch shapes (see ``autotune``); None takes the tuned
         shape stored for the nearest network shape.  graph_levels: launch each level as one
-        captured hipGraph (every form; the stage times of the stats are then not taken).  reduced_vertices (fp32 form): "newton" (default)
-        gives each vertex solved from the reduced planes one FP64 Newton step after the march;
-        "fp64" re-solves every vertex from FP64 planes; "estimate" keeps the vertex solved from
-        the reduced planes where the face kernel's error estimate puts it within 1e-5 of |v|."""
+        captured hipGraph (every form; the stage times of the stats are then not taken).  reduced_vertices (fp32 form):
+        "newton_where_needed" (default) gives one FP64 Newton step after the march to each vertex
+        solved from the reduced planes that the face kernel's error estimate does not put within
+        1e-5 of |v|; "newton" gives every such vertex the step; "fp64" re-solves every vertex from
+        FP64 planes; "estimate" keeps the vertices of a region solved from the reduced planes
+        where the estimate puts all of them within 1e-5 of |v|, else re-solves them from FP64."""
         m = native()
         nodes = [int(n) for n in nodes]
         D = len(nodes) - 2
@@ -172,7 +174,7 @@ class Engine:
         if table_log2 is not None:
             self._eng.set_option("table_log2", int(table_log2))
         self._eng.set_option("walk", {"ray": 0, "scan": 1}[walk])
-        self._eng.set_option("reduced_vertices", {"fp64": 0, "estimate": 1, "newton": 2}[reduced_vertices])
+        self._eng.set_option("reduced_vertices", {"fp64": 0, "estimate": 1, "newton": 2, "newton_where_needed": 3}[reduced_vertices])
         from .autotune import lookup
         tuned = lookup(nodes)
         self.ray_waves = int(ray_waves if ray_waves is not None else tuned["ray_waves"])

@@ -148,13 +148,15 @@ am_status am_topology_fingerprint(am_ctx* ctx, unsigned long long* out5);
  * "walk" (0 ray walk, 1
  * candidate scan, for the reduced forms), "ray_waves_reduced" (waves per
  * region of the reduced forms' ray walk: 2, 4 (default) or 8; 4 measured
- * fastest on the 4x256 net, 2 and 8 give the same records), "reduced_vertices" (FP32 form: 2
- * default, each vertex solved in FP64 from the reduced planes takes one FP64
+ * fastest on the 4x256 net, 2 and 8 give the same records), "reduced_vertices" (FP32 form: 3
+ * default, each vertex solved in FP64 from the reduced planes that the face
+ * kernel's error estimate does not put within 1e-5 of |v| takes one FP64
  * Newton step after the march, its residuals from an FP64 forward pass of the
- * network under its region's mask; 0 every vertex of a region the reduced walk
- * decides is re-solved from FP64 planes; 1 the reduced-plane vertex is kept
- * where the face kernel's error estimate puts it within 1e-5 of |v|, the other
- * regions' are re-solved; the split-BF16 form always re-solves),
+ * network under its region's mask; 2 every such vertex takes the step; 0 every
+ * vertex of a region the reduced walk decides is re-solved from FP64 planes;
+ * 1 a region's reduced-plane vertices are kept where the estimate puts all of
+ * them within 1e-5 of |v|, the other regions' are re-solved; the split-BF16
+ * form always re-solves),
  * "adjoint_tiled" (1 default: the
  * adjoint on the tiled FP64 chain and GEMMs; 0 one kernel per region),
  * "adjoint_mfma" (1 default: those GEMMs on the FP64 matrix cores; 0 VALU),
