@@ -545,6 +545,7 @@ __device__ __forceinline__ void chain_tile_half(const NetDev& net, const ChainIO
 }
 
 __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net, ChainIO io) {
+  if (io.zero && blockIdx.x == 0 && threadIdx.x == 0) *io.zero = 0;
   const int r0 = blockIdx.x * AM_CT;
   const int cnt = launch_count(io);
   if (r0 >= cnt) return;
@@ -553,6 +554,7 @@ __global__ __launch_bounds__(AM_CHAIN_THREADS) void k_chain_tile_fp64(NetDev net
 }
 
 __global__ __launch_bounds__(AM_HALF_THREADS) void k_chain_tile_mfma(NetDev net, ChainIO io) {
+  if (io.zero && blockIdx.x == 0 && threadIdx.x == 0) *io.zero = 0;
   const int r0 = blockIdx.x * AM_HT;
   const int cnt = launch_count(io);
   if (r0 >= cnt) return;
@@ -681,15 +683,28 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
 
 // NW waves per region (the records do not depend on NW: every reduction breaks
 // ties by constraint index and each serial step runs on one wave).
+// With `left` (the counter zeroed before the launch), a region the walk
+// leaves undecided puts its chunk-local slot on the candidate scan's list and
+// counts itself and its reasons in left.run (as k_collect_slots would).
 template <int NW>
 __global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp,
                                                         const double* planes, const double* eqb, int Cs_hbm,
-                                                        int only_undecided) {
+                                                        int only_undecided, SlotList left) {
   if ((int)blockIdx.x >= launch_count(io)) return;
   const int rid = io.ids ? io.ids[blockIdx.x] : rid0 + blockIdx.x;
   // a region an earlier walk decided keeps its records (its flag is 0 and stays 0)
   if (only_undecided && !(io.flags[rid - io.first] & AM_UNDECIDED)) return;
   walk_ray64_body<NW>(net, io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
+  // thread 0 wrote the region's flag (finish_region)
+  if (left.cnt && threadIdx.x == 0) {
+    const unsigned f = io.flags[rid - io.first];
+    if (f & AM_UNDECIDED) {
+      left.slots[atomicAdd(left.cnt, 1)] = (int)blockIdx.x;
+      atomicAdd(left.run, 1);
+      for (int w = 0; w < AM_WHY_COUNT; ++w)
+        if (f & AM_WHY_BIT(w)) atomicAdd(left.run + 1 + w, 1);
+    }
+  }
 }
 
 template <int NW>
@@ -701,9 +716,12 @@ __global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray64_multi(const Net
   walk_ray64_body<NW>(nets[region_net(io, Wd, rid)], io, rid, blockIdx.x, wp, planes, eqb, Cs_hbm);
 }
 
-template __global__ void k_walk_ray64<2>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int);
-template __global__ void k_walk_ray64<4>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int);
-template __global__ void k_walk_ray64<8>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int);
+template __global__ void k_walk_ray64<2>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int,
+                                         SlotList);
+template __global__ void k_walk_ray64<4>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int,
+                                         SlotList);
+template __global__ void k_walk_ray64<8>(NetDev, RegionIO, int, WalkParams, const double*, const double*, int, int,
+                                         SlotList);
 template __global__ void k_walk_ray64_multi<2>(const NetDev*, RegionIO, int, int, WalkParams, const double*,
                                                const double*, int);
 template __global__ void k_walk_ray64_multi<4>(const NetDev*, RegionIO, int, int, WalkParams, const double*,

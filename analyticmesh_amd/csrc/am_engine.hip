@@ -102,6 +102,17 @@ void ensure(DevBuf<T>& b, size_t n, bool keep, hipStream_t st) {
 
 inline unsigned blocks(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
 
+// As ensure, for state that must read zero before its first use: a grown
+// buffer is cleared by a fill kernel (no memset node, should a caller capture
+// the stream later on).
+void ensure_zeroed(DevBuf<unsigned long long>& b, size_t n, hipStream_t st) {
+  if (b.n >= n) return;
+  ensure(b, n, false, st);
+  hipLaunchKernelGGL(k_fill_u32, dim3(blocks(b.n * 2, 256)), dim3(256), 0, st, reinterpret_cast<uint32_t*>(b.p),
+                     b.n * 2, 0u);
+  HIPCHK(hipGetLastError());
+}
+
 }  // namespace
 
 struct am_ctx {
@@ -194,6 +205,7 @@ struct am_ctx {
   std::vector<DevBuf<double>> adj_dW, adj_db, adj_G, adj_P;
   DevBuf<int> tslots, tcnt;               // undecided slots of a chunk and their count
   DevBuf<int> redo_ids, redo_cnt;         // regions a reduced-precision walk left to FP64, their count
+  DevBuf<unsigned long long> redo_state;  // tile state of the one-pass scan listing them (zero between uses)
   DevBuf<uint16_t> reg_jinv;              // [R][vm][9] bf16: the reduced walk's inverse vertex systems
   bool jinv_on = false;                   // this run keeps them (reduced form, Newton vertices)
   DevBuf<int> runacc;                     // per run: FP64 ray walk undecided count, then per reason
@@ -526,21 +538,21 @@ ChainIO chain_io(const am_ctx* c, const int* ids, int first, int n, int Ch) {
 
 // The FP64 ray walk with the context's waves per region.
 void launch_ray64(am_ctx* c, int n, size_t lds, const RegionIO& io, int rid0, int Ch, int only_undecided,
-                  const WalkParams& kRay64) {
+                  const WalkParams& kRay64, const SlotList& left = SlotList{}) {
   const double* P = c->tplanes.p;
   const double* E = c->teq.p;
   switch (c->ray_waves) {
     case 2:
       hipLaunchKernelGGL(k_walk_ray64<2>, dim3(n), dim3(128), lds, c->stream, c->net, io, rid0, kRay64, P, E, Ch,
-                         only_undecided);
+                         only_undecided, left);
       break;
     case 8:
       hipLaunchKernelGGL(k_walk_ray64<8>, dim3(n), dim3(512), lds, c->stream, c->net, io, rid0, kRay64, P, E, Ch,
-                         only_undecided);
+                         only_undecided, left);
       break;
     default:
       hipLaunchKernelGGL(k_walk_ray64<4>, dim3(n), dim3(256), lds, c->stream, c->net, io, rid0, kRay64, P, E, Ch,
-                         only_undecided);
+                         only_undecided, left);
   }
 }
 
@@ -565,20 +577,6 @@ size_t fp64_prepare(am_ctx* c, int count) {
   return chunk;
 }
 
-// Regions a reduced-precision walk left to FP64 (undecided, or decided with a
-// vertex to re-solve from FP64 planes), compacted in region order.
-struct RedoSrc {
-  const unsigned* flags;
-  __device__ int operator()(size_t i) const { return (flags[i] & (AM_UNDECIDED | AM_VERT64)) ? 1 : 0; }
-};
-struct RedoSink {
-  int* ids;
-  int first;
-  __device__ void operator()(size_t i, int prefix, int v) const {
-    if (v) ids[prefix] = first + (int)i;
-  }
-};
-
 void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int* ids, int mode, Fp64Counts& t,
                       long long undecided = -1, const int* dcount = nullptr) {
   if (!c->tile_ok) {
@@ -601,11 +599,19 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
   const size_t chunk = fp64_prepare(c, count);
   const size_t clds = chain_lds_bytes(c), hlds = chain_half_lds_bytes(c);
   StageTimer& tm = c->timer;
+  // regions the reduced walk may have decided with a vertex left to FP64 (the
+  // FP32 form's Newton modes leave none)
+  const WalkParams& rw = c->fast.wp;
+  const bool vert64 = rw.lam_v < 0.f || (rw.lam_v > 0.f && !rw.vsel);
+  // the FP64 ray walk lists what it leaves undecided for the candidate scan
+  // itself (the chain zeroes the list's counter)
+  const bool fused = mode == 1;
   for (int off = 0; off < count; off += (int)chunk) {
     const int n = std::min<int>((int)chunk, count - off);
     ChainIO cio = chain_io(c, ids ? ids + off : nullptr, first + off, n, Ch);
     cio.dcount = dcount;
     cio.dbase = off;
+    if (fused) cio.zero = c->tcnt.p;
     RegionIO io;
     io.reg_mask = c->reg_mask.p;
     io.reg_start = c->reg_start.p;
@@ -641,7 +647,7 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
     const size_t m1 = tm.mark(c->stream);
     tm.span(m0, m1, T_CHAIN64);
     size_t m1v = m1;
-    if (after_walk && undecided != count) {
+    if (after_walk && undecided != count && vert64) {
       hipLaunchKernelGGL(k_vertex_fp64, dim3(n), dim3(64), 0, c->stream, c->net, io, first + off,
                          (const double*)c->tplanes.p, (const double*)c->teq.p, Ch);
       HIPCHK(hipGetLastError());
@@ -658,16 +664,18 @@ void run_fp64_regions(am_ctx* c, int lvl_first, int first, int count, const int*
       continue;
     }
     if (mode == 1) {
-      launch_ray64(c, n, rlds, io, first + off, Ch, after_walk ? 1 : 0, mf ? kRay64Mf : kRay64);
+      launch_ray64(c, n, rlds, io, first + off, Ch, after_walk ? 1 : 0, mf ? kRay64Mf : kRay64,
+                   SlotList{c->tcnt.p, c->tslots.p, c->runacc.p});
       HIPCHK(hipGetLastError());
       const size_t m2 = tm.mark(c->stream);
       tm.span(m1v, m2, T_RAY64);
+    } else {
+      hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->tcnt.p, (size_t)1, 0);
+      hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream,
+                         (const unsigned*)c->rflags.p, n, dcount, off, (const int*)(ids ? ids + off : nullptr),
+                         first + off, lvl_first, c->tcnt.p, c->tslots.p, c->runacc.p);
+      HIPCHK(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, c->stream, c->tcnt.p, (size_t)1, 0);
-    hipLaunchKernelGGL(k_collect_slots, dim3((n + 255) / 256), dim3(256), 0, c->stream, (const unsigned*)c->rflags.p,
-                       n, dcount, off, (const int*)(ids ? ids + off : nullptr), first + off, lvl_first, c->tcnt.p,
-                       c->tslots.p, c->runacc.p);
-    HIPCHK(hipGetLastError());
     io.flags = nullptr;
     const size_t m3 = tm.mark(c->stream);
     hipLaunchKernelGGL(k_walk_fp64, dim3(std::min(n, c->scan_blocks)), dim3(64), wlds, c->stream, c->net, io, first + off,
@@ -1374,14 +1382,14 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         fast_level_prepare(c->fast, c->fbuf, n, (int)lvl_count);
         ensure(c->redo_ids, (size_t)lvl_count, false, st);
         ensure(c->redo_cnt, 1, false, st);
-        ensure(c->scan_part, scan_blocks((size_t)lvl_count) + 1, false, st);
+        ensure_zeroed(c->redo_state, redo_state_words((int)lvl_count), st);
         const InsertPlan plan = insert_prepare(c, ncand);
         const FastLevel fl = fast_level_of();
+        const RedoList redo{c->redo_ids.p, c->redo_cnt.p, c->redo_state.p};
         FastTimes ft{};
         launch_as_graph(c, (int)levels, [&]() {
-          fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT, true);
-          scan_launch<int>(RedoSrc{c->rflags.p}, (size_t)lvl_count, RedoSink{c->redo_ids.p, (int)lvl_first},
-                           c->scan_part.p, c->redo_cnt.p, st);
+          fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT, redo,
+                     true);
           run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, c->redo_ids.p, 2, f64, -1,
                            c->redo_cnt.p);
           insert_launch(c, plan);
@@ -1401,20 +1409,20 @@ am_status am_run(am_ctx* c, const void* const* weights, const void* const* biase
         launches += 1;
       } else {
         const FastLevel fl = fast_level_of();
+        ensure(c->redo_ids, (size_t)lvl_count, false, st);
+        ensure(c->redo_cnt, 1, false, st);
+        ensure_zeroed(c->redo_state, redo_state_words((int)lvl_count), st);
+        const RedoList redo{c->redo_ids.p, c->redo_cnt.p, c->redo_state.p};
         FastTimes ft{};
-        fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT);
-        // FP64 planes of the regions the walk left to FP64 only (their list
-        // and count stay on the device; the launches are sized for the whole
+        fast_level(c->fast, c->fbuf, n, fl, st, c->timer, ft, c->rflags.p, c->runacc.p + 1 + AM_WHY_COUNT, redo);
+        // FP64 planes of the regions the walk left to FP64 only (their list,
+        // made by fast_level's last launch, and count stay on the device;
+        // the launches are sized for the whole
         // level and their blocks past the count exit): the ones it decided
         // with a vertex it could not vouch for get their vertices re-solved
         // from them; the undecided ones are re-decided in place (records and
         // candidate slots rewritten) by the checked FP64 ray walk, and by the
         // candidate scan where that is undecided too.
-        ensure(c->redo_ids, (size_t)lvl_count, false, st);
-        ensure(c->redo_cnt, 1, false, st);
-        ensure(c->scan_part, scan_blocks((size_t)lvl_count) + 1, false, st);
-        scan_launch<int>(RedoSrc{c->rflags.p}, (size_t)lvl_count, RedoSink{c->redo_ids.p, (int)lvl_first},
-                         c->scan_part.p, c->redo_cnt.p, st);
         run_fp64_regions(c, (int)lvl_first, (int)lvl_first, (int)lvl_count, c->redo_ids.p, 2, f64, -1,
                          c->redo_cnt.p);
         launches += ft.launches;

@@ -85,15 +85,29 @@ struct FastTimes {
 // (float or double as `dtype`), the layouts are built on stream st.
 void fast_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int precision, const void* const* weights,
                   const void* const* biases, int dtype, hipStream_t st);
+// The regions of a level that a reduced walk left to FP64: ids[0, *count)
+// (device), and the tile state of the one-pass scan that lists them
+// (redo_state_words(level size) words, zero before the first use; every
+// launch leaves it zero again).
+struct RedoList {
+  int* ids;
+  int* count;
+  unsigned long long* state;
+};
+size_t redo_state_words(int count);
+
 // Planes + checked walk for one level; the walk marks each region in
-// flags[0, lvl.count) (AM_UNDECIDED plus reason bits, or 0 when it wrote the
-// region's records).  The undecided regions are added to acc[0] and their
-// reasons to acc[1 + w] on the device; nothing here waits.
+// flags[0, lvl.count) (AM_UNDECIDED plus reason bits, AM_VERT64, or 0 when it
+// wrote the region's records).  The last launch lists the regions marked
+// undecided or AM_VERT64 into `redo` (in region order) and adds the undecided
+// ones to acc[0], their reasons to acc[1 + w] and the AM_VERT64 ones to
+// acc[1 + AM_WHY_COUNT] on the device; nothing here waits.
 // Kernel times go to `timer` (T_GEMM, T_WALK).
 // With `prepared`, fast_level_prepare(f, b, net, lvl.count) has run before
 // (then fast_level only enqueues launches: it can be captured into a graph).
 void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc, bool prepared = false);
+                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc, const RedoList& redo,
+                bool prepared = false);
 // Buffers and launch attributes of fast_level for a level of `count` regions.
 void fast_level_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int count);
 // Planes of n given regions (their masks in `reg_mask`) with the prepared

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "am_fast.h"
+#include "am_scan.h"
 
 namespace am {
 
@@ -487,22 +488,77 @@ __global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray_f32(NetDev net, R
   walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec, with_sigma ? eb[4] : 0.f);
 }
 
-// Regions the walk left undecided, added to cnt[0], their reasons, added to
-// cnt[1 + w], and the decided ones with a vertex left to FP64, added to
-// cnt[1 + AM_WHY_COUNT] (counted per wave, one atomic per wave and counter).
-__global__ void k_count_undecided(const unsigned* flags, int n, int* cnt) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const unsigned f = i < n ? flags[i] : 0u;
-  const bool lead = (threadIdx.x & 63) == 0;
-  const int u = __popcll(__ballot((f & AM_UNDECIDED) != 0u));
-  if (lead && u) atomicAdd(cnt, u);
-  const int vv = __popcll(__ballot((f & (AM_UNDECIDED | AM_VERT64)) == AM_VERT64));
-  if (lead && vv) atomicAdd(cnt + 1 + AM_WHY_COUNT, vv);
-  for (int w = 0; w < AM_WHY_COUNT; ++w) {
-    const int k = __popcll(__ballot((f & AM_UNDECIDED) && (f & AM_WHY_BIT(w))));
-    if (lead && k) atomicAdd(cnt + 1 + w, k);
+// The level's regions the walk left to FP64 in one launch: their ids (first +
+// level index, in region order, through a one-pass scan of 2048-region
+// tiles) and count, and the counters: undecided regions added to acc[0],
+// their reasons to acc[1 + w], the decided ones with a vertex left to FP64 to
+// acc[1 + AM_WHY_COUNT] (one atomic per wave and counter).
+constexpr int kRedoThreads = 256, kRedoItems = 8, kRedoTile = kRedoThreads * kRedoItems;
+__global__ __launch_bounds__(kRedoThreads) void k_redo_list(const unsigned* flags, int n, int first, int* acc,
+                                                           RedoList out, int tiles) {
+  __shared__ int s_tile, s_excl;
+  __shared__ int s_wave[kRedoThreads / 64];
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid == 0) s_tile = (int)atomicAdd(&out.state[tiles], 1ull);
+  __syncthreads();
+  const int t = s_tile;
+  const int base = t * kRedoTile + tid * kRedoItems;
+  unsigned f[kRedoItems];
+  if (base + kRedoItems <= n) {
+    const uint4* q = reinterpret_cast<const uint4*>(flags + base);
+    const uint4 a = q[0], b = q[1];
+    f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w;
+    f[4] = b.x; f[5] = b.y; f[6] = b.z; f[7] = b.w;
+  } else {
+#pragma unroll
+    for (int it = 0; it < kRedoItems; ++it) f[it] = base + it < n ? flags[base + it] : 0u;
   }
+  int sum = 0, und = 0, v64 = 0;
+  int why[AM_WHY_COUNT];
+#pragma unroll
+  for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] = 0;
+#pragma unroll
+  for (int it = 0; it < kRedoItems; ++it) {
+    const unsigned x = f[it];
+    sum += (x & (AM_UNDECIDED | AM_VERT64)) ? 1 : 0;
+    if (x & AM_UNDECIDED) {
+      und += 1;
+#pragma unroll
+      for (int w = 0; w < AM_WHY_COUNT; ++w) why[w] += (x & AM_WHY_BIT(w)) ? 1 : 0;
+    } else if (x & AM_VERT64) {
+      v64 += 1;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    und += __shfl_xor(und, d, 64);
+    v64 += __shfl_xor(v64, d, 64);
+  }
+  if (lane == 0 && und) atomicAdd(acc, und);
+  if (lane == 0 && v64) atomicAdd(acc + 1 + AM_WHY_COUNT, v64);
+  if (__ballot(und != 0)) {
+#pragma unroll
+    for (int w = 0; w < AM_WHY_COUNT; ++w) {
+      int k = why[w];
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) k += __shfl_xor(k, d, 64);
+      if (lane == 0 && k) atomicAdd(acc + 1 + w, k);
+    }
+  }
+  int tot;
+  const int off = block_excl_scan<int, kRedoThreads>(sum, s_wave, &tot);
+  if (tid < 64) {
+    const int excl = lookback_1pass(out.state, tiles, t, tot, out.count);
+    if (lane == 0) s_excl = excl;
+  }
+  __syncthreads();
+  int run = s_excl + off;
+#pragma unroll
+  for (int it = 0; it < kRedoItems; ++it)
+    if (f[it] & (AM_UNDECIDED | AM_VERT64)) out.ids[run++] = first + base + it;
 }
+
+size_t redo_state_words(int count) { return (size_t)std::max(1, (count + kRedoTile - 1) / kRedoTile) + 2; }
 
 // ---------------------------------------------------------------------------
 // host side
@@ -726,7 +782,7 @@ void fast_level_prepare(FastNet& f, FastBuffers& b, const NetDev& net, int count
 }
 
 void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& lvl, hipStream_t st,
-                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc, bool prepared) {
+                StageTimer& timer, FastTimes& t, unsigned* flags, int* acc, const RedoList& redo, bool prepared) {
   const FastNetDev& d = f.dev;
   const int prec = f.precision == 1 ? 1 : 0;
   t.launches = 0;
@@ -793,7 +849,9 @@ void fast_level(FastNet& f, FastBuffers& b, const NetDev& net, const FastLevel& 
     timer.span(m1, timer.mark(st), T_WALK);
     t.launches += 1;
   }
-  hipLaunchKernelGGL(k_count_undecided, dim3((lvl.count + 255) / 256), dim3(256), 0, st, flags, lvl.count, acc);
+  const int tiles = (int)redo_state_words(lvl.count) - 2;
+  hipLaunchKernelGGL(k_redo_list, dim3(tiles), dim3(kRedoThreads), 0, st, (const unsigned*)flags, lvl.count,
+                     lvl.first, acc, redo, tiles);
   FCHK(hipGetLastError());
 }
 

@@ -86,6 +86,16 @@ struct ChainIO {
   int noutp[AM_MAX_LINEAR];         // n_out rounded up to AM_CR
   const int* dcount = nullptr;      // as in RegionIO: regions past *dcount - dbase are not valid
   int dbase = 0;
+  int* zero = nullptr;              // set to 0 by the launch (a counter that later launches of the chunk fill)
+};
+
+// A list of chunk-local slots filled by atomics (cnt zeroed before), with
+// counters of what was listed: run[0] the entries, run[1 + w] those with
+// reason bit w.
+struct SlotList {
+  int* cnt = nullptr;
+  int* slots = nullptr;
+  int* run = nullptr;
 };
 
 // The regions and plane buffers of one tiled-chain launch.
@@ -114,10 +124,11 @@ __global__ void k_walk_fp64(NetDev net, RegionIO io, int rid0, const int* count,
                             const double* planes, const double* eq, int Cs_hbm);
 struct WalkParams;
 // only_undecided: blocks whose region is not flagged AM_UNDECIDED in io.flags exit.
-// NW waves per region: 2, 4 or 8.
+// NW waves per region: 2, 4 or 8.  With left.cnt, the regions the walk leaves
+// undecided are listed as k_collect_slots lists them.
 template <int NW>
 __global__ void k_walk_ray64(NetDev net, RegionIO io, int rid0, WalkParams wp, const double* planes,
-                             const double* eq, int Cs_hbm, int only_undecided);
+                             const double* eq, int Cs_hbm, int only_undecided, SlotList left);
 // Batch-of-networks forms (region rid0 + b; its network is mask word Wd - 1;
 // the scan's LDS layout follows each network's C, the launch is sized for the
 // widest network).

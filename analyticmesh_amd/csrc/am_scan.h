@@ -119,6 +119,54 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(Src src, size_t n, c
   }
 }
 
+// Decoupled look-back (Merrill and Garland, 2016) of tile t whose own total
+// is tot, run by one whole wave (the block's first); returns the sum of the
+// tiles before t.  state[0, tiles) hold one word per tile, flag << 32 | value
+// (flag 1: the tile's own total, 2: its inclusive prefix); state[tiles] is
+// the counter the blocks took their tile numbers from (in launch order, so a
+// tile only waits on tiles already running) and state[tiles + 1] counts the
+// tiles done looking back.  The last of those sets all tiles + 2 words back
+// to zero: the next launch on the same state needs no clearing.  *total (when
+// not null) receives the sum of every tile.  Values stay below 2^31.
+__device__ __forceinline__ int lookback_1pass(unsigned long long* state, int tiles, int t, int tot, int* total) {
+  constexpr unsigned long long F1 = 1ull << 32, F2 = 2ull << 32;
+  const int lane = threadIdx.x & 63;
+  if (lane == 0)
+    __hip_atomic_store(&state[t], (t == 0 ? F2 : F1) | (unsigned)tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  int excl = 0;
+  if (t > 0) {
+    for (int look = t - 1;; look -= 64) {
+      const int idx = look - lane;   // lane 0: the nearest tile
+      unsigned long long st =
+          idx >= 0 ? __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : F2;
+      while (__ballot((st >> 32) == 0) != 0ull) {
+        if ((st >> 32) == 0) {
+          __builtin_amdgcn_s_sleep(1);
+          st = __hip_atomic_load(&state[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
+      const unsigned long long b2 = __ballot((st >> 32) == 2);
+      const int stop = b2 ? __ffsll((long long)b2) - 1 : 63;
+      int x = lane <= stop ? (int)(unsigned)st : 0;
+#pragma unroll
+      for (int d = 32; d > 0; d >>= 1) x += __shfl_xor(x, d, 64);
+      excl += x;
+      if (b2) break;
+    }
+    if (lane == 0)
+      __hip_atomic_store(&state[t], F2 | (unsigned)(excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (lane == 0 && t == tiles - 1 && total) *total = excl + tot;
+  // this tile reads no more state: the last one done clears it
+  int done = 0;
+  if (lane == 0) done = (int)atomicAdd(&state[tiles + 1], 1ull);
+  done = __shfl(done, 0, 64);
+  if (done == tiles - 1)
+    for (int i = lane; i < tiles + 2; i += 64)
+      __hip_atomic_store(&state[i], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
 // Plain sources and sinks.
 template <typename T>
 struct ArraySrc {

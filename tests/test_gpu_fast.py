@@ -57,3 +57,27 @@ def test_fast_deep_vs_fp64(prec):
     if prec != "fp64ray":
         assert int(g.stats[9]) + int(g.stats[33]) <= len(a)
     assert err < VERT_TOL[prec]
+
+
+@pytest.mark.parametrize("mode", ["newton_where_needed", "newton", "estimate", "fp64"])
+def test_fp32_vertex_modes(mode):
+    """Each way the FP32 form gets its vertices to 1e-5: the same topology as the FP64 form and
+    vertices within 1e-5 of its (relative to |v|); the Newton step runs on at most every vertex."""
+    from analyticmesh_amd.engine import march
+    spec, pts, st = workload("deep", n_seeds=64)
+    _say(f"[fp32 reduced_vertices={mode}] deep x64 seeds")
+    ref = march(spec, pts, st, precision="fp64", max_regions=30000)
+    g = march(spec, pts, st, precision="fp32", max_regions=30000, reduced_vertices=mode)
+    a, b = g.records(), ref.records()
+    assert_same_topology(a, b)
+    err = max_vertex_error(a, b)
+    newton = int(g.stats[38])
+    _say(f"  regions {len(a)} redo {int(g.stats[9])} re-solved {int(g.stats[33])} Newton {newton} "
+         f"max vertex rel err {err:.2e}")
+    assert err < 1e-5
+    if mode in ("estimate", "fp64"):
+        assert newton == 0
+    else:
+        assert 0 < newton <= int(np.sum(a.info[:, 2]))
+    if mode == "fp64":
+        assert 0 < int(g.stats[9]) + int(g.stats[33]) <= len(a)
