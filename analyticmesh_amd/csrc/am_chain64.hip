@@ -480,9 +480,19 @@ __device__ __forceinline__ void chain_tile_half(const NetDev& net, const ChainIO
         }
       }
     }
+    // the park's biases (column c of each region: lanes with l16 % 4 == 3),
+    // loaded while the other waves finish their products
+    const bool cbias = (l16 & 3) == 3;
+    double bias[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int row = row0 + 16 * rb + l4 + 4 * v;
+        bias[rb][v] = cbias && rb < nrb && row < nout ? io.bp[li][row] : 0.0;
+      }
     __syncthreads();                  // every wave is done reading this layer's input
     {
-      const bool cbias = (l16 & 3) == 3;
 #pragma unroll
       for (int rb = 0; rb < 2; ++rb) {
         if (rb >= nrb) continue;
@@ -490,7 +500,7 @@ __device__ __forceinline__ void chain_tile_half(const NetDev& net, const ChainIO
         for (int v = 0; v < 4; ++v) {
           const int row = row0 + 16 * rb + l4 + 4 * v;
           if (row < nout) {
-            const double b = cbias ? io.bp[li][row] : 0.0;
+            const double b = bias[rb][v];
             Xs[row * AM_HXS + l16] = acc[rb][0][v] + b;
             Xs[row * AM_HXS + 16 + l16] = acc[rb][1][v] + b;
           }

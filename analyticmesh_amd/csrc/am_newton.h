@@ -23,7 +23,7 @@ struct NewtonIO {
 
 // LDS of k_newton_fp64 for a network with Wd mask words.
 inline size_t newton_lds_bytes(int Wd) {
-  return (size_t)AM_CHAIN_MAXW * AM_NT * sizeof(double) + (size_t)AM_NT * Wd * 4;
+  return (size_t)AM_CHAIN_MAXW * AM_NT * sizeof(double) + (size_t)AM_NT * (Wd | 1) * 4;
 }
 
 // Grid: one workgroup per AM_NT items of the host's upper bound; blocks past

@@ -39,7 +39,9 @@ __device__ __forceinline__ int xat(int k, int c) { return k * AM_NT + (c ^ ((k &
 __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, ChainIO cio, NewtonIO io) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   double* X = reinterpret_cast<double*>(smem_raw);                             // [AM_CHAIN_MAXW][AM_NT]
-  uint32_t* smask = reinterpret_cast<uint32_t*>(X + (size_t)AM_CHAIN_MAXW * AM_NT);   // [AM_NT][Wd]
+  // [AM_NT][Ws]: an odd row stride, so lanes reading one word of different
+  // columns' masks hit different banks
+  uint32_t* smask = reinterpret_cast<uint32_t*>(X + (size_t)AM_CHAIN_MAXW * AM_NT);
   constexpr int NPART = AM_NEWTON_THREADS / AM_NT;   // row groups of the VALU layers
   __shared__ double s_v[AM_NT][3];
   __shared__ double s_z[AM_NT][2];
@@ -52,7 +54,7 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
   const int total = io.count ? *io.count : 0;
   const int n = min(AM_NT, total - base);
   if (n <= 0) return;
-  const int Wd = net.Wd, vm = net.vm;
+  const int Wd = net.Wd, vm = net.vm, Ws = Wd | 1;
   if (tid < AM_NT) {
     int r = 0, k = 0, a = -1, b = -1;
     double v0 = 0.0, v1 = 0.0, v2 = 0.0;
@@ -73,22 +75,39 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
   __syncthreads();
   for (int idx = tid; idx < AM_NT * Wd; idx += AM_NEWTON_THREADS) {
     const int c = idx / Wd, w = idx - c * Wd;
-    smask[idx] = c < n ? io.reg_mask[(size_t)s_rk[c][0] * Wd + w] : 0u;
+    smask[c * Ws + w] = c < n ? io.reg_mask[(size_t)s_rk[c][0] * Wd + w] : 0u;
   }
   __syncthreads();
   // hidden layer 1: z_j = W0[j] . v + b0[j]; thread: column c, rows part, part + NPART, ...
   {
     const int c = tid % AM_NT, part = tid / AM_NT;
-    const uint32_t* mk = smask + (size_t)c * Wd;
+    const uint32_t* mk = smask + (size_t)c * Ws;
     const int ca = s_ab[c][0], cb = s_ab[c][1];
     const int n1 = net.nodes[1];
     const double x0 = s_v[c][0], x1 = s_v[c][1], x2 = s_v[c][2];
-    for (int j = part; j < n1; j += NPART) {
-      const double z = __builtin_fma(net.W0[j * 3 + 2], x2, __builtin_fma(net.W0[j * 3 + 1], x1,
-                                     __builtin_fma(net.W0[j * 3 + 0], x0, net.b[0][j])));
-      if (j == ca) s_z[c][0] = z;
-      if (j == cb) s_z[c][1] = z;
-      X[xat(j, c)] = mbit(mk, j) ? z : 0.0;
+    // four rows at a time, their weight loads in flight together
+    constexpr int U = 4;
+    for (int j0 = part; j0 < n1; j0 += U * NPART) {
+      double w[U][4];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * NPART;
+        const bool in = j < n1;
+        w[u][0] = in ? net.W0[j * 3 + 0] : 0.0;
+        w[u][1] = in ? net.W0[j * 3 + 1] : 0.0;
+        w[u][2] = in ? net.W0[j * 3 + 2] : 0.0;
+        w[u][3] = in ? net.b[0][j] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int j = j0 + u * NPART;
+        if (j < n1) {
+          const double z = __builtin_fma(w[u][2], x2, __builtin_fma(w[u][1], x1, __builtin_fma(w[u][0], x0, w[u][3])));
+          if (j == ca) s_z[c][0] = z;
+          if (j == cb) s_z[c][1] = z;
+          X[xat(j, c)] = mbit(mk, j) ? z : 0.0;
+        }
+      }
     }
   }
   __syncthreads();
@@ -142,6 +161,15 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
         }
       }
     }
+    // the epilogue's biases, loaded while the other waves finish their products
+    double bias[2][4];
+#pragma unroll
+    for (int rb = 0; rb < 2; ++rb)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        const int j = row0 + 16 * rb + l4 + 4 * v;
+        bias[rb][v] = rb < nrb && j < nout ? cio.bp[li][j] : 0.0;
+      }
     __syncthreads();   // every wave has read this layer's input
     const int out0 = net.off[li];
 #pragma unroll
@@ -151,12 +179,12 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
       for (int cb = 0; cb < 2; ++cb) {
         const int c = 16 * cb + l16;
         const int ac = s_ab[c][0], bc = s_ab[c][1];
-        const uint32_t* mc = smask + (size_t)c * Wd;
+        const uint32_t* mc = smask + (size_t)c * Ws;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           const int j = row0 + 16 * rb + l4 + 4 * v;
           if (j < nout) {
-            const double z = acc[rb][cb][v] + cio.bp[li][j];
+            const double z = acc[rb][cb][v] + bias[rb][v];
             const int g = out0 + j;
             if (g == ac) s_z[c][0] = z;
             if (g == bc) s_z[c][1] = z;
@@ -181,7 +209,7 @@ __global__ __launch_bounds__(AM_NEWTON_THREADS) void k_newton_fp64(NetDev net, C
   double f = cio.bp[net.D][0];
   for (int q = 0; q < NPART; ++q) f += s_part[q][tid];
   // residuals of the signed planes of the vertex's system and the step
-  const uint32_t* mk = smask + (size_t)tid * Wd;
+  const uint32_t* mk = smask + (size_t)tid * Ws;
   const int r = s_rk[tid][0], k = s_rk[tid][1];
   const double v[3] = {s_v[tid][0], s_v[tid][1], s_v[tid][2]};
   double res[3];
