@@ -222,10 +222,12 @@ __device__ __forceinline__ int checked_verdict_at(T s, T ax, T ay, T az, T c, T 
   const T pm = fmax(fmax(absT(ax), absT(ay)), fmax(absT(az), absT(c)));
   T err;
   if (sizeof(T) == 4) {
-    const T l1 = b.n1[0] * ax + b.n1[1] * ay + b.n1[2] * az;
-    const T l2 = b.n2[0] * ax + b.n2[1] * ay + b.n2[2] * az;
-    const T l3 = b.n3[0] * ax + b.n3[1] * ay + b.n3[2] * az;
-    err = b.e0 * pm + b.eg * eg + b.w1 * absT(l1) + b.w2 * absT(l2) + b.w3 * absT(l3);
+    // fused multiply-adds: one rounding per term (the bound's slack covers
+    // its own evaluation either way)
+    const T l1 = fmaT(b.n1[2], az, fmaT(b.n1[1], ay, b.n1[0] * ax));
+    const T l2 = fmaT(b.n2[2], az, fmaT(b.n2[1], ay, b.n2[0] * ax));
+    const T l3 = fmaT(b.n3[2], az, fmaT(b.n3[1], ay, b.n3[0] * ax));
+    err = fmaT(b.w3, absT(l3), fmaT(b.w2, absT(l2), fmaT(b.w1, absT(l1), fmaT(b.eg, eg, b.e0 * pm))));
   } else {
     err = b.E * pm;
   }
