@@ -674,6 +674,7 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
   const WalkRecs rec = carve_walk_recs(reinterpret_cast<unsigned char*>(Q + 2 * Cl) + ((net.Wd * 4 + 15) & ~15),
                                        net.vm, false);
   const int lane = threadIdx.x;
+  const int start_in = io.reg_start[rid];
   for (int w = lane; w < net.Wd; w += 64 * NW) smask[w] = io.reg_mask[(size_t)rid * net.Wd + w];
   const double* src = planes + (size_t)b * 4 * Cs_hbm + n1;
   for (int i = lane; i < Cl; i += 64 * NW) {
@@ -688,7 +689,7 @@ __device__ __forceinline__ void walk_ray64_body(const NetDev& net, const RegionI
                         eqb[(size_t)b * 4 + 3]};
   __syncthreads();
   const PlaneView<double> P{reinterpret_cast<const double*>(Q), Cl, n1, net.W0, net.b[0], smask};
-  walk_ray<double, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec);
+  walk_ray<double, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec, start_in);
 }
 
 // NW waves per region (the records do not depend on NW: every reduction breaks

@@ -471,6 +471,7 @@ __global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray_f32(NetDev net, R
   const int b = blockIdx.x;
   if (b >= io.count) return;
   const int rid = chunk_first + b;
+  const int start_in = io.reg_start[rid];
   const float* src = planes + (size_t)b * AM_PCOMP * Cs_hbm;
   const bool with_sigma = wp.lam > 0.f;
   for (int g = lane; g < C; g += 64 * NW) {
@@ -485,7 +486,7 @@ __global__ __launch_bounds__(64 * NW, 24 / NW) void k_walk_ray_f32(NetDev net, R
   __syncthreads();
   const PlaneView<float> P{reinterpret_cast<const float*>(Q), Cs, 0, nullptr, nullptr, smask,
                            with_sigma ? Pe : nullptr};
-  walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec, with_sigma ? eb[4] : 0.f);
+  walk_ray<float, NW>(net, io, wp, rid, rid - io.first, P, smask, eq, p, rec, start_in, with_sigma ? eb[4] : 0.f);
 }
 
 // The level's regions the walk left to FP64 in one launch: their ids (first +

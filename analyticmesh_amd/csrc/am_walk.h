@@ -356,15 +356,16 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
     io.cand_point[(slot0 + m) * 3 + 2] = (double)mid[2];
   }
   const int Wd = net.Wd;
-  for (int idx = lane; idx < vm * Wd; idx += nth) {
-    const int m = idx / Wd, w = idx - m * Wd;
+  // candidate m's mask: waves take candidates, lanes take words (no division)
+  for (int m = lane >> 6; m < vm; m += (nth + 63) >> 6) {
     int t = -1;
     if (m + 1 < ntab) t = s_tab[m];
-    if (t >= 0 && t < L) {
-      uint32_t word = smask[w];
-      if ((t >> 5) == w) word ^= 1u << (t & 31);
-      io.cand_mask[(slot0 + m) * Wd + w] = word;
-    }
+    if (t >= 0 && t < L)
+      for (int w = lane & 63; w < Wd; w += 64) {
+        uint32_t word = smask[w];
+        if ((t >> 5) == w) word ^= 1u << (t & 31);
+        io.cand_mask[(slot0 + m) * Wd + w] = word;
+      }
   }
   // each candidate's key hash: the region's sum of word hashes with the
   // flipped word's term swapped (wave 0; Wd <= 64 and vm <= 64)
@@ -1023,10 +1024,12 @@ __device__ inline WalkRecs carve_walk_recs(unsigned char* p, int vm, bool jinv) 
 // P.Pe and wp.lam > 0; the vertices are then solved in FP64 from the coarse
 // planes, and a region whose vertices the estimate cannot put within
 // AM_VERT_TOL gets the AM_VERT64 flag).
+// start_in: the region's start plane (io.reg_start[rid]), read by the caller
+// before it stages the planes, so its latency hides behind theirs.
 template <typename T, int NW>
 __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                          const PlaneView<T>& P, const uint32_t* smask, const T* eq, const T* p, const WalkRecs& rec,
-                         T eqe = T(0)) {
+                         int start_in, T eqe = T(0)) {
   const bool per_plane = sizeof(T) == 4 && wp.lam > 0.f && P.Pe != nullptr;
   bool vinacc = false;
   int* const s_tab = rec.tab;
@@ -1039,7 +1042,6 @@ __device__ void walk_ray(const NetDev& net, const RegionIO& io, const WalkParams
   __shared__ double s_vd[3];           // coarse planes: the vertex solved in FP64
   const int C = net.C, vm = net.vm;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int start_in = io.reg_start[rid];
   const int NONE = 0x7fffffff;
   bool undecided = false;
   unsigned why = 0u;
