@@ -913,6 +913,40 @@ __device__ __forceinline__ void block_argmin_or(T& t, int& g, int& bits, int& ph
   bits = b;
 }
 
+// Two argmins of (t, g) over the block with one exchange and one barrier
+// (slots alternate with `phase` as in block_argmin_or).
+template <typename T, int NW>
+__device__ __forceinline__ void block_argmin2(T& t1, int& g1, T& t2, int& g2, int& phase) {
+  wave_argmin(t1, g1);
+  wave_argmin(t2, g2);
+  if (NW == 1) return;
+  __shared__ T red_t[2][2][NW];
+  __shared__ int red_g[2][2][NW];
+  const int wave = threadIdx.x >> 6, sl = phase & 1;
+  ++phase;
+  if ((threadIdx.x & 63) == 0) {
+    red_t[sl][0][wave] = t1;
+    red_g[sl][0][wave] = g1;
+    red_t[sl][1][wave] = t2;
+    red_g[sl][1][wave] = g2;
+  }
+  __syncthreads();
+  t1 = red_t[sl][0][0];
+  g1 = red_g[sl][0][0];
+  t2 = red_t[sl][1][0];
+  g2 = red_g[sl][1][0];
+  for (int w = 1; w < NW; ++w) {
+    if (red_t[sl][0][w] < t1 || (red_t[sl][0][w] == t1 && red_g[sl][0][w] < g1)) {
+      t1 = red_t[sl][0][w];
+      g1 = red_g[sl][0][w];
+    }
+    if (red_t[sl][1][w] < t2 || (red_t[sl][1][w] == t2 && red_g[sl][1][w] < g2)) {
+      t2 = red_t[sl][1][w];
+      g2 = red_g[sl][1][w];
+    }
+  }
+}
+
 // One pass over the constraints for a proposed vertex v: its checked verdict
 // bits over all constraints but c1/c2 (bit 0 certainly violated, bit 1
 // undecided), and the nearest plane crossed by the ray v + t d, skipping
@@ -975,9 +1009,7 @@ __device__ __forceinline__ void shoot_both(const PlaneView<T>& P, int C, const T
   }
   T t1 = dp > T(0) ? np / dp : T(__builtin_inff());
   T t2 = dmn > T(0) ? nm / dmn : T(__builtin_inff());
-  int z = 0;
-  block_argmin_or<T, NW>(t1, bp, z, phase);
-  block_argmin_or<T, NW>(t2, bm, z, phase);
+  block_argmin2<T, NW>(t1, bp, t2, bm, phase);
   tp = t1; gp = bp; tm = t2; gm = bm;
 }
 
