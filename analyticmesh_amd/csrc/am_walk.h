@@ -266,6 +266,7 @@ __device__ __forceinline__ int lane_check(const T* Px, const T* Py, const T* Pz,
 // Orientation, region record and neighbour candidates of a finished walk
 // (backend/inc/kernel.h:1325-1474).  In CHECKED mode an undecided region only
 // raises its flag: the FP64 redo writes its record and candidate slots.
+// Callers reach it after a barrier that makes the walk's LDS records visible.
 template <typename T, bool CHECKED, typename V = T>
 __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkParams& wp, int rid, int slot_region,
                               const uint32_t* smask, const T* eq, int* s_tab, V* s_verts, int* s_vpair, int ntab,
@@ -287,28 +288,10 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
         if (!(absT(dir) > V(wp.eps_turn) * mag)) { undecided = true; why |= AM_WHY_BIT(AM_WHY_ORIENT); }
       }
       reversed = ((dir < V(0)) != (net.flip != 0)) ? 1 : 0;
-      __syncthreads();
-      if (!undecided && reversed && lane < nv / 2) {
-        const int a = lane, b = nv - 1 - lane;
-        for (int c = 0; c < 3; ++c) {
-          const V t = s_verts[a * 3 + c];
-          s_verts[a * 3 + c] = s_verts[b * 3 + c];
-          s_verts[b * 3 + c] = t;
-        }
-        for (int c = 0; c < 2; ++c) {
-          const int t = s_vpair[a * 2 + c];
-          s_vpair[a * 2 + c] = s_vpair[b * 2 + c];
-          s_vpair[b * 2 + c] = t;
-        }
-        if (s_jinv)
-          for (int c = 0; c < 9; ++c) {
-            const float t = s_jinv[a * 9 + c];
-            s_jinv[a * 9 + c] = s_jinv[b * 9 + c];
-            s_jinv[b * 9 + c] = t;
-          }
-      }
-      __syncthreads();
   }
+  // a reversed polygon's vertices are read back to front below (vertex k of
+  // the record is vertex nv - 1 - k of the walk), with no reordering in LDS
+  const bool rev = nv >= 3 && !undecided && reversed;
   if (CHECKED && (status != AM_ST_OK)) undecided = true;
   if (CHECKED && undecided) {
     if (lane == 0) io.flags[slot_region] = AM_UNDECIDED | why;
@@ -324,11 +307,19 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
     inf[7] = (s_jinv && io.reg_jinv) ? 1 : 0;
   }
   if (s_jinv && io.reg_jinv)
-    for (int i = lane; i < nv * 9; i += nth) io.reg_jinv[(size_t)rid * vm * 9 + i] = bf16_bits(s_jinv[i]);
+    for (int i = lane; i < nv * 9; i += nth) {
+      const int k = i / 9, c = i - 9 * k;
+      io.reg_jinv[(size_t)rid * vm * 9 + i] = bf16_bits(s_jinv[(rev ? nv - 1 - k : k) * 9 + c]);
+    }
   for (int i = lane; i < vm + 1; i += nth) io.reg_tab[(size_t)rid * (vm + 1) + i] = i < ntab ? s_tab[i] : 0;
-  for (int i = lane; i < vm * 3; i += nth)
-    io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[i] : 0.0;
-  for (int i = lane; i < vm * 2; i += nth) io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[i] : 0;
+  for (int i = lane; i < vm * 3; i += nth) {
+    const int k = i / 3, c = i - 3 * k;
+    io.reg_verts[(size_t)rid * vm * 3 + i] = i < nv * 3 ? (double)s_verts[(rev ? nv - 1 - k : k) * 3 + c] : 0.0;
+  }
+  for (int i = lane; i < vm * 2; i += nth) {
+    const int k = i >> 1, c = i & 1;
+    io.reg_vpair[(size_t)rid * vm * 2 + i] = i < nv * 2 ? s_vpair[(rev ? nv - 1 - k : k) * 2 + c] : 0;
+  }
 
   // ---- neighbour candidates (backend/inc/kernel.h:1390-1474)
   const size_t slot0 = (size_t)slot_region * vm;
@@ -346,8 +337,9 @@ __device__ void finish_region(const NetDev& net, const RegionIO& io, const WalkP
         if (i == m_) v1 = cnt;
       }
       if (nv > 0 && v1 >= 0) {
-        const int v2 = (v1 + 1) % nv;
-        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[v1 * 3 + c] + s_verts[v2 * 3 + c]) / V(2);
+        int v2 = (v1 + 1) % nv;
+        const int w1 = rev ? nv - 1 - v1 : v1, w2 = rev ? nv - 1 - v2 : v2;
+        for (int c = 0; c < 3; ++c) mid[c] = (s_verts[w1 * 3 + c] + s_verts[w2 * 3 + c]) / V(2);
       }
     }
     io.cand_start[slot0 + m] = valid ? t : -2;
@@ -726,6 +718,7 @@ __device__ void walk_region(const NetDev& net, const RegionIO& io, const WalkPar
       if (!closed || status != AM_ST_OK) undecided = true;   // truncated or stuck: FP64 decides
     }
   }
+  __syncthreads();   // the last records are visible to every thread
   finish_region<T, CHECKED>(net, io, wp, rid, slot_region, smask, eq, s_tab, s_verts, s_vpair, ntab, nv, start_in,
                             start, status, undecided, undecided ? AM_WHY_BIT(AM_WHY_SCAN) : 0u, vinacc);
 }
