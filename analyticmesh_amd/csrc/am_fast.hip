@@ -287,6 +287,15 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
         }
       }
     }
+    // the park's biases, loaded while the other waves finish their products
+    float bv[AM_GRBW][4];
+#pragma unroll
+    for (int u = 0; u < AM_GRBW; ++u)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int rb = wave + 4 * u, row = rb * 16 + 4 * (lane >> 4) + i;
+        bv[u][i] = rb < nrb && row < nout ? bias[row] : 0.f;
+      }
     // the outputs go over X, so every wave must be done reading it
     __syncthreads();
 #pragma unroll
@@ -299,7 +308,7 @@ __global__ __launch_bounds__(256, PREC == 0 ? 4 : 1) void k_gemm(FastNetDev f, G
       for (int i = 0; i < 4; ++i) {
         const int row = rb * 16 + 4 * (lane >> 4) + i;
         if (row < nout) {
-          const float b = bias[row];
+          const float b = bv[u][i];
           Ys[row * AM_XS + c0] = acc0[i] + ((c0 & 3) == 3 ? b : 0.f);
           Ys[row * AM_XS + 16 + c0] = acc1[i] + (((16 + c0) & 3) == 3 ? b : 0.f);
           // V of plane row, region c0, in row row's padding
